@@ -1,0 +1,182 @@
+/*
+ * sharpclaw_amd: MI355X (gfx950) implementation of SharpClaw's semi-discrete
+ * step -- WENO5 reconstruction, Riemann solve and SSP Runge-Kutta stage --
+ * behind a plain C ABI.  Every array crossing this boundary is a host array
+ * in the reference's Fortran layout: column-major, lower bound 1-mbc, one
+ * contiguous plane per component (src/1d/qalloc1.f90:10-29,
+ * src/2d/qalloc2.f90:11-30), i.e. q(1-mbc:nx+mbc[,1-mbc:ny+mbc], meqn).
+ *
+ * All entry points return 0 on success and a nonzero SC_ERR_* code on
+ * failure; sc_last_error() then describes the failure.  The reference
+ * reports the same conditions with write(6,*) + stop (src/flux1.f90:111-115,
+ * src/1d/bc1.f90:34-38, src/main.f90:140-145).
+ *
+ * Threading: a solver handle is used by one host thread at a time; its work
+ * is ordered on one HIP stream owned by the handle.
+ */
+#ifndef SHARPCLAW_AMD_H
+#define SHARPCLAW_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SC_OK 0
+#define SC_ERR_ARG 1         /* invalid argument or option (reference: stop) */
+#define SC_ERR_UNSUPPORTED 2 /* option the device path does not implement */
+#define SC_ERR_HIP 3         /* HIP runtime failure */
+#define SC_ERR_NOMEM 4       /* device allocation failed (qalloc STAT check) */
+#define SC_ERR_STATE 5       /* call out of order (e.g. no q uploaded) */
+
+/* physics: the Riemann solver / eigenvector plugin pair on the device.
+ * Each id names the reference plugin it replaces:
+ *   ADVECTION      apps/advection/1d/rp1.f + evec.f            (meqn=mwaves=1)
+ *   ACOUSTICS      apps/acoustics/1d/homogeneous/rp1.f + evec.f (meqn=mwaves=2)
+ *   EULER     1D   apps/euler/shockentropy/rp1eu.f + Euler evec (meqn=mwaves=3)
+ *             2D   Roe rpn2 for Euler + direction-aware evec   (meqn=4, mwaves=3)
+ *   SHALLOW_WATER 1D  .../1dRadialSymm-qwave/rp1sw.f + evec     (meqn=mwaves=2)
+ *                 2D  apps/shallowWater/2dRadialDam/rpn2sw.f + evec (meqn=mwaves=3) */
+#define SC_PHYS_ADVECTION 1
+#define SC_PHYS_ACOUSTICS 2
+#define SC_PHYS_EULER 3
+#define SC_PHYS_SHALLOW_WATER 4
+
+/* arithmetic of reconstruction + Riemann solve.  FP32 keeps float64 state,
+ * float64 stage accumulation, dt/CFL selection and conservation sums. */
+#define SC_FP64 0
+#define SC_FP32 1
+
+/* Everything module ClawParams (src/ClawParams.f90:4-14) and the physics
+ * common blocks (/comrp/, /param/, /cparam/) carry into the hot path. */
+typedef struct sc_params {
+    int ndim;            /* 1 or 2 */
+    int nx[2];           /* interior cells per direction (nx[1] ignored in 1D) */
+    int meqn;
+    int mwaves;
+    int mbc;             /* must be 3 (reconstruct.f90:85,158 hard-code it) */
+    int maux;            /* must be 0 on the device path */
+    int mcapa;           /* must be 0 on the device path */
+    int lim_type;        /* must be 2 (WENO5) on the device path */
+    int char_decomp;     /* 0 component-wise weno5, 2 characteristic weno5_char */
+    int tfluct_solver;   /* must be 0 (swapped-Riemann in-cell path, flux1.f90:155-188) */
+    int time_integrator; /* 1 SSP22, 2 SSP42, 3 SSP33, 4 SSP(10,4) */
+    int src_term;        /* must be 0 (default src: dq = 0 in the interior) */
+    int mthbc[4];        /* 1 extrapolation, 2 periodic, 3 wall (bc1.f90, bc2.f90) */
+    double xlower[2];
+    double dx[2];
+    int physics;         /* SC_PHYS_* */
+    int efix;            /* -1: the reference solver's default; 0 off; 1 on */
+    double gamma;        /* Euler (/param/ gamma; gamma1 = gamma-1) */
+    double grav;         /* shallow water (/comrp/ grav) */
+    double u;            /* advection speed (/comrp/ u) */
+    double rho, bulk;    /* acoustics (/cparam/: cc = sqrt(bulk/rho), zz = cc*rho) */
+    int precision;       /* SC_FP64 or SC_FP32 */
+    int device;          /* HIP device ordinal */
+} sc_params;
+
+/* Time-step control of src/sharpclaw.f90:2-243 (read from sharpclaw.data). */
+typedef struct sc_run_ctl {
+    double dt_max;
+    double cfl_max;
+    double cfl_desired;
+    int max_steps;
+    int dt_variable;     /* 1: variable dt.  qold is refreshed every step in
+                            both modes (the reference only does so for 1). */
+} sc_run_ctl;
+
+/* What sharpclaw() returns to main (sharpclaw.f90:230-235) plus counters. */
+typedef struct sc_run_stats {
+    double t;            /* time reached (tend on success) */
+    double cfl_maxused, cfl_last;
+    double dt_minused, dt_maxused, dt_last;
+    int steps_taken;
+    int info;            /* 0, 3 (cfl_desired > cfl_max), 11 (max_steps) */
+    int rejected;        /* rejected step attempts */
+    long long stage_evals; /* semi-discrete evaluations, rejected ones included */
+    double gpu_ms;       /* device time of the stage work (HIP events) */
+} sc_run_stats;
+
+typedef struct sc_solver sc_solver;
+
+const char *sc_last_error(void);
+int sc_version(void);
+
+/* ---- solver handle: device-resident q, qold and RK registers ---------- */
+
+/* Allocates q, qold and the RK registers on params->device
+ * (replaces qalloc, src/1d/qalloc1.f90:10-29, src/2d/qalloc2.f90:11-30). */
+int sc_create(const sc_params *params, sc_solver **out);
+int sc_destroy(sc_solver *s);
+
+/* Host <-> device copies of the whole q array, ghosts included. */
+int sc_set_q(sc_solver *s, const double *q);
+int sc_get_q(sc_solver *s, double *q);
+
+/* qold = q (sharpclaw.f90:134) and q = qold (sharpclaw.f90:201).  Both are
+ * buffer renamings on the device; no data moves. */
+int sc_save_qold(sc_solver *s);
+int sc_restore_qold(sc_solver *s);
+
+/* step(q,g,dq,aux,dt,cfl,t,rp,src,tfluct) (src/1d/step1d.f90:2-31,
+ * src/2d/step2d.f90:2-41): fills q's ghost cells in place (bc), sets
+ * dq = dt*L(q) in the interior and returns the CFL number.  dq may be NULL
+ * (result stays on the device); otherwise it receives the full array with
+ * zero ghost entries. */
+int sc_step(sc_solver *s, double dt, double t, double *dq, double *cfl);
+
+/* One SSP Runge-Kutta time step from q (= qold after sc_save_qold) with
+ * every stage fused: ghost fill + flux + stage combination in one kernel
+ * (ts_ssp22/42/33/104, src/sharpclaw.f90:247-464).  cfl receives the last
+ * stage's CFL number, as the reference's loop uses it (sharpclaw.f90:184-216). */
+int sc_ts(sc_solver *s, double dt, double told, double *cfl);
+
+/* The whole time loop of sharpclaw(tstart,tend,...) (sharpclaw.f90:2-243):
+ * dt clamp to tend, qold save, SSP step, CFL-driven dt update and
+ * reject/retake.  *dt is dt_initial on entry and dt_last on return, as
+ * main.f90 carries it from frame to frame. tend < tstart means one step. */
+int sc_advance(sc_solver *s, const sc_run_ctl *ctl, double tstart, double tend,
+               double *dt, sc_run_stats *stats);
+
+/* Conservation sums  sum_cells q(.,m) * dx [* dy]  per component, in float64. */
+int sc_sums(sc_solver *s, double *sums);
+
+/* Device time of n back-to-back fused stages (SSP33 stage-2 form) on the
+ * current q, in ms per stage, measured with HIP events on the solver's
+ * stream.  Leaves q unchanged. */
+int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stage);
+
+/* ---- single routines on host arrays (each copies in, runs, copies out) */
+
+/* rp(ixy,maxm,meqn,mwaves,mbc,mx,ql,qr,auxl,auxr,wave,s,amdq,apdq)
+ * (e.g. apps/euler/shockentropy/rp1eu.f:5-6): i-th problem has left state
+ * qr(i-1,:), right state ql(i,:), i = 2-mbc..mx+mbc.  Arrays are
+ * (1-mbc:mx+mbc, ...) with mbc = params->mbc. */
+int sc_rp(const sc_params *params, int ixy, int mx, const double *ql, const double *qr,
+          double *wave, double *s, double *amdq, double *apdq);
+
+/* evec(maxm,meqn,mbc,mx,q,auxl,auxr,evl,evr) as called by src/flux1.f90:105,
+ * plus the sweep direction ixy: left/right eigenvectors at interface i-1/2,
+ * i = 2-mbc..mx+mbc; evl(i,p,m), evr(i,m,p) are (1-mbc:mx+mbc, meqn, meqn). */
+int sc_evec(const sc_params *params, int ixy, int mx, const double *q, double *evl,
+            double *evr);
+
+/* weno5(q,ql,qr) / weno5_char(q,ql,qr,evl,evr) (src/reconstruct.f90:77-143,
+ * 146-241) on one slice of mx2 = mx+2*mbc cells (1-based, assumed shape).
+ * Writes ql(i), qr(i-1) for the interfaces i = mbc+1..mx2-2 (entries outside
+ * that range are left unchanged). */
+int sc_weno5(const sc_params *params, int mx2, const double *q, double *ql, double *qr);
+int sc_weno5_char(const sc_params *params, int mx2, const double *q, const double *evl,
+                  const double *evr, double *ql, double *qr);
+
+/* flux1(q1d,g,dq1d,aux,dt,cfl,t,rp,tfluct,ixy) (src/flux1.f90:2-190) on one
+ * slice q1d(1-mbc:mx+mbc, meqn): dq1d(1:mx) = -dt/dx(ixy) * (A-dq(i+1) +
+ * A+dq(i) + in-cell fluctuations) and the slice CFL number.  dq1d has the
+ * same shape as q1d; its ghost entries are set to zero. */
+int sc_flux1(const sc_params *params, int ixy, int mx, const double *q1d, double *dq1d,
+             double dt, double *cfl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHARPCLAW_AMD_H */

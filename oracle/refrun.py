@@ -1,0 +1,183 @@
+"""ORACLE -- test infrastructure only.
+
+Runs the reference SharpClaw, compiled from its own sources by
+oracle/ref/Makefile into oracle/_ref, on inputs written here:
+
+  harness_<app> rp|evec|weno|step   single reference routines
+  xsclaw_<app>                       the reference program src/main.f90
+
+The binaries exist only where they were built (make -C oracle ref);
+callers skip when ref_available() is false.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from sharpclaw_amd._abi import (SC_PHYS_ACOUSTICS, SC_PHYS_ADVECTION, SC_PHYS_EULER,
+                                SC_PHYS_SHALLOW_WATER)
+from sharpclaw_amd.clawdata import ClawData, write_values
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = os.path.join(HERE, "_ref")
+
+APPS = {
+    (SC_PHYS_ADVECTION, 1): "adv1d",
+    (SC_PHYS_ACOUSTICS, 1): "ac1d",
+    (SC_PHYS_EULER, 1): "eu1d",
+    (SC_PHYS_SHALLOW_WATER, 1): "sw1d",
+    (SC_PHYS_EULER, 2): "eu2d",
+    (SC_PHYS_SHALLOW_WATER, 2): "sw2d",
+}
+
+
+def app_of(p):
+    return APPS[(p.physics, p.ndim)]
+
+
+def ref_available(app=None):
+    apps = [app] if app else list(APPS.values())
+    return all(os.path.exists(os.path.join(REF, f"harness_{a}")) and
+               os.path.exists(os.path.join(REF, f"xsclaw_{a}")) for a in apps)
+
+
+def write_setprob(p, d, beta=200.0):
+    """setprob.data in the format each app's setprob reads."""
+    path = os.path.join(d, "setprob.data")
+    if p.physics == SC_PHYS_ADVECTION:       # apps/advection/1d/setprob.f: u, beta, ic
+        write_values(path, [float(p.u), float(beta), 1])
+    elif p.physics == SC_PHYS_ACOUSTICS:     # acoustics/1d/homogeneous/setprob.f: rho, K, beta
+        write_values(path, [float(p.rho), float(p.bulk), float(beta)])
+    elif p.physics == SC_PHYS_EULER:         # euler/shockentropy/setprob.f: gamma
+        write_values(path, [float(p.gamma)])
+    else:                                    # hooks/setprob_sw.f90: g
+        write_values(path, [float(p.grav)])
+
+
+def _run(exe, args, cwd):
+    r = subprocess.run([os.path.join(REF, exe)] + args, cwd=cwd, capture_output=True, text=True,
+                       timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"{exe} failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    return r
+
+
+def _i32(*v):
+    return np.asarray(v, dtype="<i4").tobytes()
+
+
+def _f64(a):
+    return np.asfortranarray(a, dtype="<f8").tobytes(order="F")
+
+
+def rp(p, ixy, mx, ql, qr):
+    L = mx + 2 * p.mbc
+    with tempfile.TemporaryDirectory() as d:
+        write_setprob(p, d)
+        with open(os.path.join(d, "in.bin"), "wb") as f:
+            f.write(_i32(ixy, mx, p.meqn, p.mwaves, p.mbc) + _f64(ql) + _f64(qr))
+        _run(f"harness_{app_of(p)}", ["rp", "in.bin", "out.bin"], d)
+        b = np.fromfile(os.path.join(d, "out.bin"), dtype="<f8")
+    n1 = L * p.meqn
+    nw = n1 * p.mwaves
+    wave = b[:nw].reshape((L, p.meqn, p.mwaves), order="F")
+    s = b[nw:nw + L * p.mwaves].reshape((L, p.mwaves), order="F")
+    o = nw + L * p.mwaves
+    return wave, s, b[o:o + n1].reshape((L, p.meqn), order="F"), b[o + n1:o + 2 * n1].reshape(
+        (L, p.meqn), order="F")
+
+
+def evec(p, ixy, mx, q):
+    L = mx + 2 * p.mbc
+    with tempfile.TemporaryDirectory() as d:
+        write_setprob(p, d)
+        with open(os.path.join(d, "in.bin"), "wb") as f:
+            f.write(_i32(ixy, mx, p.meqn, p.mbc) + _f64(q))
+        _run(f"harness_{app_of(p)}", ["evec", "in.bin", "out.bin"], d)
+        b = np.fromfile(os.path.join(d, "out.bin"), dtype="<f8")
+    n = L * p.meqn * p.meqn
+    return (b[:n].reshape((L, p.meqn, p.meqn), order="F"),
+            b[n:2 * n].reshape((L, p.meqn, p.meqn), order="F"))
+
+
+def weno(p, q, evl=None, evr=None):
+    q = np.asfortranarray(q)
+    mx2, meqn = q.shape
+    charf = 2 if evl is not None else 0
+    with tempfile.TemporaryDirectory() as d:
+        write_setprob(p, d)
+        with open(os.path.join(d, "in.bin"), "wb") as f:
+            f.write(_i32(mx2, meqn, charf) + _f64(q))
+            if charf:
+                f.write(_f64(evl) + _f64(evr))
+        _run(f"harness_{app_of(p)}", ["weno", "in.bin", "out.bin"], d)
+        b = np.fromfile(os.path.join(d, "out.bin"), dtype="<f8")
+    n = mx2 * meqn
+    return b[:n].reshape((mx2, meqn), order="F"), b[n:2 * n].reshape((mx2, meqn), order="F")
+
+
+def step(p, q, dt, t=0.0):
+    """Reference step(): returns (q after bc, dq, cfl)."""
+    shape = q.shape
+    with tempfile.TemporaryDirectory() as d:
+        write_setprob(p, d)
+        hdr = _i32(p.ndim, p.nx[0], p.nx[1], p.meqn, p.mwaves, p.char_decomp, *list(p.mthbc))
+        rh = np.array([p.dx[0], p.dx[1], p.xlower[0], p.xlower[1], dt, t], dtype="<f8").tobytes()
+        with open(os.path.join(d, "in.bin"), "wb") as f:
+            f.write(hdr + rh + _f64(q))
+        _run(f"harness_{app_of(p)}", ["step", "in.bin", "out.bin"], d)
+        b = np.fromfile(os.path.join(d, "out.bin"), dtype="<f8")
+    n = int(np.prod(shape))
+    return (b[1:1 + n].reshape(shape, order="F"), b[1 + n:1 + 2 * n].reshape(shape, order="F"),
+            float(b[0]))
+
+
+def clawdata_for(p, **kw):
+    cd = ClawData()
+    cd.ndim = p.ndim
+    cd.nx = [p.nx[0], p.nx[1]][: p.ndim]
+    cd.meqn, cd.mwaves = p.meqn, p.mwaves
+    cd.mthlim = [5] * p.mwaves
+    cd.char_decomp = p.char_decomp
+    cd.time_integrator = p.time_integrator
+    cd.xlower = [p.xlower[d] for d in range(p.ndim)]
+    cd.xupper = [p.xlower[d] + p.dx[d] * p.nx[d] for d in range(p.ndim)]
+    cd.mthbc = list(p.mthbc)[: 2 * p.ndim]
+    for k, v in kw.items():
+        setattr(cd, k, v)
+    return cd
+
+
+def read_frame(path):
+    b = open(path, "rb").read()
+    ndim = int(np.frombuffer(b[:4], "<i4")[0])
+    if ndim == 1:
+        mx, meqn = np.frombuffer(b[4:12], "<i4")
+        t = float(np.frombuffer(b[12:20], "<f8")[0])
+        q = np.frombuffer(b[20:], "<f8").reshape((mx, meqn), order="F")
+    else:
+        nx, ny, meqn = np.frombuffer(b[4:16], "<i4")
+        t = float(np.frombuffer(b[16:24], "<f8")[0])
+        q = np.frombuffer(b[24:], "<f8").reshape((nx, ny, meqn), order="F")
+    return t, q.copy(order="F")
+
+
+def run(p, cd, qinit_lines, beta=200.0):
+    """Run the reference program.  Returns (frames [(t, q_interior)], dtlog
+    (N x 2: told, dt of each new step), fort.info text)."""
+    with tempfile.TemporaryDirectory() as d:
+        write_setprob(p, d, beta=beta)
+        cd.write(os.path.join(d, "sharpclaw.data"))
+        with open(os.path.join(d, "qinit.data"), "w") as f:
+            f.write("\n".join(qinit_lines) + "\n")
+        _run(f"xsclaw_{app_of(p)}", [], d)
+        frames = []
+        k = 0
+        while os.path.exists(os.path.join(d, f"fort.b{k:04d}")):
+            frames.append(read_frame(os.path.join(d, f"fort.b{k:04d}")))
+            k += 1
+        lp = os.path.join(d, "dtlog.bin")
+        dtlog = np.fromfile(lp, dtype="<f8").reshape(-1, 2) if os.path.exists(lp) else np.zeros((0, 2))
+        info = open(os.path.join(d, "fort.info")).read()
+    return frames, dtlog, info

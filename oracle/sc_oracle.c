@@ -1,0 +1,1084 @@
+/*
+ * ORACLE -- test infrastructure only.  A plain-C restatement of SharpClaw's
+ * semi-discrete step, used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker; never linked into or called
+ * by the product library.
+ *
+ * Each function restates the reference routine named in its comment in the
+ * reference's own operation order (the same parenthesisation, the same
+ * division/multiplication choices, float64 throughout), so that gcc without
+ * FMA contraction reproduces the flang -O3 build of the reference (built by
+ * oracle/ref/Makefile into oracle/_ref) bit for bit on identical inputs.
+ *
+ * Array conventions follow the reference: column-major, Fortran lower bound
+ * 1-mbc for slices and grids, one contiguous plane per component.  The
+ * reconstruction routines use the assumed-shape (1-based) indexing of
+ * src/reconstruct.f90.
+ *
+ * Pinning (how each function is checked, tests/test_oracle_*.py):
+ *   weno5, weno5_char, rp (all physics), evec, flux1/flux2, bc, step,
+ *   ts_ssp*, the sharpclaw loop: bit comparison with oracle/_ref and with the
+ *   committed fixtures in tests/golden/ made from it; plus invariants
+ *   (constant states, polynomial exactness, mirror symmetry, L*R = I,
+ *   Roe property, SSP stability polynomials, conservation).
+ *   Euler/shallow-water evec and the 2D Euler rpn2 have no reference
+ *   implementation (the reference's evec1eu.f does not compile, no rpn2 for
+ *   Euler exists); they are pinned by L*R = I, A*R = R*Lambda, the Roe
+ *   property and rotational invariance, and by the Fortran hooks in
+ *   oracle/ref/hooks run through the reference's flux1/flux2.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../include/sharpclaw_amd.h"
+
+/* reconstruct.f90:6: "epweno = 1.e-36" is a default-real literal, i.e. the
+ * float nearest 1e-36 widened to double. */
+#define EPWENO ((double)1.0e-36f)
+
+typedef struct {
+    double gamma, gamma1, grav, u, rho, bulk, cc, zz;
+    int efix;
+} phys_t;
+
+static phys_t mkphys(const sc_params *p)
+{
+    phys_t f;
+    f.gamma = p->gamma;
+    f.gamma1 = p->gamma - 1.0;           /* setprob.f: gamma1 = gamma - 1.d0 */
+    f.grav = p->grav;
+    f.u = p->u;
+    f.rho = p->rho;
+    f.bulk = p->bulk;
+    f.cc = sqrt(p->bulk / p->rho);       /* acoustics setprob.f: cc = dsqrt(bulk/rho) */
+    f.zz = f.cc * p->rho;                /* zz = cc*rho */
+    if (p->efix >= 0)
+        f.efix = p->efix;
+    else if (p->physics == SC_PHYS_EULER)
+        f.efix = 1;                      /* rp1eu.f:40 */
+    else if (p->physics == SC_PHYS_SHALLOW_WATER)
+        f.efix = (p->ndim == 1);         /* rp1sw.f:40 on, rpn2sw.f:48 off */
+    else
+        f.efix = 0;
+    return f;
+}
+
+/* slice/grid addressing: Fortran index i in [1-mbc, mx+mbc] */
+#define A1(a, i, m) (a)[(size_t)((i) + mbc - 1) + (size_t)(m) * (size_t)L]
+#define W3(a, i, m, w) (a)[(size_t)((i) + mbc - 1) + (size_t)L * ((size_t)(m) + (size_t)meqn * (size_t)(w))]
+#define S2(a, i, w) (a)[(size_t)((i) + mbc - 1) + (size_t)(w) * (size_t)L]
+
+/* ------------------------------------------------------------------ rp */
+
+/* apps/advection/1d/rp1.f:35-43 */
+static void rp_advection(const phys_t *f, int mbc, int mx, int L, int meqn, const double *ql,
+                         const double *qr, double *wave, double *s, double *amdq, double *apdq)
+{
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double w = A1(ql, i, 0) - A1(qr, i - 1, 0);
+        W3(wave, i, 0, 0) = w;
+        S2(s, i, 0) = f->u;
+        A1(amdq, i, 0) = fmin(f->u, 0.0) * w;
+        A1(apdq, i, 0) = fmax(f->u, 0.0) * w;
+    }
+}
+
+/* apps/acoustics/1d/homogeneous/rp1.f:50-76 */
+static void rp_acoustics(const phys_t *f, int mbc, int mx, int L, int meqn, const double *ql,
+                         const double *qr, double *wave, double *s, double *amdq, double *apdq)
+{
+    const double zz = f->zz, cc = f->cc;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double d1 = A1(ql, i, 0) - A1(qr, i - 1, 0);
+        double d2 = A1(ql, i, 1) - A1(qr, i - 1, 1);
+        double a1 = (-d1 + zz * d2) / (2.0 * zz);
+        double a2 = (d1 + zz * d2) / (2.0 * zz);
+        W3(wave, i, 0, 0) = -(a1 * zz);
+        W3(wave, i, 1, 0) = a1;
+        S2(s, i, 0) = -cc;
+        W3(wave, i, 0, 1) = a2 * zz;
+        W3(wave, i, 1, 1) = a2;
+        S2(s, i, 1) = cc;
+    }
+    for (int m = 0; m < meqn; m++)
+        for (int i = 2 - mbc; i <= mx + mbc; i++) {
+            A1(amdq, i, m) = S2(s, i, 0) * W3(wave, i, m, 0);
+            A1(apdq, i, m) = S2(s, i, 1) * W3(wave, i, m, 1);
+        }
+}
+
+/* apdq = sum_mw s*wave - amdq (rp1eu.f:206-213, rp1sw.f:176-183) */
+static void apdq_from_total(int mbc, int mx, int L, int meqn, int mwaves, const double *wave,
+                            const double *s, const double *amdq, double *apdq)
+{
+    for (int m = 0; m < meqn; m++)
+        for (int i = 2 - mbc; i <= mx + mbc; i++) {
+            double df = 0.0;
+            for (int mw = 0; mw < mwaves; mw++)
+                df = df + S2(s, i, mw) * W3(wave, i, m, mw);
+            A1(apdq, i, m) = df - A1(amdq, i, m);
+        }
+}
+
+/* no entropy fix: split by the sign of s (rp1eu.f:100-111, rpn2sw.f:132-143) */
+static void split_by_sign(int mbc, int mx, int L, int meqn, int mwaves, const double *wave,
+                          const double *s, double *amdq, double *apdq)
+{
+    for (int m = 0; m < meqn; m++)
+        for (int i = 2 - mbc; i <= mx + mbc; i++) {
+            double am = 0.0, ap = 0.0;
+            for (int mw = 0; mw < mwaves; mw++) {
+                if (S2(s, i, mw) < 0.0)
+                    am = am + S2(s, i, mw) * W3(wave, i, m, mw);
+                else
+                    ap = ap + S2(s, i, mw) * W3(wave, i, m, mw);
+            }
+            A1(amdq, i, m) = am;
+            A1(apdq, i, m) = ap;
+        }
+}
+
+/* apps/euler/shockentropy/rp1eu.f:44-213 */
+static void rp_euler1(const phys_t *f, int mbc, int mx, int L, int meqn, int mwaves,
+                      const double *ql, const double *qr, double *wave, double *s, double *amdq,
+                      double *apdq)
+{
+    const double gamma = f->gamma, gamma1 = f->gamma1;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double rl = A1(qr, i - 1, 0), ml = A1(qr, i - 1, 1), el = A1(qr, i - 1, 2);
+        double rr = A1(ql, i, 0), mr = A1(ql, i, 1), er = A1(ql, i, 2);
+        double rhsqrtl = sqrt(rl);
+        double rhsqrtr = sqrt(rr);
+        double pl = gamma1 * (el - 0.5 * (ml * ml) / rl);
+        double pr = gamma1 * (er - 0.5 * (mr * mr) / rr);
+        double rhsq2 = rhsqrtl + rhsqrtr;
+        double u = (ml / rhsqrtl + mr / rhsqrtr) / rhsq2;
+        double enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+        double a2 = gamma1 * (enth - 0.5 * (u * u));
+        double a = sqrt(a2);
+        double d1 = rr - rl, d2 = mr - ml, d3 = er - el;
+        double b2 = gamma1 / (a * a) * ((enth - u * u) * d1 + u * d2 - d3);
+        double b3 = (d2 + (a - u) * d1 - a * b2) / (2.0 * a);
+        double b1 = d1 - b2 - b3;
+        W3(wave, i, 0, 0) = b1;
+        W3(wave, i, 1, 0) = b1 * (u - a);
+        W3(wave, i, 2, 0) = b1 * (enth - u * a);
+        S2(s, i, 0) = u - a;
+        W3(wave, i, 0, 1) = b2;
+        W3(wave, i, 1, 1) = b2 * u;
+        W3(wave, i, 2, 1) = b2 * 0.5 * (u * u);
+        S2(s, i, 1) = u;
+        W3(wave, i, 0, 2) = b3;
+        W3(wave, i, 1, 2) = b3 * (u + a);
+        W3(wave, i, 2, 2) = b3 * (enth + u * a);
+        S2(s, i, 2) = u + a;
+    }
+    if (!f->efix) {
+        split_by_sign(mbc, mx, L, meqn, mwaves, wave, s, amdq, apdq);
+        return;
+    }
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double rhoim1 = A1(qr, i - 1, 0);
+        double pim1 = gamma1 * (A1(qr, i - 1, 2) - 0.5 * (A1(qr, i - 1, 1) * A1(qr, i - 1, 1)) / rhoim1);
+        double cim1 = sqrt(gamma * pim1 / rhoim1);
+        double s0 = A1(qr, i - 1, 1) / rhoim1 - cim1;
+        if (s0 >= 0.0 && S2(s, i, 0) > 0.0) {
+            for (int m = 0; m < 3; m++) A1(amdq, i, m) = 0.0;
+            continue;
+        }
+        double rho1 = A1(qr, i - 1, 0) + W3(wave, i, 0, 0);
+        double rhou1 = A1(qr, i - 1, 1) + W3(wave, i, 1, 0);
+        double en1 = A1(qr, i - 1, 2) + W3(wave, i, 2, 0);
+        double p1 = gamma1 * (en1 - 0.5 * (rhou1 * rhou1) / rho1);
+        double c1 = sqrt(gamma * p1 / rho1);
+        double s1 = rhou1 / rho1 - c1;
+        double sfract;
+        if (s0 < 0.0 && s1 > 0.0)
+            sfract = s0 * (s1 - S2(s, i, 0)) / (s1 - s0);
+        else if (S2(s, i, 0) < 0.0)
+            sfract = S2(s, i, 0);
+        else
+            sfract = 0.0;
+        for (int m = 0; m < 3; m++) A1(amdq, i, m) = sfract * W3(wave, i, m, 0);
+        if (S2(s, i, 1) >= 0.0) continue;
+        for (int m = 0; m < 3; m++) A1(amdq, i, m) = A1(amdq, i, m) + S2(s, i, 1) * W3(wave, i, m, 1);
+        double rhoi = A1(ql, i, 0);
+        double pi = gamma1 * (A1(ql, i, 2) - 0.5 * (A1(ql, i, 1) * A1(ql, i, 1)) / rhoi);
+        double ci = sqrt(gamma * pi / rhoi);
+        double s3 = A1(ql, i, 1) / rhoi + ci;
+        double rho2 = A1(ql, i, 0) - W3(wave, i, 0, 2);
+        double rhou2 = A1(ql, i, 1) - W3(wave, i, 1, 2);
+        double en2 = A1(ql, i, 2) - W3(wave, i, 2, 2);
+        double p2 = gamma1 * (en2 - 0.5 * (rhou2 * rhou2) / rho2);
+        double c2 = sqrt(gamma * p2 / rho2);
+        double s2 = rhou2 / rho2 + c2;
+        if (s2 < 0.0 && s3 > 0.0)
+            sfract = s2 * (s3 - S2(s, i, 2)) / (s3 - s2);
+        else if (S2(s, i, 2) < 0.0)
+            sfract = S2(s, i, 2);
+        else
+            continue;
+        for (int m = 0; m < 3; m++) A1(amdq, i, m) = A1(amdq, i, m) + sfract * W3(wave, i, m, 2);
+    }
+    apdq_from_total(mbc, mx, L, meqn, mwaves, wave, s, amdq, apdq);
+}
+
+/* Roe solver for 2D Euler (oracle/ref/hooks/rpn2_euler.f90): normal/tangent
+ * selection of rpn2sw.f:60-66, Roe averages and entropy fix of rp1eu.f. */
+static void rp_euler2(const phys_t *f, int ixy, int mbc, int mx, int L, int meqn, int mwaves,
+                      const double *ql, const double *qr, double *wave, double *s, double *amdq,
+                      double *apdq)
+{
+    const double gamma = f->gamma, gamma1 = f->gamma1;
+    const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double rl = A1(qr, i - 1, 0), ul_ = A1(qr, i - 1, mu), vl_ = A1(qr, i - 1, mv), el = A1(qr, i - 1, 3);
+        double rr = A1(ql, i, 0), ur_ = A1(ql, i, mu), vr_ = A1(ql, i, mv), er = A1(ql, i, 3);
+        double rhsqrtl = sqrt(rl);
+        double rhsqrtr = sqrt(rr);
+        double pl = gamma1 * (el - 0.5 * (ul_ * ul_ + vl_ * vl_) / rl);
+        double pr = gamma1 * (er - 0.5 * (ur_ * ur_ + vr_ * vr_) / rr);
+        double rhsq2 = rhsqrtl + rhsqrtr;
+        double u = (ul_ / rhsqrtl + ur_ / rhsqrtr) / rhsq2;
+        double v = (vl_ / rhsqrtl + vr_ / rhsqrtr) / rhsq2;
+        double enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+        double u2v2 = u * u + v * v;
+        double a2 = gamma1 * (enth - 0.5 * u2v2);
+        double a = sqrt(a2);
+        double d1 = rr - rl, d2 = ur_ - ul_, d3 = vr_ - vl_, d4 = er - el;
+        double a3 = d3 - v * d1;
+        double a2s = gamma1 / a2 * ((enth - u2v2) * d1 + u * d2 + v * d3 - d4);
+        double a4 = (d2 + (a - u) * d1 - a * a2s) / (2.0 * a);
+        double a1 = d1 - a2s - a4;
+        W3(wave, i, 0, 0) = a1;
+        W3(wave, i, mu, 0) = a1 * (u - a);
+        W3(wave, i, mv, 0) = a1 * v;
+        W3(wave, i, 3, 0) = a1 * (enth - u * a);
+        S2(s, i, 0) = u - a;
+        W3(wave, i, 0, 1) = a2s;
+        W3(wave, i, mu, 1) = a2s * u;
+        W3(wave, i, mv, 1) = a2s * v + a3;
+        W3(wave, i, 3, 1) = a2s * 0.5 * u2v2 + a3 * v;
+        S2(s, i, 1) = u;
+        W3(wave, i, 0, 2) = a4;
+        W3(wave, i, mu, 2) = a4 * (u + a);
+        W3(wave, i, mv, 2) = a4 * v;
+        W3(wave, i, 3, 2) = a4 * (enth + u * a);
+        S2(s, i, 2) = u + a;
+    }
+    if (!f->efix) {
+        split_by_sign(mbc, mx, L, meqn, mwaves, wave, s, amdq, apdq);
+        return;
+    }
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double rhoim1 = A1(qr, i - 1, 0);
+        double qmu = A1(qr, i - 1, mu), qmv = A1(qr, i - 1, mv);
+        double pim1 = gamma1 * (A1(qr, i - 1, 3) - 0.5 * (qmu * qmu + qmv * qmv) / rhoim1);
+        double cim1 = sqrt(gamma * pim1 / rhoim1);
+        double s0 = qmu / rhoim1 - cim1;
+        if (s0 >= 0.0 && S2(s, i, 0) > 0.0) {
+            for (int m = 0; m < meqn; m++) A1(amdq, i, m) = 0.0;
+            continue;
+        }
+        double rho1 = A1(qr, i - 1, 0) + W3(wave, i, 0, 0);
+        double rhou1 = qmu + W3(wave, i, mu, 0);
+        double rhov1 = qmv + W3(wave, i, mv, 0);
+        double en1 = A1(qr, i - 1, 3) + W3(wave, i, 3, 0);
+        double p1 = gamma1 * (en1 - 0.5 * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
+        double c1 = sqrt(gamma * p1 / rho1);
+        double s1 = rhou1 / rho1 - c1;
+        double sfract;
+        if (s0 < 0.0 && s1 > 0.0)
+            sfract = s0 * (s1 - S2(s, i, 0)) / (s1 - s0);
+        else if (S2(s, i, 0) < 0.0)
+            sfract = S2(s, i, 0);
+        else
+            sfract = 0.0;
+        for (int m = 0; m < meqn; m++) A1(amdq, i, m) = sfract * W3(wave, i, m, 0);
+        if (S2(s, i, 1) >= 0.0) continue;
+        for (int m = 0; m < meqn; m++) A1(amdq, i, m) = A1(amdq, i, m) + S2(s, i, 1) * W3(wave, i, m, 1);
+        double rhoi = A1(ql, i, 0);
+        double rmu = A1(ql, i, mu), rmv = A1(ql, i, mv);
+        double pi = gamma1 * (A1(ql, i, 3) - 0.5 * (rmu * rmu + rmv * rmv) / rhoi);
+        double ci = sqrt(gamma * pi / rhoi);
+        double s3 = rmu / rhoi + ci;
+        double rho2 = A1(ql, i, 0) - W3(wave, i, 0, 2);
+        double rhou2 = rmu - W3(wave, i, mu, 2);
+        double rhov2 = rmv - W3(wave, i, mv, 2);
+        double en2 = A1(ql, i, 3) - W3(wave, i, 3, 2);
+        double p2 = gamma1 * (en2 - 0.5 * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
+        double c2 = sqrt(gamma * p2 / rho2);
+        double s2 = rhou2 / rho2 + c2;
+        if (s2 < 0.0 && s3 > 0.0)
+            sfract = s2 * (s3 - S2(s, i, 2)) / (s3 - s2);
+        else if (S2(s, i, 2) < 0.0)
+            sfract = S2(s, i, 2);
+        else
+            continue;
+        for (int m = 0; m < meqn; m++) A1(amdq, i, m) = A1(amdq, i, m) + sfract * W3(wave, i, m, 2);
+    }
+    apdq_from_total(mbc, mx, L, meqn, mwaves, wave, s, amdq, apdq);
+}
+
+/* apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/rp1sw.f:43-183 */
+static void rp_sw1(const phys_t *f, int mbc, int mx, int L, int meqn, int mwaves, const double *ql,
+                   const double *qr, double *wave, double *s, double *amdq, double *apdq)
+{
+    const double grav = f->grav;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double hl = A1(qr, i - 1, 0), hul = A1(qr, i - 1, 1);
+        double hr = A1(ql, i, 0), hur = A1(ql, i, 1);
+        double ubar = (hul / sqrt(hl) + hur / sqrt(hr)) / (sqrt(hl) + sqrt(hr));
+        double cbar = sqrt(0.5 * grav * (hl + hr));
+        double d1 = hr - hl, d2 = hur - hul;
+        double a1 = 0.5 * (-d2 + (ubar + cbar) * d1) / cbar;
+        double a2 = 0.5 * (d2 - (ubar - cbar) * d1) / cbar;
+        W3(wave, i, 0, 0) = a1;
+        W3(wave, i, 1, 0) = a1 * (ubar - cbar);
+        S2(s, i, 0) = ubar - cbar;
+        W3(wave, i, 0, 1) = a2;
+        W3(wave, i, 1, 1) = a2 * (ubar + cbar);
+        S2(s, i, 1) = ubar + cbar;
+    }
+    if (!f->efix) {
+        split_by_sign(mbc, mx, L, meqn, mwaves, wave, s, amdq, apdq);
+        return;
+    }
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double s0 = A1(qr, i - 1, 1) / A1(qr, i - 1, 0) - sqrt(grav * A1(qr, i - 1, 0));
+        if (s0 >= 0.0 && S2(s, i, 0) > 0.0) {
+            for (int m = 0; m < 2; m++) A1(amdq, i, m) = 0.0;
+            continue;
+        }
+        double hr1 = A1(qr, i - 1, 0) + W3(wave, i, 0, 0);
+        double uhr1 = A1(qr, i - 1, 1) + W3(wave, i, 1, 0);
+        double s1 = uhr1 / hr1 - sqrt(grav * hr1);
+        double sfract;
+        if (s0 < 0.0 && s1 > 0.0)
+            sfract = s0 * (s1 - S2(s, i, 0)) / (s1 - s0);
+        else if (S2(s, i, 0) < 0.0)
+            sfract = S2(s, i, 0);
+        else
+            sfract = 0.0;
+        for (int m = 0; m < 2; m++) A1(amdq, i, m) = sfract * W3(wave, i, m, 0);
+        double s3 = A1(ql, i, 1) / A1(ql, i, 0) + sqrt(grav * A1(ql, i, 0));
+        double hl2 = A1(ql, i, 0) - W3(wave, i, 0, 1);
+        double uhl2 = A1(ql, i, 1) - W3(wave, i, 1, 1);
+        double s2 = uhl2 / hl2 + sqrt(grav * hl2);
+        if (s2 < 0.0 && s3 > 0.0)
+            sfract = s2 * (s3 - S2(s, i, 1)) / (s3 - s2);
+        else if (S2(s, i, 1) < 0.0)
+            sfract = S2(s, i, 1);
+        else
+            continue;
+        for (int m = 0; m < 2; m++) A1(amdq, i, m) = A1(amdq, i, m) + sfract * W3(wave, i, m, 1);
+    }
+    apdq_from_total(mbc, mx, L, meqn, mwaves, wave, s, amdq, apdq);
+}
+
+/* apps/shallowWater/2dRadialDam/rpn2sw.f:60-143 (efix off by default) */
+static void rp_sw2(const phys_t *f, int ixy, int mbc, int mx, int L, int meqn, int mwaves,
+                   const double *ql, const double *qr, double *wave, double *s, double *amdq,
+                   double *apdq)
+{
+    const double g = f->grav;
+    const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double h = (A1(qr, i - 1, 0) + A1(ql, i, 0)) * 0.50;
+        double hsqrtl = sqrt(A1(qr, i - 1, 0));
+        double hsqrtr = sqrt(A1(ql, i, 0));
+        double hsq2 = hsqrtl + hsqrtr;
+        double u = (A1(qr, i - 1, mu) / hsqrtl + A1(ql, i, mu) / hsqrtr) / hsq2;
+        double v = (A1(qr, i - 1, mv) / hsqrtl + A1(ql, i, mv) / hsqrtr) / hsq2;
+        double a = sqrt(g * h);
+        double d1 = A1(ql, i, 0) - A1(qr, i - 1, 0);
+        double d2 = A1(ql, i, mu) - A1(qr, i - 1, mu);
+        double d3 = A1(ql, i, mv) - A1(qr, i - 1, mv);
+        double a1 = ((u + a) * d1 - d2) * (0.50 / a);
+        double a2 = -(v * d1) + d3;
+        double a3 = (-((u - a) * d1) + d2) * (0.50 / a);
+        W3(wave, i, 0, 0) = a1;
+        W3(wave, i, mu, 0) = a1 * (u - a);
+        W3(wave, i, mv, 0) = a1 * v;
+        S2(s, i, 0) = u - a;
+        W3(wave, i, 0, 1) = 0.0;
+        W3(wave, i, mu, 1) = 0.0;
+        W3(wave, i, mv, 1) = a2;
+        S2(s, i, 1) = u;
+        W3(wave, i, 0, 2) = a3;
+        W3(wave, i, mu, 2) = a3 * (u + a);
+        W3(wave, i, mv, 2) = a3 * v;
+        S2(s, i, 2) = u + a;
+    }
+    if (!f->efix) {
+        split_by_sign(mbc, mx, L, meqn, mwaves, wave, s, amdq, apdq);
+        return;
+    }
+    /* rpn2sw.f:158-228 */
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double him1 = A1(qr, i - 1, 0);
+        double s0 = A1(qr, i - 1, mu) / him1 - sqrt(g * him1);
+        if (s0 > 0.0 && S2(s, i, 0) > 0.0) {
+            for (int m = 0; m < 3; m++) A1(amdq, i, m) = 0.0;
+            continue;
+        }
+        double h1 = A1(qr, i - 1, 0) + W3(wave, i, 0, 0);
+        double hu1 = A1(qr, i - 1, mu) + W3(wave, i, mu, 0);
+        double s1 = hu1 / h1 - sqrt(g * h1);
+        double sfract;
+        if (s0 < 0.0 && s1 > 0.0)
+            sfract = s0 * ((s1 - S2(s, i, 0)) / (s1 - s0));
+        else if (S2(s, i, 0) < 0.0)
+            sfract = S2(s, i, 0);
+        else
+            sfract = 0.0;
+        for (int m = 0; m < 3; m++) A1(amdq, i, m) = sfract * W3(wave, i, m, 0);
+        if (S2(s, i, 1) > 0.0) continue;
+        for (int m = 0; m < 3; m++) A1(amdq, i, m) = A1(amdq, i, m) + S2(s, i, 1) * W3(wave, i, m, 1);
+        double hi = A1(ql, i, 0);
+        double s03 = A1(ql, i, mu) / hi + sqrt(g * hi);
+        double h3 = A1(ql, i, 0) - W3(wave, i, 0, 2);
+        double hu3 = A1(ql, i, mu) - W3(wave, i, mu, 2);
+        double s3 = hu3 / h3 + sqrt(g * h3);
+        if (s3 < 0.0 && s03 > 0.0)
+            sfract = s3 * ((s03 - S2(s, i, 2)) / (s03 - s3));
+        else if (S2(s, i, 2) < 0.0)
+            sfract = S2(s, i, 2);
+        else
+            continue;
+        for (int m = 0; m < 3; m++) A1(amdq, i, m) = A1(amdq, i, m) + sfract * W3(wave, i, m, 2);
+    }
+    apdq_from_total(mbc, mx, L, meqn, mwaves, wave, s, amdq, apdq);
+}
+
+/* Dispatch on the physics, as the link-time choice of the rp plugin does.
+ * Arrays have leading dimension L = mx + 2*mbc. */
+int orc_rp(const sc_params *p, int ixy, int mx, const double *ql, const double *qr, double *wave,
+           double *s, double *amdq, double *apdq)
+{
+    phys_t f = mkphys(p);
+    int mbc = p->mbc, meqn = p->meqn, mwaves = p->mwaves, L = mx + 2 * p->mbc;
+    switch (p->physics) {
+    case SC_PHYS_ADVECTION:
+        rp_advection(&f, mbc, mx, L, meqn, ql, qr, wave, s, amdq, apdq);
+        return 0;
+    case SC_PHYS_ACOUSTICS:
+        rp_acoustics(&f, mbc, mx, L, meqn, ql, qr, wave, s, amdq, apdq);
+        return 0;
+    case SC_PHYS_EULER:
+        if (p->ndim == 1)
+            rp_euler1(&f, mbc, mx, L, meqn, mwaves, ql, qr, wave, s, amdq, apdq);
+        else
+            rp_euler2(&f, ixy, mbc, mx, L, meqn, mwaves, ql, qr, wave, s, amdq, apdq);
+        return 0;
+    case SC_PHYS_SHALLOW_WATER:
+        if (p->ndim == 1)
+            rp_sw1(&f, mbc, mx, L, meqn, mwaves, ql, qr, wave, s, amdq, apdq);
+        else
+            rp_sw2(&f, ixy, mbc, mx, L, meqn, mwaves, ql, qr, wave, s, amdq, apdq);
+        return 0;
+    }
+    return SC_ERR_ARG;
+}
+
+/* ---------------------------------------------------------------- evec */
+
+#define E3(a, i, r, c) (a)[(size_t)((i) + mbc - 1) + (size_t)L * ((size_t)(r) + (size_t)meqn * (size_t)(c))]
+
+/* evl(i,p,m) = E3(evl,i,p,m), evr(i,m,p) = E3(evr,i,m,p) for the interface
+ * between cells i-1 and i, i = 2-mbc..mx+mbc. */
+int orc_evec(const sc_params *p, int ixy, int mx, const double *q, double *evl, double *evr)
+{
+    phys_t f = mkphys(p);
+    int mbc = p->mbc, meqn = p->meqn, L = mx + 2 * p->mbc;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        switch (p->physics) {
+        case SC_PHYS_ADVECTION: /* apps/advection/1d/evec.f:74-77 */
+            E3(evr, i, 0, 0) = 1.0;
+            E3(evl, i, 0, 0) = 1.0;
+            break;
+        case SC_PHYS_ACOUSTICS: { /* apps/acoustics/1d/homogeneous/evec.f:32-42 */
+            double zz = f.zz;
+            E3(evr, i, 0, 0) = -zz;
+            E3(evr, i, 0, 1) = zz;
+            E3(evr, i, 1, 0) = 1.0;
+            E3(evr, i, 1, 1) = 1.0;
+            double z = 1.0 / (zz + zz);
+            E3(evl, i, 0, 0) = -z;
+            E3(evl, i, 0, 1) = zz * z;
+            E3(evl, i, 1, 0) = 1.0 * z;
+            E3(evl, i, 1, 1) = zz * z;
+            break;
+        }
+        case SC_PHYS_EULER:
+            if (p->ndim == 1) { /* oracle/ref/hooks/evec_euler1d.f90 */
+                double gamma1 = f.gamma1;
+                double rl = A1(q, i - 1, 0), ml = A1(q, i - 1, 1), el = A1(q, i - 1, 2);
+                double rr = A1(q, i, 0), mr = A1(q, i, 1), er = A1(q, i, 2);
+                double rhsqrtl = sqrt(rl), rhsqrtr = sqrt(rr);
+                double pl = gamma1 * (el - 0.5 * (ml * ml) / rl);
+                double pr = gamma1 * (er - 0.5 * (mr * mr) / rr);
+                double rhsq2 = rhsqrtl + rhsqrtr;
+                double u = (ml / rhsqrtl + mr / rhsqrtr) / rhsq2;
+                double enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+                double a2 = gamma1 * (enth - 0.5 * (u * u));
+                double a = sqrt(a2);
+                E3(evr, i, 0, 0) = 1.0; E3(evr, i, 0, 1) = 1.0; E3(evr, i, 0, 2) = 1.0;
+                E3(evr, i, 1, 0) = u - a; E3(evr, i, 1, 1) = u; E3(evr, i, 1, 2) = u + a;
+                E3(evr, i, 2, 0) = enth - u * a;
+                E3(evr, i, 2, 1) = 0.5 * (u * u);
+                E3(evr, i, 2, 2) = enth + u * a;
+                double b = gamma1, h2 = 0.5 / a2, bk = 0.5 * b * (u * u);
+                E3(evl, i, 0, 0) = (bk + u * a) * h2;
+                E3(evl, i, 0, 1) = -((b * u + a) * h2);
+                E3(evl, i, 0, 2) = b * h2;
+                E3(evl, i, 1, 0) = 1.0 - bk / a2;
+                E3(evl, i, 1, 1) = b * u / a2;
+                E3(evl, i, 1, 2) = -(b / a2);
+                E3(evl, i, 2, 0) = (bk - u * a) * h2;
+                E3(evl, i, 2, 1) = -((b * u - a) * h2);
+                E3(evl, i, 2, 2) = b * h2;
+            } else { /* oracle/ref/hooks/evec_euler2d.f90 */
+                int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+                double gamma1 = f.gamma1;
+                double rl = A1(q, i - 1, 0), ul_ = A1(q, i - 1, mu), vl_ = A1(q, i - 1, mv), el = A1(q, i - 1, 3);
+                double rr = A1(q, i, 0), ur_ = A1(q, i, mu), vr_ = A1(q, i, mv), er = A1(q, i, 3);
+                double rhsqrtl = sqrt(rl), rhsqrtr = sqrt(rr);
+                double pl = gamma1 * (el - 0.5 * (ul_ * ul_ + vl_ * vl_) / rl);
+                double pr = gamma1 * (er - 0.5 * (ur_ * ur_ + vr_ * vr_) / rr);
+                double rhsq2 = rhsqrtl + rhsqrtr;
+                double u = (ul_ / rhsqrtl + ur_ / rhsqrtr) / rhsq2;
+                double v = (vl_ / rhsqrtl + vr_ / rhsqrtr) / rhsq2;
+                double enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+                double u2v2 = u * u + v * v;
+                double a2 = gamma1 * (enth - 0.5 * u2v2);
+                double a = sqrt(a2);
+                E3(evr, i, 0, 0) = 1.0; E3(evr, i, mu, 0) = u - a; E3(evr, i, mv, 0) = v;
+                E3(evr, i, 3, 0) = enth - u * a;
+                E3(evr, i, 0, 1) = 1.0; E3(evr, i, mu, 1) = u; E3(evr, i, mv, 1) = v;
+                E3(evr, i, 3, 1) = 0.5 * u2v2;
+                E3(evr, i, 0, 2) = 0.0; E3(evr, i, mu, 2) = 0.0; E3(evr, i, mv, 2) = 1.0;
+                E3(evr, i, 3, 2) = v;
+                E3(evr, i, 0, 3) = 1.0; E3(evr, i, mu, 3) = u + a; E3(evr, i, mv, 3) = v;
+                E3(evr, i, 3, 3) = enth + u * a;
+                double b = gamma1, h2 = 0.5 / a2, bk = 0.5 * b * u2v2;
+                E3(evl, i, 0, 0) = (bk + u * a) * h2;
+                E3(evl, i, 0, mu) = -((b * u + a) * h2);
+                E3(evl, i, 0, mv) = -(b * v * h2);
+                E3(evl, i, 0, 3) = b * h2;
+                E3(evl, i, 1, 0) = 1.0 - bk / a2;
+                E3(evl, i, 1, mu) = b * u / a2;
+                E3(evl, i, 1, mv) = b * v / a2;
+                E3(evl, i, 1, 3) = -(b / a2);
+                E3(evl, i, 2, 0) = -v;
+                E3(evl, i, 2, mu) = 0.0;
+                E3(evl, i, 2, mv) = 1.0;
+                E3(evl, i, 2, 3) = 0.0;
+                E3(evl, i, 3, 0) = (bk - u * a) * h2;
+                E3(evl, i, 3, mu) = -((b * u - a) * h2);
+                E3(evl, i, 3, mv) = -(b * v * h2);
+                E3(evl, i, 3, 3) = b * h2;
+            }
+            break;
+        case SC_PHYS_SHALLOW_WATER:
+            if (p->ndim == 1) { /* oracle/ref/hooks/evec_sw1d.f90 */
+                double hl = A1(q, i - 1, 0), hul = A1(q, i - 1, 1);
+                double hr = A1(q, i, 0), hur = A1(q, i, 1);
+                double ubar = (hul / sqrt(hl) + hur / sqrt(hr)) / (sqrt(hl) + sqrt(hr));
+                double cbar = sqrt(0.5 * f.grav * (hl + hr));
+                E3(evr, i, 0, 0) = 1.0; E3(evr, i, 0, 1) = 1.0;
+                E3(evr, i, 1, 0) = ubar - cbar; E3(evr, i, 1, 1) = ubar + cbar;
+                double z = 0.5 / cbar;
+                E3(evl, i, 0, 0) = (ubar + cbar) * z;
+                E3(evl, i, 0, 1) = -z;
+                E3(evl, i, 1, 0) = -((ubar - cbar) * z);
+                E3(evl, i, 1, 1) = z;
+            } else { /* oracle/ref/hooks/evec_sw2d.f90 */
+                int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+                double h = (A1(q, i - 1, 0) + A1(q, i, 0)) * 0.50;
+                double hsqrtl = sqrt(A1(q, i - 1, 0)), hsqrtr = sqrt(A1(q, i, 0));
+                double hsq2 = hsqrtl + hsqrtr;
+                double u = (A1(q, i - 1, mu) / hsqrtl + A1(q, i, mu) / hsqrtr) / hsq2;
+                double v = (A1(q, i - 1, mv) / hsqrtl + A1(q, i, mv) / hsqrtr) / hsq2;
+                double a = sqrt(f.grav * h);
+                E3(evr, i, 0, 0) = 1.0; E3(evr, i, mu, 0) = u - a; E3(evr, i, mv, 0) = v;
+                E3(evr, i, 0, 1) = 0.0; E3(evr, i, mu, 1) = 0.0; E3(evr, i, mv, 1) = 1.0;
+                E3(evr, i, 0, 2) = 1.0; E3(evr, i, mu, 2) = u + a; E3(evr, i, mv, 2) = v;
+                double z = 0.5 / a;
+                E3(evl, i, 0, 0) = (u + a) * z; E3(evl, i, 0, mu) = -z; E3(evl, i, 0, mv) = 0.0;
+                E3(evl, i, 1, 0) = -v; E3(evl, i, 1, mu) = 0.0; E3(evl, i, 1, mv) = 1.0;
+                E3(evl, i, 2, 0) = -((u - a) * z); E3(evl, i, 2, mu) = z; E3(evl, i, 2, mv) = 0.0;
+            }
+            break;
+        default:
+            return SC_ERR_ARG;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------ reconstruction */
+
+/* one WENO5 evaluation on the jumps of one side (reconstruct.f90:113-132
+ * and 208-226).  dm2..dp1 are the jumps at offsets intwo, inone, 0, ione
+ * and im the side sign; returns the nonlinear correction (without the
+ * central 4-point part). */
+static double weno_side(double im, double a_intwo, double a_inone, double a0, double a_ione)
+{
+    double t1 = im * (a_intwo - a_inone);
+    double t2 = im * (a_inone - a0);
+    double t3 = im * (a0 - a_ione);
+    double tt1 = 13.0 * (t1 * t1) + 3.0 * ((a_intwo - 3.0 * a_inone) * (a_intwo - 3.0 * a_inone));
+    double tt2 = 13.0 * (t2 * t2) + 3.0 * ((a_inone + a0) * (a_inone + a0));
+    double tt3 = 13.0 * (t3 * t3) + 3.0 * ((3.0 * a0 - a_ione) * (3.0 * a0 - a_ione));
+    tt1 = (EPWENO + tt1) * (EPWENO + tt1);
+    tt2 = (EPWENO + tt2) * (EPWENO + tt2);
+    tt3 = (EPWENO + tt3) * (EPWENO + tt3);
+    double s1 = tt2 * tt3;
+    double s2 = 6.0 * tt1 * tt3;
+    double s3 = 3.0 * tt1 * tt2;
+    double t0 = 1.0 / (s1 + s2 + s3);
+    s1 = s1 * t0;
+    s3 = s3 * t0;
+    return (s1 * (t2 - t1) + (0.5 * s3 - 0.25) * (t3 - t2)) / 3.0;
+}
+
+#define Q2(a, i, m) (a)[(size_t)((i) - 1) + (size_t)(m) * (size_t)LD]
+
+/* src/reconstruct.f90:77-143 (weno5), 1-based assumed-shape indexing,
+ * leading dimension LD >= mx2.  Writes qr(i-1), ql(i) for i = 3..mx2-2;
+ * the reference's uninitialised dq1m(1) is taken as 0 (it only reaches
+ * qr(2), the left state of ghost interface 0). */
+void orc_weno5(int mx2, int meqn, int LD, const double *q, double *ql, double *qr)
+{
+    const int mbc = 3;
+    double *dq1m = (double *)calloc((size_t)mx2 + 2, sizeof(double));
+    for (int m = 0; m < meqn; m++) {
+        dq1m[1] = 0.0;
+        for (int i = 2; i <= mx2; i++) dq1m[i] = Q2(q, i, m) - Q2(q, i - 1, m);
+        for (int i = mbc; i <= mx2 - mbc + 1; i++) {
+            double central = (-Q2(q, i - 2, m) + 7.0 * (Q2(q, i - 1, m) + Q2(q, i, m)) - Q2(q, i + 1, m)) / 12.0;
+            /* m1 = 1: im = 1 -> qr(i-1) */
+            double c1 = weno_side(1.0, dq1m[i - 2], dq1m[i - 1], dq1m[i], dq1m[i + 1]);
+            /* m1 = 2: im = -1 -> ql(i) */
+            double c2 = weno_side(-1.0, dq1m[i + 2], dq1m[i + 1], dq1m[i], dq1m[i - 1]);
+            Q2(qr, i - 1, m) = c1 + central;
+            Q2(ql, i, m) = c2 + central;
+        }
+    }
+    free(dq1m);
+}
+
+/* src/reconstruct.f90:146-241 (weno5_char).  evl(i,ip,m), evr(i,m,ip) with
+ * leading dimension LD.  hh(3,:), uninitialised in the reference, is taken
+ * as 0 (it only reaches qr(2), ql(3): ghost interface 0). */
+void orc_weno5_char(int mx2, int meqn, int LD, const double *q, const double *evl, const double *evr,
+                    double *ql, double *qr)
+{
+    const int mbc = 3;
+#define EV(a, i, r, c) (a)[(size_t)((i) - 1) + (size_t)LD * ((size_t)(r) + (size_t)meqn * (size_t)(c))]
+    double *dq = (double *)calloc((size_t)(mx2 + 2) * (size_t)meqn, sizeof(double));
+    double *hh = (double *)calloc((size_t)(mx2 + 2) * 5, sizeof(double));
+#define DQ(i, m) dq[(size_t)(i) + (size_t)(m) * (size_t)(mx2 + 2)]
+#define HH(i, k) hh[(size_t)(i) + (size_t)((k) + 2) * (size_t)(mx2 + 2)]
+    for (int m = 0; m < meqn; m++)
+        for (int i = 2; i <= mx2; i++) DQ(i, m) = Q2(q, i, m) - Q2(q, i - 1, m);
+    for (int m = 0; m < meqn; m++)
+        for (int i = 3; i <= mx2 - 1; i++) {
+            Q2(qr, i - 1, m) = (-Q2(q, i - 2, m) + 7.0 * (Q2(q, i - 1, m) + Q2(q, i, m)) - Q2(q, i + 1, m)) / 12.0;
+            Q2(ql, i, m) = Q2(qr, i - 1, m);
+        }
+    for (int ip = 0; ip < meqn; ip++) {
+        for (int m2 = -2; m2 <= 2; m2++)
+            for (int i = mbc + 1; i <= mx2 - 2; i++) {
+                double acc = 0.0;
+                for (int m = 0; m < meqn; m++) acc = acc + EV(evl, i, ip, m) * DQ(i + m2, m);
+                HH(i, m2) = acc;
+            }
+        for (int i = mbc; i <= mx2 - mbc + 1; i++) {
+            double u1 = weno_side(1.0, HH(i, -2), HH(i, -1), HH(i, 0), HH(i, 1));
+            double u2 = weno_side(-1.0, HH(i, 2), HH(i, 1), HH(i, 0), HH(i, -1));
+            for (int m = 0; m < meqn; m++) {
+                Q2(qr, i - 1, m) = Q2(qr, i - 1, m) + EV(evr, i, m, ip) * u1;
+                Q2(ql, i, m) = Q2(ql, i, m) + EV(evr, i, m, ip) * u2;
+            }
+        }
+    }
+#undef DQ
+#undef HH
+#undef EV
+    free(dq);
+    free(hh);
+}
+
+/* --------------------------------------------------------------- flux1 */
+
+/* src/flux1.f90:2-190 for lim_type=2, char_decomp 0/2, tfluct_solver=0,
+ * mcapa=0, on one slice q1d(1-mbc:mx+mbc, meqn).  In 1D dq1d is the step's
+ * dq (interior zeroed by src); for ndim > 1 it is zeroed here
+ * (flux1.f90:62).  Returns the slice CFL number (flux1.f90:126-133). */
+double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double *dq1d, double dt)
+{
+    const int mbc = p->mbc, meqn = p->meqn, mwaves = p->mwaves, L = mx + 2 * mbc;
+    const size_t n1 = (size_t)L * meqn;
+    double dtdx = dt / p->dx[ixy - 1];
+    if (p->ndim > 1) memset(dq1d, 0, n1 * sizeof(double));
+    double *ql = (double *)calloc(n1, sizeof(double));
+    double *qr = (double *)calloc(n1, sizeof(double));
+    double *amdq = (double *)calloc(n1, sizeof(double));
+    double *apdq = (double *)calloc(n1, sizeof(double));
+    double *amdq2 = (double *)calloc(n1, sizeof(double));
+    double *apdq2 = (double *)calloc(n1, sizeof(double));
+    double *s = (double *)calloc((size_t)L * mwaves, sizeof(double));
+    double *wave = (double *)calloc(n1 * mwaves, sizeof(double));
+    if (p->char_decomp == 2) {
+        double *evl = (double *)calloc(n1 * meqn, sizeof(double));
+        double *evr = (double *)calloc(n1 * meqn, sizeof(double));
+        orc_evec(p, ixy, mx, q1d, evl, evr);
+        orc_weno5_char(L, meqn, L, q1d, evl, evr, ql, qr);
+        free(evl);
+        free(evr);
+    } else {
+        orc_weno5(L, meqn, L, q1d, ql, qr);
+    }
+    orc_rp(p, ixy, mx, ql, qr, wave, s, amdq, apdq);
+    double cfl = 0.0;
+    for (int mw = 0; mw < mwaves; mw++)
+        for (int i = 1; i <= mx + 1; i++) {
+            double a = dtdx * S2(s, i, mw), b = -dtdx * S2(s, i, mw);
+            cfl = fmax(cfl, fmax(a, b));
+        }
+    /* flux1.f90:163-168: qr(i-1) <- ql(i), ql(i) <- qr(i) */
+    for (int i = 1 - mbc + 1; i <= mx + mbc; i++)
+        for (int m = 0; m < meqn; m++) {
+            A1(qr, i - 1, m) = A1(ql, i, m);
+            A1(ql, i, m) = A1(qr, i, m);
+        }
+    orc_rp(p, ixy, mx, ql, qr, wave, s, amdq2, apdq2);
+    for (int m = 0; m < meqn; m++)
+        for (int i = 1; i <= mx; i++)
+            A1(dq1d, i, m) = A1(dq1d, i, m) - dtdx * (A1(amdq, i + 1, m) + A1(apdq, i, m) + A1(amdq2, i, m) + A1(apdq2, i, m));
+    free(ql); free(qr); free(amdq); free(apdq); free(amdq2); free(apdq2); free(s); free(wave);
+    return cfl;
+}
+
+/* ------------------------------------------------------------------ bc */
+
+/* src/1d/bc1.f90:32-107 */
+static int orc_bc1(const sc_params *p, double *q)
+{
+    const int mbc = p->mbc, meqn = p->meqn, mx = p->nx[0], L = mx + 2 * mbc;
+    /* a wall negates component 2 (bc1.f90:63-65); with meqn < 2 the
+     * reference writes past q, so the option is rejected */
+    if ((p->mthbc[0] == 3 || p->mthbc[1] == 3) && meqn < 2) return SC_ERR_ARG;
+    switch (p->mthbc[0]) {
+    case 1: for (int m = 0; m < meqn; m++) for (int ibc = 1; ibc <= mbc; ibc++) A1(q, 1 - ibc, m) = A1(q, 1, m); break;
+    case 2: for (int m = 0; m < meqn; m++) for (int ibc = 1; ibc <= mbc; ibc++) A1(q, 1 - ibc, m) = A1(q, mx + 1 - ibc, m); break;
+    case 3:
+        for (int m = 0; m < meqn; m++) for (int ibc = 1; ibc <= mbc; ibc++) A1(q, 1 - ibc, m) = A1(q, ibc, m);
+        for (int ibc = 1; ibc <= mbc; ibc++) A1(q, 1 - ibc, 1) = -A1(q, ibc, 1);
+        break;
+    default: return SC_ERR_ARG;
+    }
+    switch (p->mthbc[1]) {
+    case 1: for (int m = 0; m < meqn; m++) for (int ibc = 1; ibc <= mbc; ibc++) A1(q, mx + ibc, m) = A1(q, mx, m); break;
+    case 2: for (int m = 0; m < meqn; m++) for (int ibc = 1; ibc <= mbc; ibc++) A1(q, mx + ibc, m) = A1(q, ibc, m); break;
+    case 3:
+        for (int m = 0; m < meqn; m++) for (int ibc = 1; ibc <= mbc; ibc++) A1(q, mx + ibc, m) = A1(q, mx + 1 - ibc, m);
+        for (int ibc = 1; ibc <= mbc; ibc++) A1(q, mx + ibc, 1) = -A1(q, mx + 1 - ibc, 1);
+        break;
+    default: return SC_ERR_ARG;
+    }
+    return 0;
+}
+
+/* src/2d/bc2.f90:31-206: x ghosts over all rows first, then y ghosts over
+ * all columns (so the corners take the y rule applied to x-ghost cells). */
+static int orc_bc2(const sc_params *p, double *q)
+{
+    const int mbc = p->mbc, meqn = p->meqn, nx = p->nx[0], ny = p->nx[1];
+    const int LX = nx + 2 * mbc, LY = ny + 2 * mbc;
+    /* x walls negate component 2, y walls component 3 (bc2.f90:65-67,155-157) */
+    if ((p->mthbc[0] == 3 || p->mthbc[1] == 3) && meqn < 2) return SC_ERR_ARG;
+    if ((p->mthbc[2] == 3 || p->mthbc[3] == 3) && meqn < 3) return SC_ERR_ARG;
+#define G(i, j, m) q[(size_t)((i) + mbc - 1) + (size_t)LX * ((size_t)((j) + mbc - 1) + (size_t)LY * (size_t)(m))]
+    for (int side = 0; side < 2; side++) {
+        int bct = p->mthbc[side];
+        if (bct < 1 || bct > 3) return SC_ERR_ARG;
+        for (int m = 0; m < meqn; m++)
+            for (int ibc = 1; ibc <= mbc; ibc++)
+                for (int j = 1 - mbc; j <= ny + mbc; j++) {
+                    int dst = side == 0 ? 1 - ibc : nx + ibc;
+                    int src = bct == 1 ? (side == 0 ? 1 : nx)
+                            : bct == 2 ? (side == 0 ? nx + 1 - ibc : ibc)
+                                       : (side == 0 ? ibc : nx + 1 - ibc);
+                    G(dst, j, m) = G(src, j, m);
+                }
+        if (bct == 3)
+            for (int ibc = 1; ibc <= mbc; ibc++)
+                for (int j = 1 - mbc; j <= ny + mbc; j++) {
+                    int dst = side == 0 ? 1 - ibc : nx + ibc;
+                    int src = side == 0 ? ibc : nx + 1 - ibc;
+                    G(dst, j, 1) = -G(src, j, 1);
+                }
+    }
+    for (int side = 0; side < 2; side++) {
+        int bct = p->mthbc[2 + side];
+        if (bct < 1 || bct > 3) return SC_ERR_ARG;
+        for (int m = 0; m < meqn; m++)
+            for (int jbc = 1; jbc <= mbc; jbc++)
+                for (int i = 1 - mbc; i <= nx + mbc; i++) {
+                    int dst = side == 0 ? 1 - jbc : ny + jbc;
+                    int src = bct == 1 ? (side == 0 ? 1 : ny)
+                            : bct == 2 ? (side == 0 ? ny + 1 - jbc : jbc)
+                                       : (side == 0 ? jbc : ny + 1 - jbc);
+                    G(i, dst, m) = G(i, src, m);
+                }
+        if (bct == 3)
+            for (int jbc = 1; jbc <= mbc; jbc++)
+                for (int i = 1 - mbc; i <= nx + mbc; i++) {
+                    int dst = side == 0 ? 1 - jbc : ny + jbc;
+                    int src = side == 0 ? jbc : ny + 1 - jbc;
+                    G(i, dst, 2) = -G(i, src, 2);
+                }
+    }
+#undef G
+    return 0;
+}
+
+int orc_bc(const sc_params *p, double *q)
+{
+    return p->ndim == 1 ? orc_bc1(p, q) : orc_bc2(p, q);
+}
+
+/* -------------------------------------------------------------- step */
+
+static size_t grid_size(const sc_params *p)
+{
+    size_t n = (size_t)(p->nx[0] + 2 * p->mbc) * (size_t)p->meqn;
+    if (p->ndim == 2) n *= (size_t)(p->nx[1] + 2 * p->mbc);
+    return n;
+}
+
+/* src/2d/flux2.f90:2-94.  with_ghost_sweeps = 1 sweeps rows j=0..ny+1 and
+ * columns i=0..nx+1 exactly like the reference (the ghost sweeps only feed
+ * the CFL number); 0 sweeps the interior rows/columns only. */
+double orc_flux2(const sc_params *p, const double *q, double *dq, double dt, int with_ghost_sweeps)
+{
+    const int mbc = p->mbc, meqn = p->meqn, nx = p->nx[0], ny = p->nx[1];
+    const int LX = nx + 2 * mbc, LY = ny + 2 * mbc;
+    const int g = with_ghost_sweeps ? 1 : 0;
+    int Lmax = LX > LY ? LX : LY;
+    double *q1d = (double *)malloc((size_t)Lmax * meqn * sizeof(double));
+    double *dq1d = (double *)malloc((size_t)Lmax * meqn * sizeof(double));
+#define G(a, i, j, m) (a)[(size_t)((i) + mbc - 1) + (size_t)LX * ((size_t)((j) + mbc - 1) + (size_t)LY * (size_t)(m))]
+    double cfl = 0.0;
+    for (int j = 1 - g; j <= ny + g; j++) {
+        for (int m = 0; m < meqn; m++)
+            for (int i = 1 - mbc; i <= nx + mbc; i++) q1d[(size_t)(i + mbc - 1) + (size_t)m * LX] = G(q, i, j, m);
+        double c = orc_flux1(p, 1, nx, q1d, dq1d, dt);
+        cfl = fmax(cfl, c);
+        for (int m = 0; m < meqn; m++)
+            for (int i = 1; i <= nx; i++) G(dq, i, j, m) = G(dq, i, j, m) + dq1d[(size_t)(i + mbc - 1) + (size_t)m * LX];
+    }
+    for (int i = 1 - g; i <= nx + g; i++) {
+        for (int m = 0; m < meqn; m++)
+            for (int j = 1 - mbc; j <= ny + mbc; j++) q1d[(size_t)(j + mbc - 1) + (size_t)m * LY] = G(q, i, j, m);
+        double c = orc_flux1(p, 2, ny, q1d, dq1d, dt);
+        cfl = fmax(cfl, c);
+        for (int m = 0; m < meqn; m++)
+            for (int j = 1; j <= ny; j++) G(dq, i, j, m) = G(dq, i, j, m) + dq1d[(size_t)(j + mbc - 1) + (size_t)m * LY];
+    }
+#undef G
+    free(q1d);
+    free(dq1d);
+    return cfl;
+}
+
+/* step1d.f90 / step2d.f90: bc, src (dq = 0 in the interior), flux.
+ * Returns the CFL number, or -1 for an invalid boundary option. */
+double orc_step(const sc_params *p, double *q, double *dq, double dt, double t)
+{
+    (void)t;
+    const int mbc = p->mbc, meqn = p->meqn;
+    if (orc_bc(p, q)) return -1.0;
+    if (p->ndim == 1) {
+        const int mx = p->nx[0], L = mx + 2 * mbc;
+        for (int m = 0; m < meqn; m++)
+            for (int i = 1; i <= mx; i++) A1(dq, i, m) = 0.0;
+        return orc_flux1(p, 1, mx, q, dq, dt);
+    }
+    const int nx = p->nx[0], ny = p->nx[1], LX = nx + 2 * mbc, LY = ny + 2 * mbc;
+    for (int m = 0; m < meqn; m++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                dq[(size_t)(i + mbc - 1) + (size_t)LX * ((size_t)(j + mbc - 1) + (size_t)LY * (size_t)m)] = 0.0;
+    return orc_flux2(p, q, dq, dt, 1);
+}
+
+/* ------------------------------------------------- SSP Runge-Kutta */
+
+/* whole-array loop, as the reference's array statements (ghost entries
+ * included; bc refills q's ghosts at the start of every step) */
+#define FOR_ALL(k) for (size_t k = 0; k < n; k++)
+
+/* One time step ts_ssp22/42/33/104 (src/sharpclaw.f90:247-464).  qold holds
+ * q^n on entry (as sharpclaw.f90:134 leaves it).  Returns the last
+ * stage's CFL number; counts semi-discrete evaluations into *nevals. */
+double orc_ts(const sc_params *p, double *q, const double *qold, double *r1, double *r2, double dt,
+              double told, long long *nevals)
+{
+    const size_t n = grid_size(p);
+    double cfl = 0.0;
+    const double third = 1.0 / 3.0, sixth = 1.0 / 6.0;
+    switch (p->time_integrator) {
+    case 1: /* ts_ssp22 */
+        cfl = orc_step(p, q, r1, dt, told);
+        FOR_ALL(k) q[k] = q[k] + r1[k];
+        cfl = orc_step(p, q, r1, dt, told + dt);
+        FOR_ALL(k) q[k] = 0.5 * (qold[k] + q[k] + r1[k]);
+        *nevals += 2;
+        break;
+    case 2: /* ts_ssp42 */
+        cfl = orc_step(p, q, r1, dt, told);
+        FOR_ALL(k) q[k] = q[k] + third * r1[k];
+        cfl = orc_step(p, q, r1, dt, told + third * dt);
+        FOR_ALL(k) q[k] = q[k] + third * r1[k];
+        cfl = orc_step(p, q, r1, dt, told + 2.0 * third * dt);
+        FOR_ALL(k) q[k] = q[k] + third * r1[k];
+        cfl = orc_step(p, q, r1, dt, told + dt);
+        FOR_ALL(k) q[k] = 0.25 * qold[k] + 0.75 * q[k] + 0.25 * r1[k];
+        *nevals += 4;
+        break;
+    case 3: /* ts_ssp33 */
+        cfl = orc_step(p, q, r1, dt, told);
+        FOR_ALL(k) q[k] = q[k] + r1[k];
+        cfl = orc_step(p, q, r1, dt, told + dt);
+        FOR_ALL(k) q[k] = 0.75 * qold[k] + 0.25 * (q[k] + r1[k]);
+        cfl = orc_step(p, q, r1, dt, told + 0.5 * dt);
+        FOR_ALL(k) q[k] = (qold[k] + 2.0 * (q[k] + r1[k])) / 3.0;
+        *nevals += 3;
+        break;
+    case 4: /* ts_ssp104 */
+        for (int st = 0; st < 4; st++) {
+            cfl = orc_step(p, q, r1, dt, told);
+            FOR_ALL(k) q[k] = q[k] + sixth * r1[k];
+        }
+        cfl = orc_step(p, q, r1, dt, told);
+        FOR_ALL(k)
+        {
+            r2[k] = q[k] + sixth * r1[k];
+            r1[k] = 0.04 * qold[k] + 0.36 * r2[k];
+            q[k] = 0.6 * qold[k] + 0.4 * r2[k];
+        }
+        for (int st = 0; st < 4; st++) {
+            cfl = orc_step(p, q, r2, dt, told);
+            FOR_ALL(k) q[k] = q[k] + sixth * r2[k];
+        }
+        cfl = orc_step(p, q, r2, dt, told);
+        FOR_ALL(k) q[k] = r1[k] + 0.6 * q[k] + 0.1 * r2[k];
+        *nevals += 10;
+        break;
+    default:
+        return -1.0;
+    }
+    return cfl;
+}
+
+/* src/sharpclaw.f90:35-235, the time loop of one output interval.  qold is
+ * refreshed at every new step (the reference does so only for
+ * dt_variable == 1, sharpclaw.f90:132-135, which breaks fixed-dt runs).
+ * *dt: dt_initial on entry, dt_last on return. */
+int orc_sharpclaw(const sc_params *p, const sc_run_ctl *c, double *q, double tstart, double tend,
+                  double *dt_io, sc_run_stats *st)
+{
+    size_t n = grid_size(p);
+    double *qold = (double *)calloc(n, sizeof(double));
+    double *r1 = (double *)calloc(n, sizeof(double));
+    double *r2 = (double *)calloc(n, sizeof(double));
+    double t = tstart, dt = *dt_io, told = tstart, cflmax = 0.0, cfl = 0.0;
+    double dtmin = dt, dtmax = dt;
+    int maxn = c->max_steps, info = 0, steps_taken = 0, retake = 0;
+    memset(st, 0, sizeof(*st));
+    if (c->dt_variable == 0) {
+        if (tend < tstart) {
+            maxn = 1;
+        } else {
+            maxn = (int)((tend - tstart + 1e-10) / dt);
+            if (fabs(maxn * dt - (tend - tstart)) > 1e-5 * (tend - tstart)) {
+                info = 2;
+                goto done;
+            }
+        }
+    }
+    if (c->dt_variable == 1 && c->cfl_desired > c->cfl_max) {
+        info = 3;
+        goto done;
+    }
+    for (int it = 1; it <= maxn; it++) {
+        if (!retake) {
+            told = t;
+            if (told + dt > tend && tstart < tend) dt = tend - told;
+            memcpy(qold, q, n * sizeof(double));
+        }
+        retake = 0;
+        t = told + dt;
+        cfl = orc_ts(p, q, qold, r1, r2, dt, told, &st->stage_evals);
+        if (c->dt_variable == 1) {
+            if (cfl > 0.0) {
+                dt = fmin(c->dt_max, dt * c->cfl_desired / cfl);
+                dtmin = fmin(dt, dtmin);
+                dtmax = fmax(dt, dtmax);
+            } else {
+                dt = c->dt_max;
+            }
+        }
+        if (cfl <= c->cfl_max) {
+            cflmax = fmax(cfl, cflmax);
+        } else {
+            t = told;
+            memcpy(q, qold, n * sizeof(double));
+            st->rejected++;
+            if (c->dt_variable == 1) {
+                retake = 1;
+            } else {
+                cflmax = fmax(cfl, cflmax);
+                break;
+            }
+        }
+        steps_taken++;
+        if (t >= tend) break;
+    }
+    if (c->dt_variable == 1 && t < tend && steps_taken == maxn) info = 11;
+    if (c->dt_variable == 0 && cflmax > c->cfl_max) info = 12;
+done:
+    st->t = t;
+    st->cfl_maxused = cflmax;
+    st->cfl_last = cfl;
+    st->dt_minused = dtmin;
+    st->dt_maxused = dtmax;
+    st->dt_last = dt;
+    st->steps_taken = steps_taken;
+    st->info = info;
+    *dt_io = dt;
+    free(qold);
+    free(r1);
+    free(r2);
+    return info;
+}
+
+/* conservation sums: (sum over interior cells of q(.,m)) * dx [* dy] */
+void orc_sums(const sc_params *p, const double *q, double *sums)
+{
+    int ny = p->ndim == 2 ? p->nx[1] : 1;
+    size_t LX = (size_t)(p->nx[0] + 2 * p->mbc), LY = p->ndim == 2 ? (size_t)(p->nx[1] + 2 * p->mbc) : 1;
+    for (int m = 0; m < p->meqn; m++) {
+        double acc = 0.0;
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= p->nx[0]; i++) {
+                size_t jj = p->ndim == 2 ? (size_t)(j + p->mbc - 1) : 0;
+                acc += q[(size_t)(i + p->mbc - 1) + LX * (jj + LY * (size_t)m)];
+            }
+        sums[m] = p->ndim == 2 ? acc * p->dx[0] * p->dx[1] : acc * p->dx[0];
+    }
+}
