@@ -1,0 +1,193 @@
+"""Benchmark: cell-updates/s per RK stage of the fused 2D WENO5-char stage.
+
+Workload (default): 2D Euler, dimension-by-dimension characteristic WENO5 +
+Roe rpn2, 1024 x 1024 on the unit square, float64, SSPRK3, four-quadrant
+Riemann initial data (rho in U[0.1,1.5], u,v in U[-1,1], p in U[0.03,1.5],
+seeded), extrapolation boundaries.  A step is one SSPRK3 time step (three
+fused stage launches plus the CFL readback that drives dt), exactly as the
+reference's time loop takes it.  Inputs are resident on the device before
+the timed region.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+
+  python bench.py [--gpus 1] [--steps 200] [--warmup 5] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, vendor figure
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def quadrant_states(seed):
+    rng = np.random.default_rng(seed)
+    # rows: NE, NW, SW, SE; columns: rho, u, v, p
+    return np.stack([rng.uniform(0.1, 1.5, 4), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4),
+                     rng.uniform(0.03, 1.5, 4)], axis=1)
+
+
+def quadrant_q(p, st):
+    from sharpclaw_amd import grid_shape, interior
+    g1 = p.gamma - 1.0
+    x = p.xlower[0] + (np.arange(1, p.nx[0] + 1) - 0.5) * p.dx[0]
+    y = p.xlower[1] + (np.arange(1, p.nx[1] + 1) - 0.5) * p.dx[1]
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    k = np.where(X > 0.5, np.where(Y > 0.5, 0, 3), np.where(Y > 0.5, 1, 2))
+    rho, u, v, pr = (st[k, c] for c in range(4))
+    q = np.zeros(grid_shape(p), order="F")
+    qi = interior(p, q)
+    qi[..., 0] = rho
+    qi[..., 1] = rho * u
+    qi[..., 2] = rho * v
+    qi[..., 3] = pr / g1 + 0.5 * rho * (u * u + v * v)
+    return q
+
+
+def traffic_from_profile():
+    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "stage2d_euler2d_1024_fp64_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(n, st, steps):
+    """The reference Fortran (oracle/_ref, flang -O3, serial) on a bounded
+    sample of the same workload; the C restatement when it is not built."""
+    from oracle import refrun
+    from sharpclaw_amd import ScRunCtl, make_params
+    p = make_params("euler", (n, n), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    cells = n * n
+    if refrun.ref_available("eu2d"):
+        cd = refrun.clawdata_for(p, nout=steps, outstyle=3, nstepout=steps, nstop=steps,
+                                 dt_initial=1e-4, cfl_max=0.5, cfl_desired=0.45)
+        lines = ["1", "0.5 0.5"] + [" ".join(repr(float(v)) for v in row) for row in st]
+        t0 = time.perf_counter()
+        refrun.run(p, cd, lines)
+        wall = time.perf_counter() - t0
+        kind = "reference"
+        what = "oracle/_ref/xsclaw_eu2d (reference src/ + project hooks, flang -O3, 1 thread)"
+    else:
+        from oracle import oracle
+        q0 = quadrant_q(p, st)
+        ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=steps, dt_variable=1)
+        t0 = time.perf_counter()
+        oracle.sharpclaw(p, ctl, q0, 0.0, 1e9, 1e-4)
+        wall = time.perf_counter() - t0
+        kind = "port"
+        what = "oracle/sc_oracle.c (C restatement, gcc -O2, 1 thread)"
+    value = cells * 3 * steps / wall
+    return {"value": value, "unit": "cell-updates/s per RK stage", "cores": 1, "kind": kind,
+            "sample": f"{n}x{n}, {steps} SSPRK3 steps ({3 * steps} stages), whole-process wall "
+                      f"{wall:.2f} s incl. input/qinit/output; {what}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--fp32", action="store_true", help="float32 reconstruction + Riemann")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent solves shard without exchange; one grid would need halo "
+              "exchange, which is not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from sharpclaw_amd import SC_FP32, SC_FP64, ScRunCtl, Solver, make_params
+    n = args.n
+    p = make_params("euler", (n, n), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1),
+                    precision=SC_FP32 if args.fp32 else SC_FP64)
+    st = quadrant_states(args.seed)
+    q0 = quadrant_q(p, st)
+    cells = n * n
+    log(f"bench: 2D Euler {n}x{n} {'fp32' if args.fp32 else 'fp64'} char-WENO5 SSPRK3, "
+        f"warmup {args.warmup}, steps {args.steps}")
+    with Solver(p) as s:
+        s.set_q(q0)
+        dt = 1e-4
+        t = 0.0
+        if args.warmup > 0:
+            ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=args.warmup,
+                           dt_variable=1)
+            dt, wst = s.advance(ctl, 0.0, 1e9, dt)
+            t = wst.t
+        log(f"bench: warmup done (t={t:.4e}, dt={dt:.4e})")
+        ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=args.steps, dt_variable=1)
+        w0 = time.perf_counter()
+        dt_out, rst = s.advance(ctl, t, 1e9, dt)
+        wall = time.perf_counter() - w0
+        if rst.steps_taken != args.steps:
+            raise RuntimeError(f"took {rst.steps_taken} steps, expected {args.steps}")
+        log(f"bench: timed {args.steps} steps, gpu {rst.gpu_ms:.1f} ms, wall {wall * 1e3:.1f} ms, "
+            f"rejected {rst.rejected}, cfl_last {rst.cfl_last:.4f}")
+        value = cells * rst.stage_evals / (rst.gpu_ms * 1e-3)
+        # kernel-only: the fused stage launches of 50 SSPRK3 steps back to back
+        ms_launch = s.time_stage(50, dt_out)
+        log(f"bench: fused stage kernel {ms_launch * 1e3:.1f} us per launch")
+        sums = s.sums()
+
+    meqn = 4
+    bytes_per_cell_stage = meqn * 8 * (2 + 3 + 3) / 3.0  # SSPRK3: state reads+writes per stage
+    alg_bytes = bytes_per_cell_stage * cells
+    achieved = alg_bytes / (ms_launch * 1e-3) / 1e9
+    traffic = traffic_from_profile()
+    out = {
+        "metric": "cell-updates/s per RK stage",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.fp32 else "f64",
+        "data": "synthetic (seeded four-quadrant Riemann initial data)",
+        "config": {"workload": f"2D Euler {n}x{n}, dimension-by-dimension characteristic WENO5 + "
+                               f"Roe rpn2, SSPRK3, extrapolation BCs, "
+                               f"{'float32 arithmetic / float64 state' if args.fp32 else 'float64'}",
+                   "cells": cells, "meqn": meqn, "stages_timed": int(rst.stage_evals),
+                   "seed": args.seed},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "stage2d_kernel<EU2,char,f64,16x16>",
+                     "alg_bytes_per_launch": alg_bytes,
+                     "us_per_launch": ms_launch * 1e3},
+        "gpu_ms_timed": rst.gpu_ms,
+        "pct_hbm_peak": 100.0 * achieved / HBM_PEAK_GBS,
+        "conservation_sums": [float(x) for x in sums],
+    }
+    if not args.no_cpu:
+        log(f"bench: cpu baseline ({n}x{n}, {args.cpu_steps} steps)")
+        out["cpu_baseline"] = cpu_baseline(n, st, args.cpu_steps)
+        out["cpu_baseline"]["cores"] = 1
+        log(f"bench: cpu baseline {out['cpu_baseline']['value']:.3e} cell-stage/s")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

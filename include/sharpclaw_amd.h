@@ -141,9 +141,10 @@ int sc_advance(sc_solver *s, const sc_run_ctl *ctl, double tstart, double tend,
 /* Conservation sums  sum_cells q(.,m) * dx [* dy]  per component, in float64. */
 int sc_sums(sc_solver *s, double *sums);
 
-/* Device time of n back-to-back fused stages (SSP33 stage-2 form) on the
- * current q, in ms per stage, measured with HIP events on the solver's
- * stream.  Leaves q unchanged. */
+/* Device time of the fused stage launches of n back-to-back time steps of
+ * params->time_integrator from the current q (every step restarts from q;
+ * no CFL readback), in ms per stage launch, measured with HIP events on the
+ * solver's stream.  Leaves q unchanged. */
 int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stage);
 
 /* ---- single routines on host arrays (each copies in, runs, copies out) */

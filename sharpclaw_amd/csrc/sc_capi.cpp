@@ -1,0 +1,724 @@
+// C ABI of sharpclaw_amd (include/sharpclaw_amd.h).
+//
+// The solver handle owns the device copies of q, qold and the RK registers
+// that module Global / type rkreg hold in the reference
+// (src/1d/Global1.f90, src/1d/qalloc1.f90:10-29).  Three q-sized buffers
+// rotate through the stages so that "qold = q" and "q = qold"
+// (sharpclaw.f90:134,201) are renamings; a fourth holds the SSP(10,4)
+// register r1 (sharpclaw.f90:425), a fifth dq for sc_step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/sharpclaw_amd.h"
+#include "sc_kernels.h"
+
+using namespace sc;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(SC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
+    } while (0)
+
+extern "C" const char *sc_last_error(void) { return g_err.c_str(); }
+extern "C" int sc_version(void) { return 1; }
+
+// --------------------------------------------------------- validation
+
+static int phys_id(const sc_params *p)
+{
+    switch (p->physics) {
+    case SC_PHYS_ADVECTION: return p->ndim == 1 ? ADV1 : -1;
+    case SC_PHYS_ACOUSTICS: return p->ndim == 1 ? AC1 : -1;
+    case SC_PHYS_EULER: return p->ndim == 1 ? EU1 : EU2;
+    case SC_PHYS_SHALLOW_WATER: return p->ndim == 1 ? SW1 : SW2;
+    }
+    return -1;
+}
+
+static int shape_m(int id)
+{
+    static const int M[] = {1, 2, 3, 2, 4, 3};
+    return M[id];
+}
+static int shape_w(int id)
+{
+    static const int W[] = {1, 2, 3, 2, 3, 3};
+    return W[id];
+}
+
+// Options the device path implements; everything else is rejected with the
+// reason, as the reference stops on an option it does not know.
+static int check_params(const sc_params *p, bool grid)
+{
+    if (!p) return fail(SC_ERR_ARG, "params is NULL");
+    if (p->ndim != 1 && p->ndim != 2) return fail(SC_ERR_ARG, "ndim must be 1 or 2 (got %d)", p->ndim);
+    int id = phys_id(p);
+    if (id < 0) return fail(SC_ERR_UNSUPPORTED, "physics %d has no %dD device solver", p->physics, p->ndim);
+    if (p->meqn != shape_m(id) || p->mwaves != shape_w(id))
+        return fail(SC_ERR_ARG, "physics %d in %dD needs meqn=%d mwaves=%d (got %d, %d)", p->physics,
+                    p->ndim, shape_m(id), shape_w(id), p->meqn, p->mwaves);
+    if (p->mbc != 3) return fail(SC_ERR_ARG, "mbc must be 3 for WENO5 (reconstruct.f90:85)");
+    if (p->maux != 0) return fail(SC_ERR_UNSUPPORTED, "maux > 0 is not supported on the device path");
+    if (p->mcapa != 0) return fail(SC_ERR_UNSUPPORTED, "mcapa > 0 is not supported on the device path");
+    if (p->lim_type != 2) return fail(SC_ERR_UNSUPPORTED, "lim_type %d: only 2 (WENO5) on the device path", p->lim_type);
+    if (p->char_decomp != 0 && p->char_decomp != 2)
+        return fail(SC_ERR_UNSUPPORTED, "char_decomp %d: device path implements 0 and 2", p->char_decomp);
+    if (p->tfluct_solver != 0) return fail(SC_ERR_UNSUPPORTED, "tfluct_solver 1 is not supported on the device path");
+    if (p->src_term != 0) return fail(SC_ERR_UNSUPPORTED, "src_term != 0 is not supported on the device path");
+    if (p->precision != SC_FP64 && p->precision != SC_FP32) return fail(SC_ERR_ARG, "precision must be SC_FP64 or SC_FP32");
+    if (!grid) return SC_OK;
+    if (p->time_integrator < 1 || p->time_integrator > 4)
+        return fail(SC_ERR_ARG, "time_integrator should be between 1 and 4 (sharpclaw.f90:172-174)");
+    for (int d = 0; d < p->ndim; d++) {
+        if (p->nx[d] < 1) return fail(SC_ERR_ARG, "nx[%d] = %d", d, p->nx[d]);
+        if (!(p->dx[d] > 0.0)) return fail(SC_ERR_ARG, "dx[%d] must be positive", d);
+        int bl = p->mthbc[2 * d], br = p->mthbc[2 * d + 1];
+        if (bl == 0 || br == 0)
+            return fail(SC_ERR_UNSUPPORTED, "mthbc=0 (user boundary conditions) has no device implementation");
+        if (bl < 1 || bl > 3 || br < 1 || br > 3) return fail(SC_ERR_ARG, "mthbc must be 1, 2 or 3");
+        if ((bl == 2) != (br == 2))
+            return fail(SC_ERR_ARG, "periodic boundary conditions require mthbc(%d) and mthbc(%d) both be 2 (main.f90:140-145)",
+                        2 * d + 1, 2 * d + 2);
+        if ((bl == 2 || bl == 3 || br == 3) && p->nx[d] < 3)
+            return fail(SC_ERR_ARG, "periodic/wall boundaries need at least 3 cells");
+        int mnorm = d == 0 ? 2 : 3;
+        if ((bl == 3 || br == 3) && p->meqn < mnorm)
+            return fail(SC_ERR_ARG, "a wall negates component %d, which meqn=%d does not have", mnorm, p->meqn);
+    }
+    return SC_OK;
+}
+
+template <class T> static PhysConst<T> mkconst(const sc_params *p)
+{
+    PhysConst<T> c;
+    c.gamma = T(p->gamma);
+    c.gamma1 = T(p->gamma - 1.0);
+    c.grav = T(p->grav);
+    c.u = T(p->u);
+    double cc = std::sqrt(p->bulk / p->rho);
+    c.cc = T(cc);
+    c.zz = T(cc * p->rho);
+    if (p->efix >= 0)
+        c.efix = p->efix;
+    else if (p->physics == SC_PHYS_EULER)
+        c.efix = 1;  // rp1eu.f:40
+    else if (p->physics == SC_PHYS_SHALLOW_WATER)
+        c.efix = p->ndim == 1;  // rp1sw.f:40 on, rpn2sw.f:48 off
+    else
+        c.efix = 0;
+    return c;
+}
+
+// ------------------------------------------------------------- handle
+
+static constexpr int NSLOT = 1024;  // CFL words (2 each) before the ring is cleared
+
+struct sc_solver {
+    sc_params p;
+    PhysSel sel;
+    PhysConst<double> cd;
+    PhysConst<float> cf;
+    int dev;
+    hipStream_t st;
+    size_t n;          // doubles per grid array
+    double *buf[3];    // rotating q buffers
+    double *qb;        // SSP(10,4) register
+    double *dq;        // step() output
+    double *partial;   // sums scratch
+    double *sums;      // sums result
+    int nblk_sums;
+    unsigned long long *slots;  // CFL words (device)
+    unsigned long long *hslot;  // pinned readback
+    long long slot_ctr;
+    int cur, old;      // buffer indices of q and qold
+    bool have_q;
+    hipEvent_t ev0, ev1;
+};
+
+struct DevGuard {
+    int prev;
+    explicit DevGuard(int d) { hipGetDevice(&prev); hipSetDevice(d); }
+    ~DevGuard() { hipSetDevice(prev); }
+};
+
+static size_t grid_n(const sc_params *p)
+{
+    size_t n = (size_t)(p->nx[0] + 2 * p->mbc) * p->meqn;
+    if (p->ndim == 2) n *= (size_t)(p->nx[1] + 2 * p->mbc);
+    return n;
+}
+
+extern "C" int sc_create(const sc_params *params, sc_solver **out)
+{
+    if (!out) return fail(SC_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(params, true);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(SC_ERR_HIP, "no HIP device available");
+    if (params->device < 0 || params->device >= ndev)
+        return fail(SC_ERR_ARG, "device %d out of range (%d devices)", params->device, ndev);
+    DevGuard g(params->device);
+    sc_solver *s = new sc_solver();
+    memset((void *)s, 0, sizeof(*s));
+    s->p = *params;
+    if (s->p.ndim == 1) {
+        s->p.nx[1] = 1;
+        s->p.dx[1] = 1.0;
+    }
+    s->sel.phys = phys_id(params);
+    s->sel.chr = params->char_decomp;
+    s->sel.fp32 = params->precision == SC_FP32;
+    s->cd = mkconst<double>(params);
+    s->cf = mkconst<float>(params);
+    s->dev = params->device;
+    s->n = grid_n(&s->p);
+    s->nblk_sums = 256;
+    bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
+    for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&s->buf[k], s->n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->qb, s->n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->dq, s->n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->partial, (size_t)s->nblk_sums * s->p.meqn * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->sums, s->p.meqn * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->slots, NSLOT * 2 * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipHostMalloc(&s->hslot, 2 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
+    if (ok) {
+        for (int k = 0; k < 3 && ok; k++)
+            ok = hipMemsetAsync(s->buf[k], 0, s->n * sizeof(double), s->st) == hipSuccess;
+        ok = ok && hipMemsetAsync(s->qb, 0, s->n * sizeof(double), s->st) == hipSuccess;
+        ok = ok && hipMemsetAsync(s->dq, 0, s->n * sizeof(double), s->st) == hipSuccess;
+        ok = ok && hipMemsetAsync(s->slots, 0, NSLOT * 2 * sizeof(unsigned long long), s->st) == hipSuccess;
+        ok = ok && hipStreamSynchronize(s->st) == hipSuccess;
+    }
+    if (!ok) {
+        sc_destroy(s);
+        return fail(SC_ERR_NOMEM, "device allocation of %zu doubles x 5 failed", grid_n(params));
+    }
+    s->cur = 0;
+    s->old = 0;
+    *out = s;
+    return SC_OK;
+}
+
+extern "C" int sc_destroy(sc_solver *s)
+{
+    if (!s) return SC_OK;
+    DevGuard g(s->dev);
+    if (s->st) hipStreamSynchronize(s->st);
+    for (int k = 0; k < 3; k++) if (s->buf[k]) hipFree(s->buf[k]);
+    if (s->qb) hipFree(s->qb);
+    if (s->dq) hipFree(s->dq);
+    if (s->partial) hipFree(s->partial);
+    if (s->sums) hipFree(s->sums);
+    if (s->slots) hipFree(s->slots);
+    if (s->hslot) hipHostFree(s->hslot);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->st) hipStreamDestroy(s->st);
+    delete s;
+    return SC_OK;
+}
+
+static int free_buf(const sc_solver *s, int a, int b)
+{
+    for (int k = 0; k < 3; k++)
+        if (k != a && k != b) return k;
+    return -1;
+}
+
+extern "C" int sc_set_q(sc_solver *s, const double *q)
+{
+    if (!s || !q) return fail(SC_ERR_ARG, "NULL argument");
+    DevGuard g(s->dev);
+    if (s->cur == s->old) s->cur = free_buf(s, s->old, s->old);  // keep qold intact
+    HIPCHK(hipMemcpyAsync(s->buf[s->cur], q, s->n * sizeof(double), hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->have_q = true;
+    return SC_OK;
+}
+
+// ghosts of the current q hold the boundary fill of its interior
+extern "C" int sc_get_q(sc_solver *s, double *q)
+{
+    if (!s || !q) return fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    DevGuard g(s->dev);
+    if (s->cur == s->old) {
+        // q and qold share a buffer: fill ghosts of that buffer (harmless for qold)
+    }
+    HIPCHK(launch_bc(s->p.ndim, s->p.meqn, s->buf[s->cur], s->p.nx[0], s->p.nx[1],
+                     s->p.nx[0] + 6, s->p.ndim == 2 ? s->p.nx[1] + 6 : 1, s->p.mthbc, s->st));
+    HIPCHK(hipMemcpyAsync(q, s->buf[s->cur], s->n * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return SC_OK;
+}
+
+extern "C" int sc_save_qold(sc_solver *s)
+{
+    if (!s) return fail(SC_ERR_ARG, "NULL solver");
+    s->old = s->cur;
+    return SC_OK;
+}
+
+extern "C" int sc_restore_qold(sc_solver *s)
+{
+    if (!s) return fail(SC_ERR_ARG, "NULL solver");
+    s->cur = s->old;
+    return SC_OK;
+}
+
+static StageArgs base_args(const sc_solver *s)
+{
+    StageArgs a;
+    memset(&a, 0, sizeof a);
+    a.nx = s->p.nx[0];
+    a.ny = s->p.ndim == 2 ? s->p.nx[1] : 1;
+    a.LX = s->p.nx[0] + 2 * s->p.mbc;
+    a.LY = s->p.ndim == 2 ? s->p.nx[1] + 2 * s->p.mbc : 1;
+    for (int k = 0; k < 4; k++) a.bc[k] = s->p.mthbc[k];
+    return a;
+}
+
+static hipError_t run_stage(sc_solver *s, const StageArgs &a)
+{
+    if (s->p.ndim == 1) return launch_stage1d(s->sel, 1, a, s->cd, s->cf, s->st);
+    return launch_stage2d(s->sel, a, s->cd, s->cf, s->st);
+}
+
+// next CFL word pair, cleared ring-wise
+static unsigned long long *next_slot(sc_solver *s)
+{
+    if (s->slot_ctr % NSLOT == 0 && s->slot_ctr > 0)
+        hipMemsetAsync(s->slots, 0, NSLOT * 2 * sizeof(unsigned long long), s->st);
+    unsigned long long *w = s->slots + 2 * (s->slot_ctr % NSLOT);
+    s->slot_ctr++;
+    return w;
+}
+
+static int read_cfl(sc_solver *s, unsigned long long *w, double dt, double *cfl)
+{
+    HIPCHK(hipMemcpyAsync(s->hslot, w, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    double sx, sy;
+    memcpy(&sx, &s->hslot[0], 8);
+    memcpy(&sy, &s->hslot[1], 8);
+    // cfl = max over interfaces of dtdx*|s| (flux1.f90:126-133), max of the
+    // x and y sweeps in 2D (flux2.f90:47,80); dtdx*|s| is monotone in |s|
+    double c = (dt / s->p.dx[0]) * sx;
+    if (s->p.ndim == 2) c = std::fmax(c, (dt / s->p.dx[1]) * sy);
+    *cfl = c;
+    return SC_OK;
+}
+
+extern "C" int sc_step(sc_solver *s, double dt, double t, double *dq, double *cfl)
+{
+    (void)t;
+    if (!s || !cfl) return fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    DevGuard g(s->dev);
+    StageArgs a = base_args(s);
+    double *q = s->buf[s->cur];
+    HIPCHK(launch_bc(s->p.ndim, s->p.meqn, q, a.nx, a.ny, a.LX, a.LY, s->p.mthbc, s->st));
+    HIPCHK(hipMemsetAsync(s->dq, 0, s->n * sizeof(double), s->st));
+    a.qin = q;
+    a.qout = s->dq;
+    a.op = OP_DQ;
+    a.dtdx = dt / s->p.dx[0];
+    a.dtdy = dt / s->p.dx[1];
+    a.smax = next_slot(s);
+    HIPCHK(run_stage(s, a));
+    int rc = read_cfl(s, a.smax, dt, cfl);
+    if (rc) return rc;
+    if (dq) {
+        HIPCHK(hipMemcpyAsync(dq, s->dq, s->n * sizeof(double), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+    }
+    return SC_OK;
+}
+
+// The stage sequence of ts_ssp22/42/33/104 as (op, reads qold, uses register)
+struct StagePlan {
+    int n;
+    int op[10];
+};
+
+static StagePlan plan_of(int ti)
+{
+    StagePlan P;
+    switch (ti) {
+    case 1: P = {2, {OP_ADD1, OP_SSP22_2}}; break;
+    case 2: P = {4, {OP_ADD_THIRD, OP_ADD_THIRD, OP_ADD_THIRD, OP_SSP42_4}}; break;
+    case 3: P = {3, {OP_ADD1, OP_SSP33_2, OP_SSP33_3}}; break;
+    default:
+        P = {10, {OP_ADD_SIXTH, OP_ADD_SIXTH, OP_ADD_SIXTH, OP_ADD_SIXTH, OP_SSP104_5, OP_ADD_SIXTH,
+                  OP_ADD_SIXTH, OP_ADD_SIXTH, OP_ADD_SIXTH, OP_SSP104_10}};
+    }
+    return P;
+}
+
+// enqueue one time step; returns the CFL word of its last stage
+static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
+{
+    StageArgs a = base_args(s);
+    a.dtdx = dt / s->p.dx[0];
+    a.dtdy = dt / s->p.dx[1];
+    a.qold = s->buf[s->old];
+    a.qbin = s->qb;
+    a.qbout = s->qb;
+    StagePlan P = plan_of(s->p.time_integrator);
+    int in = s->cur;
+    for (int k = 0; k < P.n; k++) {
+        int outb = free_buf(s, in, s->old);
+        a.qin = s->buf[in];
+        a.qout = s->buf[outb];
+        a.op = P.op[k];
+        a.smax = (k == P.n - 1) ? next_slot(s) : nullptr;
+        if (a.smax) *slot = a.smax;
+        HIPCHK(run_stage(s, a));
+        in = outb;
+    }
+    s->cur = in;
+    return SC_OK;
+}
+
+extern "C" int sc_ts(sc_solver *s, double dt, double told, double *cfl)
+{
+    (void)told;
+    if (!s || !cfl) return fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    DevGuard g(s->dev);
+    unsigned long long *slot = nullptr;
+    int rc = enqueue_ts(s, dt, &slot);
+    if (rc) return rc;
+    return read_cfl(s, slot, dt, cfl);
+}
+
+static int stages_of(int ti)
+{
+    static const int n[] = {0, 2, 4, 3, 10};
+    return n[ti];
+}
+
+// src/sharpclaw.f90:35-235 (qold refreshed at every new step)
+extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, double tend, double *dt_io,
+                          sc_run_stats *st)
+{
+    if (!s || !c || !dt_io || !st) return fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    DevGuard g(s->dev);
+    memset(st, 0, sizeof *st);
+    double t = tstart, dt = *dt_io, told = tstart, cflmax = 0.0, cfl = 0.0;
+    double dtmin = dt, dtmax = dt;
+    int maxn = c->max_steps, info = 0, steps_taken = 0;
+    bool retake = false;
+    HIPCHK(hipEventRecord(s->ev0, s->st));
+    if (c->dt_variable == 0) {
+        if (tend < tstart) {
+            maxn = 1;
+        } else {
+            maxn = (int)((tend - tstart + 1e-10) / dt);
+            if (std::fabs(maxn * dt - (tend - tstart)) > 1e-5 * (tend - tstart)) {
+                info = 2;
+                goto done;
+            }
+        }
+    }
+    if (c->dt_variable == 1 && c->cfl_desired > c->cfl_max) {
+        info = 3;
+        goto done;
+    }
+    for (int it = 1; it <= maxn; it++) {
+        if (!retake) {
+            told = t;
+            if (told + dt > tend && tstart < tend) dt = tend - told;
+            s->old = s->cur;  // qold = q
+        }
+        retake = false;
+        t = told + dt;
+        unsigned long long *slot = nullptr;
+        int rc = enqueue_ts(s, dt, &slot);
+        if (rc) return rc;
+        st->stage_evals += stages_of(s->p.time_integrator);
+        rc = read_cfl(s, slot, dt, &cfl);
+        if (rc) return rc;
+        if (c->dt_variable == 1) {
+            if (cfl > 0.0) {
+                dt = std::fmin(c->dt_max, dt * c->cfl_desired / cfl);
+                dtmin = std::fmin(dt, dtmin);
+                dtmax = std::fmax(dt, dtmax);
+            } else {
+                dt = c->dt_max;
+            }
+        }
+        if (cfl <= c->cfl_max) {
+            cflmax = std::fmax(cfl, cflmax);
+        } else {
+            t = told;
+            s->cur = s->old;  // q = qold
+            st->rejected++;
+            if (c->dt_variable == 1) {
+                retake = true;
+            } else {
+                cflmax = std::fmax(cfl, cflmax);
+                break;
+            }
+        }
+        steps_taken++;
+        if (t >= tend) break;
+    }
+    if (c->dt_variable == 1 && t < tend && steps_taken == maxn) info = 11;
+    if (c->dt_variable == 0 && cflmax > c->cfl_max) info = 12;
+done:
+    HIPCHK(hipEventRecord(s->ev1, s->st));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    st->gpu_ms = ms;
+    st->t = t;
+    st->cfl_maxused = cflmax;
+    st->cfl_last = cfl;
+    st->dt_minused = dtmin;
+    st->dt_maxused = dtmax;
+    st->dt_last = dt;
+    st->steps_taken = steps_taken;
+    st->info = info;
+    *dt_io = dt;
+    return SC_OK;
+}
+
+extern "C" int sc_sums(sc_solver *s, double *sums)
+{
+    if (!s || !sums) return fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    DevGuard g(s->dev);
+    StageArgs a = base_args(s);
+    HIPCHK(launch_sums(s->p.ndim, s->p.meqn, s->buf[s->cur], a.nx, a.ny, a.LX, a.LY, s->partial,
+                       s->nblk_sums, s->sums, s->st));
+    HIPCHK(hipMemcpyAsync(sums, s->sums, s->p.meqn * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    double area = s->p.ndim == 2 ? s->p.dx[0] * s->p.dx[1] : s->p.dx[0];
+    for (int m = 0; m < s->p.meqn; m++)
+        sums[m] = s->p.ndim == 2 ? sums[m] * s->p.dx[0] * s->p.dx[1] : sums[m] * area;
+    return SC_OK;
+}
+
+extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stage)
+{
+    if (!s || !ms_per_stage || n < 1) return fail(SC_ERR_ARG, "bad argument");
+    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    DevGuard g(s->dev);
+    StageArgs a = base_args(s);
+    a.dtdx = dt / s->p.dx[0];
+    a.dtdy = dt / s->p.dx[1];
+    const int q0 = s->cur;
+    a.qold = s->buf[q0];
+    a.qbin = s->qb;
+    a.qbout = s->qb;
+    StagePlan P = plan_of(s->p.time_integrator);
+    // every step restarts from q0, which no stage writes; as in a real time
+    // step only the last stage reduces the CFL (into a scratch word)
+    auto one_step = [&]() -> hipError_t {
+        int in = q0;
+        for (int k = 0; k < P.n; k++) {
+            int outb = free_buf(s, in, q0);
+            a.qin = s->buf[in];
+            a.qout = s->buf[outb];
+            a.op = P.op[k];
+            a.smax = k == P.n - 1 ? s->slots : nullptr;
+            hipError_t e = run_stage(s, a);
+            if (e != hipSuccess) return e;
+            in = outb;
+        }
+        return hipSuccess;
+    };
+    HIPCHK(one_step());  // warm-up
+    HIPCHK(hipEventRecord(s->ev0, s->st));
+    for (int k = 0; k < n; k++) HIPCHK(one_step());
+    HIPCHK(hipEventRecord(s->ev1, s->st));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    *ms_per_stage = ms / ((double)n * P.n);
+    HIPCHK(hipMemsetAsync(s->slots, 0, NSLOT * 2 * sizeof(unsigned long long), s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->slot_ctr = 0;
+    return SC_OK;
+}
+
+// ------------------------------------------------------ single routines
+
+namespace {
+struct DevBuf {
+    double *p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+    hipError_t alloc(size_t n) { return hipMalloc(&p, n * sizeof(double) + 8); }
+};
+}  // namespace
+
+static int one_call_setup(const sc_params *params)
+{
+    int rc = check_params(params, false);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SC_ERR_HIP, "no HIP device available");
+    if (params->device < 0 || params->device >= ndev) return fail(SC_ERR_ARG, "device out of range");
+    return SC_OK;
+}
+
+#define UP(d, h, n) HIPCHK(hipMemcpy((d).p, (h), (n) * sizeof(double), hipMemcpyHostToDevice))
+#define DOWN(h, d, n) HIPCHK(hipMemcpy((h), (d).p, (n) * sizeof(double), hipMemcpyDeviceToHost))
+
+extern "C" int sc_rp(const sc_params *params, int ixy, int mx, const double *ql, const double *qr,
+                     double *wave, double *s, double *amdq, double *apdq)
+{
+    int rc = one_call_setup(params);
+    if (rc) return rc;
+    if (mx < 1 || (ixy != 1 && ixy != 2) || !ql || !qr || !wave || !s || !amdq || !apdq)
+        return fail(SC_ERR_ARG, "bad argument");
+    DevGuard g(params->device);
+    const int M = params->meqn, W = params->mwaves, L = mx + 2 * params->mbc;
+    PhysSel sel{phys_id(params), 0, params->precision == SC_FP32};
+    DevBuf dql, dqr, dw, ds, dam, dap;
+    HIPCHK(dql.alloc((size_t)L * M)); HIPCHK(dqr.alloc((size_t)L * M));
+    HIPCHK(dw.alloc((size_t)L * M * W)); HIPCHK(ds.alloc((size_t)L * W));
+    HIPCHK(dam.alloc((size_t)L * M)); HIPCHK(dap.alloc((size_t)L * M));
+    UP(dql, ql, (size_t)L * M);
+    UP(dqr, qr, (size_t)L * M);
+    // entries the reference leaves untouched (i = 1-mbc) keep the caller's values
+    UP(dw, wave, (size_t)L * M * W);
+    UP(ds, s, (size_t)L * W);
+    UP(dam, amdq, (size_t)L * M);
+    UP(dap, apdq, (size_t)L * M);
+    HIPCHK(launch_rp(sel, ixy, mx, params->mbc, dql.p, dqr.p, dw.p, ds.p, dam.p, dap.p,
+                     mkconst<double>(params), mkconst<float>(params), 0));
+    HIPCHK(hipDeviceSynchronize());
+    DOWN(wave, dw, (size_t)L * M * W);
+    DOWN(s, ds, (size_t)L * W);
+    DOWN(amdq, dam, (size_t)L * M);
+    DOWN(apdq, dap, (size_t)L * M);
+    return SC_OK;
+}
+
+extern "C" int sc_evec(const sc_params *params, int ixy, int mx, const double *q, double *evl,
+                       double *evr)
+{
+    int rc = one_call_setup(params);
+    if (rc) return rc;
+    if (mx < 1 || (ixy != 1 && ixy != 2) || !q || !evl || !evr) return fail(SC_ERR_ARG, "bad argument");
+    DevGuard g(params->device);
+    const int M = params->meqn, L = mx + 2 * params->mbc;
+    PhysSel sel{phys_id(params), 2, params->precision == SC_FP32};
+    DevBuf dq, dl, dr;
+    HIPCHK(dq.alloc((size_t)L * M)); HIPCHK(dl.alloc((size_t)L * M * M)); HIPCHK(dr.alloc((size_t)L * M * M));
+    UP(dq, q, (size_t)L * M);
+    UP(dl, evl, (size_t)L * M * M);
+    UP(dr, evr, (size_t)L * M * M);
+    HIPCHK(launch_evec(sel, ixy, mx, params->mbc, dq.p, dl.p, dr.p, mkconst<double>(params),
+                       mkconst<float>(params), 0));
+    HIPCHK(hipDeviceSynchronize());
+    DOWN(evl, dl, (size_t)L * M * M);
+    DOWN(evr, dr, (size_t)L * M * M);
+    return SC_OK;
+}
+
+static int weno_common(const sc_params *params, int mx2, const double *q, const double *evl,
+                       const double *evr, double *ql, double *qr, int chr)
+{
+    if (!params) return fail(SC_ERR_ARG, "params is NULL");
+    if (params->meqn < 1 || params->meqn > 4) return fail(SC_ERR_UNSUPPORTED, "meqn must be 1..4");
+    if (mx2 < 6 || !q || !ql || !qr || (chr == 2 && (!evl || !evr))) return fail(SC_ERR_ARG, "bad argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SC_ERR_HIP, "no HIP device available");
+    DevGuard g(params->device);
+    const int M = params->meqn;
+    DevBuf dq, dl, dr, dql, dqr;
+    HIPCHK(dq.alloc((size_t)mx2 * M)); HIPCHK(dql.alloc((size_t)mx2 * M)); HIPCHK(dqr.alloc((size_t)mx2 * M));
+    UP(dq, q, (size_t)mx2 * M);
+    UP(dql, ql, (size_t)mx2 * M);
+    UP(dqr, qr, (size_t)mx2 * M);
+    if (chr == 2) {
+        HIPCHK(dl.alloc((size_t)mx2 * M * M)); HIPCHK(dr.alloc((size_t)mx2 * M * M));
+        UP(dl, evl, (size_t)mx2 * M * M);
+        UP(dr, evr, (size_t)mx2 * M * M);
+    }
+    HIPCHK(launch_weno(M, chr, params->precision == SC_FP32, mx2, dq.p, dl.p, dr.p, dql.p, dqr.p, 0));
+    HIPCHK(hipDeviceSynchronize());
+    DOWN(ql, dql, (size_t)mx2 * M);
+    DOWN(qr, dqr, (size_t)mx2 * M);
+    return SC_OK;
+}
+
+extern "C" int sc_weno5(const sc_params *params, int mx2, const double *q, double *ql, double *qr)
+{
+    return weno_common(params, mx2, q, nullptr, nullptr, ql, qr, 0);
+}
+
+extern "C" int sc_weno5_char(const sc_params *params, int mx2, const double *q, const double *evl,
+                             const double *evr, double *ql, double *qr)
+{
+    return weno_common(params, mx2, q, evl, evr, ql, qr, 2);
+}
+
+extern "C" int sc_flux1(const sc_params *params, int ixy, int mx, const double *q1d, double *dq1d,
+                        double dt, double *cfl)
+{
+    int rc = one_call_setup(params);
+    if (rc) return rc;
+    if (mx < 1 || (ixy != 1 && ixy != 2) || (ixy == 2 && params->ndim != 2) || !q1d || !dq1d || !cfl)
+        return fail(SC_ERR_ARG, "bad argument");
+    DevGuard g(params->device);
+    const int M = params->meqn, L = mx + 2 * params->mbc;
+    PhysSel sel{phys_id(params), params->char_decomp, params->precision == SC_FP32};
+    DevBuf dq, ddq;
+    HIPCHK(dq.alloc((size_t)L * M));
+    HIPCHK(ddq.alloc((size_t)L * M));
+    unsigned long long *slot = nullptr;
+    HIPCHK(hipMalloc(&slot, 2 * sizeof(unsigned long long)));
+    UP(dq, q1d, (size_t)L * M);
+    HIPCHK(hipMemset(ddq.p, 0, (size_t)L * M * sizeof(double)));
+    HIPCHK(hipMemset(slot, 0, 2 * sizeof(unsigned long long)));
+    StageArgs a;
+    memset(&a, 0, sizeof a);
+    a.qin = dq.p;
+    a.qout = ddq.p;
+    a.smax = slot;
+    a.nx = mx;
+    a.ny = 1;
+    a.LX = L;
+    a.LY = 1;
+    a.op = OP_DQ;
+    a.dtdx = dt / params->dx[ixy - 1];
+    hipError_t e = launch_stage1d(sel, ixy, a, mkconst<double>(params), mkconst<float>(params), 0);
+    unsigned long long w[2] = {0, 0};
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(w, slot, sizeof w, hipMemcpyDeviceToHost);
+    hipFree(slot);
+    if (e != hipSuccess) return fail(SC_ERR_HIP, "flux1: %s", hipGetErrorString(e));
+    DOWN(dq1d, ddq, (size_t)L * M);
+    double sx;
+    memcpy(&sx, &w[0], 8);
+    *cfl = a.dtdx * sx;
+    return SC_OK;
+}

@@ -1,0 +1,68 @@
+// Kernel arguments and host-side launchers of sc_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+#include "sc_physics.h"
+
+namespace sc {
+
+// How a fused stage combines q^n (qold), the stage input q, r = dt*L(q)
+// and the SSP(10,4) register qb (src/sharpclaw.f90:247-464).
+enum StageOp {
+    OP_DQ = 0,        // out = r                                (step: dq = dt*L(q))
+    OP_ADD1 = 1,      // out = q + r                            (SSP22/33 stage 1)
+    OP_SSP33_2 = 2,   // out = 0.75*qold + 0.25*(q + r)         (sharpclaw.f90:365)
+    OP_SSP33_3 = 3,   // out = (qold + 2*(q + r))/3             (sharpclaw.f90:374)
+    OP_SSP22_2 = 4,   // out = 0.5*(qold + q + r)               (sharpclaw.f90:274)
+    OP_ADD_THIRD = 5, // out = q + r/3                          (sharpclaw.f90:304,313,322)
+    OP_SSP42_4 = 6,   // out = 0.25*qold + 0.75*q + 0.25*r      (sharpclaw.f90:331)
+    OP_ADD_SIXTH = 7, // out = q + r/6                          (sharpclaw.f90:396-455)
+    OP_SSP104_5 = 8,  // r2 = q + r/6; qb = 0.04 qold + 0.36 r2; out = 0.6 qold + 0.4 r2 (424-426)
+    OP_SSP104_10 = 9, // out = qb + 0.6*q + 0.1*r               (sharpclaw.f90:462)
+};
+
+struct StageArgs {
+    const double *qin;   // stage input, grid layout with ghosts
+    const double *qold;  // q^n
+    const double *qbin;  // SSP(10,4) register read by OP_SSP104_10
+    double *qout;        // stage output (dq for OP_DQ)
+    double *qbout;       // SSP(10,4) register written by OP_SSP104_5
+    unsigned long long *smax;  // [0] max|s| x-sweeps, [1] y-sweeps; NULL = no CFL
+    int nx, ny;          // interior cells
+    int LX, LY;          // extents with ghosts (nx+2mbc, ny+2mbc)
+    int bc[4];           // 0: ghosts stored in qin; 1 extrap, 2 periodic, 3 wall
+    double dtdx, dtdy;
+    int op;
+    int ntx, nty;        // tiles per direction (2D)
+};
+
+// physics selector of the launchers (sc_physics.h PhysId)
+struct PhysSel {
+    int phys;   // PhysId
+    int chr;    // 0 or 2
+    int fp32;   // 0 float64, 1 float32 arithmetic
+};
+
+hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const PhysConst<double> &cd,
+                          const PhysConst<float> &cf, hipStream_t st);
+hipError_t launch_stage2d(const PhysSel &s, const StageArgs &a, const PhysConst<double> &cd,
+                          const PhysConst<float> &cf, hipStream_t st);
+// ghost fill in place (bc1.f90 / bc2.f90 semantics, corners included)
+hipError_t launch_bc(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY, const int *bc,
+                     hipStream_t st);
+// single routines on device arrays in the reference layouts
+hipError_t launch_rp(const PhysSel &s, int ixy, int mx, int mbc, const double *ql, const double *qr,
+                     double *wave, double *sp, double *amdq, double *apdq,
+                     const PhysConst<double> &cd, const PhysConst<float> &cf, hipStream_t st);
+hipError_t launch_evec(const PhysSel &s, int ixy, int mx, int mbc, const double *q, double *evl,
+                       double *evr, const PhysConst<double> &cd, const PhysConst<float> &cf,
+                       hipStream_t st);
+hipError_t launch_weno(int meqn, int chr, int fp32, int mx2, const double *q, const double *evl,
+                       const double *evr, double *ql, double *qr, hipStream_t st);
+// deterministic two-pass sums of the interior, per component
+hipError_t launch_sums(int ndim, int meqn, const double *q, int nx, int ny, int LX, int LY,
+                       double *partial, int nblk, double *out, hipStream_t st);
+hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t st);
+
+}  // namespace sc

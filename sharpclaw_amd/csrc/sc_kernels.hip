@@ -1,0 +1,741 @@
+// gfx950 kernels of the semi-discrete step.
+//
+// Fused stage kernels: one pass reads the stage input q (plus q^n and the
+// SSP(10,4) register where the stage needs them), synthesises ghost cells
+// for the boundary conditions in LDS, reconstructs both sides of every
+// interface (WENO5, component-wise or characteristic), solves the interface
+// Riemann problems and the in-cell problems in registers, forms
+// dq = dt*L(q), combines it into the next RK stage and writes that stage.
+// The maximum wave speed is reduced across each wavefront with shuffles,
+// across the block in LDS, and merged into a device word with one atomic
+// per block.
+#include "sc_kernels.h"
+#include "sc_weno.h"
+
+namespace sc {
+
+// --------------------------------------------------------------- helpers
+
+__device__ __forceinline__ void combine(int op, double qo, double q, double r, double qbi,
+                                        double &out, double &qbo)
+{
+    switch (op) {
+    case OP_DQ: out = r; break;
+    case OP_ADD1: out = q + r; break;
+    case OP_SSP33_2: out = 0.75 * qo + 0.25 * (q + r); break;
+    case OP_SSP33_3: out = (qo + 2.0 * (q + r)) / 3.0; break;
+    case OP_SSP22_2: out = 0.5 * (qo + q + r); break;
+    case OP_ADD_THIRD: out = q + (1.0 / 3.0) * r; break;
+    case OP_SSP42_4: out = 0.25 * qo + 0.75 * q + 0.25 * r; break;
+    case OP_ADD_SIXTH: out = q + (1.0 / 6.0) * r; break;
+    case OP_SSP104_5: {
+        double r2 = q + (1.0 / 6.0) * r;
+        qbo = 0.04 * qo + 0.36 * r2;
+        out = 0.6 * qo + 0.4 * r2;
+        break;
+    }
+    default: out = qbi + 0.6 * q + 0.1 * r; break;  // OP_SSP104_10
+    }
+}
+
+__device__ __forceinline__ bool op_reads_qold(int op)
+{
+    return op == OP_SSP33_2 || op == OP_SSP33_3 || op == OP_SSP22_2 || op == OP_SSP42_4 ||
+           op == OP_SSP104_5;
+}
+
+// Source cell of (possibly ghost) cell j for one direction of length n
+// (bc1.f90:40-107, bc2.f90:39-204); neg: component mnorm is negated.
+__device__ __forceinline__ int bc_src(int j, int n, int bl, int br, bool &neg)
+{
+    neg = false;
+    if (j < 1) {
+        if (bl == 1) return 1;
+        if (bl == 2) return n + j;
+        if (bl == 3) { neg = true; return 1 - j; }
+        return j;
+    }
+    if (j > n) {
+        if (br == 1) return n;
+        if (br == 2) return j - n;
+        if (br == 3) { neg = true; return 2 * n + 1 - j; }
+        return j;
+    }
+    return j;
+}
+
+template <class T> __device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// max|s| of the block merged into a device word with one atomic per block
+// (doubles >= 0 order like their bit patterns).  red: NW doubles of LDS.
+template <class T, int NW>
+__device__ __forceinline__ void merge_smax(unsigned long long *slot, T v, double *red)
+{
+    double d = (double)wave_max(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+#pragma unroll
+        for (int k = 1; k < NW; k++) m = fmax(m, red[k]);
+        if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// ------------------------------------------------------ 1D fused stage
+
+// One block of NT threads solves NT interfaces i = c0..c0+NT-1 and updates
+// the NT-1 cells c0..c0+NT-2.  The cells c0-3..c0+NT+1 are staged in LDS.
+template <int P, int IXY, int CHAR, class T, int NT>
+__global__ __launch_bounds__(NT) void stage1d_kernel(StageArgs a, PhysConst<T> c)
+{
+    constexpr int M = Shape<P>::M, W = Shape<P>::W;
+    constexpr int NC = NT - 1;
+    constexpr int NL = NT + 5;
+    constexpr int MN = IXY == 1 ? 1 : 2;  // normal momentum for a wall
+    __shared__ T sq[M][NL];
+    __shared__ T sx[2 * M][NT];
+    const int t = threadIdx.x;
+    const int c0 = 1 + blockIdx.x * NC;
+    const int n = a.nx;
+
+    for (int k = t; k < NL; k += NT) {
+        int j = min(max(c0 - 3 + k, -2), n + 3);
+        bool neg;
+        int js = bc_src(j, n, a.bc[0], a.bc[1], neg);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            double val = a.qin[(size_t)(js + 2) + (size_t)m * a.LX];
+            if (m == MN && neg) val = -val;
+            sq[m][k] = T(val);
+        }
+    }
+    __syncthreads();
+
+    const int i = c0 + t;  // interface i-1/2
+    T v[6][M];
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+#pragma unroll
+        for (int m = 0; m < M; m++) v[k][m] = sq[m][t + k];
+    T lf[M], rt[M], wave[M * W], s[W], am[M], ap[M];
+    reconstruct_iface<P, IXY, CHAR, T>(c, v, lf, rt);
+    riemann<P, IXY, T>(c, lf, rt, wave, s, am, ap);
+    T smax = T(0);
+    if (i <= n + 1) {
+#pragma unroll
+        for (int w = 0; w < W; w++) smax = fmax(smax, fabs(s[w]));
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        sx[m][t] = lf[m];
+        sx[M + m][t] = am[m];
+    }
+    __syncthreads();
+
+    if (t < NC && i <= n) {
+        // in-cell problem of cell i: left ql(i) = rt, right qr(i) from interface i+1/2
+        T qrc[M], am2[M], ap2[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) qrc[m] = sx[m][t + 1];
+        riemann<P, IXY, T>(c, rt, qrc, wave, s, am2, ap2);
+        const bool rq = op_reads_qold(a.op);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            T X = ((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m];
+            double r = -(a.dtdx * (double)X);
+            size_t g = (size_t)(i + 2) + (size_t)m * a.LX;
+            double q = a.qin[g];
+            double qo = rq ? a.qold[g] : 0.0;
+            double qbi = a.op == OP_SSP104_10 ? a.qbin[g] : 0.0;
+            double out, qbo = 0.0;
+            combine(a.op, qo, q, r, qbi, out, qbo);
+            a.qout[g] = out;
+            if (a.op == OP_SSP104_5) a.qbout[g] = qbo;
+        }
+    }
+    __shared__ double red[NT / 64];
+    if (a.smax) merge_smax<T, NT / 64>(a.smax, smax, red);
+}
+
+// ------------------------------------------------------ 2D fused stage
+
+// A block updates a TX x TY tile.  q of the tile plus 3-cell halos along x
+// and y is staged in LDS (the corner blocks are not needed: every stencil
+// is one-directional).  x-interfaces, x in-cell problems, y-interfaces and
+// y in-cell problems run in turn on the same staged tile; the interface
+// results pass through LDS to the cells on either side.
+// CFL: the reference also sweeps the ghost rows j = 0, ny+1 and columns
+// i = 0, nx+1 (flux2.f90:36,68); for boundary options 1-3 their states are
+// copies (wall: with the tangential momentum negated) of the adjacent
+// interior row/column, whose wave speeds they therefore repeat exactly, so
+// the interior sweeps give the same maximum.
+template <int P, int CHAR, class T, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void stage2d_kernel(StageArgs a, PhysConst<T> c)
+{
+    constexpr int M = Shape<P>::M, W = Shape<P>::W;
+    constexpr int NT = TX * TY;
+    constexpr int LXT = TX + 6, LYT = TY + 6;
+    constexpr int NIX = (TX + 1) * TY, NIY = TX * (TY + 1);
+    constexpr int NI = NIX > NIY ? NIX : NIY;
+    __shared__ T sq[M][LYT][LXT];
+    __shared__ T sL[M][NI], sR[M][NI], sAm[M][NI], sAp[M][NI];
+    const int t = threadIdx.x;
+    const int nx = a.nx, ny = a.ny;
+
+    // XCD-aware tile order: consecutive workgroups go to different XCDs, so
+    // give each XCD a contiguous band of tiles (halo rows stay in its L2)
+    const int ntiles = a.ntx * a.nty;
+    int b = blockIdx.x;
+    if ((ntiles & 7) == 0) b = (b & 7) * (ntiles >> 3) + (b >> 3);
+    const int i0 = 1 + (b % a.ntx) * TX, j0 = 1 + (b / a.ntx) * TY;
+
+    for (int k = t; k < LYT * LXT; k += NT) {
+        int ly = k / LXT, lx = k - ly * LXT;
+        bool cx = lx < 3 || lx >= TX + 3, cy = ly < 3 || ly >= TY + 3;
+        if (cx && cy) continue;
+        int i = min(max(i0 - 3 + lx, -2), nx + 3);
+        int j = min(max(j0 - 3 + ly, -2), ny + 3);
+        bool negx = false, negy = false;
+        if (j >= 1 && j <= ny) {
+            i = bc_src(i, nx, a.bc[0], a.bc[1], negx);
+        } else {
+            i = min(max(i, 1), nx);
+            j = bc_src(j, ny, a.bc[2], a.bc[3], negy);
+        }
+        size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
+        size_t plane = (size_t)a.LX * a.LY;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            double val = a.qin[g + m * plane];
+            if ((m == 1 && negx) || (m == 2 && negy)) val = -val;
+            sq[m][ly][lx] = T(val);
+        }
+    }
+    __syncthreads();
+
+    T smx = T(0), smy = T(0);
+    T wave[M * W], s[W];
+    // x-interfaces
+    for (int k = t; k < NIX; k += NT) {
+        int r = k / (TX + 1), ci = k - r * (TX + 1);
+        T v[6][M], lf[M], rt[M], am[M], ap[M];
+#pragma unroll
+        for (int kk = 0; kk < 6; kk++)
+#pragma unroll
+            for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + 3][ci + kk];
+        reconstruct_iface<P, 1, CHAR, T>(c, v, lf, rt);
+        riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
+        if (i0 + ci <= nx + 1 && j0 + r <= ny) {
+#pragma unroll
+            for (int w = 0; w < W; w++) smx = fmax(smx, fabs(s[w]));
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            sL[m][k] = lf[m];
+            sR[m][k] = rt[m];
+            sAm[m][k] = am[m];
+            sAp[m][k] = ap[m];
+        }
+    }
+    __syncthreads();
+    const int cx = t % TX, cy = t / TX;
+    double dqx[M];
+    {
+        int kl = cy * (TX + 1) + cx, kr = kl + 1;
+        T ql_[M], qr_[M], am2[M], ap2[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            ql_[m] = sR[m][kl];
+            qr_[m] = sL[m][kr];
+        }
+        riemann<P, 1, T>(c, ql_, qr_, wave, s, am2, ap2);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            T X = ((sAm[m][kr] + sAp[m][kl]) + am2[m]) + ap2[m];
+            dqx[m] = -(a.dtdx * (double)X);
+        }
+    }
+    __syncthreads();
+    // y-interfaces
+    for (int k = t; k < NIY; k += NT) {
+        int r = k / TX, ci = k - r * TX;
+        T v[6][M], lf[M], rt[M], am[M], ap[M];
+#pragma unroll
+        for (int kk = 0; kk < 6; kk++)
+#pragma unroll
+            for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + kk][ci + 3];
+        reconstruct_iface<P, 2, CHAR, T>(c, v, lf, rt);
+        riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
+        if (i0 + ci <= nx && j0 + r <= ny + 1) {
+#pragma unroll
+            for (int w = 0; w < W; w++) smy = fmax(smy, fabs(s[w]));
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            sL[m][k] = lf[m];
+            sR[m][k] = rt[m];
+            sAm[m][k] = am[m];
+            sAp[m][k] = ap[m];
+        }
+    }
+    __syncthreads();
+    {
+        const int i = i0 + cx, j = j0 + cy;
+        int kb = cy * TX + cx, kt = kb + TX;
+        T ql_[M], qr_[M], am2[M], ap2[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            ql_[m] = sR[m][kb];
+            qr_[m] = sL[m][kt];
+        }
+        riemann<P, 2, T>(c, ql_, qr_, wave, s, am2, ap2);
+        if (i <= nx && j <= ny) {
+            const bool rq = op_reads_qold(a.op);
+            size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
+            size_t plane = (size_t)a.LX * a.LY;
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                T Y = ((sAm[m][kt] + sAp[m][kb]) + am2[m]) + ap2[m];
+                double r = dqx[m] + -(a.dtdy * (double)Y);
+                size_t gm = g + m * plane;
+                double q = a.qin[gm];
+                double qo = rq ? a.qold[gm] : 0.0;
+                double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
+                double out, qbo = 0.0;
+                combine(a.op, qo, q, r, qbi, out, qbo);
+                a.qout[gm] = out;
+                if (a.op == OP_SSP104_5) a.qbout[gm] = qbo;
+            }
+        }
+    }
+    __shared__ double red[2][NT / 64];
+    if (a.smax) {
+        merge_smax<T, NT / 64>(a.smax, smx, red[0]);
+        merge_smax<T, NT / 64>(a.smax + 1, smy, red[1]);
+    }
+}
+
+// ------------------------------------------------------------ ghost fill
+
+__global__ void bc1_kernel(double *q, int meqn, int n, int LX, int bl, int br)
+{
+    int t = threadIdx.x;  // 0..5 -> ghost cells -2..0, n+1..n+3
+    if (t >= 6) return;
+    int j = t < 3 ? t - 2 : n + t - 2;
+    bool neg;
+    int js = bc_src(j, n, bl, br, neg);
+    for (int m = 0; m < meqn; m++) {
+        double v = q[(size_t)(js + 2) + (size_t)m * LX];
+        q[(size_t)(j + 2) + (size_t)m * LX] = (m == 1 && neg) ? -v : v;
+    }
+}
+
+// x ghosts for every row (ghost rows included), as bc2.f90:31-116
+__global__ void bc2x_kernel(double *q, int meqn, int nx, int ny, int LX, int LY, int bl, int br)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 6 * LY) return;
+    int jj = idx / 6, t = idx % 6;
+    int j = jj - 2;
+    int i = t < 3 ? t - 2 : nx + t - 2;
+    bool neg;
+    int is = bc_src(i, nx, bl, br, neg);
+    size_t plane = (size_t)LX * LY;
+    for (int m = 0; m < meqn; m++) {
+        double v = q[(size_t)(is + 2) + (size_t)LX * (j + 2) + m * plane];
+        q[(size_t)(i + 2) + (size_t)LX * (j + 2) + m * plane] = (m == 1 && neg) ? -v : v;
+    }
+}
+
+// then y ghosts for every column (ghost columns included), bc2.f90:118-206
+__global__ void bc2y_kernel(double *q, int meqn, int nx, int ny, int LX, int LY, int bl, int br)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 6 * LX) return;
+    int ii = idx % LX, t = idx / LX;
+    int i = ii - 2;
+    int j = t < 3 ? t - 2 : ny + t - 2;
+    bool neg;
+    int js = bc_src(j, ny, bl, br, neg);
+    size_t plane = (size_t)LX * LY;
+    for (int m = 0; m < meqn; m++) {
+        double v = q[(size_t)(i + 2) + (size_t)LX * (js + 2) + m * plane];
+        q[(size_t)(i + 2) + (size_t)LX * (j + 2) + m * plane] = (m == 2 && neg) ? -v : v;
+    }
+}
+
+// ------------------------------------------------- single-routine kernels
+
+// rp on slices in the reference layout: i = 2-mbc..mx+mbc
+template <int P, int IXY, class T>
+__global__ void rp_kernel(int mx, int mbc, const double *ql, const double *qr, double *wave,
+                          double *sp, double *amdq, double *apdq, PhysConst<T> c)
+{
+    constexpr int M = Shape<P>::M, W = Shape<P>::W;
+    const int L = mx + 2 * mbc;
+    int i = 2 - mbc + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i > mx + mbc) return;
+    T lf[M], rt[M], wv[M * W], s[W], am[M], ap[M];
+    int o = i + mbc - 1;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        lf[m] = T(qr[o - 1 + m * L]);
+        rt[m] = T(ql[o + m * L]);
+    }
+    riemann<P, IXY, T>(c, lf, rt, wv, s, am, ap);
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        sp[o + w * L] = s[w];
+#pragma unroll
+        for (int m = 0; m < M; m++) wave[o + L * (m + M * w)] = wv[m + M * w];
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        amdq[o + m * L] = am[m];
+        apdq[o + m * L] = ap[m];
+    }
+}
+
+template <int P, int IXY, class T>
+__global__ void evec_kernel(int mx, int mbc, const double *q, double *evl, double *evr,
+                            PhysConst<T> c)
+{
+    constexpr int M = Shape<P>::M;
+    const int L = mx + 2 * mbc;
+    int i = 2 - mbc + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i > mx + mbc) return;
+    int o = i + mbc - 1;
+    T qa[M], qb[M], l[M * M], r[M * M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        qa[m] = T(q[o - 1 + m * L]);
+        qb[m] = T(q[o + m * L]);
+    }
+    eigvec<P, IXY, T>(c, qa, qb, l, r);
+#pragma unroll
+    for (int p = 0; p < M; p++)
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            evl[o + L * (p + M * m)] = l[p * M + m];  // evl(i,p,m)
+            evr[o + L * (m + M * p)] = r[m * M + p];  // evr(i,m,p)
+        }
+}
+
+// weno5 / weno5_char on one slice in assumed-shape layout (1..mx2); the
+// interfaces I = 4..mx2-2 (both sides defined in the reference)
+template <int M, int CHAR, class T>
+__global__ void weno_kernel(int mx2, const double *q, const double *evl, const double *evr,
+                            double *ql, double *qr)
+{
+    int I = 4 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (I > mx2 - 2) return;
+    T v[6][M];
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+#pragma unroll
+        for (int m = 0; m < M; m++) v[k][m] = T(q[(I - 4 + k) + m * mx2]);
+    T d[5][M];
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+        for (int m = 0; m < M; m++) d[k][m] = v[k + 1][m] - v[k][m];
+    T lf[M], rt[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        T central = (-v[1][m] + T(7) * (v[2][m] + v[3][m]) - v[4][m]) * (T(1) / T(12));
+        lf[m] = central;
+        rt[m] = central;
+    }
+    if constexpr (CHAR == 0) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            lf[m] = weno_corr<T>(T(1), d[0][m], d[1][m], d[2][m], d[3][m]) + lf[m];
+            rt[m] = weno_corr<T>(T(-1), d[4][m], d[3][m], d[2][m], d[1][m]) + rt[m];
+        }
+    } else {
+        const int mi = I - 1;
+#pragma unroll
+        for (int p = 0; p < M; p++) {
+            T hh[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                T acc = T(0);
+#pragma unroll
+                for (int m = 0; m < M; m++) acc = acc + T(evl[mi + mx2 * (p + M * m)]) * d[k][m];
+                hh[k] = acc;
+            }
+            T u1 = weno_corr<T>(T(1), hh[0], hh[1], hh[2], hh[3]);
+            T u2 = weno_corr<T>(T(-1), hh[4], hh[3], hh[2], hh[1]);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                T e = T(evr[mi + mx2 * (m + M * p)]);
+                lf[m] = lf[m] + e * u1;
+                rt[m] = rt[m] + e * u2;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        qr[(I - 2) + m * mx2] = lf[m];  // qr(I-1)
+        ql[(I - 1) + m * mx2] = rt[m];  // ql(I)
+    }
+}
+
+// ------------------------------------------------------------ reductions
+
+__global__ void sums_partial(int ndim, const double *q, int nx, int ny, int LX, int LY,
+                             double *partial, int nblk)
+{
+    const int m = blockIdx.y;
+    const long long ncell = (long long)nx * (ndim == 2 ? ny : 1);
+    double acc = 0.0;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < ncell;
+         k += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(k % nx) + 1;
+        int j = ndim == 2 ? (int)(k / nx) + 1 : 1;
+        size_t g = (size_t)(i + 2) + (ndim == 2 ? (size_t)LX * (j + 2) + (size_t)m * LX * LY
+                                                  : (size_t)m * LX);
+        acc += q[g];
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(size_t)m * nblk + blockIdx.x] = red[0];
+}
+
+__global__ void sums_final(const double *partial, int nblk, double *out)
+{
+    const int m = blockIdx.x;
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int b = 0; b < nblk; b++) acc += partial[(size_t)m * nblk + b];
+        out[m] = acc;
+    }
+}
+
+__global__ void copy_kernel(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n2)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n2;
+         k += (size_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
+// -------------------------------------------------------------- launchers
+
+constexpr int NT1 = 256;
+constexpr int TX2 = 16, TY2 = 16;
+
+template <int P, int IXY, int CHAR, class T>
+static hipError_t go1(const StageArgs &a, const PhysConst<T> &c, hipStream_t st)
+{
+    int nblk = (a.nx + NT1 - 2) / (NT1 - 1);
+    hipLaunchKernelGGL((stage1d_kernel<P, IXY, CHAR, T, NT1>), dim3(nblk), dim3(NT1), 0, st, a, c);
+    return hipGetLastError();
+}
+
+template <int P, int IXY, int CHAR>
+static hipError_t go1p(const PhysSel &s, const StageArgs &a, const PhysConst<double> &cd,
+                       const PhysConst<float> &cf, hipStream_t st)
+{
+    return s.fp32 ? go1<P, IXY, CHAR, float>(a, cf, st) : go1<P, IXY, CHAR, double>(a, cd, st);
+}
+
+template <int P, int IXY>
+static hipError_t go1c(const PhysSel &s, const StageArgs &a, const PhysConst<double> &cd,
+                       const PhysConst<float> &cf, hipStream_t st)
+{
+    return s.chr == 2 ? go1p<P, IXY, 2>(s, a, cd, cf, st) : go1p<P, IXY, 0>(s, a, cd, cf, st);
+}
+
+hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const PhysConst<double> &cd,
+                          const PhysConst<float> &cf, hipStream_t st)
+{
+    switch (s.phys) {
+    case ADV1: return go1c<ADV1, 1>(s, a, cd, cf, st);
+    case AC1: return go1c<AC1, 1>(s, a, cd, cf, st);
+    case EU1: return go1c<EU1, 1>(s, a, cd, cf, st);
+    case SW1: return go1c<SW1, 1>(s, a, cd, cf, st);
+    case EU2: return ixy == 1 ? go1c<EU2, 1>(s, a, cd, cf, st) : go1c<EU2, 2>(s, a, cd, cf, st);
+    case SW2: return ixy == 1 ? go1c<SW2, 1>(s, a, cd, cf, st) : go1c<SW2, 2>(s, a, cd, cf, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int P, int CHAR, class T>
+static hipError_t go2(const StageArgs &a, const PhysConst<T> &c, hipStream_t st)
+{
+    hipLaunchKernelGGL((stage2d_kernel<P, CHAR, T, TX2, TY2>), dim3(a.ntx * a.nty), dim3(TX2 * TY2),
+                       0, st, a, c);
+    return hipGetLastError();
+}
+
+template <int P>
+static hipError_t go2c(const PhysSel &s, const StageArgs &a, const PhysConst<double> &cd,
+                       const PhysConst<float> &cf, hipStream_t st)
+{
+    if (s.chr == 2)
+        return s.fp32 ? go2<P, 2, float>(a, cf, st) : go2<P, 2, double>(a, cd, st);
+    return s.fp32 ? go2<P, 0, float>(a, cf, st) : go2<P, 0, double>(a, cd, st);
+}
+
+hipError_t launch_stage2d(const PhysSel &s, const StageArgs &a0, const PhysConst<double> &cd,
+                          const PhysConst<float> &cf, hipStream_t st)
+{
+    StageArgs a = a0;
+    a.ntx = (a.nx + TX2 - 1) / TX2;
+    a.nty = (a.ny + TY2 - 1) / TY2;
+    switch (s.phys) {
+    case EU2: return go2c<EU2>(s, a, cd, cf, st);
+    case SW2: return go2c<SW2>(s, a, cd, cf, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_bc(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY, const int *bc,
+                     hipStream_t st)
+{
+    if (ndim == 1) {
+        hipLaunchKernelGGL(bc1_kernel, dim3(1), dim3(64), 0, st, q, meqn, nx, LX, bc[0], bc[1]);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(bc2x_kernel, dim3((6 * LY + 255) / 256), dim3(256), 0, st, q, meqn, nx, ny, LX,
+                       LY, bc[0], bc[1]);
+    hipLaunchKernelGGL(bc2y_kernel, dim3((6 * LX + 255) / 256), dim3(256), 0, st, q, meqn, nx, ny, LX,
+                       LY, bc[2], bc[3]);
+    return hipGetLastError();
+}
+
+template <int P, int IXY, class T>
+static hipError_t gorp(int mx, int mbc, const double *ql, const double *qr, double *wave, double *sp,
+                       double *amdq, double *apdq, const PhysConst<T> &c, hipStream_t st)
+{
+    int n = mx + 2 * mbc - 1;
+    hipLaunchKernelGGL((rp_kernel<P, IXY, T>), dim3((n + 127) / 128), dim3(128), 0, st, mx, mbc, ql,
+                       qr, wave, sp, amdq, apdq, c);
+    return hipGetLastError();
+}
+
+template <int P, int IXY>
+static hipError_t gorpp(const PhysSel &s, int mx, int mbc, const double *ql, const double *qr,
+                        double *wave, double *sp, double *amdq, double *apdq,
+                        const PhysConst<double> &cd, const PhysConst<float> &cf, hipStream_t st)
+{
+    return s.fp32 ? gorp<P, IXY, float>(mx, mbc, ql, qr, wave, sp, amdq, apdq, cf, st)
+                  : gorp<P, IXY, double>(mx, mbc, ql, qr, wave, sp, amdq, apdq, cd, st);
+}
+
+hipError_t launch_rp(const PhysSel &s, int ixy, int mx, int mbc, const double *ql, const double *qr,
+                     double *wave, double *sp, double *amdq, double *apdq,
+                     const PhysConst<double> &cd, const PhysConst<float> &cf, hipStream_t st)
+{
+#define RPA s, mx, mbc, ql, qr, wave, sp, amdq, apdq, cd, cf, st
+    switch (s.phys) {
+    case ADV1: return gorpp<ADV1, 1>(RPA);
+    case AC1: return gorpp<AC1, 1>(RPA);
+    case EU1: return gorpp<EU1, 1>(RPA);
+    case SW1: return gorpp<SW1, 1>(RPA);
+    case EU2: return ixy == 1 ? gorpp<EU2, 1>(RPA) : gorpp<EU2, 2>(RPA);
+    case SW2: return ixy == 1 ? gorpp<SW2, 1>(RPA) : gorpp<SW2, 2>(RPA);
+    }
+#undef RPA
+    return hipErrorInvalidValue;
+}
+
+template <int P, int IXY, class T>
+static hipError_t goev(int mx, int mbc, const double *q, double *evl, double *evr,
+                       const PhysConst<T> &c, hipStream_t st)
+{
+    int n = mx + 2 * mbc - 1;
+    hipLaunchKernelGGL((evec_kernel<P, IXY, T>), dim3((n + 127) / 128), dim3(128), 0, st, mx, mbc, q,
+                       evl, evr, c);
+    return hipGetLastError();
+}
+
+template <int P, int IXY>
+static hipError_t goevp(const PhysSel &s, int mx, int mbc, const double *q, double *evl, double *evr,
+                        const PhysConst<double> &cd, const PhysConst<float> &cf, hipStream_t st)
+{
+    return s.fp32 ? goev<P, IXY, float>(mx, mbc, q, evl, evr, cf, st)
+                  : goev<P, IXY, double>(mx, mbc, q, evl, evr, cd, st);
+}
+
+hipError_t launch_evec(const PhysSel &s, int ixy, int mx, int mbc, const double *q, double *evl,
+                       double *evr, const PhysConst<double> &cd, const PhysConst<float> &cf,
+                       hipStream_t st)
+{
+#define EVA s, mx, mbc, q, evl, evr, cd, cf, st
+    switch (s.phys) {
+    case ADV1: return goevp<ADV1, 1>(EVA);
+    case AC1: return goevp<AC1, 1>(EVA);
+    case EU1: return goevp<EU1, 1>(EVA);
+    case SW1: return goevp<SW1, 1>(EVA);
+    case EU2: return ixy == 1 ? goevp<EU2, 1>(EVA) : goevp<EU2, 2>(EVA);
+    case SW2: return ixy == 1 ? goevp<SW2, 1>(EVA) : goevp<SW2, 2>(EVA);
+    }
+#undef EVA
+    return hipErrorInvalidValue;
+}
+
+template <int M, int CHAR, class T>
+static hipError_t gow(int mx2, const double *q, const double *evl, const double *evr, double *ql,
+                      double *qr, hipStream_t st)
+{
+    int n = mx2 - 5;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((weno_kernel<M, CHAR, T>), dim3((n + 127) / 128), dim3(128), 0, st, mx2, q, evl,
+                       evr, ql, qr);
+    return hipGetLastError();
+}
+
+template <int M>
+static hipError_t gowm(int chr, int fp32, int mx2, const double *q, const double *evl,
+                       const double *evr, double *ql, double *qr, hipStream_t st)
+{
+    if (chr == 2)
+        return fp32 ? gow<M, 2, float>(mx2, q, evl, evr, ql, qr, st)
+                    : gow<M, 2, double>(mx2, q, evl, evr, ql, qr, st);
+    return fp32 ? gow<M, 0, float>(mx2, q, evl, evr, ql, qr, st)
+                : gow<M, 0, double>(mx2, q, evl, evr, ql, qr, st);
+}
+
+hipError_t launch_weno(int meqn, int chr, int fp32, int mx2, const double *q, const double *evl,
+                       const double *evr, double *ql, double *qr, hipStream_t st)
+{
+    switch (meqn) {
+    case 1: return gowm<1>(chr, fp32, mx2, q, evl, evr, ql, qr, st);
+    case 2: return gowm<2>(chr, fp32, mx2, q, evl, evr, ql, qr, st);
+    case 3: return gowm<3>(chr, fp32, mx2, q, evl, evr, ql, qr, st);
+    case 4: return gowm<4>(chr, fp32, mx2, q, evl, evr, ql, qr, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sums(int ndim, int meqn, const double *q, int nx, int ny, int LX, int LY,
+                       double *partial, int nblk, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(sums_partial, dim3(nblk, meqn), dim3(256), 0, st, ndim, q, nx, ny, LX, LY,
+                       partial, nblk);
+    hipLaunchKernelGGL(sums_final, dim3(meqn), dim3(64), 0, st, partial, nblk, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t st)
+{
+    size_t n2 = n / 2;
+    hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, st, (double2 *)dst,
+                       (const double2 *)src, n2);
+    return hipGetLastError();
+}
+
+}  // namespace sc

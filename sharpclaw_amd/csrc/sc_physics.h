@@ -1,0 +1,500 @@
+// Device physics for gfx950: the Riemann-solver plugins (rp) and the
+// eigenvector plugins (evec) of the reference, one interface at a time.
+//
+// A physics P fixes meqn (M) and mwaves (W).  rp<P,IXY>(c, lf, rt, ...)
+// solves the Riemann problem whose left state is qr(i-1,:) and right state
+// ql(i,:) (the reference's i-th problem, e.g. rp1eu.f:19-20); IXY selects the
+// normal momentum for the 2D solvers (rpn2sw.f:60-66).  The arithmetic
+// follows each reference solver's expression order; the compiler may fuse
+// multiply-adds, so results agree with the reference to rounding, not bit
+// for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sc {
+
+enum PhysId { ADV1 = 0, AC1 = 1, EU1 = 2, SW1 = 3, EU2 = 4, SW2 = 5 };
+
+template <int P> struct Shape;
+template <> struct Shape<ADV1> { static constexpr int M = 1, W = 1, DIM = 1; };
+template <> struct Shape<AC1> { static constexpr int M = 2, W = 2, DIM = 1; };
+template <> struct Shape<EU1> { static constexpr int M = 3, W = 3, DIM = 1; };
+template <> struct Shape<SW1> { static constexpr int M = 2, W = 2, DIM = 1; };
+template <> struct Shape<EU2> { static constexpr int M = 4, W = 3, DIM = 2; };
+template <> struct Shape<SW2> { static constexpr int M = 3, W = 3, DIM = 2; };
+
+// physical constants in the compute precision
+template <class T> struct PhysConst {
+    T gamma, gamma1;  // /param/ (setprob.f: gamma1 = gamma - 1)
+    T grav;           // /comrp/ grav
+    T u;              // /comrp/ u (advection)
+    T cc, zz;         // /cparam/ sound speed and impedance
+    int efix;
+};
+
+template <class T> __device__ __forceinline__ T dsq(T x) { return x * x; }
+
+// ---------------------------------------------------------------- rp
+
+// apps/advection/1d/rp1.f:35-43
+template <class T>
+__device__ __forceinline__ void rp_adv(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
+                                       T *s, T *amdq, T *apdq)
+{
+    T w = rt[0] - lf[0];
+    wave[0] = w;
+    s[0] = c.u;
+    amdq[0] = fmin(c.u, T(0)) * w;
+    apdq[0] = fmax(c.u, T(0)) * w;
+}
+
+// apps/acoustics/1d/homogeneous/rp1.f:50-76
+template <class T>
+__device__ __forceinline__ void rp_ac(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
+                                      T *s, T *amdq, T *apdq)
+{
+    const T zz = c.zz, cc = c.cc;
+    T d1 = rt[0] - lf[0], d2 = rt[1] - lf[1];
+    T a1 = (-d1 + zz * d2) / (T(2) * zz);
+    T a2 = (d1 + zz * d2) / (T(2) * zz);
+    // wave(m, mw) stored at wave[m + M*mw]
+    wave[0] = -(a1 * zz);
+    wave[1] = a1;
+    wave[2] = a2 * zz;
+    wave[3] = a2;
+    s[0] = -cc;
+    s[1] = cc;
+    amdq[0] = s[0] * wave[0];
+    amdq[1] = s[0] * wave[1];
+    apdq[0] = s[1] * wave[2];
+    apdq[1] = s[1] * wave[3];
+}
+
+template <int M, int W, class T>
+__device__ __forceinline__ void split_by_sign(const T *wave, const T *s, T *amdq, T *apdq)
+{
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        T am = T(0), ap = T(0);
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            if (s[w] < T(0))
+                am = am + s[w] * wave[m + M * w];
+            else
+                ap = ap + s[w] * wave[m + M * w];
+        }
+        amdq[m] = am;
+        apdq[m] = ap;
+    }
+}
+
+template <int M, int W, class T>
+__device__ __forceinline__ void apdq_total(const T *wave, const T *s, const T *amdq, T *apdq)
+{
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        T df = T(0);
+#pragma unroll
+        for (int w = 0; w < W; w++) df = df + s[w] * wave[m + M * w];
+        apdq[m] = df - amdq[m];
+    }
+}
+
+// apps/euler/shockentropy/rp1eu.f:44-213 (efix: rp1eu.f:126-201)
+template <class T>
+__device__ __forceinline__ void rp_eu1(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
+                                       T *s, T *amdq, T *apdq)
+{
+    const T g = c.gamma, g1 = c.gamma1, h = T(0.5);
+    T rl = lf[0], ml = lf[1], el = lf[2];
+    T rr = rt[0], mr = rt[1], er = rt[2];
+    T rhsqrtl = sqrt(rl), rhsqrtr = sqrt(rr);
+    T pl = g1 * (el - h * (ml * ml) / rl);
+    T pr = g1 * (er - h * (mr * mr) / rr);
+    T rhsq2 = rhsqrtl + rhsqrtr;
+    T u = (ml / rhsqrtl + mr / rhsqrtr) / rhsq2;
+    T enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+    T a2 = g1 * (enth - h * (u * u));
+    T a = sqrt(a2);
+    T d1 = rr - rl, d2 = mr - ml, d3 = er - el;
+    T b2 = g1 / (a * a) * ((enth - u * u) * d1 + u * d2 - d3);
+    T b3 = (d2 + (a - u) * d1 - a * b2) / (T(2) * a);
+    T b1 = d1 - b2 - b3;
+    wave[0] = b1;
+    wave[1] = b1 * (u - a);
+    wave[2] = b1 * (enth - u * a);
+    s[0] = u - a;
+    wave[3] = b2;
+    wave[4] = b2 * u;
+    wave[5] = b2 * h * (u * u);
+    s[1] = u;
+    wave[6] = b3;
+    wave[7] = b3 * (u + a);
+    wave[8] = b3 * (enth + u * a);
+    s[2] = u + a;
+    if (!c.efix) {
+        split_by_sign<3, 3>(wave, s, amdq, apdq);
+        return;
+    }
+    do {
+        T s0 = ml / rl - sqrt(g * (g1 * (el - h * (ml * ml) / rl)) / rl);
+        if (s0 >= T(0) && s[0] > T(0)) {
+            amdq[0] = amdq[1] = amdq[2] = T(0);
+            break;
+        }
+        T rho1 = rl + wave[0], rhou1 = ml + wave[1], en1 = el + wave[2];
+        T p1 = g1 * (en1 - h * (rhou1 * rhou1) / rho1);
+        T s1 = rhou1 / rho1 - sqrt(g * p1 / rho1);
+        T sfract;
+        if (s0 < T(0) && s1 > T(0))
+            sfract = s0 * (s1 - s[0]) / (s1 - s0);
+        else if (s[0] < T(0))
+            sfract = s[0];
+        else
+            sfract = T(0);
+        amdq[0] = sfract * wave[0];
+        amdq[1] = sfract * wave[1];
+        amdq[2] = sfract * wave[2];
+        if (s[1] >= T(0)) break;
+        amdq[0] = amdq[0] + s[1] * wave[3];
+        amdq[1] = amdq[1] + s[1] * wave[4];
+        amdq[2] = amdq[2] + s[1] * wave[5];
+        T s3 = mr / rr + sqrt(g * (g1 * (er - h * (mr * mr) / rr)) / rr);
+        T rho2 = rr - wave[6], rhou2 = mr - wave[7], en2 = er - wave[8];
+        T p2 = g1 * (en2 - h * (rhou2 * rhou2) / rho2);
+        T s2 = rhou2 / rho2 + sqrt(g * p2 / rho2);
+        if (s2 < T(0) && s3 > T(0))
+            sfract = s2 * (s3 - s[2]) / (s3 - s2);
+        else if (s[2] < T(0))
+            sfract = s[2];
+        else
+            break;
+        amdq[0] = amdq[0] + sfract * wave[6];
+        amdq[1] = amdq[1] + sfract * wave[7];
+        amdq[2] = amdq[2] + sfract * wave[8];
+    } while (0);
+    apdq_total<3, 3>(wave, s, amdq, apdq);
+}
+
+// Roe solver for 2D Euler: normal/tangential selection of rpn2sw.f:60-66,
+// Roe averages, wave strengths and entropy fix of rp1eu.f extended by the
+// tangential momentum; the entropy and shear jumps share wave 2 (speed u).
+template <int IXY, class T>
+__device__ __forceinline__ void rp_eu2(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
+                                       T *s, T *amdq, T *apdq)
+{
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T g = c.gamma, g1 = c.gamma1, h = T(0.5);
+    T rl = lf[0], ul = lf[mu], vl = lf[mv], el = lf[3];
+    T rr = rt[0], ur = rt[mu], vr = rt[mv], er = rt[3];
+    T rhsqrtl = sqrt(rl), rhsqrtr = sqrt(rr);
+    T pl = g1 * (el - h * (ul * ul + vl * vl) / rl);
+    T pr = g1 * (er - h * (ur * ur + vr * vr) / rr);
+    T rhsq2 = rhsqrtl + rhsqrtr;
+    T u = (ul / rhsqrtl + ur / rhsqrtr) / rhsq2;
+    T v = (vl / rhsqrtl + vr / rhsqrtr) / rhsq2;
+    T enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+    T u2v2 = u * u + v * v;
+    T a2 = g1 * (enth - h * u2v2);
+    T a = sqrt(a2);
+    T d1 = rr - rl, d2 = ur - ul, d3 = vr - vl, d4 = er - el;
+    T a3 = d3 - v * d1;
+    T a2s = g1 / a2 * ((enth - u2v2) * d1 + u * d2 + v * d3 - d4);
+    T a4 = (d2 + (a - u) * d1 - a * a2s) / (T(2) * a);
+    T a1 = d1 - a2s - a4;
+    // wave(m, w) at wave[m + 4*w]
+    wave[0] = a1;
+    wave[mu] = a1 * (u - a);
+    wave[mv] = a1 * v;
+    wave[3] = a1 * (enth - u * a);
+    s[0] = u - a;
+    wave[4] = a2s;
+    wave[4 + mu] = a2s * u;
+    wave[4 + mv] = a2s * v + a3;
+    wave[7] = a2s * h * u2v2 + a3 * v;
+    s[1] = u;
+    wave[8] = a4;
+    wave[8 + mu] = a4 * (u + a);
+    wave[8 + mv] = a4 * v;
+    wave[11] = a4 * (enth + u * a);
+    s[2] = u + a;
+    if (!c.efix) {
+        split_by_sign<4, 3>(wave, s, amdq, apdq);
+        return;
+    }
+    do {
+        T s0 = ul / rl - sqrt(g * pl / rl);
+        if (s0 >= T(0) && s[0] > T(0)) {
+            amdq[0] = amdq[1] = amdq[2] = amdq[3] = T(0);
+            break;
+        }
+        T rho1 = rl + wave[0], rhou1 = ul + wave[mu], rhov1 = vl + wave[mv], en1 = el + wave[3];
+        T p1 = g1 * (en1 - h * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
+        T s1 = rhou1 / rho1 - sqrt(g * p1 / rho1);
+        T sfract;
+        if (s0 < T(0) && s1 > T(0))
+            sfract = s0 * (s1 - s[0]) / (s1 - s0);
+        else if (s[0] < T(0))
+            sfract = s[0];
+        else
+            sfract = T(0);
+#pragma unroll
+        for (int m = 0; m < 4; m++) amdq[m] = sfract * wave[m];
+        if (s[1] >= T(0)) break;
+#pragma unroll
+        for (int m = 0; m < 4; m++) amdq[m] = amdq[m] + s[1] * wave[4 + m];
+        T s3 = ur / rr + sqrt(g * pr / rr);
+        T rho2 = rr - wave[8], rhou2 = ur - wave[8 + mu], rhov2 = vr - wave[8 + mv], en2 = er - wave[11];
+        T p2 = g1 * (en2 - h * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
+        T s2 = rhou2 / rho2 + sqrt(g * p2 / rho2);
+        if (s2 < T(0) && s3 > T(0))
+            sfract = s2 * (s3 - s[2]) / (s3 - s2);
+        else if (s[2] < T(0))
+            sfract = s[2];
+        else
+            break;
+#pragma unroll
+        for (int m = 0; m < 4; m++) amdq[m] = amdq[m] + sfract * wave[8 + m];
+    } while (0);
+    apdq_total<4, 3>(wave, s, amdq, apdq);
+}
+
+// apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/rp1sw.f:43-183
+template <class T>
+__device__ __forceinline__ void rp_sw1(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
+                                       T *s, T *amdq, T *apdq)
+{
+    const T gr = c.grav, h = T(0.5);
+    T hl = lf[0], hul = lf[1], hr = rt[0], hur = rt[1];
+    T sl = sqrt(hl), sr = sqrt(hr);
+    T ubar = (hul / sl + hur / sr) / (sl + sr);
+    T cbar = sqrt(h * gr * (hl + hr));
+    T d1 = hr - hl, d2 = hur - hul;
+    T a1 = h * (-d2 + (ubar + cbar) * d1) / cbar;
+    T a2 = h * (d2 - (ubar - cbar) * d1) / cbar;
+    wave[0] = a1;
+    wave[1] = a1 * (ubar - cbar);
+    s[0] = ubar - cbar;
+    wave[2] = a2;
+    wave[3] = a2 * (ubar + cbar);
+    s[1] = ubar + cbar;
+    if (!c.efix) {
+        split_by_sign<2, 2>(wave, s, amdq, apdq);
+        return;
+    }
+    do {
+        T s0 = hul / hl - sqrt(gr * hl);
+        if (s0 >= T(0) && s[0] > T(0)) {
+            amdq[0] = amdq[1] = T(0);
+            break;
+        }
+        T hr1 = hl + wave[0], uhr1 = hul + wave[1];
+        T s1 = uhr1 / hr1 - sqrt(gr * hr1);
+        T sfract;
+        if (s0 < T(0) && s1 > T(0))
+            sfract = s0 * (s1 - s[0]) / (s1 - s0);
+        else if (s[0] < T(0))
+            sfract = s[0];
+        else
+            sfract = T(0);
+        amdq[0] = sfract * wave[0];
+        amdq[1] = sfract * wave[1];
+        T s3 = hur / hr + sqrt(gr * hr);
+        T hl2 = hr - wave[2], uhl2 = hur - wave[3];
+        T s2 = uhl2 / hl2 + sqrt(gr * hl2);
+        if (s2 < T(0) && s3 > T(0))
+            sfract = s2 * (s3 - s[1]) / (s3 - s2);
+        else if (s[1] < T(0))
+            sfract = s[1];
+        else
+            break;
+        amdq[0] = amdq[0] + sfract * wave[2];
+        amdq[1] = amdq[1] + sfract * wave[3];
+    } while (0);
+    apdq_total<2, 2>(wave, s, amdq, apdq);
+}
+
+// apps/shallowWater/2dRadialDam/rpn2sw.f:60-228
+template <int IXY, class T>
+__device__ __forceinline__ void rp_sw2(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
+                                       T *s, T *amdq, T *apdq)
+{
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    const T gr = c.grav, h = T(0.5);
+    T hh = (lf[0] + rt[0]) * h;
+    T hsqrtl = sqrt(lf[0]), hsqrtr = sqrt(rt[0]);
+    T hsq2 = hsqrtl + hsqrtr;
+    T u = (lf[mu] / hsqrtl + rt[mu] / hsqrtr) / hsq2;
+    T v = (lf[mv] / hsqrtl + rt[mv] / hsqrtr) / hsq2;
+    T a = sqrt(gr * hh);
+    T d1 = rt[0] - lf[0], d2 = rt[mu] - lf[mu], d3 = rt[mv] - lf[mv];
+    T a1 = ((u + a) * d1 - d2) * (h / a);
+    T a2 = -(v * d1) + d3;
+    T a3 = (-((u - a) * d1) + d2) * (h / a);
+    wave[0] = a1;
+    wave[mu] = a1 * (u - a);
+    wave[mv] = a1 * v;
+    s[0] = u - a;
+    wave[3] = T(0);
+    wave[3 + mu] = T(0);
+    wave[3 + mv] = a2;
+    s[1] = u;
+    wave[6] = a3;
+    wave[6 + mu] = a3 * (u + a);
+    wave[6 + mv] = a3 * v;
+    s[2] = u + a;
+    if (!c.efix) {
+        split_by_sign<3, 3>(wave, s, amdq, apdq);
+        return;
+    }
+    do {
+        T him1 = lf[0];
+        T s0 = lf[mu] / him1 - sqrt(gr * him1);
+        if (s0 > T(0) && s[0] > T(0)) {
+            amdq[0] = amdq[1] = amdq[2] = T(0);
+            break;
+        }
+        T h1 = lf[0] + wave[0], hu1 = lf[mu] + wave[mu];
+        T s1 = hu1 / h1 - sqrt(gr * h1);
+        T sfract;
+        if (s0 < T(0) && s1 > T(0))
+            sfract = s0 * ((s1 - s[0]) / (s1 - s0));
+        else if (s[0] < T(0))
+            sfract = s[0];
+        else
+            sfract = T(0);
+#pragma unroll
+        for (int m = 0; m < 3; m++) amdq[m] = sfract * wave[m];
+        if (s[1] > T(0)) break;
+#pragma unroll
+        for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + s[1] * wave[3 + m];
+        T hi = rt[0];
+        T s03 = rt[mu] / hi + sqrt(gr * hi);
+        T h3 = rt[0] - wave[6], hu3 = rt[mu] - wave[6 + mu];
+        T s3 = hu3 / h3 + sqrt(gr * h3);
+        if (s3 < T(0) && s03 > T(0))
+            sfract = s3 * ((s03 - s[2]) / (s03 - s3));
+        else if (s[2] < T(0))
+            sfract = s[2];
+        else
+            break;
+#pragma unroll
+        for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + sfract * wave[6 + m];
+    } while (0);
+    apdq_total<3, 3>(wave, s, amdq, apdq);
+}
+
+template <int P, int IXY, class T>
+__device__ __forceinline__ void riemann(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
+                                        T *s, T *amdq, T *apdq)
+{
+    if constexpr (P == ADV1) rp_adv(c, lf, rt, wave, s, amdq, apdq);
+    else if constexpr (P == AC1) rp_ac(c, lf, rt, wave, s, amdq, apdq);
+    else if constexpr (P == EU1) rp_eu1(c, lf, rt, wave, s, amdq, apdq);
+    else if constexpr (P == SW1) rp_sw1(c, lf, rt, wave, s, amdq, apdq);
+    else if constexpr (P == EU2) rp_eu2<IXY>(c, lf, rt, wave, s, amdq, apdq);
+    else rp_sw2<IXY>(c, lf, rt, wave, s, amdq, apdq);
+}
+
+// --------------------------------------------------------------- evec
+//
+// Left (evl[p][m]) and right (evr[m][p]) eigenvectors at the interface
+// between cells qa = q(i-1) and qb = q(i); stored row-major as
+// evl[p*M + m], evr[m*M + p].
+
+template <int P, int IXY, class T>
+__device__ __forceinline__ void eigvec(const PhysConst<T> &c, const T *qa, const T *qb, T *evl,
+                                       T *evr)
+{
+    constexpr int M = Shape<P>::M;
+    const T h = T(0.5);
+    if constexpr (P == ADV1) {  // apps/advection/1d/evec.f:74-77
+        evl[0] = T(1);
+        evr[0] = T(1);
+    } else if constexpr (P == AC1) {  // apps/acoustics/1d/homogeneous/evec.f:32-42
+        const T zz = c.zz;
+        evr[0 * M + 0] = -zz;
+        evr[0 * M + 1] = zz;
+        evr[1 * M + 0] = T(1);
+        evr[1 * M + 1] = T(1);
+        T z = T(1) / (zz + zz);
+        evl[0 * M + 0] = -z;
+        evl[0 * M + 1] = zz * z;
+        evl[1 * M + 0] = z;
+        evl[1 * M + 1] = zz * z;
+    } else if constexpr (P == EU1) {  // R of evec1eu.f:49-57, L = R^-1
+        const T g1 = c.gamma1;
+        T rl = qa[0], ml = qa[1], el = qa[2], rr = qb[0], mr = qb[1], er = qb[2];
+        T rhsqrtl = sqrt(rl), rhsqrtr = sqrt(rr);
+        T pl = g1 * (el - h * (ml * ml) / rl);
+        T pr = g1 * (er - h * (mr * mr) / rr);
+        T rhsq2 = rhsqrtl + rhsqrtr;
+        T u = (ml / rhsqrtl + mr / rhsqrtr) / rhsq2;
+        T enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+        T a2 = g1 * (enth - h * (u * u));
+        T a = sqrt(a2);
+        evr[0] = T(1); evr[1] = T(1); evr[2] = T(1);
+        evr[3] = u - a; evr[4] = u; evr[5] = u + a;
+        evr[6] = enth - u * a; evr[7] = h * (u * u); evr[8] = enth + u * a;
+        T b = g1, h2 = h / a2, bk = h * b * (u * u);
+        evl[0] = (bk + u * a) * h2; evl[1] = -((b * u + a) * h2); evl[2] = b * h2;
+        evl[3] = T(1) - bk / a2;   evl[4] = b * u / a2;          evl[5] = -(b / a2);
+        evl[6] = (bk - u * a) * h2; evl[7] = -((b * u - a) * h2); evl[8] = b * h2;
+    } else if constexpr (P == EU2) {  // fields (u-c, entropy, shear, u+c)
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        const T g1 = c.gamma1;
+        T rl = qa[0], ul = qa[mu], vl = qa[mv], el = qa[3];
+        T rr = qb[0], ur = qb[mu], vr = qb[mv], er = qb[3];
+        T rhsqrtl = sqrt(rl), rhsqrtr = sqrt(rr);
+        T pl = g1 * (el - h * (ul * ul + vl * vl) / rl);
+        T pr = g1 * (er - h * (ur * ur + vr * vr) / rr);
+        T rhsq2 = rhsqrtl + rhsqrtr;
+        T u = (ul / rhsqrtl + ur / rhsqrtr) / rhsq2;
+        T v = (vl / rhsqrtl + vr / rhsqrtr) / rhsq2;
+        T enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+        T u2v2 = u * u + v * v;
+        T a2 = g1 * (enth - h * u2v2);
+        T a = sqrt(a2);
+        // evr[m*4 + p]
+        evr[0 * 4 + 0] = T(1); evr[mu * 4 + 0] = u - a; evr[mv * 4 + 0] = v; evr[3 * 4 + 0] = enth - u * a;
+        evr[0 * 4 + 1] = T(1); evr[mu * 4 + 1] = u;     evr[mv * 4 + 1] = v; evr[3 * 4 + 1] = h * u2v2;
+        evr[0 * 4 + 2] = T(0); evr[mu * 4 + 2] = T(0);  evr[mv * 4 + 2] = T(1); evr[3 * 4 + 2] = v;
+        evr[0 * 4 + 3] = T(1); evr[mu * 4 + 3] = u + a; evr[mv * 4 + 3] = v; evr[3 * 4 + 3] = enth + u * a;
+        T b = g1, h2 = h / a2, bk = h * b * u2v2;
+        evl[0 * 4 + 0] = (bk + u * a) * h2; evl[0 * 4 + mu] = -((b * u + a) * h2);
+        evl[0 * 4 + mv] = -(b * v * h2);    evl[0 * 4 + 3] = b * h2;
+        evl[1 * 4 + 0] = T(1) - bk / a2;    evl[1 * 4 + mu] = b * u / a2;
+        evl[1 * 4 + mv] = b * v / a2;       evl[1 * 4 + 3] = -(b / a2);
+        evl[2 * 4 + 0] = -v;                evl[2 * 4 + mu] = T(0);
+        evl[2 * 4 + mv] = T(1);             evl[2 * 4 + 3] = T(0);
+        evl[3 * 4 + 0] = (bk - u * a) * h2; evl[3 * 4 + mu] = -((b * u - a) * h2);
+        evl[3 * 4 + mv] = -(b * v * h2);    evl[3 * 4 + 3] = b * h2;
+    } else if constexpr (P == SW1) {  // Roe averages of rp1sw.f:47-49
+        T hl = qa[0], hul = qa[1], hr = qb[0], hur = qb[1];
+        T sl = sqrt(hl), sr = sqrt(hr);
+        T ubar = (hul / sl + hur / sr) / (sl + sr);
+        T cbar = sqrt(h * c.grav * (hl + hr));
+        evr[0] = T(1); evr[1] = T(1);
+        evr[2] = ubar - cbar; evr[3] = ubar + cbar;
+        T z = h / cbar;
+        evl[0] = (ubar + cbar) * z; evl[1] = -z;
+        evl[2] = -((ubar - cbar) * z); evl[3] = z;
+    } else {  // SW2, Roe averages of rpn2sw.f:80-87; fields (u-c, shear, u+c)
+        constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+        T hh = (qa[0] + qb[0]) * h;
+        T hsqrtl = sqrt(qa[0]), hsqrtr = sqrt(qb[0]);
+        T hsq2 = hsqrtl + hsqrtr;
+        T u = (qa[mu] / hsqrtl + qb[mu] / hsqrtr) / hsq2;
+        T v = (qa[mv] / hsqrtl + qb[mv] / hsqrtr) / hsq2;
+        T a = sqrt(c.grav * hh);
+        evr[0 * 3 + 0] = T(1); evr[mu * 3 + 0] = u - a; evr[mv * 3 + 0] = v;
+        evr[0 * 3 + 1] = T(0); evr[mu * 3 + 1] = T(0);  evr[mv * 3 + 1] = T(1);
+        evr[0 * 3 + 2] = T(1); evr[mu * 3 + 2] = u + a; evr[mv * 3 + 2] = v;
+        T z = h / a;
+        evl[0 * 3 + 0] = (u + a) * z; evl[0 * 3 + mu] = -z; evl[0 * 3 + mv] = T(0);
+        evl[1 * 3 + 0] = -v;          evl[1 * 3 + mu] = T(0); evl[1 * 3 + mv] = T(1);
+        evl[2 * 3 + 0] = -((u - a) * z); evl[2 * 3 + mu] = z; evl[2 * 3 + mv] = T(0);
+    }
+}
+
+}  // namespace sc

@@ -1,0 +1,136 @@
+// WENO5 reconstruction on the device (src/reconstruct.f90:77-241).
+//
+// For an interface i-1/2 the stencil is the six cells v[k] = q(i-3+k),
+// k = 0..5, with jumps d[k] = v[k+1] - v[k] (d[k] = dq(i-2+k) of the
+// reference).  weno5 / weno5_char produce both sides of that interface:
+//   lf = qr(i-1)  (m1 = 1 in the reference, jumps d0..d3)
+//   rt = ql(i)    (m1 = 2, jumps d4..d1 mirrored)
+// The characteristic version projects the jumps with the left eigenvectors
+// at the same interface, reconstructs each field and adds R * correction to
+// the 4th-order central value (reconstruct.f90:171-237).
+#pragma once
+#include "sc_physics.h"
+
+namespace sc {
+
+// epweno = 1.e-36 is a default-real literal (reconstruct.f90:6)
+#define SC_EPWENO ((double)1.0e-36f)
+
+// One side of reconstruct.f90:113-132 in float64: the nonlinear correction
+// (without the central part) from the jumps at offsets intwo, inone, 0, ione.
+__device__ __forceinline__ double weno_side(double im, double a2, double a1, double a0, double am1)
+{
+    double t1 = im * (a2 - a1);
+    double t2 = im * (a1 - a0);
+    double t3 = im * (a0 - am1);
+    double tt1 = 13.0 * (t1 * t1) + 3.0 * dsq(a2 - 3.0 * a1);
+    double tt2 = 13.0 * (t2 * t2) + 3.0 * dsq(a1 + a0);
+    double tt3 = 13.0 * (t3 * t3) + 3.0 * dsq(3.0 * a0 - am1);
+    tt1 = dsq(SC_EPWENO + tt1);
+    tt2 = dsq(SC_EPWENO + tt2);
+    tt3 = dsq(SC_EPWENO + tt3);
+    double s1 = tt2 * tt3;
+    double s2 = 6.0 * tt1 * tt3;
+    double s3 = 3.0 * tt1 * tt2;
+    double t0 = 1.0 / (s1 + s2 + s3);
+    s1 = s1 * t0;
+    s3 = s3 * t0;
+    return (s1 * (t2 - t1) + (0.5 * s3 - 0.25) * (t3 - t2)) * (1.0 / 3.0);
+}
+
+// The same correction in float32.  The float64 form squares the smoothness
+// indicators and multiplies them pairwise, which underflows float32 for
+// small jumps (tt ~ jump^4, s ~ jump^8) and gives 0/0 in flat regions.  The
+// weights are homogeneous of degree 0 in (jumps, sqrt(eps)) and the
+// correction of degree 1, so the jumps are scaled by 1/M (M = max |jump|),
+// eps by 1/M^2, the weights formed as alpha_k = gamma_k / (eps + beta_k)^2
+// (w1 = s1/sum, w3 = s3/sum of the reference), and the correction scaled
+// back by M.  eps/M^2 is clamped to [1e-15, 1e15]; below the clamp the
+// reference weights differ only where the correction itself is below
+// float32 resolution.
+__device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, float a0, float am1)
+{
+    float M = fmaxf(fmaxf(fabsf(a2), fabsf(a1)), fmaxf(fabsf(a0), fabsf(am1)));
+    if (!(M > 0.0f)) return 0.0f;
+    float r = 1.0f / M;
+    a2 *= r;
+    a1 *= r;
+    a0 *= r;
+    am1 *= r;
+    float eps = fminf(fmaxf(1.0e-36f * r * r, 1.0e-15f), 1.0e15f);
+    float t1 = im * (a2 - a1);
+    float t2 = im * (a1 - a0);
+    float t3 = im * (a0 - am1);
+    float b1 = 13.0f * (t1 * t1) + 3.0f * dsq(a2 - 3.0f * a1);
+    float b2 = 13.0f * (t2 * t2) + 3.0f * dsq(a1 + a0);
+    float b3 = 13.0f * (t3 * t3) + 3.0f * dsq(3.0f * a0 - am1);
+    float al1 = 1.0f / dsq(eps + b1);
+    float al2 = 6.0f / dsq(eps + b2);
+    float al3 = 3.0f / dsq(eps + b3);
+    float rs = 1.0f / (al1 + al2 + al3);
+    float w1 = al1 * rs, w3 = al3 * rs;
+    return ((w1 * (t2 - t1) + (0.5f * w3 - 0.25f) * (t3 - t2)) * (1.0f / 3.0f)) * M;
+}
+
+template <class T> __device__ __forceinline__ T weno_corr(T im, T a2, T a1, T a0, T am1);
+template <> __device__ __forceinline__ double weno_corr<double>(double im, double a2, double a1,
+                                                                double a0, double am1)
+{
+    return weno_side(im, a2, a1, a0, am1);
+}
+template <> __device__ __forceinline__ float weno_corr<float>(float im, float a2, float a1, float a0,
+                                                              float am1)
+{
+    return weno_side_f32(im, a2, a1, a0, am1);
+}
+
+// Both sides of interface i-1/2 from the six cells v[k][m] = q(i-3+k, m).
+template <int P, int IXY, int CHAR, class T>
+__device__ __forceinline__ void reconstruct_iface(const PhysConst<T> &c, const T (&v)[6][Shape<P>::M],
+                                                  T (&lf)[Shape<P>::M], T (&rt)[Shape<P>::M])
+{
+    constexpr int M = Shape<P>::M;
+    T d[5][M];
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+        for (int m = 0; m < M; m++) d[k][m] = v[k + 1][m] - v[k][m];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        T central = (-v[1][m] + T(7) * (v[2][m] + v[3][m]) - v[4][m]) * (T(1) / T(12));
+        lf[m] = central;
+        rt[m] = central;
+    }
+    if constexpr (CHAR == 0 || P == ADV1) {
+        // component-wise (reconstruct.f90:94-140); with the advection
+        // eigenvectors L = R = 1 the characteristic form is the same
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            lf[m] = weno_corr<T>(T(1), d[0][m], d[1][m], d[2][m], d[3][m]) + lf[m];
+            rt[m] = weno_corr<T>(T(-1), d[4][m], d[3][m], d[2][m], d[1][m]) + rt[m];
+        }
+    } else {
+        T evl[M * M], evr[M * M];
+        eigvec<P, IXY>(c, v[2], v[3], evl, evr);
+#pragma unroll
+        for (int p = 0; p < M; p++) {
+            T hh[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                T acc = T(0);
+#pragma unroll
+                for (int m = 0; m < M; m++) acc = acc + evl[p * M + m] * d[k][m];
+                hh[k] = acc;
+            }
+            T u1 = weno_corr<T>(T(1), hh[0], hh[1], hh[2], hh[3]);
+            T u2 = weno_corr<T>(T(-1), hh[4], hh[3], hh[2], hh[1]);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                lf[m] = lf[m] + evr[m * M + p] * u1;
+                rt[m] = rt[m] + evr[m * M + p] * u2;
+            }
+        }
+    }
+}
+
+}  // namespace sc

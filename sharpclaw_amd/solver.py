@@ -1,0 +1,147 @@
+"""Host-side mirror of the reference's hot-path interface over the C ABI.
+
+  Solver            the device-resident q/qold/RK registers of module Global
+                    (src/1d/Global1.f90) with step(), ts(), advance() --
+                    step / ts_ssp* / the sharpclaw() loop of src/sharpclaw.f90
+  rp, evec, weno5, weno5_char, flux1
+                    single routines with the reference's argument meaning
+                    (rp1eu.f:5-6, flux1.f90:105, reconstruct.f90:77,146,
+                    flux1.f90:2), on host arrays in the reference layout
+
+Arrays are numpy float64 in Fortran order: q(1-mbc:nx+mbc[,1-mbc:ny+mbc], meqn).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._abi import ScParams, ScRunCtl, ScRunStats, grid_shape
+from ._lib import check, lib
+
+
+def _f(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+class Solver:
+    """One grid on one device (sc_create/sc_destroy)."""
+
+    def __init__(self, params: ScParams):
+        self.p = params
+        h = C.c_void_p()
+        check(lib().sc_create(C.byref(params), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return grid_shape(self.p)
+
+    def set_q(self, q):
+        q = _f(q)
+        if q.shape != self.shape:
+            raise ValueError(f"q has shape {q.shape}, expected {self.shape}")
+        check(lib().sc_set_q(self._h, q))
+
+    def get_q(self):
+        q = np.zeros(self.shape, order="F")
+        check(lib().sc_get_q(self._h, q))
+        return q
+
+    def save_qold(self):
+        check(lib().sc_save_qold(self._h))
+
+    def restore_qold(self):
+        check(lib().sc_restore_qold(self._h))
+
+    def step(self, dt, t=0.0, want_dq=True):
+        """step(): bc on q, dq = dt*L(q); returns (dq or None, cfl)."""
+        cfl = C.c_double()
+        if want_dq:
+            dq = np.zeros(self.shape, order="F")
+            check(lib().sc_step(self._h, dt, t, dq.ctypes.data_as(C.c_void_p), C.byref(cfl)))
+            return dq, cfl.value
+        check(lib().sc_step(self._h, dt, t, None, C.byref(cfl)))
+        return None, cfl.value
+
+    def ts(self, dt, told=0.0):
+        """One fused SSP time step from q (call save_qold first, as
+        sharpclaw.f90:134 does); returns the last stage's CFL number."""
+        cfl = C.c_double()
+        check(lib().sc_ts(self._h, dt, told, C.byref(cfl)))
+        return cfl.value
+
+    def advance(self, ctl: ScRunCtl, tstart, tend, dt):
+        """sharpclaw(tstart, tend, ...): returns (dt_last, stats)."""
+        d = C.c_double(dt)
+        st = ScRunStats()
+        check(lib().sc_advance(self._h, C.byref(ctl), tstart, tend, C.byref(d), C.byref(st)))
+        return d.value, st
+
+    def sums(self):
+        out = np.zeros(self.p.meqn, order="F")
+        check(lib().sc_sums(self._h, out))
+        return out
+
+    def time_stage(self, n, dt):
+        ms = C.c_double()
+        check(lib().sc_time_stage(self._h, n, dt, C.byref(ms)))
+        return ms.value
+
+
+def rp(p, ixy, mx, ql, qr):
+    """rp(ixy,...,ql,qr,...) -> (wave, s, amdq, apdq), reference layout."""
+    L = mx + 2 * p.mbc
+    wave = np.zeros((L, p.meqn, p.mwaves), order="F")
+    s = np.zeros((L, p.mwaves), order="F")
+    amdq = np.zeros((L, p.meqn), order="F")
+    apdq = np.zeros((L, p.meqn), order="F")
+    check(lib().sc_rp(C.byref(p), ixy, mx, _f(ql), _f(qr), wave, s, amdq, apdq))
+    return wave, s, amdq, apdq
+
+
+def evec(p, ixy, mx, q):
+    L = mx + 2 * p.mbc
+    evl = np.zeros((L, p.meqn, p.meqn), order="F")
+    evr = np.zeros((L, p.meqn, p.meqn), order="F")
+    check(lib().sc_evec(C.byref(p), ixy, mx, _f(q), evl, evr))
+    return evl, evr
+
+
+def weno5(p, q):
+    q = _f(q)
+    ql = np.zeros_like(q, order="F")
+    qr = np.zeros_like(q, order="F")
+    check(lib().sc_weno5(C.byref(p), q.shape[0], q, ql, qr))
+    return ql, qr
+
+
+def weno5_char(p, q, evl, evr):
+    q = _f(q)
+    ql = np.zeros_like(q, order="F")
+    qr = np.zeros_like(q, order="F")
+    check(lib().sc_weno5_char(C.byref(p), q.shape[0], q, _f(evl), _f(evr), ql, qr))
+    return ql, qr
+
+
+def flux1(p, ixy, mx, q1d, dt):
+    q1d = _f(q1d)
+    dq = np.zeros_like(q1d, order="F")
+    cfl = C.c_double()
+    check(lib().sc_flux1(C.byref(p), ixy, mx, q1d, dq, dt, C.byref(cfl)))
+    return dq, cfl.value

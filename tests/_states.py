@@ -5,14 +5,17 @@ from sharpclaw_amd._abi import (SC_PHYS_ACOUSTICS, SC_PHYS_ADVECTION, SC_PHYS_EU
                                 SC_PHYS_SHALLOW_WATER)
 
 
-def random_cells(p, shape, rng, smooth=False):
+def random_cells(p, shape, rng, smooth=False, amp=1.0):
     """Conserved variables of physical states, shape + (meqn,), Fortran order.
 
     smooth=True superposes a few sine modes (smooth WENO regime); otherwise
-    cell values are independent (every stencil non-smooth)."""
+    cell values are independent (every stencil non-smooth).  amp < 1 shrinks
+    every range about its midpoint (gentle data for very small grids)."""
     shape = tuple(shape)
 
     def field(lo, hi):
+        mid = 0.5 * (lo + hi)
+        lo, hi = mid - amp * (mid - lo), mid + amp * (hi - mid)
         if not smooth:
             return rng.uniform(lo, hi, size=shape)
         grids = np.meshgrid(*[np.linspace(0, 1, n) for n in shape], indexing="ij")
@@ -55,7 +58,7 @@ def random_cells(p, shape, rng, smooth=False):
     return q
 
 
-def grid(p, rng, smooth=False):
+def grid(p, rng, smooth=False, amp=1.0):
     if p.ndim == 1:
-        return random_cells(p, (p.nx[0] + 2 * p.mbc,), rng, smooth)
-    return random_cells(p, (p.nx[0] + 2 * p.mbc, p.nx[1] + 2 * p.mbc), rng, smooth)
+        return random_cells(p, (p.nx[0] + 2 * p.mbc,), rng, smooth, amp)
+    return random_cells(p, (p.nx[0] + 2 * p.mbc, p.nx[1] + 2 * p.mbc), rng, smooth, amp)

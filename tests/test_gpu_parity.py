@@ -1,0 +1,345 @@
+"""Parity of the HIP path (through the C ABI) with the oracle.
+
+Tolerances (normwise relative L-infinity, max|gpu - oracle| / max|oracle|):
+  float64 single routines and single steps   1e-12
+  float64 whole runs (hundreds of steps)     max(1e-10, 8 x the oracle's own change
+                                             when its initial data moves by 1 ulp)
+  float32 option: relative L1 error  sum|gpu - oracle| / sum|oracle|  <= 1e-5
+"""
+import numpy as np
+import pytest
+
+import _states
+from conftest import rel_linf
+from oracle import oracle as O
+from sharpclaw_amd import SC_FP32, ScRunCtl, Solver, interior, make_params
+from sharpclaw_amd import solver as S
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL_RUN = 1e-10
+
+
+def run_tolerance(p, ctl, q0, tstart, tend, dt, rq):
+    """Whole-run tolerance: WENO weights near discontinuities amplify
+    rounding differences, so a run's own conditioning is measured by
+    re-running the oracle from q0 perturbed by one ulp; the GPU may differ
+    from the oracle by 8x that (and always by 1e-10)."""
+    qp = q0.copy(order="F")
+    interior(p, qp)[:] = np.nextafter(interior(p, qp), np.inf)
+    rp_, _, _ = O.sharpclaw(p, ctl, qp, tstart, tend, dt)
+    return max(TOL_RUN, 8.0 * rel_linf(interior(p, rp_), interior(p, rq)))
+
+PHYS = [("advection", 1), ("acoustics", 1), ("euler", 1), ("shallow_water", 1), ("euler", 2),
+        ("shallow_water", 2)]
+
+
+def _p(name, ndim, n=40, **kw):
+    nx = (n,) if ndim == 1 else (n, n)
+    return make_params(name, nx, **kw)
+
+
+def rel_l1(a, b):
+    return np.sum(np.abs(a - b)) / np.sum(np.abs(b))
+
+
+@pytest.mark.parametrize("name,ndim", PHYS)
+def test_rp(name, ndim):
+    rng = np.random.default_rng(11)
+    p = _p(name, ndim)
+    mx = 200
+    L = mx + 6
+    for ixy in ([1] if ndim == 1 else [1, 2]):
+        ql = _states.random_cells(p, (L,), rng)
+        qr = _states.random_cells(p, (L,), rng)
+        got = S.rp(p, ixy, mx, ql, qr)
+        ref = O.rp(p, ixy, mx, ql, qr)
+        for g, r, what in zip(got, ref, ("wave", "s", "amdq", "apdq")):
+            assert rel_linf(g[1:], r[1:]) <= TOL, (name, ixy, what)
+
+
+@pytest.mark.parametrize("name,ndim", PHYS)
+def test_evec(name, ndim):
+    rng = np.random.default_rng(12)
+    p = _p(name, ndim, char_decomp=2)
+    mx = 150
+    for ixy in ([1] if ndim == 1 else [1, 2]):
+        q = _states.random_cells(p, (mx + 6,), rng)
+        gl, gr = S.evec(p, ixy, mx, q)
+        rl, rr = O.evec(p, ixy, mx, q)
+        assert rel_linf(gl[1:], rl[1:]) <= TOL
+        assert rel_linf(gr[1:], rr[1:]) <= TOL
+
+
+@pytest.mark.parametrize("name,ndim", PHYS)
+@pytest.mark.parametrize("char", [0, 2])
+def test_weno(name, ndim, char):
+    rng = np.random.default_rng(13)
+    p = _p(name, ndim)
+    mx = 300
+    L = mx + 6
+    for smooth in (True, False):
+        q = _states.random_cells(p, (L,), rng, smooth=smooth)
+        if char == 2:
+            evl, evr = O.evec(p, 1, mx, q)
+            gl, gr = S.weno5_char(p, q, evl, evr)
+            rl, rr = O.weno5_char(q, evl, evr)
+        else:
+            gl, gr = S.weno5(p, q)
+            rl, rr = O.weno5(q)
+        # defined range: qr(I-1), ql(I) for I = 4..mx2-2
+        assert rel_linf(gr[2:L - 3], rr[2:L - 3]) <= TOL
+        assert rel_linf(gl[3:L - 2], rl[3:L - 2]) <= TOL
+
+
+@pytest.mark.parametrize("name,ndim", PHYS)
+@pytest.mark.parametrize("char", [0, 2])
+def test_flux1(name, ndim, char):
+    rng = np.random.default_rng(14)
+    p = _p(name, ndim, char_decomp=char)
+    for mx in (7, 254, 255, 256, 600):
+        for ixy in ([1] if ndim == 1 else [1, 2]):
+            q1d = _states.random_cells(p, (mx + 6,), rng, smooth=True, amp=0.3 if mx < 32 else 1.0)
+            dt = 0.2 * p.dx[ixy - 1]
+            g, gc = S.flux1(p, ixy, mx, q1d, dt)
+            r, rc = O.flux1(p, ixy, mx, q1d, dt)
+            assert rel_linf(g[3:3 + mx], r[3:3 + mx]) <= TOL, (mx, ixy)
+            assert abs(gc - rc) <= TOL * rc
+
+
+BCS1 = [(1, 1, 1, 1), (2, 2, 1, 1), (3, 3, 1, 1), (1, 3, 1, 1)]
+BCS2 = [(1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3), (1, 3, 2, 2), (2, 2, 3, 1)]
+
+
+def _bcs(name, ndim):
+    bcs = BCS1 if ndim == 1 else BCS2
+    if name == "advection":
+        bcs = [b for b in bcs if 3 not in b[:2]]
+    return bcs
+
+
+@pytest.mark.parametrize("name,ndim", PHYS)
+@pytest.mark.parametrize("char", [0, 2])
+def test_step(name, ndim, char):
+    rng = np.random.default_rng(15)
+    for bc in _bcs(name, ndim):
+        for n in ((37,) if ndim == 2 else (300,)):
+            p = _p(name, ndim, n=n, char_decomp=char, mthbc=bc)
+            q = _states.grid(p, rng, smooth=True)
+            dt = 0.1 * p.dx[0]
+            with Solver(p) as s:
+                s.set_q(q)
+                gdq, gc = s.step(dt)
+                gq = s.get_q()
+            rq, rdq, rc = O.step(p, q, dt)
+            assert np.array_equal(gq, rq), ("ghost fill", bc)
+            assert rel_linf(interior(p, gdq), interior(p, rdq)) <= TOL, bc
+            assert abs(gc - rc) <= TOL * rc, bc
+
+
+@pytest.mark.parametrize("name,ndim", PHYS)
+@pytest.mark.parametrize("ti", [1, 2, 3, 4])
+def test_ts(name, ndim, ti):
+    rng = np.random.default_rng(16)
+    for char in (0, 2):
+        p = _p(name, ndim, n=(45 if ndim == 2 else 500), char_decomp=char, time_integrator=ti,
+               mthbc=(2, 2, 2, 2) if name != "euler" else (1, 1, 1, 1))
+        q = _states.grid(p, rng, smooth=True)
+        dt = 0.05 * p.dx[0]
+        with Solver(p) as s:
+            s.set_q(q)
+            s.save_qold()
+            gc = s.ts(dt)
+            gq = s.get_q()
+        rq, rc, _ = O.ts(p, q, dt)
+        assert rel_linf(interior(p, gq), interior(p, rq)) <= TOL, char
+        assert abs(gc - rc) <= TOL * rc
+
+
+def _c1_params():
+    return make_params("advection", (500,), char_decomp=0, time_integrator=3, mthbc=(2, 2, 2, 2))
+
+
+def _c1_q0(p):
+    x = p.xlower[0] + (np.arange(1, p.nx[0] + 1) - 0.5) * p.dx[0]
+    q = O.new_grid(p)
+    interior(p, q)[:, 0] = np.exp(-200.0 * (x - 0.3) ** 2) + ((x >= 0.6) & (x <= 0.8))
+    return q
+
+
+def test_advance_c1_full_run():
+    """1D advection, component-wise WENO5 + SSPRK3, CFL 0.5, to t=1 (1000 steps)."""
+    p = _c1_params()
+    q0 = _c1_q0(p)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.55, cfl_desired=0.5, max_steps=50000, dt_variable=1)
+    rq, rdt, rst = O.sharpclaw(p, ctl, q0, 0.0, 1.0, 0.001)
+    with Solver(p) as s:
+        s.set_q(q0)
+        gdt, gst = s.advance(ctl, 0.0, 1.0, 0.001)
+        gq = s.get_q()
+    assert gst.steps_taken == rst.steps_taken == 1000
+    assert gst.t == rst.t == 1.0
+    tol = run_tolerance(p, ctl, q0, 0.0, 1.0, 0.001, rq)
+    err = rel_linf(interior(p, gq), interior(p, rq))
+    print(f"C1 whole run: rel Linf {err:.3e}, tolerance {tol:.3e}")
+    assert err <= tol
+    assert abs(gdt - rdt) <= 1e-12 * rdt
+
+
+def _shu_osher(p, fp32=False):
+    g1 = p.gamma - 1.0
+    x = p.xlower[0] + (np.arange(1, p.nx[0] + 1) - 0.5) * p.dx[0]
+    q = O.new_grid(p)
+    qi = interior(p, q)
+    left = x < -4.0
+    rho = np.where(left, 3.857143, 1.0 + 0.2 * np.sin(5.0 * x))
+    u = np.where(left, 2.629369, 0.0)
+    pr = np.where(left, 10.333333, 1.0)
+    qi[:, 0] = rho
+    qi[:, 1] = rho * u
+    qi[:, 2] = pr / g1 + 0.5 * rho * u * u
+    return q
+
+
+def test_advance_c2_euler_char_ssp104():
+    """1D Euler Shu-Osher, characteristic WENO5 + Roe, SSP(10,4) (to t=0.36)."""
+    p = make_params("euler", (1600,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                    time_integrator=4, mthbc=(1, 1, 1, 1))
+    q0 = _shu_osher(p)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=2.5, cfl_desired=2.45, max_steps=50000, dt_variable=1)
+    rq, rdt, rst = O.sharpclaw(p, ctl, q0, 0.0, 0.36, 0.001)
+    with Solver(p) as s:
+        s.set_q(q0)
+        gdt, gst = s.advance(ctl, 0.0, 0.36, 0.001)
+        gq = s.get_q()
+    assert (gst.steps_taken, gst.rejected) == (rst.steps_taken, rst.rejected)
+    tol = run_tolerance(p, ctl, q0, 0.0, 0.36, 0.001, rq)
+    assert rel_linf(interior(p, gq), interior(p, rq)) <= tol
+
+
+def test_per_step_along_c2_trajectory():
+    """From oracle states along the Shu-Osher run, one GPU step vs one oracle
+    step (identical inputs): the 1e-12 per-step bar on states with shocks."""
+    p = make_params("euler", (1600,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                    time_integrator=4, mthbc=(1, 1, 1, 1))
+    q = _shu_osher(p)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=2.5, cfl_desired=2.45, max_steps=50000, dt_variable=1)
+    dt = 0.001
+    t = 0.0
+    with Solver(p) as s:
+        for k in range(6):
+            q, dt, _ = O.sharpclaw(p, ctl, q, t, t + 0.05, dt)
+            t += 0.05
+            s.set_q(q)
+            s.save_qold()
+            gc = s.ts(dt)
+            gq = s.get_q()
+            rq, rc, _ = O.ts(p, q, dt)
+            assert rel_linf(interior(p, gq), interior(p, rq)) <= TOL, k
+            assert abs(gc - rc) <= TOL * rc
+
+
+def _quadrants(p, seed=7):
+    rng = np.random.default_rng(seed)
+    st = np.stack([rng.uniform(0.1, 1.5, 4), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4),
+                   rng.uniform(0.03, 1.5, 4)], axis=1)  # NE, NW, SW, SE
+    g1 = p.gamma - 1.0
+    x = p.xlower[0] + (np.arange(1, p.nx[0] + 1) - 0.5) * p.dx[0]
+    y = p.xlower[1] + (np.arange(1, p.nx[1] + 1) - 0.5) * p.dx[1]
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    k = np.where(X > 0.5, np.where(Y > 0.5, 0, 3), np.where(Y > 0.5, 1, 2))
+    rho, u, v, pr = (st[k, c] for c in range(4))
+    q = O.new_grid(p)
+    qi = interior(p, q)
+    qi[..., 0] = rho
+    qi[..., 1] = rho * u
+    qi[..., 2] = rho * v
+    qi[..., 3] = pr / g1 + 0.5 * rho * (u * u + v * v)
+    return q
+
+
+def test_advance_c3_reduced():
+    """2D Euler four-quadrant problem, char WENO5 + Roe rpn2, SSPRK3, 64x64, 25 steps."""
+    p = make_params("euler", (64, 64), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    q0 = _quadrants(p)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=25, dt_variable=1)
+    rq, rdt, rst = O.sharpclaw(p, ctl, q0, 0.0, 10.0, 1e-4)
+    with Solver(p) as s:
+        s.set_q(q0)
+        gdt, gst = s.advance(ctl, 0.0, 10.0, 1e-4)
+        gq = s.get_q()
+    assert (gst.steps_taken, gst.rejected) == (rst.steps_taken, rst.rejected)
+    tol = run_tolerance(p, ctl, q0, 0.0, 10.0, 1e-4, rq)
+    assert rel_linf(interior(p, gq), interior(p, rq)) <= tol
+
+
+def test_fp32_option_l1():
+    """float32 reconstruction + Riemann, float64 accumulation: L1 vs the FP64 oracle."""
+    p = make_params("euler", (1600,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                    time_integrator=4, mthbc=(1, 1, 1, 1))
+    q0 = _shu_osher(p)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=2.5, cfl_desired=2.45, max_steps=50000, dt_variable=1)
+    rq, _, _ = O.sharpclaw(p, ctl, q0, 0.0, 0.36, 0.001)
+    p32 = make_params("euler", (1600,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                      time_integrator=4, mthbc=(1, 1, 1, 1), precision=SC_FP32)
+    with Solver(p32) as s:
+        s.set_q(q0)
+        s.advance(ctl, 0.0, 0.36, 0.001)
+        gq = s.get_q()
+    err = rel_l1(interior(p, gq), interior(p, rq))
+    print("C2 float32 relative L1:", err)
+    assert err <= 1e-5
+
+    p2 = make_params("euler", (64, 64), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    q0 = _quadrants(p2)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=20, dt_variable=1)
+    rq, _, _ = O.sharpclaw(p2, ctl, q0, 0.0, 10.0, 1e-4)
+    p2.precision = SC_FP32
+    with Solver(p2) as s:
+        s.set_q(q0)
+        s.advance(ctl, 0.0, 10.0, 1e-4)
+        gq = s.get_q()
+    err = rel_l1(interior(p2, gq), interior(p2, rq))
+    print("2D float32 relative L1:", err)
+    assert err <= 1e-5
+
+
+def test_conservation_sums():
+    """Periodic problems conserve sum q dx to rounding; sums match the oracle."""
+    rng = np.random.default_rng(17)
+    for name, ndim in (("acoustics", 1), ("shallow_water", 2)):
+        p = _p(name, ndim, n=(96 if ndim == 2 else 512), char_decomp=2, mthbc=(2, 2, 2, 2))
+        q = _states.grid(p, rng, smooth=True)
+        ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=40, dt_variable=1)
+        with Solver(p) as s:
+            s.set_q(q)
+            s0 = s.sums()
+            assert np.allclose(s0, O.sums(p, q), rtol=1e-14, atol=1e-14)
+            s.advance(ctl, 0.0, 10.0, 1e-4)
+            s1 = s.sums()
+        assert np.allclose(s1, s0, rtol=1e-12, atol=1e-12 * np.max(np.abs(s0))), (name, s0, s1)
+
+
+def test_non_square_and_partial_tiles():
+    rng = np.random.default_rng(18)
+    for nx, ny in ((17, 40), (50, 3), (33, 31)):
+        p = make_params("shallow_water", (nx, ny), char_decomp=0, mthbc=(1, 3, 2, 2))
+        q = _states.grid(p, rng, smooth=True, amp=0.3)
+        dt = 0.1 * min(p.dx[0], p.dx[1])
+        with Solver(p) as s:
+            s.set_q(q)
+            gdq, gc = s.step(dt)
+        _, rdq, rc = O.step(p, q, dt)
+        assert rel_linf(interior(p, gdq), interior(p, rdq)) <= TOL, (nx, ny)
+        assert abs(gc - rc) <= TOL * rc
+
+
+def test_errors_are_reported():
+    from sharpclaw_amd import ScError
+    p = make_params("advection", (10,), mthbc=(3, 3, 1, 1))
+    with pytest.raises(ScError):
+        Solver(p)
+    p = make_params("euler", (10,), mthbc=(2, 1, 1, 1))
+    with pytest.raises(ScError):
+        Solver(p)
