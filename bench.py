@@ -147,6 +147,10 @@ def main():
         log(f"bench: fused stage kernel {ms_launch * 1e3:.1f} us per launch")
         sums = s.sums()
 
+    from sharpclaw_amd import copy_bench
+    copy_bytes = 1 << 30
+    copy_gbs = copy_bench(copy_bytes, 10)
+    log(f"bench: copy kernel {copy_gbs:.0f} GB/s ({copy_bytes} bytes x 10)")
     meqn = 4
     bytes_per_cell_stage = meqn * 8 * (2 + 3 + 3) / 3.0  # SSPRK3: state reads+writes per stage
     alg_bytes = bytes_per_cell_stage * cells
@@ -176,6 +180,8 @@ def main():
                      "alg_bytes_per_launch": alg_bytes,
                      "us_per_launch": ms_launch * 1e3},
         "gpu_ms_timed": rst.gpu_ms,
+        "copy_kernel_gbs": copy_gbs,
+        "frac_of_copy_bw": achieved / copy_gbs,
         "pct_hbm_peak": 100.0 * achieved / HBM_PEAK_GBS,
         "conservation_sums": [float(x) for x in sums],
     }

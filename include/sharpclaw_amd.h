@@ -17,6 +17,8 @@
 #ifndef SHARPCLAW_AMD_H
 #define SHARPCLAW_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -146,6 +148,11 @@ int sc_sums(sc_solver *s, double *sums);
  * no CFL readback), in ms per stage launch, measured with HIP events on the
  * solver's stream.  Leaves q unchanged. */
 int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stage);
+
+/* Achievable HBM bandwidth: reps device-to-device copies of nbytes by a plain
+ * copy kernel on `device`, timed with HIP events; *gbps = 2*nbytes*reps/time
+ * (bytes read + written).  Calibrates the roofline and the HBM counters. */
+int sc_copy_bench(int device, size_t nbytes, int reps, double *gbps);
 
 /* ---- single routines on host arrays (each copies in, runs, copies out) */
 

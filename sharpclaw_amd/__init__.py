@@ -6,4 +6,4 @@ from ._abi import (PHYSICS, SC_FP32, SC_FP64, SC_PHYS_ACOUSTICS, SC_PHYS_ADVECTI
                    SC_PHYS_EULER, SC_PHYS_SHALLOW_WATER, ScParams, ScRunCtl, ScRunStats,
                    grid_shape, interior, make_params)
 from ._lib import LIB_PATH, ScError, lib  # noqa: F401
-from .solver import Solver, evec, flux1, rp, weno5, weno5_char  # noqa: F401
+from .solver import Solver, copy_bench, evec, flux1, rp, weno5, weno5_char  # noqa: F401

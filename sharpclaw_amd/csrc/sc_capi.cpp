@@ -567,6 +567,40 @@ extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stag
     return SC_OK;
 }
 
+extern "C" int sc_copy_bench(int device, size_t nbytes, int reps, double *gbps)
+{
+    if (!gbps || reps < 1 || nbytes < 16) return fail(SC_ERR_ARG, "bad argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SC_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(SC_ERR_ARG, "device out of range");
+    DevGuard g(device);
+    size_t n = nbytes / sizeof(double);
+    double *a = nullptr, *b = nullptr;
+    hipEvent_t e0, e1;
+    HIPCHK(hipMalloc(&a, n * sizeof(double)));
+    if (hipMalloc(&b, n * sizeof(double)) != hipSuccess) {
+        hipFree(a);
+        return fail(SC_ERR_NOMEM, "copy bench allocation failed");
+    }
+    hipError_t e = hipMemset(a, 0, n * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_copy(b, a, n, 0);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, 0);
+    for (int k = 0; k < reps && e == hipSuccess; k++) e = launch_copy(b, a, n, 0);
+    if (e == hipSuccess) e = hipEventRecord(e1, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return fail(SC_ERR_HIP, "copy bench: %s", hipGetErrorString(e));
+    *gbps = 2.0 * (double)(n * sizeof(double)) * reps / (ms * 1e-3) / 1e9;
+    return SC_OK;
+}
+
 // ------------------------------------------------------ single routines
 
 namespace {

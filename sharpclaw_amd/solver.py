@@ -145,3 +145,10 @@ def flux1(p, ixy, mx, q1d, dt):
     cfl = C.c_double()
     check(lib().sc_flux1(C.byref(p), ixy, mx, q1d, dq, dt, C.byref(cfl)))
     return dq, cfl.value
+
+
+def copy_bench(nbytes=1 << 30, reps=10, device=0):
+    """Achievable HBM bandwidth (GB/s) of a plain device copy kernel."""
+    g = C.c_double()
+    check(lib().sc_copy_bench(device, nbytes, reps, C.byref(g)))
+    return g.value
