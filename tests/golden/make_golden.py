@@ -1,0 +1,141 @@
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+The reference SharpClaw (src/ compiled unmodified with flang -O3 by
+oracle/ref/Makefile, plus the project-written app hooks in
+oracle/ref/hooks/) is run through oracle/_ref/harness_* (single routines)
+and oracle/_ref/xsclaw_* (the reference program src/main.f90) on seeded
+inputs.  Each fixture stores the inputs and the reference's outputs.
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import _states  # noqa: E402
+from oracle import refrun as R  # noqa: E402
+from sharpclaw_amd import interior, make_params  # noqa: E402
+
+PHYS = [("advection", 1), ("acoustics", 1), ("euler", 1), ("shallow_water", 1), ("euler", 2),
+        ("shallow_water", 2)]
+
+
+def tag(name, ndim):
+    return f"{name}{ndim}d"
+
+
+def save(fname, **arrays):
+    np.savez_compressed(os.path.join(HERE, fname), **arrays)
+    print("wrote", fname, {k: getattr(v, "shape", v) for k, v in arrays.items()})
+
+
+def routines():
+    rng = np.random.default_rng(20260101)
+    for name, ndim in PHYS:
+        p = make_params(name, (24,) if ndim == 1 else (24, 24), char_decomp=2)
+        mx, L = 24, 30
+        out = {}
+        for ixy in ([1] if ndim == 1 else [1, 2]):
+            ql = _states.random_cells(p, (L,), rng)
+            qr = _states.random_cells(p, (L,), rng)
+            w, s, am, ap = R.rp(p, ixy, mx, ql, qr)
+            out.update({f"rp{ixy}_ql": ql, f"rp{ixy}_qr": qr, f"rp{ixy}_wave": w, f"rp{ixy}_s": s,
+                        f"rp{ixy}_amdq": am, f"rp{ixy}_apdq": ap})
+            q = _states.random_cells(p, (L,), rng)
+            evl, evr = R.evec(p, ixy, mx, q)
+            out.update({f"evec{ixy}_q": q, f"evec{ixy}_evl": evl, f"evec{ixy}_evr": evr})
+        q = _states.random_cells(p, (L,), rng, smooth=True)
+        evl, evr = R.evec(p, 1, mx, q)
+        ql, qr = R.weno(p, q)
+        qlc, qrc = R.weno(p, q, evl, evr)
+        out.update({"weno_q": q, "weno_ql": ql, "weno_qr": qr, "wenoc_evl": evl, "wenoc_evr": evr,
+                    "wenoc_ql": qlc, "wenoc_qr": qrc})
+        save(f"routines_{tag(name, ndim)}.npz", **out)
+
+
+def steps():
+    rng = np.random.default_rng(20260102)
+    for name, ndim in PHYS:
+        out = {}
+        bcs = [(1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3), (1, 3, 2, 2)]
+        if name == "advection":
+            bcs = [b for b in bcs if 3 not in b[:2]]
+        k = 0
+        for bc in bcs:
+            for char in (0, 2):
+                p = make_params(name, (40,) if ndim == 1 else (12, 12), char_decomp=char, mthbc=bc)
+                q = _states.grid(p, rng, smooth=True)
+                dt = 0.1 * p.dx[0]
+                qb, dq, cfl = R.step(p, q, dt)
+                out.update({f"c{k}_q": q, f"c{k}_qbc": qb, f"c{k}_dq": dq,
+                            f"c{k}_cfl": np.array(cfl), f"c{k}_dt": np.array(dt),
+                            f"c{k}_bc": np.array(bc), f"c{k}_char": np.array(char)})
+                k += 1
+        out["n"] = np.array(k)
+        save(f"step_{tag(name, ndim)}.npz", **out)
+
+
+def runs():
+    # C1: 1D advection, component-wise WENO5, SSPRK3, CFL 0.5, to t = 1
+    p = make_params("advection", (500,), char_decomp=0, time_integrator=3, mthbc=(2, 2, 2, 2))
+    cd = R.clawdata_for(p, nout=2, outstyle=1, tfinal=1.0, dt_initial=0.001, cfl_max=0.55,
+                        cfl_desired=0.5)
+    fr, dtl, info = R.run(p, cd, ["1", "200.0 0.3 0.6 0.8"])
+    save("run_c1_advection.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, ctl=np.array([1e99, 0.55, 0.5, 50000, 1]), dt_initial=np.array(0.001))
+    # C2: 1D Euler Shu-Osher, char WENO5 + Roe, SSP(10,4), to t = 0.36 (two frames)
+    p = make_params("euler", (1600,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                    time_integrator=4, mthbc=(1, 1, 1, 1))
+    cd = R.clawdata_for(p, nout=2, outstyle=1, tfinal=0.36, dt_initial=0.001, cfl_max=2.5,
+                        cfl_desired=2.45)
+    fr, dtl, info = R.run(p, cd, ["2", "3.857143 2.629369 10.333333 -4.0"])
+    save("run_c2_euler.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, ctl=np.array([1e99, 2.5, 2.45, 50000, 1]), dt_initial=np.array(0.001))
+    # C3 reduced: 2D Euler four quadrants, char WENO5 + Roe rpn2, SSPRK3, 32x32, 12 steps
+    p = make_params("euler", (32, 32), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    rng = np.random.default_rng(7)
+    st = np.stack([rng.uniform(0.1, 1.5, 4), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4),
+                   rng.uniform(0.03, 1.5, 4)], axis=1)
+    lines = ["1", "0.5 0.5"] + [" ".join(repr(float(v)) for v in row) for row in st]
+    cd = R.clawdata_for(p, nout=12, outstyle=3, nstepout=12, nstop=12, dt_initial=1e-3,
+                        cfl_max=0.5, cfl_desired=0.45)
+    fr, dtl, info = R.run(p, cd, lines)
+    save("run_c3_euler2d.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, states=st, ctl=np.array([1e99, 0.5, 0.45, 12, 1]), dt_initial=np.array(1e-3))
+    # 1D shallow-water dam break, char WENO5 + Roe (efix), SSPRK3, to t = 0.05
+    p = make_params("shallow_water", (200,), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    cd = R.clawdata_for(p, nout=1, outstyle=1, tfinal=0.05, dt_initial=1e-4, cfl_max=0.5,
+                        cfl_desired=0.45)
+    fr, dtl, info = R.run(p, cd, ["3", "2.0 0.5 0.45"])
+    save("run_sw1d_dambreak.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, ctl=np.array([1e99, 0.5, 0.45, 50000, 1]), dt_initial=np.array(1e-4))
+    # 1D acoustics pulse, walls, char WENO5, SSP(10,4), to t = 0.25
+    p = make_params("acoustics", (200,), xlower=(-1.0,), xupper=(1.0,), char_decomp=2,
+                    time_integrator=4, mthbc=(3, 3, 1, 1))
+    cd = R.clawdata_for(p, nout=1, outstyle=1, tfinal=0.25, dt_initial=1e-3, cfl_max=2.5,
+                        cfl_desired=2.4)
+    fr, dtl, info = R.run(p, cd, ["4", "1.0 100.0 0.0"])
+    save("run_ac1d_pulse.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, ctl=np.array([1e99, 2.5, 2.4, 50000, 1]), dt_initial=np.array(1e-3))
+    # 2D shallow water pulse, periodic, char WENO5, SSPRK3, 24x24, to t = 0.02
+    p = make_params("shallow_water", (24, 24), char_decomp=2, time_integrator=3,
+                    mthbc=(2, 2, 2, 2))
+    cd = R.clawdata_for(p, nout=1, outstyle=1, tfinal=0.02, dt_initial=1e-4, cfl_max=0.5,
+                        cfl_desired=0.45)
+    fr, dtl, info = R.run(p, cd, ["2", "0.3 60.0 0.4 0.55"])
+    save("run_sw2d_pulse.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, ctl=np.array([1e99, 0.5, 0.45, 50000, 1]), dt_initial=np.array(1e-4))
+
+
+if __name__ == "__main__":
+    if not R.ref_available():
+        sys.exit("oracle/_ref is not built (make -C oracle ref)")
+    routines()
+    steps()
+    runs()

@@ -1,0 +1,88 @@
+"""The C ABI library loads and exports every entry point include/*.h
+declares, and the ctypes mirror has the header's struct layouts.
+No device work is done here."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from sharpclaw_amd import _abi
+from sharpclaw_amd._lib import LIB_PATH, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "sharpclaw_amd.h")
+
+
+def declared():
+    txt = open(HDR).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sc_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared()
+    assert len(names) >= 18, names
+    L = lib()
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (sc_\w+)", out))
+    assert set(names) <= exported, set(names) - exported
+
+
+def test_version_and_no_device_error():
+    L = lib()
+    assert L.sc_version() >= 1
+    p = _abi.make_params("euler", (16,))
+    h = C.c_void_p()
+    import sharpclaw_amd
+    rc = L.sc_create(C.byref(p), C.byref(h))
+    if rc == 0:  # a GPU is present: clean up, nothing else to check here
+        L.sc_destroy(h)
+        pytest.skip("device present")
+    assert rc == sharpclaw_amd._abi.SC_ERR_HIP
+    assert b"no HIP device" in L.sc_last_error()
+
+
+def test_invalid_options_rejected_before_device_work():
+    """Option checks (the reference's stop paths) run before any HIP call."""
+    L = lib()
+    cases = [
+        dict(physics="advection", nx=(10,), mthbc=(3, 3, 1, 1)),   # wall needs component 2
+        dict(physics="euler", nx=(10,), mthbc=(2, 1, 1, 1)),       # periodic on one side only
+        dict(physics="euler", nx=(10,), time_integrator=7),
+        dict(physics="euler", nx=(10,), char_decomp=1),            # weno5_wave: not on the device path
+    ]
+    for c in cases:
+        p = _abi.make_params(c.pop("physics"), c.pop("nx"), **c)
+        h = C.c_void_p()
+        rc = L.sc_create(C.byref(p), C.byref(h))
+        assert rc in (_abi.SC_ERR_ARG, _abi.SC_ERR_UNSUPPORTED), (c, rc)
+
+
+def test_struct_layouts_match_header():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "sharpclaw_amd.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(sc_params), offsetof(sc_params, xlower),
+         offsetof(sc_params, physics), offsetof(sc_params, device), sizeof(sc_run_ctl),
+         sizeof(sc_run_stats));
+  printf("%zu %zu\n", offsetof(sc_run_stats, stage_evals), offsetof(sc_run_stats, gpu_ms));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "t.c"), "w") as f:
+            f.write(src)
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o", os.path.join(d, "t"),
+                               os.path.join(d, "t.c")])
+        out = subprocess.check_output([os.path.join(d, "t")], text=True).split()
+    P, S = _abi.ScParams, _abi.ScRunStats
+    want = [C.sizeof(P), P.xlower.offset, P.physics.offset, P.device.offset,
+            C.sizeof(_abi.ScRunCtl), C.sizeof(S), S.stage_evals.offset, S.gpu_ms.offset]
+    assert [int(x) for x in out] == want

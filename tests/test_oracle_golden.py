@@ -1,0 +1,108 @@
+"""The oracle (C restatement) against the golden fixtures written by the
+reference itself (tests/golden/make_golden.py): bit-for-bit equality."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sharpclaw_amd import ScRunCtl, interior, make_params
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+PHYS = [("advection", 1), ("acoustics", 1), ("euler", 1), ("shallow_water", 1), ("euler", 2),
+        ("shallow_water", 2)]
+
+
+def load(name):
+    return np.load(os.path.join(G, name))
+
+
+@pytest.mark.parametrize("name,ndim", PHYS)
+def test_routines_bit_exact(name, ndim):
+    d = load(f"routines_{name}{ndim}d.npz")
+    p = make_params(name, (24,) if ndim == 1 else (24, 24), char_decomp=2)
+    mx, L = 24, 30
+    for ixy in ([1] if ndim == 1 else [1, 2]):
+        got = O.rp(p, ixy, mx, d[f"rp{ixy}_ql"], d[f"rp{ixy}_qr"])
+        for g, key in zip(got, ("wave", "s", "amdq", "apdq")):
+            assert np.array_equal(g[1:], d[f"rp{ixy}_{key}"][1:]), (key, ixy)
+        evl, evr = O.evec(p, ixy, mx, d[f"evec{ixy}_q"])
+        assert np.array_equal(evl[1:], d[f"evec{ixy}_evl"][1:])
+        assert np.array_equal(evr[1:], d[f"evec{ixy}_evr"][1:])
+    ql, qr = O.weno5(d["weno_q"])
+    # interfaces I = 4..mx2-2 of the assumed-shape arrays
+    assert np.array_equal(qr[2:L - 3], d["weno_qr"][2:L - 3])
+    assert np.array_equal(ql[3:L - 2], d["weno_ql"][3:L - 2])
+    ql, qr = O.weno5_char(d["weno_q"], d["wenoc_evl"], d["wenoc_evr"])
+    assert np.array_equal(qr[2:L - 3], d["wenoc_qr"][2:L - 3])
+    assert np.array_equal(ql[3:L - 2], d["wenoc_ql"][3:L - 2])
+
+
+@pytest.mark.parametrize("name,ndim", PHYS)
+def test_step_bit_exact(name, ndim):
+    d = load(f"step_{name}{ndim}d.npz")
+    for k in range(int(d["n"])):
+        p = make_params(name, (40,) if ndim == 1 else (12, 12), char_decomp=int(d[f"c{k}_char"]),
+                        mthbc=tuple(int(b) for b in d[f"c{k}_bc"]))
+        qb, dq, cfl = O.step(p, d[f"c{k}_q"], float(d[f"c{k}_dt"]))
+        assert np.array_equal(qb, d[f"c{k}_qbc"]), k
+        assert np.array_equal(interior(p, dq), interior(p, d[f"c{k}_dq"])), k
+        assert cfl == float(d[f"c{k}_cfl"]), k
+
+
+def _replay(p, fname, single_step=False):
+    d = load(fname)
+    c = d["ctl"]
+    ctl = ScRunCtl(dt_max=float(c[0]), cfl_max=float(c[1]), cfl_desired=float(c[2]),
+                   max_steps=int(c[3]), dt_variable=int(c[4]))
+    q = O.new_grid(p)
+    interior(p, q)[:] = d["q"][0]
+    dt = float(d["dt_initial"])
+    t = float(d["t"][0])
+    if single_step:
+        # outstyle 3: one step per sharpclaw call (tend < tstart), main.f90:145
+        nsteps = int(c[3])
+        for _ in range(nsteps):
+            q, dt, st = O.sharpclaw(p, ctl, q, t, t - 1.0, dt)
+            t = st.t
+        assert t == float(d["t"][-1])
+        assert np.array_equal(interior(p, q), d["q"][-1])
+        return
+    nsteps = 0
+    for k in range(1, len(d["t"])):
+        q, dt, st = O.sharpclaw(p, ctl, q, t, float(d["t"][k]), dt)
+        t = st.t
+        nsteps += st.steps_taken - st.rejected
+        assert t == float(d["t"][k])
+        assert np.array_equal(interior(p, q), d["q"][k]), (fname, k)
+    assert nsteps == len(d["dtlog"])
+
+
+def test_run_c1_advection():
+    _replay(make_params("advection", (500,), char_decomp=0, time_integrator=3, mthbc=(2, 2, 2, 2)),
+            "run_c1_advection.npz")
+
+
+def test_run_c2_euler():
+    _replay(make_params("euler", (1600,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                        time_integrator=4, mthbc=(1, 1, 1, 1)), "run_c2_euler.npz")
+
+
+def test_run_c3_euler2d():
+    _replay(make_params("euler", (32, 32), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1)),
+            "run_c3_euler2d.npz", single_step=True)
+
+
+def test_run_sw1d_dambreak():
+    _replay(make_params("shallow_water", (200,), char_decomp=2, time_integrator=3,
+                        mthbc=(1, 1, 1, 1)), "run_sw1d_dambreak.npz")
+
+
+def test_run_ac1d_pulse():
+    _replay(make_params("acoustics", (200,), xlower=(-1.0,), xupper=(1.0,), char_decomp=2,
+                        time_integrator=4, mthbc=(3, 3, 1, 1)), "run_ac1d_pulse.npz")
+
+
+def test_run_sw2d_pulse():
+    _replay(make_params("shallow_water", (24, 24), char_decomp=2, time_integrator=3,
+                        mthbc=(2, 2, 2, 2)), "run_sw2d_pulse.npz")
