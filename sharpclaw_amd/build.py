@@ -56,5 +56,58 @@ def build(verbose=False):
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FAPPS = {
+    "adv1d": ["setprob_advection.f90", "qinit1.f90"],
+    "ac1d": ["setprob_acoustics.f90", "qinit1.f90"],
+    "eu1d": ["setprob_euler.f90", "qinit1.f90"],
+    "sw1d": ["setprob_sw.f90", "qinit1.f90"],
+    "eu2d": ["setprob_euler.f90", "qinit2.f90"],
+    "sw2d": ["setprob_sw.f90", "qinit2.f90"],
+}
+
+
+def flang():
+    for c in (os.environ.get("FC"), "/opt/rocm/lib/llvm/bin/flang", shutil.which("flang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose=False):
+    """The Fortran host driver (fortran/xsclaw_gpu.f90 + the ISO_C_BINDING
+    module) linked with each sample application's hooks into
+    fortran/_build/xsclaw_gpu_<app>, with an rpath to this package."""
+    fc = flang()
+    if fc is None:
+        if verbose:
+            print("flang not found: Fortran driver not built", flush=True)
+        return []
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+
+    def run(cmd):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd, cwd=bdir)
+
+    run([fc, "-O2", "-c", os.path.join(FORTRAN, "sharpclaw_gpu.f90"), "-o", "sharpclaw_gpu.o"])
+    run([fc, "-O2", "-c", os.path.join(FORTRAN, "xsclaw_gpu.f90"), "-o", "xsclaw_gpu.o"])
+    run([fc, "-O2", "-c", os.path.join(FORTRAN, "apps", "hooks_default.f90"), "-o", "hooks_default.o"])
+    out = []
+    for app, srcs in FAPPS.items():
+        objs = []
+        for src in srcs:
+            obj = f"{app}_{src}.o"
+            run([fc, "-O2", "-c", os.path.join(FORTRAN, "apps", src), "-o", obj])
+            objs.append(obj)
+        exe = f"xsclaw_gpu_{app}"
+        run([fc, "-O2", "-o", exe, "xsclaw_gpu.o", "sharpclaw_gpu.o", "hooks_default.o"] + objs +
+            [f"-L{HERE}", "-lsharpclaw_amd", "-Wl,-rpath,$ORIGIN/../.."])
+        out.append(os.path.join(bdir, exe))
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

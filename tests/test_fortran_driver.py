@@ -1,0 +1,119 @@
+"""The Fortran host driver (sharpclaw_amd/fortran/xsclaw_gpu.f90, the device
+path behind the ISO_C_BINDING module) against the reference program built
+with its own output routines (oracle/_ref/xsclaw_<app>_fmt): the same
+sharpclaw.data / setprob.data / qinit.data in, fort.q / fort.t / fort.a /
+fort.info out.  Frame 0 must be byte-identical (same initial data through
+the same writer layout); later frames agree to the printed precision."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import refrun as R
+from sharpclaw_amd import make_params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FB = os.path.join(ROOT, "sharpclaw_amd", "fortran", "_build")
+APPS = {"advection": 1, "acoustics": 2, "euler": 3, "shallow_water": 4}
+
+
+def have_driver(app):
+    return os.path.exists(os.path.join(FB, f"xsclaw_gpu_{app}"))
+
+
+def setup_case(d, p, cd, qlines, name, beta=200.0):
+    R.write_setprob(p, d, beta=beta)
+    cd.write(os.path.join(d, "sharpclaw.data"))
+    with open(os.path.join(d, "qinit.data"), "w") as f:
+        f.write("\n".join(qlines) + "\n")
+    with open(os.path.join(d, "gpu.data"), "w") as f:
+        f.write(f"{APPS[name]}\n0\n0\n-1\n")
+
+
+def read_q(path):
+    vals = []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    body = [ln for ln in lines if ln.strip() and not any(
+        k in ln for k in ("grid_number", "AMR_level", "mx", "my", "xlow", "ylow", "dx", "dy"))]
+    for ln in body:
+        vals.extend(float(x.replace("D", "E")) for x in ln.split())
+    return np.array(vals)
+
+
+def run_pair(p, cd, qlines, name, app, tmp):
+    dg = os.path.join(tmp, "gpu")
+    dr = os.path.join(tmp, "ref")
+    os.makedirs(dg)
+    os.makedirs(dr)
+    for d in (dg, dr):
+        setup_case(d, p, cd, qlines, name)
+    rg = subprocess.run([os.path.join(FB, f"xsclaw_gpu_{app}")], cwd=dg, capture_output=True,
+                        text=True, timeout=300)
+    assert rg.returncode == 0, rg.stdout + rg.stderr
+    rr = subprocess.run([os.path.join(R.REF, f"xsclaw_{app}_fmt")], cwd=dr, capture_output=True,
+                        text=True, timeout=300)
+    assert rr.returncode == 0, rr.stdout + rr.stderr
+    return dg, dr
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not (have_driver("adv1d") and R.ref_available("adv1d")),
+                    reason="driver or reference build missing")
+def test_driver_c1_advection():
+    p = make_params("advection", (500,), char_decomp=0, time_integrator=3, mthbc=(2, 2, 2, 2))
+    cd = R.clawdata_for(p, nout=2, outstyle=1, tfinal=1.0, dt_initial=0.001, cfl_max=0.55,
+                        cfl_desired=0.5)
+    with tempfile.TemporaryDirectory() as tmp:
+        dg, dr = run_pair(p, cd, ["1", "200.0 0.3 0.6 0.8"], "advection", "adv1d", tmp)
+        for f in ("fort.q0000", "fort.t0000", "fort.t0001", "fort.t0002"):
+            assert open(os.path.join(dg, f)).read() == open(os.path.join(dr, f)).read(), f
+        for f in ("fort.q0001", "fort.q0002"):
+            a, b = read_q(os.path.join(dg, f)), read_q(os.path.join(dr, f))
+            assert a.shape == b.shape
+            # 8 significant digits printed (out1.f90:70) plus the run's own
+            # sensitivity near the square wave (~1e-8)
+            assert np.max(np.abs(a - b)) <= 5e-8 * np.max(np.abs(b)), f
+        assert open(os.path.join(dg, "fort.info")).read() == open(os.path.join(dr, "fort.info")).read()
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not (have_driver("eu2d") and R.ref_available("eu2d")),
+                    reason="driver or reference build missing")
+def test_driver_euler2d_quadrants():
+    p = make_params("euler", (32, 32), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    rng = np.random.default_rng(7)
+    st = np.stack([rng.uniform(0.1, 1.5, 4), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4),
+                   rng.uniform(0.03, 1.5, 4)], axis=1)
+    lines = ["1", "0.5 0.5"] + [" ".join(repr(float(v)) for v in row) for row in st]
+    cd = R.clawdata_for(p, nout=12, outstyle=3, nstepout=6, nstop=12, dt_initial=1e-3, cfl_max=0.5,
+                        cfl_desired=0.45)
+    with tempfile.TemporaryDirectory() as tmp:
+        dg, dr = run_pair(p, cd, lines, "euler", "eu2d", tmp)
+        for f in ("fort.q0000", "fort.t0000", "fort.a0000", "fort.a0001"):
+            assert open(os.path.join(dg, f)).read() == open(os.path.join(dr, f)).read(), f
+        for f in ("fort.q0001", "fort.q0002"):
+            a, b = read_q(os.path.join(dg, f)), read_q(os.path.join(dr, f))
+            assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b)), f
+        ta = open(os.path.join(dg, "fort.t0002")).read().split()
+        tb = open(os.path.join(dr, "fort.t0002")).read().split()
+        assert abs(float(ta[0]) - float(tb[0])) <= 1e-14 and ta[1:] == tb[1:]
+
+
+@pytest.mark.skipif(not have_driver("eu1d"), reason="driver not built")
+def test_driver_reports_errors_like_the_reference():
+    """A failing library call stops the driver with the library's message
+    (the reference's write(6,*) + stop).  Without a GPU that is sc_create;
+    with one, an unsupported option (tfluct_solver = 1) is."""
+    p = make_params("euler", (64,), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    cd = R.clawdata_for(p, nout=1, outstyle=1, tfinal=0.01, dt_initial=1e-3, cfl_max=0.5,
+                        cfl_desired=0.45, tfluct_solver=1)
+    with tempfile.TemporaryDirectory() as d:
+        setup_case(d, p, cd, ["5", "0.1 50.0 0.0"], "euler")
+        r = subprocess.run([os.path.join(FB, "xsclaw_gpu_eu1d")], cwd=d, capture_output=True,
+                           text=True, timeout=120)
+    assert r.returncode != 0
+    assert "*** ERROR *** sc_create" in r.stdout
+    assert ("no HIP device" in r.stdout) or ("tfluct_solver" in r.stdout)
