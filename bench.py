@@ -66,6 +66,23 @@ def traffic_from_profile():
         return json.load(f).get("bytes_per_launch")
 
 
+def valu_from_profile(tx, ty, ms_launch):
+    """float64 VALU roofline of the tuned stage kernel: flop per launch from
+    the committed rocprofv3 counter summary, divided by the live launch time."""
+    path = os.path.join(ROOT, "profiles", "stage2d_euler2d_1024_fp64_valu.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        ks = json.load(f)["kernels"]
+    k = ks.get(f"stage2d_kernel<4, 2, double, {tx}, {ty}>")
+    if k is None:
+        return None
+    tf = k["fp64_flop_per_launch"] / (ms_launch * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf / FP64_PEAK_TFLOPS, "flop_per_launch": k["fp64_flop_per_launch"],
+            "valu_lane_instr_per_cell": k["valu_lane_instr_per_cell"]}
+
+
 def cpu_baseline(n, st, steps):
     """The reference Fortran (oracle/_ref, flang -O3, serial) on a bounded
     sample of the same workload; the C restatement when it is not built."""
@@ -144,7 +161,8 @@ def main():
         value = cells * rst.stage_evals / (rst.gpu_ms * 1e-3)
         # kernel-only: the fused stage launches of 50 SSPRK3 steps back to back
         ms_launch = s.time_stage(50, dt_out)
-        log(f"bench: fused stage kernel {ms_launch * 1e3:.1f} us per launch")
+        tile, (tx, ty) = s.get_tile()
+        log(f"bench: fused stage kernel {ms_launch * 1e3:.1f} us per launch (tile {tx}x{ty})")
         sums = s.sums()
 
     from sharpclaw_amd import copy_bench
@@ -173,12 +191,14 @@ def main():
                                f"Roe rpn2, SSPRK3, extrapolation BCs, "
                                f"{'float32 arithmetic / float64 state' if args.fp32 else 'float64'}",
                    "cells": cells, "meqn": meqn, "stages_timed": int(rst.stage_evals),
+                   "tile": f"{tx}x{ty} (autotuned)",
                    "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage2d_kernel<EU2,char,f64,16x16>",
+                     "kernel": f"stage2d_kernel<EU2,char,{'f32' if args.fp32 else 'f64'},{tx}x{ty}>",
                      "alg_bytes_per_launch": alg_bytes,
                      "us_per_launch": ms_launch * 1e3},
+        "valu_roofline": None if args.fp32 or n != 1024 else valu_from_profile(tx, ty, ms_launch),
         "gpu_ms_timed": rst.gpu_ms,
         "copy_kernel_gbs": copy_gbs,
         "frac_of_copy_bw": achieved / copy_gbs,

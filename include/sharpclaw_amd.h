@@ -149,6 +149,15 @@ int sc_sums(sc_solver *s, double *sums);
  * solver's stream.  Leaves q unchanged. */
 int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stage);
 
+/* Tile shape of the 2D fused stage kernel (no counterpart in the reference):
+ * -1 (default) times the candidates on this solver's grid at the first
+ * sc_ts/sc_advance/sc_time_stage and keeps the fastest; 0..3 fix it
+ * (16x16, 16x15, 15x16, 32x7 cells).  Every shape gives identical results.
+ * The environment variable SC_TILE=k fixes it for new solvers. */
+int sc_set_tile(sc_solver *s, int tile);
+/* Current tile index (-1 before tuning) and its cell extents (tx, ty may be NULL). */
+int sc_get_tile(sc_solver *s, int *tile, int *tx, int *ty);
+
 /* Achievable HBM bandwidth: reps device-to-device copies of nbytes by a plain
  * copy kernel on `device`, timed with HIP events; *gbps = 2*nbytes*reps/time
  * (bytes read + written).  Calibrates the roofline and the HBM counters. */

@@ -47,6 +47,8 @@ def lib():
         "sc_sums": ([H, dp], C.c_int),
         "sc_time_stage": ([H, C.c_int, C.c_double, dref], C.c_int),
         "sc_copy_bench": ([C.c_int, C.c_size_t, C.c_int, dref], C.c_int),
+        "sc_set_tile": ([H, C.c_int], C.c_int),
+        "sc_get_tile": ([H, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "sc_rp": ([P, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp], C.c_int),
         "sc_evec": ([P, C.c_int, C.c_int, dp, dp, dp], C.c_int),
         "sc_weno5": ([P, C.c_int, dp, dp, dp], C.c_int),

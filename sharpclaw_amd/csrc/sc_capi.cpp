@@ -11,8 +11,12 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 
 #include "../../include/sharpclaw_amd.h"
 #include "sc_kernels.h"
@@ -118,6 +122,7 @@ template <class T> static PhysConst<T> mkconst(const sc_params *p)
     double cc = std::sqrt(p->bulk / p->rho);
     c.cc = T(cc);
     c.zz = T(cc * p->rho);
+    c.hzz = T(0.5 / (cc * p->rho));
     if (p->efix >= 0)
         c.efix = p->efix;
     else if (p->physics == SC_PHYS_EULER)
@@ -152,6 +157,7 @@ struct sc_solver {
     long long slot_ctr;
     int cur, old;      // buffer indices of q and qold
     bool have_q;
+    int tile2d;        // 2D tile shape index; -1: tune at first use
     hipEvent_t ev0, ev1;
 };
 
@@ -218,6 +224,11 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     }
     s->cur = 0;
     s->old = 0;
+    s->tile2d = -1;
+    if (const char *e = getenv("SC_TILE")) {
+        int k = atoi(e);
+        if (k >= 0 && k < NTILE2D) s->tile2d = k;
+    }
     *out = s;
     return SC_OK;
 }
@@ -304,7 +315,77 @@ static StageArgs base_args(const sc_solver *s)
 static hipError_t run_stage(sc_solver *s, const StageArgs &a)
 {
     if (s->p.ndim == 1) return launch_stage1d(s->sel, 1, a, s->cd, s->cf, s->st);
-    return launch_stage2d(s->sel, a, s->cd, s->cf, s->st);
+    return launch_stage2d(s->sel, s->tile2d < 0 ? 0 : s->tile2d, a, s->cd, s->cf, s->st);
+}
+
+// Autotuner of the 2D tile shape: every candidate computes the same values
+// (a tile only changes which workgroup evaluates an interface), so the
+// choice is made by timing alone -- two launches of a three-array stage on
+// the solver's own q after a warm-up -- and remembered per (physics,
+// reconstruction, precision, grid) for the life of the process.
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int, int>, int> g_tune;
+
+static int tune_tile(sc_solver *s)
+{
+    if (s->p.ndim != 2 || s->tile2d >= 0) return SC_OK;
+    auto key = std::make_tuple(s->sel.phys, s->sel.chr, s->sel.fp32, s->p.nx[0], s->p.nx[1]);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            s->tile2d = it->second;
+            return SC_OK;
+        }
+    }
+    int best = 0;
+    if ((long long)s->p.nx[0] * s->p.nx[1] >= 128 * 128) {
+        StageArgs a = base_args(s);
+        a.dtdx = 1e-6 / s->p.dx[0];
+        a.dtdy = 1e-6 / s->p.dx[1];
+        a.qin = s->buf[s->cur];
+        a.qold = s->buf[s->cur];
+        a.qout = s->buf[free_buf(s, s->cur, s->old)];
+        a.op = OP_SSP33_2;
+        a.smax = nullptr;
+        float bestms = 1e30f;
+        for (int k = 0; k < NTILE2D; k++) {
+            HIPCHK(launch_stage2d(s->sel, k, a, s->cd, s->cf, s->st));
+            HIPCHK(hipEventRecord(s->ev0, s->st));
+            for (int r = 0; r < 2; r++) HIPCHK(launch_stage2d(s->sel, k, a, s->cd, s->cf, s->st));
+            HIPCHK(hipEventRecord(s->ev1, s->st));
+            HIPCHK(hipEventSynchronize(s->ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+            if (ms < bestms) {
+                bestms = ms;
+                best = k;
+            }
+        }
+    }
+    s->tile2d = best;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+    return SC_OK;
+}
+
+extern "C" int sc_set_tile(sc_solver *s, int tile)
+{
+    if (!s) return fail(SC_ERR_ARG, "NULL solver");
+    if (tile < -1 || tile >= NTILE2D) return fail(SC_ERR_ARG, "tile must be -1..%d", NTILE2D - 1);
+    s->tile2d = tile;
+    return SC_OK;
+}
+
+extern "C" int sc_get_tile(sc_solver *s, int *tile, int *tx, int *ty)
+{
+    if (!s || !tile) return fail(SC_ERR_ARG, "NULL argument");
+    *tile = s->tile2d;
+    int a = 0, b = 0;
+    tile2d_shape(s->tile2d < 0 ? 0 : s->tile2d, &a, &b);
+    if (tx) *tx = a;
+    if (ty) *ty = b;
+    return SC_OK;
 }
 
 // next CFL word pair, cleared ring-wise
@@ -409,8 +490,10 @@ extern "C" int sc_ts(sc_solver *s, double dt, double told, double *cfl)
     if (!s || !cfl) return fail(SC_ERR_ARG, "NULL argument");
     if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
+    int rc = tune_tile(s);
+    if (rc) return rc;
     unsigned long long *slot = nullptr;
-    int rc = enqueue_ts(s, dt, &slot);
+    rc = enqueue_ts(s, dt, &slot);
     if (rc) return rc;
     return read_cfl(s, slot, dt, cfl);
 }
@@ -428,6 +511,8 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
     if (!s || !c || !dt_io || !st) return fail(SC_ERR_ARG, "NULL argument");
     if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
+    int trc = tune_tile(s);
+    if (trc) return trc;
     memset(st, 0, sizeof *st);
     double t = tstart, dt = *dt_io, told = tstart, cflmax = 0.0, cfl = 0.0;
     double dtmin = dt, dtmax = dt;
@@ -529,6 +614,8 @@ extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stag
     if (!s || !ms_per_stage || n < 1) return fail(SC_ERR_ARG, "bad argument");
     if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
+    int trc = tune_tile(s);
+    if (trc) return trc;
     StageArgs a = base_args(s);
     a.dtdx = dt / s->p.dx[0];
     a.dtdy = dt / s->p.dx[1];

@@ -46,8 +46,11 @@ struct PhysSel {
 
 hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const PhysConst<double> &cd,
                           const PhysConst<float> &cf, hipStream_t st);
-hipError_t launch_stage2d(const PhysSel &s, const StageArgs &a, const PhysConst<double> &cd,
+// 2D tile shapes selectable per launch (index into a fixed table)
+constexpr int NTILE2D = 4;
+hipError_t launch_stage2d(const PhysSel &s, int tile, const StageArgs &a, const PhysConst<double> &cd,
                           const PhysConst<float> &cf, hipStream_t st);
+void tile2d_shape(int tile, int *tx, int *ty);
 // ghost fill in place (bc1.f90 / bc2.f90 semantics, corners included)
 hipError_t launch_bc(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY, const int *bc,
                      hipStream_t st);

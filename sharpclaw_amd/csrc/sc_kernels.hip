@@ -177,10 +177,11 @@ __global__ __launch_bounds__(NT) void stage1d_kernel(StageArgs a, PhysConst<T> c
 // interior row/column, whose wave speeds they therefore repeat exactly, so
 // the interior sweeps give the same maximum.
 template <int P, int CHAR, class T, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void stage2d_kernel(StageArgs a, PhysConst<T> c)
+__global__ __launch_bounds__(256) void stage2d_kernel(StageArgs a, PhysConst<T> c)
 {
     constexpr int M = Shape<P>::M, W = Shape<P>::W;
-    constexpr int NT = TX * TY;
+    constexpr int NT = 256;
+    static_assert(TX * TY <= NT, "one cell per thread");
     constexpr int LXT = TX + 6, LYT = TY + 6;
     constexpr int NIX = (TX + 1) * TY, NIY = TX * (TY + 1);
     constexpr int NI = NIX > NIY ? NIX : NIY;
@@ -246,8 +247,9 @@ __global__ __launch_bounds__(TX *TY) void stage2d_kernel(StageArgs a, PhysConst<
     }
     __syncthreads();
     const int cx = t % TX, cy = t / TX;
+    const bool owns_cell = t < TX * TY;
     double dqx[M];
-    {
+    if (owns_cell) {
         int kl = cy * (TX + 1) + cx, kr = kl + 1;
         T ql_[M], qr_[M], am2[M], ap2[M];
 #pragma unroll
@@ -286,7 +288,7 @@ __global__ __launch_bounds__(TX *TY) void stage2d_kernel(StageArgs a, PhysConst<
         }
     }
     __syncthreads();
-    {
+    if (owns_cell) {
         const int i = i0 + cx, j = j0 + cy;
         int kb = cy * TX + cx, kt = kb + TX;
         T ql_[M], qr_[M], am2[M], ap2[M];
@@ -534,7 +536,6 @@ __global__ void copy_kernel(double2 *__restrict__ dst, const double2 *__restrict
 // -------------------------------------------------------------- launchers
 
 constexpr int NT1 = 256;
-constexpr int TX2 = 16, TY2 = 16;
 
 template <int P, int IXY, int CHAR, class T>
 static hipError_t go1(const StageArgs &a, const PhysConst<T> &c, hipStream_t st)
@@ -572,34 +573,61 @@ hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const P
     return hipErrorInvalidValue;
 }
 
-template <int P, int CHAR, class T>
-static hipError_t go2(const StageArgs &a, const PhysConst<T> &c, hipStream_t st)
+// Tile shapes of the 2D stage kernel (256 threads each).  16x16 runs 272
+// interfaces per sweep (a second, mostly idle pass); the others fit each
+// sweep in one pass at the price of 240 or 224 cells per tile.
+struct TileShape {
+    int tx, ty;
+};
+static const TileShape kTiles[NTILE2D] = {{16, 16}, {16, 15}, {15, 16}, {32, 7}};
+
+template <int P, int CHAR, class T, int TX, int TY>
+static hipError_t go2t(const StageArgs &a0, const PhysConst<T> &c, hipStream_t st)
 {
-    hipLaunchKernelGGL((stage2d_kernel<P, CHAR, T, TX2, TY2>), dim3(a.ntx * a.nty), dim3(TX2 * TY2),
-                       0, st, a, c);
+    StageArgs a = a0;
+    a.ntx = (a.nx + TX - 1) / TX;
+    a.nty = (a.ny + TY - 1) / TY;
+    hipLaunchKernelGGL((stage2d_kernel<P, CHAR, T, TX, TY>), dim3(a.ntx * a.nty), dim3(256), 0,
+                       st, a, c);
     return hipGetLastError();
 }
 
+template <int P, int CHAR, class T>
+static hipError_t go2(int tile, const StageArgs &a, const PhysConst<T> &c, hipStream_t st)
+{
+    switch (tile) {
+    case 1: return go2t<P, CHAR, T, 16, 15>(a, c, st);
+    case 2: return go2t<P, CHAR, T, 15, 16>(a, c, st);
+    case 3: return go2t<P, CHAR, T, 32, 7>(a, c, st);
+    default: return go2t<P, CHAR, T, 16, 16>(a, c, st);
+    }
+}
+
 template <int P>
-static hipError_t go2c(const PhysSel &s, const StageArgs &a, const PhysConst<double> &cd,
+static hipError_t go2c(const PhysSel &s, int tile, const StageArgs &a, const PhysConst<double> &cd,
                        const PhysConst<float> &cf, hipStream_t st)
 {
     if (s.chr == 2)
-        return s.fp32 ? go2<P, 2, float>(a, cf, st) : go2<P, 2, double>(a, cd, st);
-    return s.fp32 ? go2<P, 0, float>(a, cf, st) : go2<P, 0, double>(a, cd, st);
+        return s.fp32 ? go2<P, 2, float>(tile, a, cf, st) : go2<P, 2, double>(tile, a, cd, st);
+    return s.fp32 ? go2<P, 0, float>(tile, a, cf, st) : go2<P, 0, double>(tile, a, cd, st);
 }
 
-hipError_t launch_stage2d(const PhysSel &s, const StageArgs &a0, const PhysConst<double> &cd,
+hipError_t launch_stage2d(const PhysSel &s, int tile, const StageArgs &a, const PhysConst<double> &cd,
                           const PhysConst<float> &cf, hipStream_t st)
 {
-    StageArgs a = a0;
-    a.ntx = (a.nx + TX2 - 1) / TX2;
-    a.nty = (a.ny + TY2 - 1) / TY2;
+    if (tile < 0 || tile >= NTILE2D) tile = 0;
     switch (s.phys) {
-    case EU2: return go2c<EU2>(s, a, cd, cf, st);
-    case SW2: return go2c<SW2>(s, a, cd, cf, st);
+    case EU2: return go2c<EU2>(s, tile, a, cd, cf, st);
+    case SW2: return go2c<SW2>(s, tile, a, cd, cf, st);
     }
     return hipErrorInvalidValue;
+}
+
+void tile2d_shape(int tile, int *tx, int *ty)
+{
+    if (tile < 0 || tile >= NTILE2D) tile = 0;
+    *tx = kTiles[tile].tx;
+    *ty = kTiles[tile].ty;
 }
 
 hipError_t launch_bc(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY, const int *bc,

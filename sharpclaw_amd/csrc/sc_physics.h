@@ -5,11 +5,14 @@
 // solves the Riemann problem whose left state is qr(i-1,:) and right state
 // ql(i,:) (the reference's i-th problem, e.g. rp1eu.f:19-20); IXY selects the
 // normal momentum for the 2D solvers (rpn2sw.f:60-66).  The arithmetic
-// follows each reference solver's expression order; the compiler may fuse
+// follows each reference solver's expressions, with shared reciprocals
+// (sc_fastmath.h) in place of repeated divisions and with fused
 // multiply-adds, so results agree with the reference to rounding, not bit
 // for bit.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "sc_fastmath.h"
 
 namespace sc {
 
@@ -29,6 +32,7 @@ template <class T> struct PhysConst {
     T grav;           // /comrp/ grav
     T u;              // /comrp/ u (advection)
     T cc, zz;         // /cparam/ sound speed and impedance
+    T hzz;            // 1/(2 zz)
     int efix;
 };
 
@@ -55,8 +59,8 @@ __device__ __forceinline__ void rp_ac(const PhysConst<T> &c, const T *lf, const 
 {
     const T zz = c.zz, cc = c.cc;
     T d1 = rt[0] - lf[0], d2 = rt[1] - lf[1];
-    T a1 = (-d1 + zz * d2) / (T(2) * zz);
-    T a2 = (d1 + zz * d2) / (T(2) * zz);
+    T a1 = (-d1 + zz * d2) * c.hzz;
+    T a2 = (d1 + zz * d2) * c.hzz;
     // wave(m, mw) stored at wave[m + M*mw]
     wave[0] = -(a1 * zz);
     wave[1] = a1;
@@ -108,17 +112,19 @@ __device__ __forceinline__ void rp_eu1(const PhysConst<T> &c, const T *lf, const
     const T g = c.gamma, g1 = c.gamma1, h = T(0.5);
     T rl = lf[0], ml = lf[1], el = lf[2];
     T rr = rt[0], mr = rt[1], er = rt[2];
-    T rhsqrtl = sqrt(rl), rhsqrtr = sqrt(rr);
-    T pl = g1 * (el - h * (ml * ml) / rl);
-    T pr = g1 * (er - h * (mr * mr) / rr);
-    T rhsq2 = rhsqrtl + rhsqrtr;
-    T u = (ml / rhsqrtl + mr / rhsqrtr) / rhsq2;
-    T enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+    T irl = frcp(rl), irr = frcp(rr);
+    T isl = frsq(rl), isr = frsq(rr);  // 1/sqrt(rho)
+    T pl = g1 * (el - h * (ml * ml) * irl);
+    T pr = g1 * (er - h * (mr * mr) * irr);
+    T irs = frcp(rl * isl + rr * isr);  // 1/(sqrt(rhol) + sqrt(rhor))
+    T u = (ml * isl + mr * isr) * irs;
+    T enth = ((el + pl) * isl + (er + pr) * isr) * irs;
     T a2 = g1 * (enth - h * (u * u));
-    T a = sqrt(a2);
+    T ia = frsq(a2);
+    T a = a2 * ia;
     T d1 = rr - rl, d2 = mr - ml, d3 = er - el;
-    T b2 = g1 / (a * a) * ((enth - u * u) * d1 + u * d2 - d3);
-    T b3 = (d2 + (a - u) * d1 - a * b2) / (T(2) * a);
+    T b2 = g1 * (ia * ia) * ((enth - u * u) * d1 + u * d2 - d3);
+    T b3 = (d2 + (a - u) * d1 - a * b2) * (h * ia);
     T b1 = d1 - b2 - b3;
     wave[0] = b1;
     wave[1] = b1 * (u - a);
@@ -137,17 +143,18 @@ __device__ __forceinline__ void rp_eu1(const PhysConst<T> &c, const T *lf, const
         return;
     }
     do {
-        T s0 = ml / rl - sqrt(g * (g1 * (el - h * (ml * ml) / rl)) / rl);
+        T s0 = ml * irl - fsqrt(g * pl * irl);
         if (s0 >= T(0) && s[0] > T(0)) {
             amdq[0] = amdq[1] = amdq[2] = T(0);
             break;
         }
         T rho1 = rl + wave[0], rhou1 = ml + wave[1], en1 = el + wave[2];
-        T p1 = g1 * (en1 - h * (rhou1 * rhou1) / rho1);
-        T s1 = rhou1 / rho1 - sqrt(g * p1 / rho1);
+        T ir1 = frcp(rho1);
+        T p1 = g1 * (en1 - h * (rhou1 * rhou1) * ir1);
+        T s1 = rhou1 * ir1 - fsqrt(g * p1 * ir1);
         T sfract;
         if (s0 < T(0) && s1 > T(0))
-            sfract = s0 * (s1 - s[0]) / (s1 - s0);
+            sfract = s0 * fdiv(s1 - s[0], s1 - s0);
         else if (s[0] < T(0))
             sfract = s[0];
         else
@@ -159,12 +166,13 @@ __device__ __forceinline__ void rp_eu1(const PhysConst<T> &c, const T *lf, const
         amdq[0] = amdq[0] + s[1] * wave[3];
         amdq[1] = amdq[1] + s[1] * wave[4];
         amdq[2] = amdq[2] + s[1] * wave[5];
-        T s3 = mr / rr + sqrt(g * (g1 * (er - h * (mr * mr) / rr)) / rr);
+        T s3 = mr * irr + fsqrt(g * pr * irr);
         T rho2 = rr - wave[6], rhou2 = mr - wave[7], en2 = er - wave[8];
-        T p2 = g1 * (en2 - h * (rhou2 * rhou2) / rho2);
-        T s2 = rhou2 / rho2 + sqrt(g * p2 / rho2);
+        T ir2 = frcp(rho2);
+        T p2 = g1 * (en2 - h * (rhou2 * rhou2) * ir2);
+        T s2 = rhou2 * ir2 + fsqrt(g * p2 * ir2);
         if (s2 < T(0) && s3 > T(0))
-            sfract = s2 * (s3 - s[2]) / (s3 - s2);
+            sfract = s2 * fdiv(s3 - s[2], s3 - s2);
         else if (s[2] < T(0))
             sfract = s[2];
         else
@@ -187,20 +195,22 @@ __device__ __forceinline__ void rp_eu2(const PhysConst<T> &c, const T *lf, const
     const T g = c.gamma, g1 = c.gamma1, h = T(0.5);
     T rl = lf[0], ul = lf[mu], vl = lf[mv], el = lf[3];
     T rr = rt[0], ur = rt[mu], vr = rt[mv], er = rt[3];
-    T rhsqrtl = sqrt(rl), rhsqrtr = sqrt(rr);
-    T pl = g1 * (el - h * (ul * ul + vl * vl) / rl);
-    T pr = g1 * (er - h * (ur * ur + vr * vr) / rr);
-    T rhsq2 = rhsqrtl + rhsqrtr;
-    T u = (ul / rhsqrtl + ur / rhsqrtr) / rhsq2;
-    T v = (vl / rhsqrtl + vr / rhsqrtr) / rhsq2;
-    T enth = ((el + pl) / rhsqrtl + (er + pr) / rhsqrtr) / rhsq2;
+    T irl = frcp(rl), irr = frcp(rr);
+    T isl = frsq(rl), isr = frsq(rr);
+    T pl = g1 * (el - h * (ul * ul + vl * vl) * irl);
+    T pr = g1 * (er - h * (ur * ur + vr * vr) * irr);
+    T irs = frcp(rl * isl + rr * isr);
+    T u = (ul * isl + ur * isr) * irs;
+    T v = (vl * isl + vr * isr) * irs;
+    T enth = ((el + pl) * isl + (er + pr) * isr) * irs;
     T u2v2 = u * u + v * v;
     T a2 = g1 * (enth - h * u2v2);
-    T a = sqrt(a2);
+    T ia = frsq(a2);
+    T a = a2 * ia;
     T d1 = rr - rl, d2 = ur - ul, d3 = vr - vl, d4 = er - el;
     T a3 = d3 - v * d1;
-    T a2s = g1 / a2 * ((enth - u2v2) * d1 + u * d2 + v * d3 - d4);
-    T a4 = (d2 + (a - u) * d1 - a * a2s) / (T(2) * a);
+    T a2s = g1 * (ia * ia) * ((enth - u2v2) * d1 + u * d2 + v * d3 - d4);
+    T a4 = (d2 + (a - u) * d1 - a * a2s) * (h * ia);
     T a1 = d1 - a2s - a4;
     // wave(m, w) at wave[m + 4*w]
     wave[0] = a1;
@@ -223,17 +233,18 @@ __device__ __forceinline__ void rp_eu2(const PhysConst<T> &c, const T *lf, const
         return;
     }
     do {
-        T s0 = ul / rl - sqrt(g * pl / rl);
+        T s0 = ul * irl - fsqrt(g * pl * irl);
         if (s0 >= T(0) && s[0] > T(0)) {
             amdq[0] = amdq[1] = amdq[2] = amdq[3] = T(0);
             break;
         }
         T rho1 = rl + wave[0], rhou1 = ul + wave[mu], rhov1 = vl + wave[mv], en1 = el + wave[3];
-        T p1 = g1 * (en1 - h * (rhou1 * rhou1 + rhov1 * rhov1) / rho1);
-        T s1 = rhou1 / rho1 - sqrt(g * p1 / rho1);
+        T ir1 = frcp(rho1);
+        T p1 = g1 * (en1 - h * (rhou1 * rhou1 + rhov1 * rhov1) * ir1);
+        T s1 = rhou1 * ir1 - fsqrt(g * p1 * ir1);
         T sfract;
         if (s0 < T(0) && s1 > T(0))
-            sfract = s0 * (s1 - s[0]) / (s1 - s0);
+            sfract = s0 * fdiv(s1 - s[0], s1 - s0);
         else if (s[0] < T(0))
             sfract = s[0];
         else
@@ -243,12 +254,13 @@ __device__ __forceinline__ void rp_eu2(const PhysConst<T> &c, const T *lf, const
         if (s[1] >= T(0)) break;
 #pragma unroll
         for (int m = 0; m < 4; m++) amdq[m] = amdq[m] + s[1] * wave[4 + m];
-        T s3 = ur / rr + sqrt(g * pr / rr);
+        T s3 = ur * irr + fsqrt(g * pr * irr);
         T rho2 = rr - wave[8], rhou2 = ur - wave[8 + mu], rhov2 = vr - wave[8 + mv], en2 = er - wave[11];
-        T p2 = g1 * (en2 - h * (rhou2 * rhou2 + rhov2 * rhov2) / rho2);
-        T s2 = rhou2 / rho2 + sqrt(g * p2 / rho2);
+        T ir2 = frcp(rho2);
+        T p2 = g1 * (en2 - h * (rhou2 * rhou2 + rhov2 * rhov2) * ir2);
+        T s2 = rhou2 * ir2 + fsqrt(g * p2 * ir2);
         if (s2 < T(0) && s3 > T(0))
-            sfract = s2 * (s3 - s[2]) / (s3 - s2);
+            sfract = s2 * fdiv(s3 - s[2], s3 - s2);
         else if (s[2] < T(0))
             sfract = s[2];
         else
@@ -266,12 +278,14 @@ __device__ __forceinline__ void rp_sw1(const PhysConst<T> &c, const T *lf, const
 {
     const T gr = c.grav, h = T(0.5);
     T hl = lf[0], hul = lf[1], hr = rt[0], hur = rt[1];
-    T sl = sqrt(hl), sr = sqrt(hr);
-    T ubar = (hul / sl + hur / sr) / (sl + sr);
-    T cbar = sqrt(h * gr * (hl + hr));
+    T ihl = frcp(hl), ihr = frcp(hr);
+    T isl = frsq(hl), isr = frsq(hr);
+    T ubar = (hul * isl + hur * isr) * frcp(hl * isl + hr * isr);
+    T cbar = fsqrt(h * gr * (hl + hr));
+    T icb = frcp(cbar);
     T d1 = hr - hl, d2 = hur - hul;
-    T a1 = h * (-d2 + (ubar + cbar) * d1) / cbar;
-    T a2 = h * (d2 - (ubar - cbar) * d1) / cbar;
+    T a1 = h * (-d2 + (ubar + cbar) * d1) * icb;
+    T a2 = h * (d2 - (ubar - cbar) * d1) * icb;
     wave[0] = a1;
     wave[1] = a1 * (ubar - cbar);
     s[0] = ubar - cbar;
@@ -283,27 +297,27 @@ __device__ __forceinline__ void rp_sw1(const PhysConst<T> &c, const T *lf, const
         return;
     }
     do {
-        T s0 = hul / hl - sqrt(gr * hl);
+        T s0 = hul * ihl - fsqrt(gr * hl);
         if (s0 >= T(0) && s[0] > T(0)) {
             amdq[0] = amdq[1] = T(0);
             break;
         }
         T hr1 = hl + wave[0], uhr1 = hul + wave[1];
-        T s1 = uhr1 / hr1 - sqrt(gr * hr1);
+        T s1 = fdiv(uhr1, hr1) - fsqrt(gr * hr1);
         T sfract;
         if (s0 < T(0) && s1 > T(0))
-            sfract = s0 * (s1 - s[0]) / (s1 - s0);
+            sfract = s0 * fdiv(s1 - s[0], s1 - s0);
         else if (s[0] < T(0))
             sfract = s[0];
         else
             sfract = T(0);
         amdq[0] = sfract * wave[0];
         amdq[1] = sfract * wave[1];
-        T s3 = hur / hr + sqrt(gr * hr);
+        T s3 = hur * ihr + fsqrt(gr * hr);
         T hl2 = hr - wave[2], uhl2 = hur - wave[3];
-        T s2 = uhl2 / hl2 + sqrt(gr * hl2);
+        T s2 = fdiv(uhl2, hl2) + fsqrt(gr * hl2);
         if (s2 < T(0) && s3 > T(0))
-            sfract = s2 * (s3 - s[1]) / (s3 - s2);
+            sfract = s2 * fdiv(s3 - s[1], s3 - s2);
         else if (s[1] < T(0))
             sfract = s[1];
         else
@@ -322,15 +336,16 @@ __device__ __forceinline__ void rp_sw2(const PhysConst<T> &c, const T *lf, const
     constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
     const T gr = c.grav, h = T(0.5);
     T hh = (lf[0] + rt[0]) * h;
-    T hsqrtl = sqrt(lf[0]), hsqrtr = sqrt(rt[0]);
-    T hsq2 = hsqrtl + hsqrtr;
-    T u = (lf[mu] / hsqrtl + rt[mu] / hsqrtr) / hsq2;
-    T v = (lf[mv] / hsqrtl + rt[mv] / hsqrtr) / hsq2;
-    T a = sqrt(gr * hh);
+    T isl = frsq(lf[0]), isr = frsq(rt[0]);
+    T irs = frcp(lf[0] * isl + rt[0] * isr);
+    T u = (lf[mu] * isl + rt[mu] * isr) * irs;
+    T v = (lf[mv] * isl + rt[mv] * isr) * irs;
+    T ia = frsq(gr * hh);
+    T a = gr * hh * ia;
     T d1 = rt[0] - lf[0], d2 = rt[mu] - lf[mu], d3 = rt[mv] - lf[mv];
-    T a1 = ((u + a) * d1 - d2) * (h / a);
+    T a1 = ((u + a) * d1 - d2) * (h * ia);
     T a2 = -(v * d1) + d3;
-    T a3 = (-((u - a) * d1) + d2) * (h / a);
+    T a3 = (-((u - a) * d1) + d2) * (h * ia);
     wave[0] = a1;
     wave[mu] = a1 * (u - a);
     wave[mv] = a1 * v;
@@ -349,16 +364,16 @@ __device__ __forceinline__ void rp_sw2(const PhysConst<T> &c, const T *lf, const
     }
     do {
         T him1 = lf[0];
-        T s0 = lf[mu] / him1 - sqrt(gr * him1);
+        T s0 = fdiv(lf[mu], him1) - fsqrt(gr * him1);
         if (s0 > T(0) && s[0] > T(0)) {
             amdq[0] = amdq[1] = amdq[2] = T(0);
             break;
         }
         T h1 = lf[0] + wave[0], hu1 = lf[mu] + wave[mu];
-        T s1 = hu1 / h1 - sqrt(gr * h1);
+        T s1 = fdiv(hu1, h1) - fsqrt(gr * h1);
         T sfract;
         if (s0 < T(0) && s1 > T(0))
-            sfract = s0 * ((s1 - s[0]) / (s1 - s0));
+            sfract = s0 * fdiv(s1 - s[0], s1 - s0);
         else if (s[0] < T(0))
             sfract = s[0];
         else
@@ -369,11 +384,11 @@ __device__ __forceinline__ void rp_sw2(const PhysConst<T> &c, const T *lf, const
 #pragma unroll
         for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + s[1] * wave[3 + m];
         T hi = rt[0];
-        T s03 = rt[mu] / hi + sqrt(gr * hi);
+        T s03 = fdiv(rt[mu], hi) + fsqrt(gr * hi);
         T h3 = rt[0] - wave[6], hu3 = rt[mu] - wave[6 + mu];
-        T s3 = hu3 / h3 + sqrt(gr * h3);
+        T s3 = fdiv(hu3, h3) + fsqrt(gr * h3);
         if (s3 < T(0) && s03 > T(0))
-            sfract = s3 * ((s03 - s[2]) / (s03 - s3));
+            sfract = s3 * fdiv(s03 - s[2], s03 - s3);
         else if (s[2] < T(0))
             sfract = s[2];
         else
@@ -496,5 +511,181 @@ __device__ __forceinline__ void eigvec(const PhysConst<T> &c, const T *qa, const
         evl[2 * 3 + 0] = -((u - a) * z); evl[2 * 3 + mu] = z; evl[2 * 3 + mv] = T(0);
     }
 }
+
+// -------------------------------------- structured eigen-decompositions
+//
+// Eig<P,IXY,T> holds the Roe state of one interface and applies L (to_char)
+// and R (from_char) without forming the matrices: the eigenvectors of
+// eigvec() have zeros, ones and shared sub-expressions that a dense M x M
+// product would multiply out.  With b = gamma-1, q2 = |u|^2/2, K = b/c^2,
+// h2 = 1/(2c^2) and jumps (d0, dn, dt, dE) in (rho, normal, tangential, E):
+//   Y = q2 d0 - u dn - v dt + dE,  Z = u d0 - dn
+//   w(u-c) = h2 (bY + cZ),  w(entropy) = d0 - K Y,  w(shear) = dt - v d0,
+//   w(u+c) = h2 (bY - cZ)
+// which are the rows of L = R^-1 in eigvec() regrouped.
+
+template <int P, int IXY, class T> struct Eig;
+
+template <int IXY, class T> struct Eig<ADV1, IXY, T> {
+    __device__ __forceinline__ void init(const PhysConst<T> &, const T *, const T *) {}
+    __device__ __forceinline__ void to_char(const T *d, T *w) const { w[0] = d[0]; }
+    __device__ __forceinline__ void from_char(const T *c, T *o) const { o[0] = c[0]; }
+};
+
+template <int IXY, class T> struct Eig<AC1, IXY, T> {
+    T zz, z;
+    __device__ __forceinline__ void init(const PhysConst<T> &c, const T *, const T *)
+    {
+        zz = c.zz;
+        z = c.hzz;
+    }
+    __device__ __forceinline__ void to_char(const T *d, T *w) const
+    {
+        w[0] = z * (zz * d[1] - d[0]);
+        w[1] = z * (d[0] + zz * d[1]);
+    }
+    __device__ __forceinline__ void from_char(const T *c, T *o) const
+    {
+        o[0] = zz * (c[1] - c[0]);
+        o[1] = c[0] + c[1];
+    }
+};
+
+// Roe average of two cell states for Euler (u, v, H, c); v = 0 in 1D
+template <int MU, int MV, int ME, class T>
+__device__ __forceinline__ void roe_euler(const PhysConst<T> &c, const T *qa, const T *qb, T &u, T &v,
+                                          T &H, T &a2)
+{
+    const T g1 = c.gamma1, h = T(0.5);
+    T rl = qa[0], rr = qb[0];
+    T irl = frcp(rl), irr = frcp(rr);
+    T isl = frsq(rl), isr = frsq(rr);
+    T ul = qa[MU], ur = qb[MU];
+    T vl = MV >= 0 ? qa[MV] : T(0), vr = MV >= 0 ? qb[MV] : T(0);
+    T pl = g1 * (qa[ME] - h * (ul * ul + vl * vl) * irl);
+    T pr = g1 * (qb[ME] - h * (ur * ur + vr * vr) * irr);
+    T irs = frcp(rl * isl + rr * isr);
+    u = (ul * isl + ur * isr) * irs;
+    v = (vl * isl + vr * isr) * irs;
+    H = ((qa[ME] + pl) * isl + (qb[ME] + pr) * isr) * irs;
+    a2 = g1 * (H - h * (u * u + v * v));
+}
+
+template <int IXY, class T> struct Eig<EU1, IXY, T> {
+    T u, a, H, q2, b, h2, K;
+    __device__ __forceinline__ void init(const PhysConst<T> &c, const T *qa, const T *qb)
+    {
+        T v, a2;
+        roe_euler<1, -1, 2>(c, qa, qb, u, v, H, a2);
+        T ia = frsq(a2);
+        a = a2 * ia;
+        b = c.gamma1;
+        K = b * (ia * ia);
+        h2 = T(0.5) * (ia * ia);
+        q2 = T(0.5) * (u * u);
+    }
+    __device__ __forceinline__ void to_char(const T *d, T *w) const
+    {
+        T Y = q2 * d[0] - u * d[1] + d[2];
+        T Z = u * d[0] - d[1];
+        T bY = b * Y;
+        w[0] = h2 * (bY + a * Z);
+        w[1] = d[0] - K * Y;
+        w[2] = h2 * (bY - a * Z);
+    }
+    __device__ __forceinline__ void from_char(const T *c, T *o) const
+    {
+        T S = c[0] + c[1] + c[2], D = c[2] - c[0];
+        o[0] = S;
+        o[1] = u * S + a * D;
+        o[2] = H * (c[0] + c[2]) + q2 * c[1] + (u * a) * D;
+    }
+};
+
+template <int IXY, class T> struct Eig<EU2, IXY, T> {
+    static constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    T u, v, a, H, q2, b, h2, K;
+    __device__ __forceinline__ void init(const PhysConst<T> &c, const T *qa, const T *qb)
+    {
+        T a2;
+        roe_euler<mu, mv, 3>(c, qa, qb, u, v, H, a2);
+        T ia = frsq(a2);
+        a = a2 * ia;
+        b = c.gamma1;
+        K = b * (ia * ia);
+        h2 = T(0.5) * (ia * ia);
+        q2 = T(0.5) * (u * u + v * v);
+    }
+    __device__ __forceinline__ void to_char(const T *d, T *w) const
+    {
+        T Y = q2 * d[0] - u * d[mu] - v * d[mv] + d[3];
+        T Z = u * d[0] - d[mu];
+        T bY = b * Y;
+        w[0] = h2 * (bY + a * Z);
+        w[1] = d[0] - K * Y;
+        w[2] = d[mv] - v * d[0];
+        w[3] = h2 * (bY - a * Z);
+    }
+    __device__ __forceinline__ void from_char(const T *c, T *o) const
+    {
+        T S = c[0] + c[1] + c[3], D = c[3] - c[0];
+        o[0] = S;
+        o[mu] = u * S + a * D;
+        o[mv] = v * S + c[2];
+        o[3] = H * (c[0] + c[3]) + q2 * c[1] + v * c[2] + (u * a) * D;
+    }
+};
+
+template <int IXY, class T> struct Eig<SW1, IXY, T> {
+    T u, a, z;
+    __device__ __forceinline__ void init(const PhysConst<T> &c, const T *qa, const T *qb)
+    {
+        T isl = frsq(qa[0]), isr = frsq(qb[0]);
+        u = (qa[1] * isl + qb[1] * isr) * frcp(qa[0] * isl + qb[0] * isr);
+        T ia = frsq(T(0.5) * c.grav * (qa[0] + qb[0]));
+        a = T(0.5) * c.grav * (qa[0] + qb[0]) * ia;
+        z = T(0.5) * ia;
+    }
+    __device__ __forceinline__ void to_char(const T *d, T *w) const
+    {
+        w[0] = z * ((u + a) * d[0] - d[1]);
+        w[1] = z * (d[1] - (u - a) * d[0]);
+    }
+    __device__ __forceinline__ void from_char(const T *c, T *o) const
+    {
+        T S = c[0] + c[1];
+        o[0] = S;
+        o[1] = u * S + a * (c[1] - c[0]);
+    }
+};
+
+template <int IXY, class T> struct Eig<SW2, IXY, T> {
+    static constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    T u, v, a, z;
+    __device__ __forceinline__ void init(const PhysConst<T> &c, const T *qa, const T *qb)
+    {
+        T isl = frsq(qa[0]), isr = frsq(qb[0]);
+        T irs = frcp(qa[0] * isl + qb[0] * isr);
+        u = (qa[mu] * isl + qb[mu] * isr) * irs;
+        v = (qa[mv] * isl + qb[mv] * isr) * irs;
+        T gh = c.grav * ((qa[0] + qb[0]) * T(0.5));
+        T ia = frsq(gh);
+        a = gh * ia;
+        z = T(0.5) * ia;
+    }
+    __device__ __forceinline__ void to_char(const T *d, T *w) const
+    {
+        w[0] = z * ((u + a) * d[0] - d[mu]);
+        w[1] = d[mv] - v * d[0];
+        w[2] = z * (d[mu] - (u - a) * d[0]);
+    }
+    __device__ __forceinline__ void from_char(const T *c, T *o) const
+    {
+        T S = c[0] + c[2];
+        o[0] = S;
+        o[mu] = u * S + a * (c[2] - c[0]);
+        o[mv] = v * S + c[1];
+    }
+};
 
 }  // namespace sc

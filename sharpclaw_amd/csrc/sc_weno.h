@@ -32,7 +32,7 @@ __device__ __forceinline__ double weno_side(double im, double a2, double a1, dou
     double s1 = tt2 * tt3;
     double s2 = 6.0 * tt1 * tt3;
     double s3 = 3.0 * tt1 * tt2;
-    double t0 = 1.0 / (s1 + s2 + s3);
+    double t0 = frcp(s1 + s2 + s3);
     s1 = s1 * t0;
     s3 = s3 * t0;
     return (s1 * (t2 - t1) + (0.5 * s3 - 0.25) * (t3 - t2)) * (1.0 / 3.0);
@@ -52,7 +52,7 @@ __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, flo
 {
     float M = fmaxf(fmaxf(fabsf(a2), fabsf(a1)), fmaxf(fabsf(a0), fabsf(am1)));
     if (!(M > 0.0f)) return 0.0f;
-    float r = 1.0f / M;
+    float r = frcp(M);
     a2 *= r;
     a1 *= r;
     a0 *= r;
@@ -64,10 +64,10 @@ __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, flo
     float b1 = 13.0f * (t1 * t1) + 3.0f * dsq(a2 - 3.0f * a1);
     float b2 = 13.0f * (t2 * t2) + 3.0f * dsq(a1 + a0);
     float b3 = 13.0f * (t3 * t3) + 3.0f * dsq(3.0f * a0 - am1);
-    float al1 = 1.0f / dsq(eps + b1);
-    float al2 = 6.0f / dsq(eps + b2);
-    float al3 = 3.0f / dsq(eps + b3);
-    float rs = 1.0f / (al1 + al2 + al3);
+    float al1 = frcp(dsq(eps + b1));
+    float al2 = 6.0f * frcp(dsq(eps + b2));
+    float al3 = 3.0f * frcp(dsq(eps + b3));
+    float rs = frcp(al1 + al2 + al3);
     float w1 = al1 * rs, w3 = al3 * rs;
     return ((w1 * (t2 - t1) + (0.5f * w3 - 0.25f) * (t3 - t2)) * (1.0f / 3.0f)) * M;
 }
@@ -110,25 +110,31 @@ __device__ __forceinline__ void reconstruct_iface(const PhysConst<T> &c, const T
             rt[m] = weno_corr<T>(T(-1), d[4][m], d[3][m], d[2][m], d[1][m]) + rt[m];
         }
     } else {
-        T evl[M * M], evr[M * M];
-        eigvec<P, IXY>(c, v[2], v[3], evl, evr);
+        // characteristic-wise (reconstruct.f90:178-237): project the five
+        // jumps with L of this interface, reconstruct each field on both
+        // sides, return with R
+        Eig<P, IXY, T> e;
+        e.init(c, v[2], v[3]);
+        T hh[M][5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            T w[M];
+            e.to_char(d[k], w);
+#pragma unroll
+            for (int p = 0; p < M; p++) hh[p][k] = w[p];
+        }
+        T cl[M], cr[M], ol[M], orr[M];
 #pragma unroll
         for (int p = 0; p < M; p++) {
-            T hh[5];
+            cl[p] = weno_corr<T>(T(1), hh[p][0], hh[p][1], hh[p][2], hh[p][3]);
+            cr[p] = weno_corr<T>(T(-1), hh[p][4], hh[p][3], hh[p][2], hh[p][1]);
+        }
+        e.from_char(cl, ol);
+        e.from_char(cr, orr);
 #pragma unroll
-            for (int k = 0; k < 5; k++) {
-                T acc = T(0);
-#pragma unroll
-                for (int m = 0; m < M; m++) acc = acc + evl[p * M + m] * d[k][m];
-                hh[k] = acc;
-            }
-            T u1 = weno_corr<T>(T(1), hh[0], hh[1], hh[2], hh[3]);
-            T u2 = weno_corr<T>(T(-1), hh[4], hh[3], hh[2], hh[1]);
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                lf[m] = lf[m] + evr[m * M + p] * u1;
-                rt[m] = rt[m] + evr[m * M + p] * u2;
-            }
+        for (int m = 0; m < M; m++) {
+            lf[m] = lf[m] + ol[m];
+            rt[m] = rt[m] + orr[m];
         }
     }
 }

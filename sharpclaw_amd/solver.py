@@ -98,6 +98,15 @@ class Solver:
         check(lib().sc_sums(self._h, out))
         return out
 
+    def set_tile(self, tile):
+        """2D tile shape: -1 autotune at first use, 0..3 fixed."""
+        check(lib().sc_set_tile(self._h, tile))
+
+    def get_tile(self):
+        k, tx, ty = C.c_int(), C.c_int(), C.c_int()
+        check(lib().sc_get_tile(self._h, C.byref(k), C.byref(tx), C.byref(ty)))
+        return k.value, (tx.value, ty.value)
+
     def time_stage(self, n, dt):
         ms = C.c_double()
         check(lib().sc_time_stage(self._h, n, dt, C.byref(ms)))

@@ -343,3 +343,30 @@ def test_errors_are_reported():
     p = make_params("euler", (10,), mthbc=(2, 1, 1, 1))
     with pytest.raises(ScError):
         Solver(p)
+
+
+def test_tile_shapes_identical_and_autotune():
+    """Every 2D tile shape gives bit-identical stages; the autotuner picks one."""
+    rng = np.random.default_rng(19)
+    for name, char in (("euler", 2), ("shallow_water", 0)):
+        p = make_params(name, (150, 133), char_decomp=char, time_integrator=3, mthbc=(1, 3, 2, 2))
+        q = _states.grid(p, rng, smooth=True, amp=0.5)
+        outs = []
+        for tile in (0, 1, 2, 3):
+            with Solver(p) as s:
+                s.set_tile(tile)
+                s.set_q(q)
+                s.save_qold()
+                c = s.ts(0.05 * p.dx[0])
+                outs.append((s.get_q(), c))
+        for g, c in outs[1:]:
+            assert np.array_equal(g, outs[0][0]) and c == outs[0][1]
+        rq, rc, _ = O.ts(p, q, 0.05 * p.dx[0])
+        assert rel_linf(interior(p, outs[0][0]), interior(p, rq)) <= TOL
+    p = make_params("euler", (256, 256), char_decomp=2)
+    with Solver(p) as s:
+        s.set_q(_states.grid(p, rng, smooth=True, amp=0.5))
+        s.save_qold()
+        s.ts(1e-5)
+        k, (tx, ty) = s.get_tile()
+        assert 0 <= k <= 3 and tx * ty <= 256
