@@ -114,6 +114,109 @@ def cpu_baseline(n, st, steps):
                       f"{wall:.2f} s incl. input/qinit/output; {what}"}
 
 
+def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
+    """The batched workload: nsolves 1D shallow-water dam breaks of uneven
+    size (sharpclaw_amd.workloads.dambreak_batch) to t = 0.2, float32
+    characteristic WENO5 + Roe, SSPRK3, one persistent launch.  Beside it:
+    the float64 single-solve path (Solver.advance per grid) on a stratified
+    sample, and the reference Fortran on a stratified sample over the host's
+    cores."""
+    from sharpclaw_amd import SC_FP32, SC_FP64, Batch, ScRunCtl, Solver, make_params
+    from sharpclaw_amd.workloads import dambreak_batch, dambreak_q
+    mx, hl, hr, x0 = dambreak_batch(nsolves, seed)
+    xl, xu = np.zeros(nsolves), np.ones(nsolves)
+    tend, dt0 = 0.2, 1e-5
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=1000000, dt_variable=1)
+    out = {"workload": f"{nsolves} 1D shallow-water dam breaks, mx log-uniform [64,4096] "
+                       f"(sum {int(mx.sum())} cells), char WENO5 + Roe (efix), SSPRK3, to t=0.2",
+           "seed": seed}
+    for prec, name in ((SC_FP32, "fp32"), (SC_FP64, "fp64")):
+        p = make_params("shallow_water", (8,), char_decomp=2, time_integrator=3, precision=prec)
+        with Batch(p, mx, xl, xu) as b:
+            for i in range(nsolves):
+                b.set_q(i, dambreak_q(int(mx[i]), hl[i], hr[i], x0[i]))
+            t0 = time.perf_counter()
+            res, ms = b.run(ctl, 0.0, tend, dt0)
+            first = time.perf_counter() - t0
+            for i in range(nsolves):  # second run from the same data: tuner already done
+                b.set_q(i, dambreak_q(int(mx[i]), hl[i], hr[i], x0[i]))
+            res, ms = b.run(ctl, 0.0, tend, dt0)
+            block = b.block
+        bad = [i for i, r in enumerate(res) if r.info != 0 or abs(r.t - tend) > 1e-12]
+        if bad:
+            raise RuntimeError(f"batched solves {bad[:5]} did not reach t={tend}")
+        cs = float(sum(int(mx[i]) * r.stage_evals for i, r in enumerate(res)))
+        done = np.array([r.done_ms for r in res])
+        out[name] = {"cell_stages": cs, "launch_ms": ms, "value": cs / (ms * 1e-3),
+                     "unit": "cell-updates/s per RK stage",
+                     "latency_ms": {"p50": float(np.percentile(done, 50)),
+                                    "p99": float(np.percentile(done, 99)),
+                                    "max": float(done.max())},
+                     "steps_total": int(sum(r.steps_taken for r in res)),
+                     "rejected_total": int(sum(r.rejected for r in res)),
+                     "block": block, "first_run_wall_s_incl_tuning": first}
+        log(f"bench: batch {name}: {ms:.1f} ms, {cs / (ms * 1e-3):.3e} cell-stage/s, "
+            f"p50 {out[name]['latency_ms']['p50']:.1f} ms p99 {out[name]['latency_ms']['p99']:.1f} ms, "
+            f"block {block}")
+    # float64 single-solve path on a stratified sample (quantiles of mx)
+    order = np.argsort(mx)
+    pick = [int(order[int(round(f * (nsolves - 1)))]) for f in np.linspace(0.05, 0.95, single_sample)]
+    cs1 = ms1 = 0.0
+    for i in pick:
+        p = make_params("shallow_water", (int(mx[i]),), char_decomp=2, time_integrator=3)
+        with Solver(p) as s:
+            s.set_q(dambreak_q(int(mx[i]), hl[i], hr[i], x0[i]))
+            _, st = s.advance(ctl, 0.0, tend, dt0)
+        cs1 += int(mx[i]) * st.stage_evals
+        ms1 += st.gpu_ms
+    single = cs1 / (ms1 * 1e-3)
+    out["fp64_single_solve_path"] = {"value": single, "unit": "cell-updates/s per RK stage",
+                                     "sample": f"{len(pick)} solves at mx quantiles 5..95%: "
+                                               f"{sorted(int(mx[i]) for i in pick)}",
+                                     "gpu_ms": ms1}
+    out["fp32_batched_vs_fp64_single"] = out["fp32"]["value"] / single
+    log(f"bench: fp64 single-solve path {single:.3e} cell-stage/s; batched fp32 is "
+        f"{out['fp32_batched_vs_fp64_single']:.1f}x")
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(mx, hl, hr, x0, order, cpu_sample)
+    return out
+
+
+def cpu_baseline_batch(mx, hl, hr, x0, order, k):
+    """Reference Fortran (flang -O3), one serial process per solve over the
+    host cores, on k solves at mx quantiles 2..90% (bounded time)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import refrun
+    from sharpclaw_amd import make_params
+    if not refrun.ref_available("sw1d"):
+        return None
+    n = len(mx)
+    pick = [int(order[int(round(f * (n - 1)))]) for f in np.linspace(0.02, 0.9, k)]
+    cores = min(16, len(os.sched_getaffinity(0)))
+
+    def one(i):
+        p = make_params("shallow_water", (int(mx[i]),), char_decomp=2, time_integrator=3)
+        cd = refrun.clawdata_for(p, nout=1, outstyle=1, tfinal=0.2, dt_initial=1e-5, cfl_max=0.5,
+                                 cfl_desired=0.45, max_steps=1000000)
+        t0 = time.perf_counter()
+        _, dtlog, _ = refrun.run(p, cd, ["3", f"{float(hl[i])!r} {float(hr[i])!r} {float(x0[i])!r}"])
+        return time.perf_counter() - t0, int(mx[i]) * 3 * len(dtlog)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        rs = list(ex.map(one, pick))
+    wall = time.perf_counter() - t0
+    cs = sum(c for _, c in rs)
+    per_core = cs / sum(w for w, _ in rs)
+    total = float(np.sum(9.0 * np.asarray(mx, dtype=np.float64) ** 2))
+    return {"value": cs / wall, "unit": "cell-updates/s per RK stage", "cores": cores,
+            "kind": "reference", "per_core_value": per_core,
+            "full_batch_estimate_s": total / (per_core * cores),
+            "sample": f"{k} dam breaks at mx quantiles 2..90% {sorted(int(mx[i]) for i in pick)}, "
+                      f"one serial oracle/_ref/xsclaw_sw1d process each (flang -O3) over {cores} "
+                      f"threads, {wall:.1f} s wall"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +227,8 @@ def main():
     ap.add_argument("--fp32", action="store_true", help="float32 reconstruction + Riemann")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched dam-break workload")
+    ap.add_argument("--batch", type=int, default=4096, help="solves in the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -212,6 +317,9 @@ def main():
         log(f"bench: cpu baseline {out['cpu_baseline']['value']:.3e} cell-stage/s")
     else:
         out["cpu_baseline"] = None
+    if not args.no_batch:
+        log(f"bench: batched workload ({args.batch} dam breaks)")
+        out["batched"] = batched_dambreaks(args.batch, args.seed, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -193,6 +193,43 @@ int sc_weno5_char(const sc_params *params, int mx2, const double *q, const doubl
 int sc_flux1(const sc_params *params, int ixy, int mx, const double *q1d, double *dq1d,
              double dt, double *cfl);
 
+/* ---- batched solves: many independent 1D grids in one persistent launch --
+ * No counterpart in the reference, which solves one grid per process; each
+ * solve follows the reference's time loop (sharpclaw.f90:35-235) exactly as
+ * sc_advance does, with all its control decisions taken on the device. */
+
+typedef struct sc_batch sc_batch;
+
+/* Per-solve outcome of sc_batch_run (the fields of sc_run_stats plus timing). */
+typedef struct sc_batch_result {
+    double t;
+    double cfl_maxused, cfl_last;
+    double dt_minused, dt_maxused, dt_last;
+    int steps_taken, rejected, info;
+    int buffer;               /* device buffer holding the final q (internal) */
+    long long stage_evals;
+    long long t_begin, t_end; /* device wall-clock ticks */
+    double done_ms;           /* completion time after the launch began, ms */
+} sc_batch_result;
+
+/* nsolves 1D grids sharing params (physics, reconstruction, precision,
+ * integrator, boundary options); solve i has mx[i] cells on
+ * [xlower[i], xupper[i]]. */
+int sc_batch_create(const sc_params *params, int nsolves, const int *mx, const double *xlower,
+                    const double *xupper, sc_batch **out);
+int sc_batch_destroy(sc_batch *b);
+/* q(1-mbc:mx+mbc, meqn) of solve i */
+int sc_batch_set_q(sc_batch *b, int i, const double *q);
+int sc_batch_get_q(sc_batch *b, int i, double *q);
+/* Advance every solve from tstart to tend (dt_initial each) in one launch;
+ * results[i] for solve i; gpu_ms: HIP-event time of the launch.  The first
+ * run of a batch also times workgroup sizes 64/128/256 on one step of every
+ * solve (states untouched) and keeps the fastest. */
+int sc_batch_run(sc_batch *b, const sc_run_ctl *ctl, double tstart, double tend, double dt_initial,
+                 sc_batch_result *results, double *gpu_ms);
+/* workgroup size chosen by the tuner (0 before the first run) */
+int sc_batch_block(sc_batch *b, int *block);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,28 @@ class ScRunStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ScBatchResult(C.Structure):
+    _fields_ = [
+        ("t", C.c_double),
+        ("cfl_maxused", C.c_double),
+        ("cfl_last", C.c_double),
+        ("dt_minused", C.c_double),
+        ("dt_maxused", C.c_double),
+        ("dt_last", C.c_double),
+        ("steps_taken", C.c_int),
+        ("rejected", C.c_int),
+        ("info", C.c_int),
+        ("buffer", C.c_int),
+        ("stage_evals", C.c_longlong),
+        ("t_begin", C.c_longlong),
+        ("t_end", C.c_longlong),
+        ("done_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # meqn / mwaves of each device physics by dimension
 SHAPES = {
     (SC_PHYS_ADVECTION, 1): (1, 1),

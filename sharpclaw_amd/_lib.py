@@ -6,7 +6,7 @@ import os
 
 import numpy as np
 
-from ._abi import ScParams, ScRunCtl, ScRunStats
+from ._abi import ScBatchResult, ScParams, ScRunCtl, ScRunStats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsharpclaw_amd.so")
@@ -54,6 +54,13 @@ def lib():
         "sc_weno5": ([P, C.c_int, dp, dp, dp], C.c_int),
         "sc_weno5_char": ([P, C.c_int, dp, dp, dp, dp, dp], C.c_int),
         "sc_flux1": ([P, C.c_int, C.c_int, dp, dp, C.c_double, dref], C.c_int),
+        "sc_batch_create": ([P, C.c_int, C.POINTER(C.c_int), dref, dref, C.POINTER(H)], C.c_int),
+        "sc_batch_destroy": ([H], C.c_int),
+        "sc_batch_set_q": ([H, C.c_int, dp], C.c_int),
+        "sc_batch_get_q": ([H, C.c_int, dp], C.c_int),
+        "sc_batch_run": ([H, C.POINTER(ScRunCtl), C.c_double, C.c_double, C.c_double,
+                          C.POINTER(ScBatchResult), dref], C.c_int),
+        "sc_batch_block": ([H, C.POINTER(C.c_int)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

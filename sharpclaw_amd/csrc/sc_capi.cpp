@@ -19,13 +19,14 @@
 #include <tuple>
 
 #include "../../include/sharpclaw_amd.h"
+#include "sc_internal.h"
 #include "sc_kernels.h"
 
 using namespace sc;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...)
+int sc_fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -40,7 +41,7 @@ static int fail(int code, const char *fmt, ...)
     do {                                                                                     \
         hipError_t e_ = (call);                                                              \
         if (e_ != hipSuccess)                                                                \
-            return fail(SC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
+            return sc_fail(SC_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));          \
     } while (0)
 
 extern "C" const char *sc_last_error(void) { return g_err.c_str(); }
@@ -48,7 +49,7 @@ extern "C" int sc_version(void) { return 1; }
 
 // --------------------------------------------------------- validation
 
-static int phys_id(const sc_params *p)
+int sc_phys_id(const sc_params *p)
 {
     switch (p->physics) {
     case SC_PHYS_ADVECTION: return p->ndim == 1 ? ADV1 : -1;
@@ -72,66 +73,44 @@ static int shape_w(int id)
 
 // Options the device path implements; everything else is rejected with the
 // reason, as the reference stops on an option it does not know.
-static int check_params(const sc_params *p, bool grid)
+int sc_check_params(const sc_params *p, bool grid)
 {
-    if (!p) return fail(SC_ERR_ARG, "params is NULL");
-    if (p->ndim != 1 && p->ndim != 2) return fail(SC_ERR_ARG, "ndim must be 1 or 2 (got %d)", p->ndim);
-    int id = phys_id(p);
-    if (id < 0) return fail(SC_ERR_UNSUPPORTED, "physics %d has no %dD device solver", p->physics, p->ndim);
+    if (!p) return sc_fail(SC_ERR_ARG, "params is NULL");
+    if (p->ndim != 1 && p->ndim != 2) return sc_fail(SC_ERR_ARG, "ndim must be 1 or 2 (got %d)", p->ndim);
+    int id = sc_phys_id(p);
+    if (id < 0) return sc_fail(SC_ERR_UNSUPPORTED, "physics %d has no %dD device solver", p->physics, p->ndim);
     if (p->meqn != shape_m(id) || p->mwaves != shape_w(id))
-        return fail(SC_ERR_ARG, "physics %d in %dD needs meqn=%d mwaves=%d (got %d, %d)", p->physics,
+        return sc_fail(SC_ERR_ARG, "physics %d in %dD needs meqn=%d mwaves=%d (got %d, %d)", p->physics,
                     p->ndim, shape_m(id), shape_w(id), p->meqn, p->mwaves);
-    if (p->mbc != 3) return fail(SC_ERR_ARG, "mbc must be 3 for WENO5 (reconstruct.f90:85)");
-    if (p->maux != 0) return fail(SC_ERR_UNSUPPORTED, "maux > 0 is not supported on the device path");
-    if (p->mcapa != 0) return fail(SC_ERR_UNSUPPORTED, "mcapa > 0 is not supported on the device path");
-    if (p->lim_type != 2) return fail(SC_ERR_UNSUPPORTED, "lim_type %d: only 2 (WENO5) on the device path", p->lim_type);
+    if (p->mbc != 3) return sc_fail(SC_ERR_ARG, "mbc must be 3 for WENO5 (reconstruct.f90:85)");
+    if (p->maux != 0) return sc_fail(SC_ERR_UNSUPPORTED, "maux > 0 is not supported on the device path");
+    if (p->mcapa != 0) return sc_fail(SC_ERR_UNSUPPORTED, "mcapa > 0 is not supported on the device path");
+    if (p->lim_type != 2) return sc_fail(SC_ERR_UNSUPPORTED, "lim_type %d: only 2 (WENO5) on the device path", p->lim_type);
     if (p->char_decomp != 0 && p->char_decomp != 2)
-        return fail(SC_ERR_UNSUPPORTED, "char_decomp %d: device path implements 0 and 2", p->char_decomp);
-    if (p->tfluct_solver != 0) return fail(SC_ERR_UNSUPPORTED, "tfluct_solver 1 is not supported on the device path");
-    if (p->src_term != 0) return fail(SC_ERR_UNSUPPORTED, "src_term != 0 is not supported on the device path");
-    if (p->precision != SC_FP64 && p->precision != SC_FP32) return fail(SC_ERR_ARG, "precision must be SC_FP64 or SC_FP32");
+        return sc_fail(SC_ERR_UNSUPPORTED, "char_decomp %d: device path implements 0 and 2", p->char_decomp);
+    if (p->tfluct_solver != 0) return sc_fail(SC_ERR_UNSUPPORTED, "tfluct_solver 1 is not supported on the device path");
+    if (p->src_term != 0) return sc_fail(SC_ERR_UNSUPPORTED, "src_term != 0 is not supported on the device path");
+    if (p->precision != SC_FP64 && p->precision != SC_FP32) return sc_fail(SC_ERR_ARG, "precision must be SC_FP64 or SC_FP32");
     if (!grid) return SC_OK;
     if (p->time_integrator < 1 || p->time_integrator > 4)
-        return fail(SC_ERR_ARG, "time_integrator should be between 1 and 4 (sharpclaw.f90:172-174)");
+        return sc_fail(SC_ERR_ARG, "time_integrator should be between 1 and 4 (sharpclaw.f90:172-174)");
     for (int d = 0; d < p->ndim; d++) {
-        if (p->nx[d] < 1) return fail(SC_ERR_ARG, "nx[%d] = %d", d, p->nx[d]);
-        if (!(p->dx[d] > 0.0)) return fail(SC_ERR_ARG, "dx[%d] must be positive", d);
+        if (p->nx[d] < 1) return sc_fail(SC_ERR_ARG, "nx[%d] = %d", d, p->nx[d]);
+        if (!(p->dx[d] > 0.0)) return sc_fail(SC_ERR_ARG, "dx[%d] must be positive", d);
         int bl = p->mthbc[2 * d], br = p->mthbc[2 * d + 1];
         if (bl == 0 || br == 0)
-            return fail(SC_ERR_UNSUPPORTED, "mthbc=0 (user boundary conditions) has no device implementation");
-        if (bl < 1 || bl > 3 || br < 1 || br > 3) return fail(SC_ERR_ARG, "mthbc must be 1, 2 or 3");
+            return sc_fail(SC_ERR_UNSUPPORTED, "mthbc=0 (user boundary conditions) has no device implementation");
+        if (bl < 1 || bl > 3 || br < 1 || br > 3) return sc_fail(SC_ERR_ARG, "mthbc must be 1, 2 or 3");
         if ((bl == 2) != (br == 2))
-            return fail(SC_ERR_ARG, "periodic boundary conditions require mthbc(%d) and mthbc(%d) both be 2 (main.f90:140-145)",
+            return sc_fail(SC_ERR_ARG, "periodic boundary conditions require mthbc(%d) and mthbc(%d) both be 2 (main.f90:140-145)",
                         2 * d + 1, 2 * d + 2);
         if ((bl == 2 || bl == 3 || br == 3) && p->nx[d] < 3)
-            return fail(SC_ERR_ARG, "periodic/wall boundaries need at least 3 cells");
+            return sc_fail(SC_ERR_ARG, "periodic/wall boundaries need at least 3 cells");
         int mnorm = d == 0 ? 2 : 3;
         if ((bl == 3 || br == 3) && p->meqn < mnorm)
-            return fail(SC_ERR_ARG, "a wall negates component %d, which meqn=%d does not have", mnorm, p->meqn);
+            return sc_fail(SC_ERR_ARG, "a wall negates component %d, which meqn=%d does not have", mnorm, p->meqn);
     }
     return SC_OK;
-}
-
-template <class T> static PhysConst<T> mkconst(const sc_params *p)
-{
-    PhysConst<T> c;
-    c.gamma = T(p->gamma);
-    c.gamma1 = T(p->gamma - 1.0);
-    c.grav = T(p->grav);
-    c.u = T(p->u);
-    double cc = std::sqrt(p->bulk / p->rho);
-    c.cc = T(cc);
-    c.zz = T(cc * p->rho);
-    c.hzz = T(0.5 / (cc * p->rho));
-    if (p->efix >= 0)
-        c.efix = p->efix;
-    else if (p->physics == SC_PHYS_EULER)
-        c.efix = 1;  // rp1eu.f:40
-    else if (p->physics == SC_PHYS_SHALLOW_WATER)
-        c.efix = p->ndim == 1;  // rp1sw.f:40 on, rpn2sw.f:48 off
-    else
-        c.efix = 0;
-    return c;
 }
 
 // ------------------------------------------------------------- handle
@@ -176,15 +155,15 @@ static size_t grid_n(const sc_params *p)
 
 extern "C" int sc_create(const sc_params *params, sc_solver **out)
 {
-    if (!out) return fail(SC_ERR_ARG, "out is NULL");
+    if (!out) return sc_fail(SC_ERR_ARG, "out is NULL");
     *out = nullptr;
-    int rc = check_params(params, true);
+    int rc = sc_check_params(params, true);
     if (rc) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return fail(SC_ERR_HIP, "no HIP device available");
+        return sc_fail(SC_ERR_HIP, "no HIP device available");
     if (params->device < 0 || params->device >= ndev)
-        return fail(SC_ERR_ARG, "device %d out of range (%d devices)", params->device, ndev);
+        return sc_fail(SC_ERR_ARG, "device %d out of range (%d devices)", params->device, ndev);
     DevGuard g(params->device);
     sc_solver *s = new sc_solver();
     memset((void *)s, 0, sizeof(*s));
@@ -193,11 +172,11 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
         s->p.nx[1] = 1;
         s->p.dx[1] = 1.0;
     }
-    s->sel.phys = phys_id(params);
+    s->sel.phys = sc_phys_id(params);
     s->sel.chr = params->char_decomp;
     s->sel.fp32 = params->precision == SC_FP32;
-    s->cd = mkconst<double>(params);
-    s->cf = mkconst<float>(params);
+    s->cd = sc_make_const<double>(params);
+    s->cf = sc_make_const<float>(params);
     s->dev = params->device;
     s->n = grid_n(&s->p);
     s->nblk_sums = 256;
@@ -220,7 +199,7 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     }
     if (!ok) {
         sc_destroy(s);
-        return fail(SC_ERR_NOMEM, "device allocation of %zu doubles x 5 failed", grid_n(params));
+        return sc_fail(SC_ERR_NOMEM, "device allocation of %zu doubles x 5 failed", grid_n(params));
     }
     s->cur = 0;
     s->old = 0;
@@ -261,7 +240,7 @@ static int free_buf(const sc_solver *s, int a, int b)
 
 extern "C" int sc_set_q(sc_solver *s, const double *q)
 {
-    if (!s || !q) return fail(SC_ERR_ARG, "NULL argument");
+    if (!s || !q) return sc_fail(SC_ERR_ARG, "NULL argument");
     DevGuard g(s->dev);
     if (s->cur == s->old) s->cur = free_buf(s, s->old, s->old);  // keep qold intact
     HIPCHK(hipMemcpyAsync(s->buf[s->cur], q, s->n * sizeof(double), hipMemcpyHostToDevice, s->st));
@@ -273,8 +252,8 @@ extern "C" int sc_set_q(sc_solver *s, const double *q)
 // ghosts of the current q hold the boundary fill of its interior
 extern "C" int sc_get_q(sc_solver *s, double *q)
 {
-    if (!s || !q) return fail(SC_ERR_ARG, "NULL argument");
-    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    if (!s || !q) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
     if (s->cur == s->old) {
         // q and qold share a buffer: fill ghosts of that buffer (harmless for qold)
@@ -288,14 +267,14 @@ extern "C" int sc_get_q(sc_solver *s, double *q)
 
 extern "C" int sc_save_qold(sc_solver *s)
 {
-    if (!s) return fail(SC_ERR_ARG, "NULL solver");
+    if (!s) return sc_fail(SC_ERR_ARG, "NULL solver");
     s->old = s->cur;
     return SC_OK;
 }
 
 extern "C" int sc_restore_qold(sc_solver *s)
 {
-    if (!s) return fail(SC_ERR_ARG, "NULL solver");
+    if (!s) return sc_fail(SC_ERR_ARG, "NULL solver");
     s->cur = s->old;
     return SC_OK;
 }
@@ -371,15 +350,15 @@ static int tune_tile(sc_solver *s)
 
 extern "C" int sc_set_tile(sc_solver *s, int tile)
 {
-    if (!s) return fail(SC_ERR_ARG, "NULL solver");
-    if (tile < -1 || tile >= NTILE2D) return fail(SC_ERR_ARG, "tile must be -1..%d", NTILE2D - 1);
+    if (!s) return sc_fail(SC_ERR_ARG, "NULL solver");
+    if (tile < -1 || tile >= NTILE2D) return sc_fail(SC_ERR_ARG, "tile must be -1..%d", NTILE2D - 1);
     s->tile2d = tile;
     return SC_OK;
 }
 
 extern "C" int sc_get_tile(sc_solver *s, int *tile, int *tx, int *ty)
 {
-    if (!s || !tile) return fail(SC_ERR_ARG, "NULL argument");
+    if (!s || !tile) return sc_fail(SC_ERR_ARG, "NULL argument");
     *tile = s->tile2d;
     int a = 0, b = 0;
     tile2d_shape(s->tile2d < 0 ? 0 : s->tile2d, &a, &b);
@@ -416,8 +395,8 @@ static int read_cfl(sc_solver *s, unsigned long long *w, double dt, double *cfl)
 extern "C" int sc_step(sc_solver *s, double dt, double t, double *dq, double *cfl)
 {
     (void)t;
-    if (!s || !cfl) return fail(SC_ERR_ARG, "NULL argument");
-    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    if (!s || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
     StageArgs a = base_args(s);
     double *q = s->buf[s->cur];
@@ -487,8 +466,8 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
 extern "C" int sc_ts(sc_solver *s, double dt, double told, double *cfl)
 {
     (void)told;
-    if (!s || !cfl) return fail(SC_ERR_ARG, "NULL argument");
-    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    if (!s || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
     int rc = tune_tile(s);
     if (rc) return rc;
@@ -508,8 +487,8 @@ static int stages_of(int ti)
 extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, double tend, double *dt_io,
                           sc_run_stats *st)
 {
-    if (!s || !c || !dt_io || !st) return fail(SC_ERR_ARG, "NULL argument");
-    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    if (!s || !c || !dt_io || !st) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
     int trc = tune_tile(s);
     if (trc) return trc;
@@ -595,8 +574,8 @@ done:
 
 extern "C" int sc_sums(sc_solver *s, double *sums)
 {
-    if (!s || !sums) return fail(SC_ERR_ARG, "NULL argument");
-    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    if (!s || !sums) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
     StageArgs a = base_args(s);
     HIPCHK(launch_sums(s->p.ndim, s->p.meqn, s->buf[s->cur], a.nx, a.ny, a.LX, a.LY, s->partial,
@@ -611,8 +590,8 @@ extern "C" int sc_sums(sc_solver *s, double *sums)
 
 extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stage)
 {
-    if (!s || !ms_per_stage || n < 1) return fail(SC_ERR_ARG, "bad argument");
-    if (!s->have_q) return fail(SC_ERR_STATE, "no q uploaded");
+    if (!s || !ms_per_stage || n < 1) return sc_fail(SC_ERR_ARG, "bad argument");
+    if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     DevGuard g(s->dev);
     int trc = tune_tile(s);
     if (trc) return trc;
@@ -656,10 +635,10 @@ extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stag
 
 extern "C" int sc_copy_bench(int device, size_t nbytes, int reps, double *gbps)
 {
-    if (!gbps || reps < 1 || nbytes < 16) return fail(SC_ERR_ARG, "bad argument");
+    if (!gbps || reps < 1 || nbytes < 16) return sc_fail(SC_ERR_ARG, "bad argument");
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SC_ERR_HIP, "no HIP device available");
-    if (device < 0 || device >= ndev) return fail(SC_ERR_ARG, "device out of range");
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sc_fail(SC_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return sc_fail(SC_ERR_ARG, "device out of range");
     DevGuard g(device);
     size_t n = nbytes / sizeof(double);
     double *a = nullptr, *b = nullptr;
@@ -667,7 +646,7 @@ extern "C" int sc_copy_bench(int device, size_t nbytes, int reps, double *gbps)
     HIPCHK(hipMalloc(&a, n * sizeof(double)));
     if (hipMalloc(&b, n * sizeof(double)) != hipSuccess) {
         hipFree(a);
-        return fail(SC_ERR_NOMEM, "copy bench allocation failed");
+        return sc_fail(SC_ERR_NOMEM, "copy bench allocation failed");
     }
     hipError_t e = hipMemset(a, 0, n * sizeof(double));
     if (e == hipSuccess) e = hipEventCreate(&e0);
@@ -683,7 +662,7 @@ extern "C" int sc_copy_bench(int device, size_t nbytes, int reps, double *gbps)
     hipEventDestroy(e1);
     hipFree(a);
     hipFree(b);
-    if (e != hipSuccess) return fail(SC_ERR_HIP, "copy bench: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "copy bench: %s", hipGetErrorString(e));
     *gbps = 2.0 * (double)(n * sizeof(double)) * reps / (ms * 1e-3) / 1e9;
     return SC_OK;
 }
@@ -700,11 +679,11 @@ struct DevBuf {
 
 static int one_call_setup(const sc_params *params)
 {
-    int rc = check_params(params, false);
+    int rc = sc_check_params(params, false);
     if (rc) return rc;
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SC_ERR_HIP, "no HIP device available");
-    if (params->device < 0 || params->device >= ndev) return fail(SC_ERR_ARG, "device out of range");
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sc_fail(SC_ERR_HIP, "no HIP device available");
+    if (params->device < 0 || params->device >= ndev) return sc_fail(SC_ERR_ARG, "device out of range");
     return SC_OK;
 }
 
@@ -717,10 +696,10 @@ extern "C" int sc_rp(const sc_params *params, int ixy, int mx, const double *ql,
     int rc = one_call_setup(params);
     if (rc) return rc;
     if (mx < 1 || (ixy != 1 && ixy != 2) || !ql || !qr || !wave || !s || !amdq || !apdq)
-        return fail(SC_ERR_ARG, "bad argument");
+        return sc_fail(SC_ERR_ARG, "bad argument");
     DevGuard g(params->device);
     const int M = params->meqn, W = params->mwaves, L = mx + 2 * params->mbc;
-    PhysSel sel{phys_id(params), 0, params->precision == SC_FP32};
+    PhysSel sel{sc_phys_id(params), 0, params->precision == SC_FP32};
     DevBuf dql, dqr, dw, ds, dam, dap;
     HIPCHK(dql.alloc((size_t)L * M)); HIPCHK(dqr.alloc((size_t)L * M));
     HIPCHK(dw.alloc((size_t)L * M * W)); HIPCHK(ds.alloc((size_t)L * W));
@@ -733,7 +712,7 @@ extern "C" int sc_rp(const sc_params *params, int ixy, int mx, const double *ql,
     UP(dam, amdq, (size_t)L * M);
     UP(dap, apdq, (size_t)L * M);
     HIPCHK(launch_rp(sel, ixy, mx, params->mbc, dql.p, dqr.p, dw.p, ds.p, dam.p, dap.p,
-                     mkconst<double>(params), mkconst<float>(params), 0));
+                     sc_make_const<double>(params), sc_make_const<float>(params), 0));
     HIPCHK(hipDeviceSynchronize());
     DOWN(wave, dw, (size_t)L * M * W);
     DOWN(s, ds, (size_t)L * W);
@@ -747,17 +726,17 @@ extern "C" int sc_evec(const sc_params *params, int ixy, int mx, const double *q
 {
     int rc = one_call_setup(params);
     if (rc) return rc;
-    if (mx < 1 || (ixy != 1 && ixy != 2) || !q || !evl || !evr) return fail(SC_ERR_ARG, "bad argument");
+    if (mx < 1 || (ixy != 1 && ixy != 2) || !q || !evl || !evr) return sc_fail(SC_ERR_ARG, "bad argument");
     DevGuard g(params->device);
     const int M = params->meqn, L = mx + 2 * params->mbc;
-    PhysSel sel{phys_id(params), 2, params->precision == SC_FP32};
+    PhysSel sel{sc_phys_id(params), 2, params->precision == SC_FP32};
     DevBuf dq, dl, dr;
     HIPCHK(dq.alloc((size_t)L * M)); HIPCHK(dl.alloc((size_t)L * M * M)); HIPCHK(dr.alloc((size_t)L * M * M));
     UP(dq, q, (size_t)L * M);
     UP(dl, evl, (size_t)L * M * M);
     UP(dr, evr, (size_t)L * M * M);
-    HIPCHK(launch_evec(sel, ixy, mx, params->mbc, dq.p, dl.p, dr.p, mkconst<double>(params),
-                       mkconst<float>(params), 0));
+    HIPCHK(launch_evec(sel, ixy, mx, params->mbc, dq.p, dl.p, dr.p, sc_make_const<double>(params),
+                       sc_make_const<float>(params), 0));
     HIPCHK(hipDeviceSynchronize());
     DOWN(evl, dl, (size_t)L * M * M);
     DOWN(evr, dr, (size_t)L * M * M);
@@ -767,11 +746,11 @@ extern "C" int sc_evec(const sc_params *params, int ixy, int mx, const double *q
 static int weno_common(const sc_params *params, int mx2, const double *q, const double *evl,
                        const double *evr, double *ql, double *qr, int chr)
 {
-    if (!params) return fail(SC_ERR_ARG, "params is NULL");
-    if (params->meqn < 1 || params->meqn > 4) return fail(SC_ERR_UNSUPPORTED, "meqn must be 1..4");
-    if (mx2 < 6 || !q || !ql || !qr || (chr == 2 && (!evl || !evr))) return fail(SC_ERR_ARG, "bad argument");
+    if (!params) return sc_fail(SC_ERR_ARG, "params is NULL");
+    if (params->meqn < 1 || params->meqn > 4) return sc_fail(SC_ERR_UNSUPPORTED, "meqn must be 1..4");
+    if (mx2 < 6 || !q || !ql || !qr || (chr == 2 && (!evl || !evr))) return sc_fail(SC_ERR_ARG, "bad argument");
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(SC_ERR_HIP, "no HIP device available");
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sc_fail(SC_ERR_HIP, "no HIP device available");
     DevGuard g(params->device);
     const int M = params->meqn;
     DevBuf dq, dl, dr, dql, dqr;
@@ -808,10 +787,10 @@ extern "C" int sc_flux1(const sc_params *params, int ixy, int mx, const double *
     int rc = one_call_setup(params);
     if (rc) return rc;
     if (mx < 1 || (ixy != 1 && ixy != 2) || (ixy == 2 && params->ndim != 2) || !q1d || !dq1d || !cfl)
-        return fail(SC_ERR_ARG, "bad argument");
+        return sc_fail(SC_ERR_ARG, "bad argument");
     DevGuard g(params->device);
     const int M = params->meqn, L = mx + 2 * params->mbc;
-    PhysSel sel{phys_id(params), params->char_decomp, params->precision == SC_FP32};
+    PhysSel sel{sc_phys_id(params), params->char_decomp, params->precision == SC_FP32};
     DevBuf dq, ddq;
     HIPCHK(dq.alloc((size_t)L * M));
     HIPCHK(ddq.alloc((size_t)L * M));
@@ -831,12 +810,12 @@ extern "C" int sc_flux1(const sc_params *params, int ixy, int mx, const double *
     a.LY = 1;
     a.op = OP_DQ;
     a.dtdx = dt / params->dx[ixy - 1];
-    hipError_t e = launch_stage1d(sel, ixy, a, mkconst<double>(params), mkconst<float>(params), 0);
+    hipError_t e = launch_stage1d(sel, ixy, a, sc_make_const<double>(params), sc_make_const<float>(params), 0);
     unsigned long long w[2] = {0, 0};
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(w, slot, sizeof w, hipMemcpyDeviceToHost);
     hipFree(slot);
-    if (e != hipSuccess) return fail(SC_ERR_HIP, "flux1: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "flux1: %s", hipGetErrorString(e));
     DOWN(dq1d, ddq, (size_t)L * M);
     double sx;
     memcpy(&sx, &w[0], 8);

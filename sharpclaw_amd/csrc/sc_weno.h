@@ -47,11 +47,12 @@ __device__ __forceinline__ double weno_side(double im, double a2, double a1, dou
 // (w1 = s1/sum, w3 = s3/sum of the reference), and the correction scaled
 // back by M.  eps/M^2 is clamped to [1e-15, 1e15]; below the clamp the
 // reference weights differ only where the correction itself is below
-// float32 resolution.
+// float32 resolution.  Jumps below 1e-30 (where 1/M would overflow float32
+// near its denormal range) give no correction: its size is below M.
 __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, float a0, float am1)
 {
     float M = fmaxf(fmaxf(fabsf(a2), fabsf(a1)), fmaxf(fabsf(a0), fabsf(am1)));
-    if (!(M > 0.0f)) return 0.0f;
+    if (!(M > 1.0e-30f)) return 0.0f;
     float r = frcp(M);
     a2 *= r;
     a1 *= r;

@@ -3,6 +3,8 @@
   Solver            the device-resident q/qold/RK registers of module Global
                     (src/1d/Global1.f90) with step(), ts(), advance() --
                     step / ts_ssp* / the sharpclaw() loop of src/sharpclaw.f90
+  Batch             many independent 1D grids of uneven size advanced by the
+                    sharpclaw() loop in one persistent launch
   rp, evec, weno5, weno5_char, flux1
                     single routines with the reference's argument meaning
                     (rp1eu.f:5-6, flux1.f90:105, reconstruct.f90:77,146,
@@ -14,7 +16,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import ScParams, ScRunCtl, ScRunStats, grid_shape
+from ._abi import ScBatchResult, ScParams, ScRunCtl, ScRunStats, grid_shape
 from ._lib import check, lib
 
 
@@ -111,6 +113,69 @@ class Solver:
         ms = C.c_double()
         check(lib().sc_time_stage(self._h, n, dt, C.byref(ms)))
         return ms.value
+
+
+class Batch:
+    """nsolves 1D grids sharing params (sc_batch_create); solve i has mx[i]
+    cells on [xlower[i], xupper[i]]."""
+
+    def __init__(self, params: ScParams, mx, xlower, xupper):
+        self.p = params
+        self.mx = np.ascontiguousarray(mx, dtype=np.int32)
+        n = len(self.mx)
+        xl = np.ascontiguousarray(np.broadcast_to(xlower, (n,)), dtype=np.float64)
+        xu = np.ascontiguousarray(np.broadcast_to(xupper, (n,)), dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().sc_batch_create(C.byref(params), n, self.mx.ctypes.data_as(C.POINTER(C.c_int)),
+                                    xl.ctypes.data_as(C.POINTER(C.c_double)),
+                                    xu.ctypes.data_as(C.POINTER(C.c_double)), C.byref(h)))
+        self._h = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def shape(self, i):
+        return (int(self.mx[i]) + 2 * self.p.mbc, self.p.meqn)
+
+    def set_q(self, i, q):
+        q = _f(q)
+        if q.shape != self.shape(i):
+            raise ValueError(f"q has shape {q.shape}, expected {self.shape(i)}")
+        check(lib().sc_batch_set_q(self._h, i, q))
+
+    def get_q(self, i):
+        q = np.zeros(self.shape(i), order="F")
+        check(lib().sc_batch_get_q(self._h, i, q))
+        return q
+
+    def run(self, ctl: ScRunCtl, tstart, tend, dt):
+        """Every solve: sharpclaw(tstart, tend) from dt.  Returns
+        (list of ScBatchResult, launch ms)."""
+        res = (ScBatchResult * self.n)()
+        ms = C.c_double()
+        check(lib().sc_batch_run(self._h, C.byref(ctl), tstart, tend, dt, res, C.byref(ms)))
+        return list(res), ms.value
+
+    @property
+    def block(self):
+        b = C.c_int()
+        check(lib().sc_batch_block(self._h, C.byref(b)))
+        return b.value
 
 
 def rp(p, ixy, mx, ql, qr):

@@ -63,6 +63,22 @@ def test_invalid_options_rejected_before_device_work():
         assert rc in (_abi.SC_ERR_ARG, _abi.SC_ERR_UNSUPPORTED), (c, rc)
 
 
+def test_batch_options_rejected_before_device_work():
+    L = lib()
+    p2 = _abi.make_params("euler", (8, 8))
+    mx = (C.c_int * 2)(16, 32)
+    lo = (C.c_double * 2)(0.0, 0.0)
+    hi = (C.c_double * 2)(1.0, 1.0)
+    h = C.c_void_p()
+    assert L.sc_batch_create(C.byref(p2), 2, mx, lo, hi, C.byref(h)) == _abi.SC_ERR_UNSUPPORTED
+    p1 = _abi.make_params("shallow_water", 16)
+    bad = (C.c_int * 2)(16, 2)
+    assert L.sc_batch_create(C.byref(p1), 2, bad, lo, hi, C.byref(h)) == _abi.SC_ERR_ARG
+    assert b"mx" in L.sc_last_error()
+    assert L.sc_batch_create(C.byref(p1), 0, mx, lo, hi, C.byref(h)) == _abi.SC_ERR_ARG
+    assert not h.value
+
+
 def test_struct_layouts_match_header():
     src = r'''
 #include <stdio.h>
@@ -73,6 +89,8 @@ int main(void) {
          offsetof(sc_params, physics), offsetof(sc_params, device), sizeof(sc_run_ctl),
          sizeof(sc_run_stats));
   printf("%zu %zu\n", offsetof(sc_run_stats, stage_evals), offsetof(sc_run_stats, gpu_ms));
+  printf("%zu %zu %zu %zu\n", sizeof(sc_batch_result), offsetof(sc_batch_result, buffer),
+         offsetof(sc_batch_result, stage_evals), offsetof(sc_batch_result, done_ms));
   return 0;
 }
 '''
@@ -85,4 +103,6 @@ int main(void) {
     P, S = _abi.ScParams, _abi.ScRunStats
     want = [C.sizeof(P), P.xlower.offset, P.physics.offset, P.device.offset,
             C.sizeof(_abi.ScRunCtl), C.sizeof(S), S.stage_evals.offset, S.gpu_ms.offset]
+    B = _abi.ScBatchResult
+    want += [C.sizeof(B), B.buffer.offset, B.stage_evals.offset, B.done_ms.offset]
     assert [int(x) for x in out] == want
