@@ -217,6 +217,114 @@ def cpu_baseline_batch(mx, hl, hr, x0, order, k):
                       f"threads, {wall:.1f} s wall"}
 
 
+def served_bench(n, rate, seed, cpu=True):
+    """The served workload: n requests with Poisson arrivals at `rate`/s
+    (sharpclaw_amd.workloads.served_mix), submitted at their arrival times
+    to one Server; latency = completion - submission on the server clock."""
+    from sharpclaw_amd import SC_FP64, ScRunCtl, Server, Solver
+    from sharpclaw_amd.workloads import SERVED_CTL, served_mix
+    mix = served_mix(n, seed, rate)
+    ctl = ScRunCtl(**SERVED_CTL)
+    info = {}
+    with Server(hbm_bytes=32 << 30, max_active=4096) as srv:
+        # warm-up: one small request of each kind and precision (kernel loads)
+        warm = served_mix(6, seed + 1, 1e9)
+        for r in warm:
+            srv.submit(r["params"], ctl, r["q0"], 0.0, min(r["tend"], 1e-3), r["dt0"])
+        srv.drain(120e3)
+        for rid in srv.poll():
+            srv.result(rid)
+        ids = {}
+        t0 = time.perf_counter()
+        for k, r in enumerate(mix):
+            while True:
+                wait = r["arrival"] - (time.perf_counter() - t0)
+                if wait <= 0:
+                    break
+                time.sleep(min(wait, 0.002))
+            ids[srv.submit(r["params"], ctl, r["q0"], 0.0, r["tend"], r["dt0"])] = k
+        srv.drain(600e3)
+        wall = time.perf_counter() - t0
+        done = srv.poll()
+        st = srv.stats()
+        lat, cs, first_sub, last_done = [], [], 1e300, 0.0
+        per = {}
+        for rid in done:
+            k = ids[rid]
+            q, rs = srv.result(rid)
+            if rs.run.info != 0:
+                raise RuntimeError(f"served request {k} ended with info {rs.run.info}")
+            p = mix[k]["params"]
+            cells = p.nx[0] * (p.nx[1] if p.ndim == 2 else 1)
+            lat.append(rs.done_ms - rs.submit_ms)
+            cs.append(cells * rs.run.stage_evals)
+            first_sub = min(first_sub, rs.submit_ms)
+            last_done = max(last_done, rs.done_ms)
+            key = f"{mix[k]['kind']}_{'fp64' if mix[k]['precision'] == SC_FP64 else 'fp32'}"
+            per.setdefault(key, []).append(lat[-1])
+    lat = np.array(lat)
+    span_s = (last_done - first_sub) * 1e-3
+    info = {"workload": f"{n} requests, Poisson arrivals at {rate:g}/s (seed {seed}): 1D acoustics "
+                        f"mx 128..4096, 1D Euler mx 128..1999, 2D shallow water 128^2..1024^2, "
+                        f"Gaussian bumps, float32/float64 at random, char WENO5 + SSPRK3",
+            "value": float(np.sum(cs)) / span_s, "unit": "cell-updates/s per RK stage",
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "latency_ms_by_kind": {k: {"n": len(v), "p50": float(np.percentile(v, 50)),
+                                       "p99": float(np.percentile(v, 99))} for k, v in sorted(per.items())},
+            "makespan_s": span_s, "wall_s": wall,
+            "server": {k: st[k] for k in ("rounds", "peak_bytes", "capacity_bytes", "peak_active")}}
+    log(f"bench: served {n} requests in {span_s:.2f} s, {info['value']:.3e} cell-stage/s, "
+        f"p50 {info['latency_ms']['p50']:.1f} ms p99 {info['latency_ms']['p99']:.1f} ms, "
+        f"{st['rounds']} rounds, peak {st['peak_active']} active")
+    # the float64 single-solve path on a sample: each request alone, in turn
+    sample = [r for r in served_mix(n, seed, rate)][:: max(1, n // 12)][:12]
+    cs1 = ms1 = 0.0
+    for r in sample:
+        p = r["params"]
+        p.precision = SC_FP64
+        with Solver(p) as s:
+            s.set_q(r["q0"])
+            _, rst = s.advance(ctl, 0.0, r["tend"], r["dt0"])
+        cells = p.nx[0] * (p.nx[1] if p.ndim == 2 else 1)
+        cs1 += cells * rst.stage_evals
+        ms1 += rst.gpu_ms
+    info["fp64_single_solve_path"] = {"value": cs1 / (ms1 * 1e-3), "sample": f"{len(sample)} requests of the "
+                                      "same mix, float64, one at a time through Solver.advance"}
+    info["served_vs_fp64_single"] = info["value"] / info["fp64_single_solve_path"]["value"]
+    log(f"bench: fp64 single-solve path on the mix {info['fp64_single_solve_path']['value']:.3e} "
+        f"cell-stage/s; served is {info['served_vs_fp64_single']:.1f}x")
+    if cpu:
+        info["cpu_baseline"] = cpu_baseline_served(seed)
+    return info
+
+
+def cpu_baseline_served(seed):
+    """Reference Fortran (flang -O3, one core) per request kind, on one
+    request of each kind at a reduced size; serial per-request rate."""
+    from oracle import refrun
+    from sharpclaw_amd import make_params
+    out = {}
+    # qinit kinds of oracle/ref/hooks: 4 and 5 (amp, beta, x0), 2D 2 (amp, beta, x0, y0)
+    cases = [("acoustics1d", "acoustics", (1024,), (3, 3), 0.25, ["4", "0.5 400.0 0.5"]),
+             ("euler1d", "euler", (1024,), (1, 1), 0.1, ["5", "0.5 400.0 0.5"]),
+             ("sw2d", "shallow_water", (128, 128), (1, 1, 1, 1), 0.05, ["2", "0.5 200.0 0.45 0.55"])]
+    for kind, phys, nx, bc, tend, lines in cases:
+        p = make_params(phys, nx, mthbc=bc)
+        if not refrun.ref_available(refrun.app_of(p)):
+            continue
+        cd = refrun.clawdata_for(p, nout=1, outstyle=1, tfinal=tend, dt_initial=1e-5, cfl_max=0.5,
+                                 cfl_desired=0.45, max_steps=1000000)
+        t0 = time.perf_counter()
+        _, dtlog, _ = refrun.run(p, cd, lines)
+        wall = time.perf_counter() - t0
+        cells = int(np.prod(nx))
+        out[kind] = {"value": cells * 3 * len(dtlog) / wall, "unit": "cell-updates/s per RK stage",
+                     "cores": 1, "kind": "reference",
+                     "sample": f"{'x'.join(map(str, nx))} to t={tend}, {len(dtlog)} steps, {wall:.2f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +337,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched dam-break workload")
     ap.add_argument("--batch", type=int, default=4096, help="solves in the batched workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
+    ap.add_argument("--requests", type=int, default=300, help="requests in the served workload")
+    ap.add_argument("--rate", type=float, default=100.0, help="request arrival rate (1/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -320,6 +431,9 @@ def main():
     if not args.no_batch:
         log(f"bench: batched workload ({args.batch} dam breaks)")
         out["batched"] = batched_dambreaks(args.batch, args.seed, cpu=not args.no_cpu)
+    if not args.no_serve:
+        log(f"bench: served workload ({args.requests} requests at {args.rate:g}/s)")
+        out["served"] = served_bench(args.requests, args.rate, args.seed, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -230,6 +230,55 @@ int sc_batch_run(sc_batch *b, const sc_run_ctl *ctl, double tstart, double tend,
 /* workgroup size chosen by the tuner (0 before the first run) */
 int sc_batch_block(sc_batch *b, int *block);
 
+/* ---- request server: continuous batching on one device -----------------
+ * No counterpart in the reference (one solve per process there); each
+ * request runs the reference's time loop (sharpclaw.f90:35-235) on its own
+ * grid, exactly as sc_advance / sc_batch_run do.  Requests are admitted in
+ * arrival order while their device footprint fits the server's memory
+ * budget and an active slot is free; admitted 1D requests advance together
+ * in rounds of the persistent batch kernel (one launch per group of equal
+ * options per round, a bounded number of time-loop iterations per request
+ * per round, loop state carried across rounds), 2D requests each on a
+ * stream of their own.  Arrivals join at the next round.  A request's
+ * result does not depend on what else is in flight. */
+
+typedef struct sc_server sc_server;
+
+typedef struct sc_request_stats {
+    sc_run_stats run;        /* the time loop's record (gpu_ms unused) */
+    double submit_ms;        /* server clock: ms since sc_server_create */
+    double admit_ms;
+    double done_ms;
+    long long footprint;     /* device bytes held while active */
+    int rounds;              /* batch rounds taken (1D; 0 for 2D) */
+} sc_request_stats;
+
+typedef struct sc_server_stats {
+    long long submitted, completed, rounds;
+    long long peak_bytes, capacity_bytes, used_bytes;
+    int active, queued, peak_active;
+    double clock_ms;
+} sc_server_stats;
+
+/* hbm_bytes: device memory the server holds for request state (one
+ * allocation); max_active: most requests in flight at once. */
+int sc_server_create(int device, size_t hbm_bytes, int max_active, sc_server **out);
+/* Queue a solve of q (copied) from tstart to tend; *id names it. */
+int sc_server_submit(sc_server *s, const sc_params *params, const sc_run_ctl *ctl, double tstart,
+                     double tend, double dt_initial, const double *q, long long *id);
+/* Ids of requests finished since the last poll (at most max_ids). */
+int sc_server_poll(sc_server *s, long long *ids, int max_ids, int *n);
+/* Block until a finished request awaits polling or timeout_ms passes. */
+int sc_server_wait(sc_server *s, double timeout_ms, int *n_ready);
+/* Final q (with ghost cells filled) and record of a finished request, which
+ * is then released. */
+int sc_server_result(sc_server *s, long long id, double *q, sc_request_stats *st);
+/* Block until every submitted request has finished (or timeout_ms). */
+int sc_server_drain(sc_server *s, double timeout_ms);
+int sc_server_get_stats(sc_server *s, sc_server_stats *st);
+/* Unfinished requests are abandoned. */
+int sc_server_destroy(sc_server *s);
+
 #ifdef __cplusplus
 }
 #endif

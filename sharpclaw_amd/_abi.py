@@ -109,6 +109,40 @@ class ScBatchResult(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ScRequestStats(C.Structure):
+    _fields_ = [
+        ("run", ScRunStats),
+        ("submit_ms", C.c_double),
+        ("admit_ms", C.c_double),
+        ("done_ms", C.c_double),
+        ("footprint", C.c_longlong),
+        ("rounds", C.c_int),
+    ]
+
+    def as_dict(self):
+        d = self.run.as_dict()
+        d.update({f: getattr(self, f) for f, _ in self._fields_[1:]})
+        return d
+
+
+class ScServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_longlong),
+        ("completed", C.c_longlong),
+        ("rounds", C.c_longlong),
+        ("peak_bytes", C.c_longlong),
+        ("capacity_bytes", C.c_longlong),
+        ("used_bytes", C.c_longlong),
+        ("active", C.c_int),
+        ("queued", C.c_int),
+        ("peak_active", C.c_int),
+        ("clock_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # meqn / mwaves of each device physics by dimension
 SHAPES = {
     (SC_PHYS_ADVECTION, 1): (1, 1),

@@ -6,7 +6,7 @@ import os
 
 import numpy as np
 
-from ._abi import ScBatchResult, ScParams, ScRunCtl, ScRunStats
+from ._abi import ScBatchResult, ScParams, ScRequestStats, ScRunCtl, ScRunStats, ScServerStats
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsharpclaw_amd.so")
@@ -61,6 +61,15 @@ def lib():
         "sc_batch_run": ([H, C.POINTER(ScRunCtl), C.c_double, C.c_double, C.c_double,
                           C.POINTER(ScBatchResult), dref], C.c_int),
         "sc_batch_block": ([H, C.POINTER(C.c_int)], C.c_int),
+        "sc_server_create": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(H)], C.c_int),
+        "sc_server_submit": ([H, P, C.POINTER(ScRunCtl), C.c_double, C.c_double, C.c_double, dp,
+                              C.POINTER(C.c_longlong)], C.c_int),
+        "sc_server_poll": ([H, C.POINTER(C.c_longlong), C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "sc_server_wait": ([H, C.c_double, C.POINTER(C.c_int)], C.c_int),
+        "sc_server_result": ([H, C.c_longlong, dp, C.POINTER(ScRequestStats)], C.c_int),
+        "sc_server_drain": ([H, C.c_double], C.c_int),
+        "sc_server_get_stats": ([H, C.POINTER(ScServerStats)], C.c_int),
+        "sc_server_destroy": ([H], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

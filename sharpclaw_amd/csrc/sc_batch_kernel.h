@@ -1,5 +1,6 @@
 // Batched solves, device side: the persistent kernel that advances many
-// independent 1D grids of uneven size in one launch.
+// independent 1D grids of uneven size in one launch, each by a bounded
+// number of time-loop iterations from its saved loop state.
 //
 // Each workgroup takes solves from a device work queue, largest first
 // (the host passes the order), and runs a solve's whole time loop on the
@@ -19,27 +20,40 @@
 
 namespace sc {
 
+// One solve of a launch: its grid, its slab of device memory and the
+// options of its time loop.
 struct BatchSolve {
-    long long off;   // first double of this solve's buffers (4 x len)
+    long long off;      // first double of this solve's buffers (4 x len)
     int mx;
+    int maxn;           // iteration limit of the whole loop (max_steps, or the
+                        // fixed-dt step count, as sharpclaw.f90 computes it)
     double dx;
+    double tstart, tend;
+    sc_run_ctl ctl;
+};
+
+// The loop state of one solve, kept in device memory between launches so a
+// solve can be advanced a bounded number of iterations per launch and
+// resumed later with identical arithmetic.
+struct SolveState {
+    double t, told, dt, cfl, cflmax, dtmin, dtmax;
+    long long stages, t_begin, t_end;
+    int cur, old, retake, steps, rejected, info, it, done;
 };
 
 struct BatchArgs {
     const BatchSolve *solves;
-    sc_batch_result *out;
-    const int *order;   // solve indices, largest first
-    const int *start;   // buffer holding each solve's q at launch
-    int nsolves;
+    SolveState *state;
+    const int *order;   // solves of this launch, largest first
+    int nsolves;        // entries of order
     int *next;          // work-queue counter
     double *base;
     int meqn;
     int bc[2];
     int ti;             // time integrator 1..4
-    sc_run_ctl ctl;
-    int maxn;           // step limit (sharpclaw.f90: mx steps for fixed dt)
-    int info0;          // nonzero: option error found by the host, no steps
-    double tstart, tend, dt0;
+    int step_limit;     // > 0: at most this many iterations per solve
+    int budget_chunks;  // > 0: at most max(1, budget_chunks / (chunks x stages))
+                        // iterations, chunks = ceil(mx / (NT - 1))
 };
 
 struct Plan {
@@ -93,12 +107,13 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
     while (k < A.nsolves) {
         const int b = A.order[k];
         const BatchSolve S = A.solves[b];
+        SolveState s = A.state[b];
         const int n = S.mx;
         const size_t len = (size_t)(n + 6) * A.meqn;
         double *buf[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) buf[j] = A.base + S.off + j * len;
-        long long t_begin = (long long)wall_clock64();
+        if (s.t_begin == 0) s.t_begin = (long long)wall_clock64();
 
         StageArgs a;
         a.nx = n;
@@ -113,28 +128,32 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
         a.qbin = buf[3];
         a.qbout = buf[3];
 
-        // src/sharpclaw.f90:35-235 for this solve
-        const sc_run_ctl ctl = A.ctl;
-        double tcur = A.tstart, dt = A.dt0, told = A.tstart, cflmax = 0.0, cfl = 0.0;
-        double dtmin = dt, dtmax = dt;
-        int cur = A.start[b], old = cur, steps = 0, rejected = 0, info = 0;
-        long long stages = 0;
-        bool retake = false;
-        info = A.info0;
-        for (int it = 1; info == 0 && it <= A.maxn; it++) {
-            if (!retake) {
-                told = tcur;
-                if (told + dt > A.tend && A.tstart < A.tend) dt = A.tend - told;
-                old = cur;  // qold = q
+        int lim = 0x7fffffff;
+        if (A.step_limit > 0) lim = A.step_limit;
+        else if (A.budget_chunks > 0) lim = max(1, A.budget_chunks / (((n + NT - 2) / (NT - 1)) * plan.n));
+
+        // src/sharpclaw.f90:35-235 for this solve, from its saved state
+        const sc_run_ctl ctl = S.ctl;
+        bool fin = s.done != 0;
+        for (int used = 0; !fin && used < lim; used++) {
+            if (s.info != 0 || s.it >= S.maxn) {
+                fin = true;
+                break;
             }
-            retake = false;
-            tcur = told + dt;
-            a.dtdx = dt / S.dx;
-            a.qold = buf[old];
-            int in = cur;
+            s.it++;
+            if (!s.retake) {
+                s.told = s.t;
+                if (s.told + s.dt > S.tend && S.tstart < S.tend) s.dt = S.tend - s.told;
+                s.old = s.cur;  // qold = q
+            }
+            s.retake = 0;
+            s.t = s.told + s.dt;
+            a.dtdx = s.dt / S.dx;
+            a.qold = buf[s.old];
+            int in = s.cur;
             T sm = T(0);
             for (int st = 0; st < plan.n; st++) {
-                int outb = other_buf(in, old);
+                int outb = other_buf(in, s.old);
                 a.qin = buf[in];
                 a.qout = buf[outb];
                 a.op = plan.op[st];
@@ -145,7 +164,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
                 }
                 in = outb;
             }
-            stages += plan.n;
+            s.stages += plan.n;
             // block maximum of |s| of the last stage -> CFL (flux1.f90:126-133)
             double d = (double)wave_max(sm);
             if ((t & 63) == 0) red[t >> 6] = d;
@@ -154,51 +173,43 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
 #pragma unroll
             for (int w = 1; w < NW; w++) m = fmax(m, red[w]);
             __syncthreads();
-            cfl = (dt / S.dx) * m;
+            s.cfl = (s.dt / S.dx) * m;
             if (ctl.dt_variable == 1) {
-                if (cfl > 0.0) {
-                    dt = fmin(ctl.dt_max, dt * ctl.cfl_desired / cfl);
-                    dtmin = fmin(dt, dtmin);
-                    dtmax = fmax(dt, dtmax);
+                if (s.cfl > 0.0) {
+                    s.dt = fmin(ctl.dt_max, s.dt * ctl.cfl_desired / s.cfl);
+                    s.dtmin = fmin(s.dt, s.dtmin);
+                    s.dtmax = fmax(s.dt, s.dtmax);
                 } else {
-                    dt = ctl.dt_max;
+                    s.dt = ctl.dt_max;
                 }
             }
-            if (cfl <= ctl.cfl_max) {
-                cflmax = fmax(cfl, cflmax);
-                cur = in;
+            if (s.cfl <= ctl.cfl_max) {
+                s.cflmax = fmax(s.cfl, s.cflmax);
+                s.cur = in;
             } else {
-                tcur = told;
-                cur = old;  // q = qold
-                rejected++;
+                s.t = s.told;
+                s.cur = s.old;  // q = qold
+                s.rejected++;
                 if (ctl.dt_variable == 1) {
-                    retake = true;
+                    s.retake = 1;
                 } else {
-                    cflmax = fmax(cfl, cflmax);
+                    s.cflmax = fmax(s.cfl, s.cflmax);
+                    fin = true;
                     break;
                 }
             }
-            steps++;
-            if (tcur >= A.tend) break;
+            s.steps++;
+            if (s.t >= S.tend) fin = true;
         }
-        if (info == 0 && ctl.dt_variable == 1 && tcur < A.tend && steps == A.maxn) info = 11;
-        if (info == 0 && ctl.dt_variable == 0 && cflmax > ctl.cfl_max) info = 12;
+        if (!fin && (s.info != 0 || s.it >= S.maxn)) fin = true;
+        if (fin && !s.done) {
+            if (s.info == 0 && ctl.dt_variable == 1 && s.t < S.tend && s.steps == S.maxn) s.info = 11;
+            if (s.info == 0 && ctl.dt_variable == 0 && s.cflmax > ctl.cfl_max) s.info = 12;
+            s.done = 1;
+            s.t_end = (long long)wall_clock64();
+        }
         if (t == 0) {
-            sc_batch_result r;
-            r.t = tcur;
-            r.cfl_maxused = cflmax;
-            r.cfl_last = cfl;
-            r.dt_minused = dtmin;
-            r.dt_maxused = dtmax;
-            r.dt_last = dt;
-            r.steps_taken = steps;
-            r.rejected = rejected;
-            r.info = info;
-            r.buffer = cur;
-            r.stage_evals = stages;
-            r.t_begin = t_begin;
-            r.t_end = (long long)wall_clock64();
-            A.out[b] = r;
+            A.state[b] = s;
             sb = atomicAdd(A.next, 1);
         }
         __syncthreads();

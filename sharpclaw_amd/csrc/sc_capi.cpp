@@ -137,6 +137,7 @@ struct sc_solver {
     int cur, old;      // buffer indices of q and qold
     bool have_q;
     int tile2d;        // 2D tile shape index; -1: tune at first use
+    bool ext;          // buf/qb/dq live in caller memory (sc_create_in)
     hipEvent_t ev0, ev1;
 };
 
@@ -153,7 +154,9 @@ static size_t grid_n(const sc_params *p)
     return n;
 }
 
-extern "C" int sc_create(const sc_params *params, sc_solver **out)
+// mem: NULL, or 5 grid arrays of device memory that the solver uses for its
+// q buffers, SSP(10,4) register and dq instead of allocating them.
+static int create_impl(const sc_params *params, double *mem, sc_solver **out)
 {
     if (!out) return sc_fail(SC_ERR_ARG, "out is NULL");
     *out = nullptr;
@@ -181,9 +184,16 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     s->n = grid_n(&s->p);
     s->nblk_sums = 256;
     bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
-    for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&s->buf[k], s->n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&s->qb, s->n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&s->dq, s->n * sizeof(double)) == hipSuccess;
+    if (mem) {
+        s->ext = true;
+        for (int k = 0; k < 3; k++) s->buf[k] = mem + k * s->n;
+        s->qb = mem + 3 * s->n;
+        s->dq = mem + 4 * s->n;
+    } else {
+        for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&s->buf[k], s->n * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(&s->qb, s->n * sizeof(double)) == hipSuccess;
+        ok = ok && hipMalloc(&s->dq, s->n * sizeof(double)) == hipSuccess;
+    }
     ok = ok && hipMalloc(&s->partial, (size_t)s->nblk_sums * s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->sums, s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->slots, NSLOT * 2 * sizeof(unsigned long long)) == hipSuccess;
@@ -212,14 +222,32 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     return SC_OK;
 }
 
+extern "C" int sc_create(const sc_params *params, sc_solver **out)
+{
+    return create_impl(params, nullptr, out);
+}
+
+int sc_create_in(const sc_params *params, double *mem, sc_solver **out)
+{
+    if (!mem) return sc_fail(SC_ERR_ARG, "mem is NULL");
+    return create_impl(params, mem, out);
+}
+
+size_t sc_solver_doubles(const sc_params *params)
+{
+    return 5 * grid_n(params);
+}
+
 extern "C" int sc_destroy(sc_solver *s)
 {
     if (!s) return SC_OK;
     DevGuard g(s->dev);
     if (s->st) hipStreamSynchronize(s->st);
-    for (int k = 0; k < 3; k++) if (s->buf[k]) hipFree(s->buf[k]);
-    if (s->qb) hipFree(s->qb);
-    if (s->dq) hipFree(s->dq);
+    if (!s->ext) {
+        for (int k = 0; k < 3; k++) if (s->buf[k]) hipFree(s->buf[k]);
+        if (s->qb) hipFree(s->qb);
+        if (s->dq) hipFree(s->dq);
+    }
     if (s->partial) hipFree(s->partial);
     if (s->sums) hipFree(s->sums);
     if (s->slots) hipFree(s->slots);
@@ -308,7 +336,10 @@ static std::map<std::tuple<int, int, int, int, int>, int> g_tune;
 static int tune_tile(sc_solver *s)
 {
     if (s->p.ndim != 2 || s->tile2d >= 0) return SC_OK;
-    auto key = std::make_tuple(s->sel.phys, s->sel.chr, s->sel.fp32, s->p.nx[0], s->p.nx[1]);
+    // one tuning per size class (powers of two per direction), so grids of
+    // many sizes (served requests) do not each pay for a tuning
+    auto lg = [](int n) { int k = 0; while ((2 << k) <= n) k++; return k; };
+    auto key = std::make_tuple(s->sel.phys, s->sel.chr, s->sel.fp32, lg(s->p.nx[0]), lg(s->p.nx[1]));
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(key);

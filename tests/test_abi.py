@@ -79,6 +79,15 @@ def test_batch_options_rejected_before_device_work():
     assert not h.value
 
 
+def test_server_arguments_checked_before_device_work():
+    L = lib()
+    h = C.c_void_p()
+    assert L.sc_server_create(0, 1 << 20, 0, C.byref(h)) == _abi.SC_ERR_ARG
+    assert L.sc_server_create(0, 16, 4, C.byref(h)) == _abi.SC_ERR_ARG
+    assert L.sc_server_create(0, 1 << 20, 4, None) == _abi.SC_ERR_ARG
+    assert not h.value
+
+
 def test_struct_layouts_match_header():
     src = r'''
 #include <stdio.h>
@@ -91,6 +100,8 @@ int main(void) {
   printf("%zu %zu\n", offsetof(sc_run_stats, stage_evals), offsetof(sc_run_stats, gpu_ms));
   printf("%zu %zu %zu %zu\n", sizeof(sc_batch_result), offsetof(sc_batch_result, buffer),
          offsetof(sc_batch_result, stage_evals), offsetof(sc_batch_result, done_ms));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(sc_request_stats), offsetof(sc_request_stats, submit_ms),
+         offsetof(sc_request_stats, rounds), sizeof(sc_server_stats), offsetof(sc_server_stats, clock_ms));
   return 0;
 }
 '''
@@ -105,4 +116,6 @@ int main(void) {
             C.sizeof(_abi.ScRunCtl), C.sizeof(S), S.stage_evals.offset, S.gpu_ms.offset]
     B = _abi.ScBatchResult
     want += [C.sizeof(B), B.buffer.offset, B.stage_evals.offset, B.done_ms.offset]
+    R, V = _abi.ScRequestStats, _abi.ScServerStats
+    want += [C.sizeof(R), R.submit_ms.offset, R.rounds.offset, C.sizeof(V), V.clock_ms.offset]
     assert [int(x) for x in out] == want

@@ -233,12 +233,12 @@ def test_batch_alone_vs_mixed_and_block_sizes_bitwise(fp32):
                                   1e-3, block=128)
         assert np.array_equal(alone[0], mixed[k])
         assert (ra[0].steps_taken, ra[0].dt_last, ra[0].t) == (rm[k].steps_taken, rm[k].dt_last, rm[k].t)
-    for blk in (64, 256):
+    for blk in (64, 256, 512, 1024):
         other, _, got = _run_batch(base, mx, xl, xu, qs, ctl, 0.03, 1e-3, block=blk)
         assert got == blk
         for i in range(n):
             assert np.array_equal(other[i], mixed[i]), (blk, i)
     tuned, _, got = _run_batch(base, mx, xl, xu, qs, ctl, 0.03, 1e-3)
-    assert got in (64, 128, 256)
+    assert got in (64, 128, 256, 512, 1024)
     for i in range(n):
         assert np.array_equal(tuned[i], mixed[i])

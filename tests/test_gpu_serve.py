@@ -1,0 +1,163 @@
+"""Request server (sc_server_*) through the C ABI.
+
+Continuous batching must not change a request's result: every request of a
+mixed load is compared bitwise with the same request served alone, and with
+the oracle's time loop (float64: the whole-run tolerance of test_gpu_parity,
+max(1e-10, 8x the oracle's own 1-ulp sensitivity), with identical step
+counts; float32: relative L1 <= 1e-5 against the float64 oracle).  Admission
+respects the memory budget and the slot count, and requests submitted while
+others run are admitted before those finish.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rel_linf
+from oracle import oracle as O
+from sharpclaw_amd import SC_FP32, SC_FP64, ScError, ScRunCtl, Server, interior, make_params
+from sharpclaw_amd._abi import SC_ERR_ARG, SC_ERR_NOMEM
+from sharpclaw_amd.workloads import SERVED_CTL, served_request
+
+pytestmark = pytest.mark.gpu
+
+TOL_RUN = 1e-10
+
+
+def _small_mix(seed=11):
+    """Two of each kind and precision at sizes the oracle runs quickly."""
+    rng = np.random.default_rng(seed)
+    out = []
+    dt0 = 1e-5
+    for kind in ("acoustics1d", "euler1d", "sw2d"):
+        for prec in (SC_FP64, SC_FP32):
+            for _ in range(2):
+                if kind == "sw2d":
+                    n = int(rng.integers(40, 72))
+                    p = make_params("shallow_water", (n, n), grav=9.81, precision=prec)
+                    xc = (np.arange(n) + 0.5) / n
+                    X, Y = np.meshgrid(xc, xc, indexing="ij")
+                    q = O.new_grid(p)
+                    interior(p, q)[..., 0] = 1.0 + 0.5 * np.exp(-((X - 0.45) ** 2 + (Y - 0.55) ** 2) / 0.01)
+                    tend = 0.02
+                else:
+                    mx = int(rng.integers(60, 400))
+                    name = "acoustics" if kind == "acoustics1d" else "euler"
+                    bc = (3, 3) if kind == "acoustics1d" else (1, 1)
+                    p = make_params(name, (mx,), mthbc=bc, precision=prec)
+                    xc = (np.arange(mx) + 0.5) / mx
+                    bump = 0.5 * np.exp(-((xc - rng.uniform(0.3, 0.7)) / 0.05) ** 2)
+                    q = O.new_grid(p)
+                    if kind == "acoustics1d":
+                        interior(p, q)[:, 0] = bump
+                    else:
+                        interior(p, q)[:, 0] = 1.0 + bump
+                        interior(p, q)[:, 2] = (1.0 + bump) / 0.4
+                    tend = 0.05
+                out.append({"kind": kind, "params": p, "q0": q, "tend": tend, "dt0": dt0})
+    return out
+
+
+def _serve(reqs, hbm_bytes, max_active, ctl):
+    with Server(hbm_bytes=hbm_bytes, max_active=max_active) as srv:
+        ids = [srv.submit(r["params"], ctl, r["q0"], 0.0, r["tend"], r["dt0"]) for r in reqs]
+        srv.drain(120e3)
+        res = {}
+        done = srv.poll()
+        assert sorted(done) == sorted(ids)
+        for k, rid in enumerate(ids):
+            res[k] = srv.result(rid)
+        stats = srv.stats()
+    return [res[k] for k in range(len(reqs))], stats
+
+
+def _fp64(p):
+    p64 = type(p)()
+    for f, _ in p._fields_:
+        setattr(p64, f, getattr(p, f))
+    p64.precision = SC_FP64
+    return p64
+
+
+def test_server_mix_equals_alone_and_oracle():
+    reqs = _small_mix()
+    ctl = ScRunCtl(**SERVED_CTL)
+    big = max(r["q0"].size for r in reqs) * 8 * 5
+    # a budget of ~3 large requests and 4 slots: most requests queue
+    mixed, st = _serve(reqs, hbm_bytes=3 * big, max_active=4, ctl=ctl)
+    print("server stats:", st)
+    assert st["peak_bytes"] <= st["capacity_bytes"]
+    assert st["peak_active"] <= 4
+    assert st["completed"] == len(reqs)
+    for k, r in enumerate(reqs):
+        (alone,), _ = _serve([r], hbm_bytes=3 * big, max_active=4, ctl=ctl)
+        qm, sm = mixed[k]
+        qa, sa = alone
+        assert np.array_equal(qm, qa), (k, r["kind"])
+        assert sm.run.steps_taken == sa.run.steps_taken and sm.run.dt_last == sa.run.dt_last
+        p = r["params"]
+        p64 = _fp64(p)
+        rq, _, rst = O.sharpclaw(p64, ctl, r["q0"], 0.0, r["tend"], r["dt0"])
+        assert sm.run.info == 0 and sm.run.t == pytest.approx(r["tend"], abs=1e-14)
+        if p.precision == SC_FP64:
+            qp = r["q0"].copy(order="F")
+            interior(p, qp)[:] = np.nextafter(interior(p, qp), np.inf)
+            rq2, _, _ = O.sharpclaw(p64, ctl, qp, 0.0, r["tend"], r["dt0"])
+            tol = max(TOL_RUN, 8.0 * rel_linf(interior(p, rq2), interior(p, rq)))
+            err = rel_linf(interior(p, qm), interior(p, rq))
+            assert err <= tol, (k, r["kind"], err, tol)
+            assert sm.run.steps_taken == rst.steps_taken
+        else:
+            g, o = interior(p, qm), interior(p, rq)
+            err = np.sum(np.abs(g - o)) / np.sum(np.abs(o))
+            assert err <= 1e-5, (k, r["kind"], err)
+
+
+def test_server_admission_and_continuous_arrivals():
+    rng = np.random.default_rng(5)
+    ctl = ScRunCtl(**SERVED_CTL)
+    first = []
+    for _ in range(6):
+        p, q, tend, dt0 = served_request("acoustics1d", rng, SC_FP64)
+        first.append((p, q, tend, dt0))
+    late = []
+    for _ in range(6):
+        p, q, tend, dt0 = served_request("euler1d", rng, SC_FP32)
+        late.append((p, q, tend, dt0))
+    with Server(hbm_bytes=64 << 20, max_active=8) as srv:
+        ids1 = [srv.submit(p, ctl, q, 0.0, tend, dt0) for p, q, tend, dt0 in first]
+        t0 = time.time()
+        while srv.stats()["rounds"] < 3 and time.time() - t0 < 30:
+            time.sleep(0.001)
+        ids2 = [srv.submit(p, ctl, q, 0.0, tend, dt0) for p, q, tend, dt0 in late]
+        srv.drain(120e3)
+        st = srv.stats()
+        done = srv.poll()
+        assert sorted(done) == sorted(ids1 + ids2)
+        res = {rid: srv.result(rid)[1] for rid in ids1 + ids2}
+        # a request larger than the whole budget is refused at submission
+        p = make_params("shallow_water", (1024, 1024))
+        with pytest.raises(ScError) as ei:
+            srv.submit(p, ctl, O.new_grid(p), 0.0, 0.01, 1e-4)
+        assert ei.value.code == SC_ERR_NOMEM
+    assert st["peak_active"] <= 8 and st["peak_bytes"] <= st["capacity_bytes"]
+    assert st["rounds"] >= 3
+    # late arrivals joined while earlier requests were still running
+    first_done = max(res[r].done_ms for r in ids1)
+    assert any(res[r].admit_ms < first_done for r in ids2)
+    for r in ids1 + ids2:
+        s = res[r]
+        assert s.submit_ms <= s.admit_ms <= s.done_ms and s.run.info == 0 and s.rounds >= 1
+
+
+def test_server_rejects_bad_requests():
+    ctl = ScRunCtl(**SERVED_CTL)
+    with Server(hbm_bytes=16 << 20, max_active=2) as srv:
+        p = make_params("euler", (64,), time_integrator=7)
+        with pytest.raises(ScError) as ei:
+            srv.submit(p, ctl, O.new_grid(p), 0.0, 0.01, 1e-4)
+        assert ei.value.code == SC_ERR_ARG
+        p = make_params("euler", (64,), device=3)
+        with pytest.raises(ScError):
+            srv.submit(p, ctl, O.new_grid(p), 0.0, 0.01, 1e-4)
+        assert srv.stats()["submitted"] == 0
