@@ -142,6 +142,7 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
                 b.set_q(i, dambreak_q(int(mx[i]), hl[i], hr[i], x0[i]))
             res, ms = b.run(ctl, 0.0, tend, dt0)
             block = b.block
+            tuning = b.tuning()
         bad = [i for i, r in enumerate(res) if r.info != 0 or abs(r.t - tend) > 1e-12]
         if bad:
             raise RuntimeError(f"batched solves {bad[:5]} did not reach t={tend}")
@@ -154,7 +155,8 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
                                     "max": float(done.max())},
                      "steps_total": int(sum(r.steps_taken for r in res)),
                      "rejected_total": int(sum(r.rejected for r in res)),
-                     "block": block, "first_run_wall_s_incl_tuning": first}
+                     "block": block, "tuner_predicted_ms": {str(k): v for k, v in tuning.items()},
+                     "first_run_wall_s_incl_tuning": first}
         log(f"bench: batch {name}: {ms:.1f} ms, {cs / (ms * 1e-3):.3e} cell-stage/s, "
             f"p50 {out[name]['latency_ms']['p50']:.1f} ms p99 {out[name]['latency_ms']['p99']:.1f} ms, "
             f"block {block}")

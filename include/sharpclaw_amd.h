@@ -229,6 +229,8 @@ int sc_batch_run(sc_batch *b, const sc_run_ctl *ctl, double tstart, double tend,
                  sc_batch_result *results, double *gpu_ms);
 /* workgroup size chosen by the tuner (0 before the first run) */
 int sc_batch_block(sc_batch *b, int *block);
+/* the tuner's candidates and predicted whole-run times (ms); *n <= max */
+int sc_batch_tuning(sc_batch *b, int max, int *blocks, double *predicted_ms, int *n);
 
 /* ---- request server: continuous batching on one device -----------------
  * No counterpart in the reference (one solve per process there); each

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <queue>
 #include <vector>
 
 #include "../../include/sharpclaw_amd.h"
@@ -19,10 +20,11 @@ const int kBatchBlocks[NBATCH_BLOCKS] = {64, 128, 256, 512, 1024};
 
 // ------------------------------------------------------------ launchers
 
+// resident workgroups per CU of one instantiation (same on every device here)
 template <int P, int CHAR, class T, int NT>
-static hipError_t gob(const BatchArgs &A, const PhysConst<T> &c, int ncu, int max_wg, hipStream_t st)
+static int occ_of(void)
 {
-    static int occ = -1;  // resident workgroups per CU (same on every device here)
+    static int occ = -1;
     if (occ < 0) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<P, CHAR, T, NT>, NT, 0) !=
@@ -30,11 +32,47 @@ static hipError_t gob(const BatchArgs &A, const PhysConst<T> &c, int ncu, int ma
             nb = 1;
         occ = nb;
     }
-    int nwg = std::min(A.nsolves, ncu * occ);
+    return occ;
+}
+
+template <int P, int CHAR, class T, int NT>
+static hipError_t gob(const BatchArgs &A, const PhysConst<T> &c, int ncu, int max_wg, hipStream_t st)
+{
+    int nwg = std::min(A.nsolves, ncu * occ_of<P, CHAR, T, NT>());
     if (max_wg > 0) nwg = std::min(nwg, max_wg);
     if (nwg < 1) return hipSuccess;
     hipLaunchKernelGGL((batch_kernel<P, CHAR, T, NT>), dim3(nwg), dim3(NT), 0, st, A, c);
     return hipGetLastError();
+}
+
+template <int P, int CHAR, class T>
+static int occ_nt(int nt)
+{
+    switch (nt) {
+    case 64: return occ_of<P, CHAR, T, 64>();
+    case 128: return occ_of<P, CHAR, T, 128>();
+    case 512: return occ_of<P, CHAR, T, 512>();
+    case 1024: return occ_of<P, CHAR, T, 1024>();
+    default: return occ_of<P, CHAR, T, 256>();
+    }
+}
+
+template <int P>
+static int occ_p(const PhysSel &s, int nt)
+{
+    if (s.chr == 2) return s.fp32 ? occ_nt<P, 2, float>(nt) : occ_nt<P, 2, double>(nt);
+    return s.fp32 ? occ_nt<P, 0, float>(nt) : occ_nt<P, 0, double>(nt);
+}
+
+int batch_occupancy(const PhysSel &s, int nt)
+{
+    switch (s.phys) {
+    case ADV1: return occ_p<ADV1>(s, nt);
+    case AC1: return occ_p<AC1>(s, nt);
+    case EU1: return occ_p<EU1>(s, nt);
+    case SW1: return occ_p<SW1>(s, nt);
+    }
+    return 1;
 }
 
 template <int P, int CHAR, class T>
@@ -96,6 +134,7 @@ struct sc_batch {
     double *d_base;
     size_t total;             // doubles allocated
     double tick_ms;           // wall_clock64 period in ms
+    double predicted[NBATCH_BLOCKS];  // tuner's makespan estimates (ms)
     hipEvent_t ev0, ev1;
 };
 
@@ -257,11 +296,52 @@ static int run_once(sc_batch *b, int nt, double tstart, double dt0, int info0, i
     return SC_OK;
 }
 
-// Autotuner of the workgroup size: each candidate runs one iteration of
-// every solve's time loop from the current states and the fastest is kept.
-// A trial writes only the buffers other than the one holding q (and the
-// SSP(10,4) register), and its loop states are discarded, so the solves are
-// untouched.
+// Makespan of a whole run predicted for workgroup size nt from a trial of
+// one time-loop iteration of every solve (trial_ms, st: the states after
+// it).  The trial keeps every resident workgroup busy, so it gives the time
+// c of one chunk-stage on one busy workgroup; the trial's CFL gives each
+// solve's step count, (tend - tstart) / dt_next (maxn for fixed dt).  Solves
+// are then placed largest first on the least loaded of the resident
+// workgroups (the device work queue does the same), and the largest load is
+// the makespan: it sees both the total work and the critical path of the
+// largest solves, which a one-iteration trial alone does not.
+static double predict_ms(const sc_batch *b, int nt, float trial_ms, double tstart, int stages)
+{
+    const int n = b->nsolves;
+    std::vector<double> work(n);
+    double trial = 0.0;
+    for (int i = 0; i < n; i++) {
+        const BatchSolve &S = b->solves[i];
+        const SolveState &s = b->state[i];
+        const double ch = (double)((S.mx + nt - 2) / (nt - 1)) * stages;
+        double iters = S.maxn;
+        if (S.ctl.dt_variable == 1 && s.dt > 0.0)
+            iters = std::min((double)S.maxn, std::max(1.0, std::ceil((S.tend - tstart) / s.dt)));
+        if (s.info != 0) iters = 0.0;
+        trial += s.it > 0 ? ch : 0.0;
+        work[i] = iters * ch;
+    }
+    const int slots = std::max(1, std::min(n, b->ncu * batch_occupancy(b->sel, nt)));
+    if (trial <= 0.0) return trial_ms;
+    const double c = trial_ms * slots / trial;
+    std::sort(work.begin(), work.end(), std::greater<double>());
+    std::priority_queue<double, std::vector<double>, std::greater<double>> load;
+    for (int k = 0; k < slots; k++) load.push(0.0);
+    double worst = 0.0;
+    for (double w : work) {
+        double l = load.top() + w;
+        load.pop();
+        load.push(l);
+        worst = std::max(worst, l);
+    }
+    return worst * c;
+}
+
+// Autotuner of the workgroup size: each candidate runs one time-loop
+// iteration of every solve from the current states, and the one with the
+// smallest predicted makespan (predict_ms) is kept.  A trial writes only the
+// buffers other than the one holding q (and the SSP(10,4) register), and
+// its loop states are discarded, so the solves are untouched.
 static int tune_block(sc_batch *b, double tstart, double dt0, int info0)
 {
     if (b->block) return SC_OK;
@@ -273,14 +353,17 @@ static int tune_block(sc_batch *b, double tstart, double dt0, int info0)
                 return SC_OK;
             }
     }
+    static const int nstage[] = {0, 2, 4, 3, 10};
     int best = 256;
-    float bestms = 1e30f;
+    double bestms = 1e300;
     for (int k = 0; k < NBATCH_BLOCKS; k++) {
         float ms = 0.f;
         int rc = run_once(b, kBatchBlocks[k], tstart, dt0, info0, 1, &ms);
         if (rc) return rc;
-        if (ms < bestms) {
-            bestms = ms;
+        double pred = predict_ms(b, kBatchBlocks[k], ms, tstart, nstage[b->p.time_integrator]);
+        b->predicted[k] = pred;
+        if (pred < bestms) {
+            bestms = pred;
             best = kBatchBlocks[k];
         }
     }
@@ -331,5 +414,17 @@ extern "C" int sc_batch_block(sc_batch *b, int *block)
 {
     if (!b || !block) return sc_fail(SC_ERR_ARG, "bad argument");
     *block = b->block;
+    return SC_OK;
+}
+
+extern "C" int sc_batch_tuning(sc_batch *b, int max, int *blocks, double *predicted_ms, int *n)
+{
+    if (!b || !n || (max > 0 && (!blocks || !predicted_ms))) return sc_fail(SC_ERR_ARG, "bad argument");
+    int k = 0;
+    for (; k < NBATCH_BLOCKS && k < max; k++) {
+        blocks[k] = kBatchBlocks[k];
+        predicted_ms[k] = b->predicted[k];
+    }
+    *n = k;
     return SC_OK;
 }

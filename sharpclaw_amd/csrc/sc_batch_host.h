@@ -21,6 +21,9 @@ extern const int kBatchBlocks[NBATCH_BLOCKS];
 hipError_t launch_batch(const PhysSel &s, int nt, const BatchArgs &A, const PhysConst<double> &cd,
                         const PhysConst<float> &cf, int ncu, int max_wg, hipStream_t st);
 
+// Resident workgroups per CU of the kernel for these options and size.
+int batch_occupancy(const PhysSel &s, int nt);
+
 // Iteration limit and pre-loop option errors of the time loop for one
 // interval (sharpclaw.f90: fixed dt takes (tend - tstart)/dt steps, info 2
 // when that is not whole, info 3 when cfl_desired > cfl_max).

@@ -139,9 +139,17 @@ struct Group {
     int *ho = nullptr;
 };
 
+// Workgroup size of a 1D solve by its size: large solves take wide
+// workgroups (fewer chunks per stage, so fewer dependent steps on their
+// critical path), small ones narrow (fewer idle lanes).
+int block_class(int mx)
+{
+    return mx >= 1024 ? 1024 : mx >= 384 ? 512 : 256;
+}
+
 bool same_group(const sc_params &a, const sc_params &b)
 {
-    return a.physics == b.physics && a.meqn == b.meqn && a.char_decomp == b.char_decomp &&
+    return block_class(a.nx[0]) == block_class(b.nx[0]) && a.physics == b.physics && a.meqn == b.meqn && a.char_decomp == b.char_decomp &&
            a.precision == b.precision && a.time_integrator == b.time_integrator &&
            a.mthbc[0] == b.mthbc[0] && a.mthbc[1] == b.mthbc[1] && a.gamma == b.gamma &&
            a.grav == b.grav && a.u == b.u && a.rho == b.rho && a.bulk == b.bulk && a.efix == b.efix;
@@ -222,6 +230,7 @@ Group *group_for(sc_server *S, const sc_params &p)
     g->sel.fp32 = p.precision == SC_FP32;
     g->cd = sc_make_const<double>(&p);
     g->cf = sc_make_const<float>(&p);
+    g->nt = block_class(p.nx[0]);
     if (const char *e = getenv("SC_SERVE_BLOCK")) {
         int v = atoi(e);
         for (int k = 0; k < NBATCH_BLOCKS; k++)
@@ -324,7 +333,8 @@ int launch_round(sc_server *S, Group *g)
     A.bc[0] = g->p.mthbc[0];
     A.bc[1] = g->p.mthbc[1];
     A.ti = g->p.time_integrator;
-    A.budget_chunks = S->budget_chunks;
+    // rounds of similar length whatever the workgroup size
+    A.budget_chunks = std::max(1, S->budget_chunks * 256 / g->nt);
     hipError_t e = hipMemcpyAsync(g->d_solves, g->hs, n * sizeof(BatchSolve), hipMemcpyHostToDevice, g->st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(g->d_state, g->hst, n * sizeof(SolveState), hipMemcpyHostToDevice, g->st);

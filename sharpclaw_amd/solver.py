@@ -177,6 +177,14 @@ class Batch:
         check(lib().sc_batch_block(self._h, C.byref(b)))
         return b.value
 
+    def tuning(self):
+        """{workgroup size: predicted whole-run ms} of the tuner (0: untimed)."""
+        blocks = (C.c_int * 8)()
+        pred = (C.c_double * 8)()
+        n = C.c_int()
+        check(lib().sc_batch_tuning(self._h, 8, blocks, pred, C.byref(n)))
+        return {blocks[k]: pred[k] for k in range(n.value)}
+
 
 def rp(p, ixy, mx, ql, qr):
     """rp(ixy,...,ql,qr,...) -> (wave, s, amdq, apdq), reference layout."""
