@@ -237,12 +237,14 @@ int sc_batch_tuning(sc_batch *b, int max, int *blocks, double *predicted_ms, int
  * request runs the reference's time loop (sharpclaw.f90:35-235) on its own
  * grid, exactly as sc_advance / sc_batch_run do.  Requests are admitted in
  * arrival order while their device footprint fits the server's memory
- * budget and an active slot is free; admitted 1D requests advance together
- * in rounds of the persistent batch kernel (one launch per group of equal
- * options per round, a bounded number of time-loop iterations per request
- * per round, loop state carried across rounds), 2D requests each on a
- * stream of their own.  Arrivals join at the next round.  A request's
- * result does not depend on what else is in flight. */
+ * budget and an active slot is free.  Admitted requests advance together in
+ * rounds: per round, 1D requests with equal options share one launch of the
+ * persistent batch kernel (a bounded number of time-loop iterations each),
+ * and 2D requests with equal options advance K steps with every stage
+ * launch covering all their grids' tiles and the time-loop decisions taken
+ * on the device; loop state carries across rounds.  Arrivals join at the
+ * next round.  A request's result does not depend on what else is in
+ * flight.  Boundary options 1-3 only (ghost cells are synthesised). */
 
 typedef struct sc_server sc_server;
 
@@ -252,7 +254,7 @@ typedef struct sc_request_stats {
     double admit_ms;
     double done_ms;
     long long footprint;     /* device bytes held while active */
-    int rounds;              /* batch rounds taken (1D; 0 for 2D) */
+    int rounds;              /* rounds the request took part in */
 } sc_request_stats;
 
 typedef struct sc_server_stats {

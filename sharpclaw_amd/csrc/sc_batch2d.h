@@ -1,0 +1,78 @@
+// Batched 2D grids advanced on the device: the stage launches of one time
+// step cover the tiles of every active grid, and a control kernel takes
+// each grid's time-loop decisions (sharpclaw.f90:35-235: dt clamp to tend,
+// qold save, CFL-driven dt update, accept or reject-and-retake) from its
+// CFL words, so a round of many steps runs without the host.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/sharpclaw_amd.h"
+#include "sc_batch_kernel.h"
+
+namespace sc {
+
+struct Grid2D {
+    long long off;      // first double of this grid's 4 arrays (3 q buffers, SSP(10,4) register)
+    int nx, ny, LX, LY;
+    int ntx, nty;       // tiles per direction
+    int tile0;          // first launch-wide tile index of this grid
+    int maxn;           // iteration limit of the loop
+    double dx, dy, tstart, tend;
+    sc_run_ctl ctl;
+};
+
+struct Batch2DArgs {
+    const Grid2D *grids;        // ascending tile0
+    SolveState *state;
+    unsigned long long *smax;   // 2 CFL words per grid (x, y sweeps)
+    int ngrids;
+    double *base;
+    int bc[4];
+    int ti;                     // time integrator 1..4
+    int stage;                  // stage of a stage launch
+};
+
+// Host side (sc_batch2d.hip).
+hipError_t launch_stage2d_batch(const PhysSel &s, int tile, const Batch2DArgs &A, int ntiles,
+                                const PhysConst<double> &cd, const PhysConst<float> &cf, hipStream_t st);
+// do_post: close the iteration whose stages just ran; do_pre: open the next.
+hipError_t launch_ctl2d(const Batch2DArgs &A, int do_post, int do_pre, hipStream_t st);
+__host__ __device__ int stages_of_integrator(int ti);
+
+}  // namespace sc
+
+namespace sc {
+
+// Host driver of rounds over a set of 2D grids with equal options: it owns
+// the device descriptor/state/CFL arrays and their pinned staging, and
+// enqueues a round of K iterations (K x (stages + 1) + 1 launches) on its
+// stream without waiting.
+struct Batch2DRunner {
+    PhysSel sel;
+    PhysConst<double> cd;
+    PhysConst<float> cf;
+    int tile = 1;       // tile shape (sc_kernels.h tile table)
+    int ti = 3;
+    int bc[4] = {1, 1, 1, 1};
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    int cap = 0;
+    Grid2D *d_grids = nullptr;
+    SolveState *d_state = nullptr;
+    unsigned long long *d_smax = nullptr;
+    Grid2D *h_grids = nullptr;      // pinned
+    SolveState *h_state = nullptr;  // pinned
+
+    int init(const sc_params &p);
+    int reserve(int n);
+    // h_grids[0..n) and h_state[0..n) filled by the caller (tile0 is set
+    // here); states come back into h_state when the round's event fires.
+    int launch_round(int n, int K, double *base);
+    int wait();
+    void destroy();
+};
+
+// Iterations per round so that a round is a few milliseconds of work.
+int round_iterations(int stages, long long tiles);
+
+}  // namespace sc

@@ -39,6 +39,8 @@ struct SolveState {
     double t, told, dt, cfl, cflmax, dtmin, dtmax;
     long long stages, t_begin, t_end;
     int cur, old, retake, steps, rejected, info, it, done;
+    int go;             // 2D rounds: the current iteration's stages run
+    int pad;
 };
 
 struct BatchArgs {

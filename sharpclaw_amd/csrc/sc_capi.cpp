@@ -137,7 +137,6 @@ struct sc_solver {
     int cur, old;      // buffer indices of q and qold
     bool have_q;
     int tile2d;        // 2D tile shape index; -1: tune at first use
-    bool ext;          // buf/qb/dq live in caller memory (sc_create_in)
     hipEvent_t ev0, ev1;
 };
 
@@ -154,9 +153,7 @@ static size_t grid_n(const sc_params *p)
     return n;
 }
 
-// mem: NULL, or 5 grid arrays of device memory that the solver uses for its
-// q buffers, SSP(10,4) register and dq instead of allocating them.
-static int create_impl(const sc_params *params, double *mem, sc_solver **out)
+extern "C" int sc_create(const sc_params *params, sc_solver **out)
 {
     if (!out) return sc_fail(SC_ERR_ARG, "out is NULL");
     *out = nullptr;
@@ -184,16 +181,9 @@ static int create_impl(const sc_params *params, double *mem, sc_solver **out)
     s->n = grid_n(&s->p);
     s->nblk_sums = 256;
     bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
-    if (mem) {
-        s->ext = true;
-        for (int k = 0; k < 3; k++) s->buf[k] = mem + k * s->n;
-        s->qb = mem + 3 * s->n;
-        s->dq = mem + 4 * s->n;
-    } else {
-        for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&s->buf[k], s->n * sizeof(double)) == hipSuccess;
-        ok = ok && hipMalloc(&s->qb, s->n * sizeof(double)) == hipSuccess;
-        ok = ok && hipMalloc(&s->dq, s->n * sizeof(double)) == hipSuccess;
-    }
+    for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&s->buf[k], s->n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->qb, s->n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->dq, s->n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->partial, (size_t)s->nblk_sums * s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->sums, s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->slots, NSLOT * 2 * sizeof(unsigned long long)) == hipSuccess;
@@ -222,32 +212,14 @@ static int create_impl(const sc_params *params, double *mem, sc_solver **out)
     return SC_OK;
 }
 
-extern "C" int sc_create(const sc_params *params, sc_solver **out)
-{
-    return create_impl(params, nullptr, out);
-}
-
-int sc_create_in(const sc_params *params, double *mem, sc_solver **out)
-{
-    if (!mem) return sc_fail(SC_ERR_ARG, "mem is NULL");
-    return create_impl(params, mem, out);
-}
-
-size_t sc_solver_doubles(const sc_params *params)
-{
-    return 5 * grid_n(params);
-}
-
 extern "C" int sc_destroy(sc_solver *s)
 {
     if (!s) return SC_OK;
     DevGuard g(s->dev);
     if (s->st) hipStreamSynchronize(s->st);
-    if (!s->ext) {
-        for (int k = 0; k < 3; k++) if (s->buf[k]) hipFree(s->buf[k]);
-        if (s->qb) hipFree(s->qb);
-        if (s->dq) hipFree(s->dq);
-    }
+    for (int k = 0; k < 3; k++) if (s->buf[k]) hipFree(s->buf[k]);
+    if (s->qb) hipFree(s->qb);
+    if (s->dq) hipFree(s->dq);
     if (s->partial) hipFree(s->partial);
     if (s->sums) hipFree(s->sums);
     if (s->slots) hipFree(s->slots);

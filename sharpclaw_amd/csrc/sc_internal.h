@@ -12,10 +12,7 @@ int sc_fail(int code, const char *fmt, ...);
 int sc_check_params(const sc_params *p, bool grid);
 // PhysId of the device physics, -1 if none.
 int sc_phys_id(const sc_params *p);
-// A solver whose grid arrays (5 x sc_solver_doubles/5) live in mem, device
-// memory of the caller that outlives the solver.
-int sc_create_in(const sc_params *params, double *mem, sc_solver **out);
-size_t sc_solver_doubles(const sc_params *params);
+
 
 template <class T> inline sc::PhysConst<T> sc_make_const(const sc_params *p)
 {

@@ -12,9 +12,15 @@
 //              state (sc_batch_kernel.h).  Requests admitted meanwhile join
 //              at the next round; finished ones leave at the end of the
 //              round that finished them.
-//   2D solves  each on a worker thread that runs the sharpclaw() loop
-//              (sc_advance) on its own stream, with the grid arrays carved
-//              from the arena.
+//   2D solves  in rounds too: per round, each group of 2D grids with equal
+//              options advances K time steps with no host involvement --
+//              every stage launch covers the tiles of all the group's
+//              grids, and a control kernel takes each grid's time-loop
+//              decisions between steps (sc_batch2d.h).
+//
+// Each group runs its rounds back to back on its own stream, independently
+// of the other groups (the scheduler polls the groups' round events), so a
+// slow group does not hold up the others' arrivals and completions.
 //
 // A request's arithmetic is the same whatever else is in flight (the loop
 // state is exact across rounds, and no reduction mixes solves), so its
@@ -39,6 +45,7 @@
 #include <vector>
 
 #include "../../include/sharpclaw_amd.h"
+#include "sc_batch2d.h"
 #include "sc_batch_host.h"
 #include "sc_internal.h"
 
@@ -113,8 +120,9 @@ struct Req {
     int status = 0;         // 0 queued, 1 active, 2 done
     int rc = SC_OK;
     int rounds = 0;
-    BatchSolve bs;          // 1D: descriptor and loop state
-    SolveState ss;
+    BatchSolve bs;          // 1D: descriptor
+    Grid2D g2;              // 2D: descriptor
+    SolveState ss;          // loop state
     sc_run_stats run;
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
@@ -128,7 +136,9 @@ struct Group {
     int nt = 256;
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
-    std::vector<Req *> active;
+    std::vector<Req *> active;    // members of the round in flight / next round
+    std::vector<Req *> incoming;  // admitted, join at the next launch
+    bool inflight = false;
     int cap = 0;
     BatchSolve *d_solves = nullptr;
     SolveState *d_state = nullptr;
@@ -146,6 +156,15 @@ int block_class(int mx)
 {
     return mx >= 1024 ? 1024 : mx >= 384 ? 512 : 256;
 }
+
+// 2D requests whose options select the same kernels and constants.
+struct Group2D {
+    sc_params p;
+    Batch2DRunner run;
+    std::vector<Req *> active;
+    std::vector<Req *> incoming;
+    bool inflight = false;
+};
 
 bool same_group(const sc_params &a, const sc_params &b)
 {
@@ -168,8 +187,9 @@ struct sc_server {
     std::deque<Req *> queue;
     std::unordered_map<long long, std::unique_ptr<Req>> reqs;
     std::deque<long long> completed;
-    std::vector<std::unique_ptr<Group>> groups;  // scheduler thread only
-    int n_active = 0, peak_active = 0, workers = 0;
+    std::vector<std::unique_ptr<Group>> groups;      // scheduler thread only
+    std::vector<std::unique_ptr<Group2D>> groups2d;  // scheduler thread only
+    int n_active = 0, peak_active = 0;
     long long next_id = 1, submitted = 0, n_completed = 0, rounds = 0;
     bool stop = false;
     std::string fatal;
@@ -202,21 +222,23 @@ void finish_locked(sc_server *S, Req *r, int rc)
     S->cv_work.notify_all();
 }
 
-void worker2d(sc_server *S, Req *r)
+bool same_group2d(const sc_params &a, const sc_params &b)
 {
-    (void)hipSetDevice(S->dev);
-    sc_solver *sv = nullptr;
-    double *mem = S->arena_base + r->off / (long long)sizeof(double);
-    int rc = sc_create_in(&r->p, mem, &sv);
-    if (!rc) rc = sc_set_q(sv, r->q.data());
-    double dt = r->dt0;
-    if (!rc) rc = sc_advance(sv, &r->ctl, r->tstart, r->tend, &dt, &r->run);
-    if (!rc) rc = sc_get_q(sv, r->q.data());
-    if (sv) sc_destroy(sv);
-    std::lock_guard<std::mutex> lk(S->mu);
-    finish_locked(S, r, rc);
-    S->workers--;
-    S->cv_done.notify_all();
+    return a.physics == b.physics && a.meqn == b.meqn && a.char_decomp == b.char_decomp &&
+           a.precision == b.precision && a.time_integrator == b.time_integrator &&
+           a.mthbc[0] == b.mthbc[0] && a.mthbc[1] == b.mthbc[1] && a.mthbc[2] == b.mthbc[2] &&
+           a.mthbc[3] == b.mthbc[3] && a.gamma == b.gamma && a.grav == b.grav && a.efix == b.efix;
+}
+
+Group2D *group2d_for(sc_server *S, const sc_params &p)
+{
+    for (auto &g : S->groups2d)
+        if (same_group2d(g->p, p)) return g.get();
+    auto g = std::make_unique<Group2D>();
+    g->p = p;
+    if (g->run.init(p)) return nullptr;
+    S->groups2d.push_back(std::move(g));
+    return S->groups2d.back().get();
 }
 
 Group *group_for(sc_server *S, const sc_params &p)
@@ -282,11 +304,31 @@ bool ensure_cap(Group *g, int n)
 int start_request(sc_server *S, Req *r)
 {
     if (r->p.ndim == 2) {
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            S->workers++;
-        }
-        std::thread(worker2d, S, r).detach();
+        Group2D *g = group2d_for(S, r->p);
+        if (!g) return sc_fail(SC_ERR_HIP, "server: stream/event creation failed");
+        int tx, ty;
+        tile2d_shape(g->run.tile, &tx, &ty);
+        Grid2D &G = r->g2;
+        memset(&G, 0, sizeof G);
+        G.off = r->off / (long long)sizeof(double);
+        G.nx = r->p.nx[0];
+        G.ny = r->p.nx[1];
+        G.LX = G.nx + 6;
+        G.LY = G.ny + 6;
+        G.ntx = (G.nx + tx - 1) / tx;
+        G.nty = (G.ny + ty - 1) / ty;
+        G.dx = r->p.dx[0];
+        G.dy = r->p.dx[1];
+        G.tstart = r->tstart;
+        G.tend = r->tend;
+        G.ctl = r->ctl;
+        int info0 = 0;
+        loop_limits(r->ctl, r->tstart, r->tend, r->dt0, &G.maxn, &info0);
+        r->ss = initial_state(r->tstart, r->dt0, 0, info0);
+        hipError_t e = hipMemcpyAsync(S->arena_base + G.off, r->q.data(), r->q.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, g->run.st);
+        if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "server upload: %s", hipGetErrorString(e));
+        g->incoming.push_back(r);
         return SC_OK;
     }
     Group *g = group_for(S, r->p);
@@ -306,12 +348,78 @@ int start_request(sc_server *S, Req *r)
     hipError_t e = hipMemcpyAsync(S->arena_base + r->bs.off, r->q.data(), len * sizeof(double),
                                   hipMemcpyHostToDevice, g->st);
     if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "server upload: %s", hipGetErrorString(e));
-    g->active.push_back(r);
+    g->incoming.push_back(r);
+    return SC_OK;
+}
+
+void record_run(Req *r)
+{
+    sc_batch_result br;
+    state_to_result(r->ss, 1.0, 0, &br);
+    memset(&r->run, 0, sizeof r->run);
+    r->run.t = br.t;
+    r->run.cfl_maxused = br.cfl_maxused;
+    r->run.cfl_last = br.cfl_last;
+    r->run.dt_minused = br.dt_minused;
+    r->run.dt_maxused = br.dt_maxused;
+    r->run.dt_last = br.dt_last;
+    r->run.steps_taken = br.steps_taken;
+    r->run.info = br.info;
+    r->run.rejected = br.rejected;
+    r->run.stage_evals = br.stage_evals;
+}
+
+int launch_round2d(sc_server *S, Group2D *g)
+{
+    g->active.insert(g->active.end(), g->incoming.begin(), g->incoming.end());
+    g->incoming.clear();
+    const int n = (int)g->active.size();
+    int rc = g->run.reserve(n);
+    if (rc) return rc;
+    long long tiles = 0;
+    for (int i = 0; i < n; i++) {
+        g->run.h_grids[i] = g->active[i]->g2;
+        g->run.h_state[i] = g->active[i]->ss;
+        tiles += (long long)g->active[i]->g2.ntx * g->active[i]->g2.nty;
+    }
+    const int K = round_iterations(stages_of_integrator(g->p.time_integrator), tiles);
+    return g->run.launch_round(n, K, S->arena_base);
+}
+
+int collect_round2d(sc_server *S, Group2D *g)
+{
+    int rc = g->run.wait();
+    if (rc) return rc;
+    std::vector<Req *> keep, done;
+    for (size_t i = 0; i < g->active.size(); i++) {
+        Req *r = g->active[i];
+        r->ss = g->run.h_state[i];
+        r->rounds++;
+        (r->ss.done ? done : keep).push_back(r);
+    }
+    for (Req *r : done) {
+        const Grid2D &G = r->g2;
+        const size_t n = (size_t)G.LX * G.LY * r->p.meqn;
+        double *q = S->arena_base + G.off + (long long)r->ss.cur * n;
+        hipError_t e = launch_bc(2, r->p.meqn, q, G.nx, G.ny, G.LX, G.LY, r->p.mthbc, g->run.st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(r->q.data(), q, n * sizeof(double), hipMemcpyDeviceToHost, g->run.st);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->run.st);
+        if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "server download: %s", hipGetErrorString(e));
+        record_run(r);
+    }
+    g->active.swap(keep);
+    if (!done.empty()) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (Req *r : done) finish_locked(S, r, SC_OK);
+    }
     return SC_OK;
 }
 
 int launch_round(sc_server *S, Group *g)
 {
+    g->active.insert(g->active.end(), g->incoming.begin(), g->incoming.end());
+    g->incoming.clear();
     std::stable_sort(g->active.begin(), g->active.end(),
                      [](const Req *a, const Req *b) { return a->bs.mx > b->bs.mx; });
     const int n = (int)g->active.size();
@@ -367,19 +475,7 @@ int collect_round(sc_server *S, Group *g)
         if (e == hipSuccess) e = hipStreamSynchronize(g->st);
         if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "server download: %s", hipGetErrorString(e));
         fill_ghosts_1d(r->q.data(), n, r->p.meqn, r->p.mthbc[0], r->p.mthbc[1]);
-        sc_batch_result br;
-        state_to_result(r->ss, S->tick_ms, 0, &br);
-        memset(&r->run, 0, sizeof r->run);
-        r->run.t = br.t;
-        r->run.cfl_maxused = br.cfl_maxused;
-        r->run.cfl_last = br.cfl_last;
-        r->run.dt_minused = br.dt_minused;
-        r->run.dt_maxused = br.dt_maxused;
-        r->run.dt_last = br.dt_last;
-        r->run.steps_taken = br.steps_taken;
-        r->run.info = br.info;
-        r->run.rejected = br.rejected;
-        r->run.stage_evals = br.stage_evals;
+        record_run(r);
     }
     g->active.swap(keep);
     if (!done.empty()) {
@@ -395,7 +491,15 @@ void fail_all(sc_server *S, const char *msg)
     S->fatal = msg;
     for (auto &g : S->groups) {
         for (Req *r : g->active) finish_locked(S, r, SC_ERR_HIP);
+        for (Req *r : g->incoming) finish_locked(S, r, SC_ERR_HIP);
         g->active.clear();
+        g->incoming.clear();
+    }
+    for (auto &g : S->groups2d) {
+        for (Req *r : g->active) finish_locked(S, r, SC_ERR_HIP);
+        for (Req *r : g->incoming) finish_locked(S, r, SC_ERR_HIP);
+        g->active.clear();
+        g->incoming.clear();
     }
     while (!S->queue.empty()) {
         Req *r = S->queue.front();
@@ -406,32 +510,47 @@ void fail_all(sc_server *S, const char *msg)
     S->stop = true;
 }
 
+// ready: the group's round in flight has finished (or failed: < 0)
+int round_done(hipEvent_t ev)
+{
+    hipError_t e = hipEventQuery(ev);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    sc_fail(SC_ERR_HIP, "server round: %s", hipGetErrorString(e));
+    return -1;
+}
+
+// Each group runs its rounds back to back on its own stream, independently
+// of the other groups: a group is relaunched as soon as its own round has
+// been collected, with the requests admitted meanwhile.
 void sched_loop(sc_server *S)
 {
     (void)hipSetDevice(S->dev);
     for (;;) {
         std::vector<Req *> admitted;
+        bool busy = false;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            for (;;) {
-                if (S->stop) return;
-                // admission in arrival order: footprint and a free slot
-                while (!S->queue.empty() && S->n_active < S->max_active) {
-                    Req *r = S->queue.front();
-                    long long off = S->arena.alloc(r->bytes);
-                    if (off < 0) break;
-                    S->queue.pop_front();
-                    r->off = off;
-                    r->status = 1;
-                    r->t_admit = S->now_ms();
-                    S->n_active++;
-                    S->peak_active = std::max(S->peak_active, S->n_active);
-                    admitted.push_back(r);
-                }
-                bool has1d = !admitted.empty();
-                for (auto &g : S->groups) has1d = has1d || !g->active.empty();
-                if (has1d) break;
+            if (S->stop) return;
+            // admission in arrival order: footprint and a free slot
+            while (!S->queue.empty() && S->n_active < S->max_active) {
+                Req *r = S->queue.front();
+                long long off = S->arena.alloc(r->bytes);
+                if (off < 0) break;
+                S->queue.pop_front();
+                r->off = off;
+                r->status = 1;
+                r->t_admit = S->now_ms();
+                S->n_active++;
+                S->peak_active = std::max(S->peak_active, S->n_active);
+                admitted.push_back(r);
+            }
+            busy = !admitted.empty();
+            for (auto &g : S->groups) busy = busy || g->inflight || !g->active.empty() || !g->incoming.empty();
+            for (auto &g : S->groups2d) busy = busy || g->inflight || !g->active.empty() || !g->incoming.empty();
+            if (!busy) {
                 S->cv_work.wait_for(lk, std::chrono::milliseconds(20));
+                continue;
             }
         }
         for (Req *r : admitted) {
@@ -441,25 +560,55 @@ void sched_loop(sc_server *S)
                 finish_locked(S, r, rc);
             }
         }
-        bool any = false;
+        bool progressed = false;
+        long long finished_rounds = 0;
+        for (auto &g : S->groups2d) {
+            if (g->inflight || (g->active.empty() && g->incoming.empty())) continue;
+            if (launch_round2d(S, g.get())) {
+                fail_all(S, "server round launch failed");
+                return;
+            }
+            g->inflight = true;
+            progressed = true;
+        }
         for (auto &g : S->groups) {
-            if (g->active.empty()) continue;
+            if (g->inflight || (g->active.empty() && g->incoming.empty())) continue;
             if (launch_round(S, g.get())) {
                 fail_all(S, "server round launch failed");
                 return;
             }
-            any = true;
+            g->inflight = true;
+            progressed = true;
         }
-        if (!any) continue;
         for (auto &g : S->groups) {
-            if (g->active.empty()) continue;
-            if (collect_round(S, g.get())) {
+            if (!g->inflight) continue;
+            int d = round_done(g->ev);
+            if (d < 0 || (d > 0 && collect_round(S, g.get()))) {
                 fail_all(S, "server round failed");
                 return;
             }
+            if (d > 0) {
+                g->inflight = false;
+                progressed = true;
+                finished_rounds++;
+            }
         }
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->rounds++;
+        for (auto &g : S->groups2d) {
+            if (!g->inflight) continue;
+            int d = round_done(g->run.ev);
+            if (d < 0 || (d > 0 && collect_round2d(S, g.get()))) {
+                fail_all(S, "server round failed");
+                return;
+            }
+            if (d > 0) {
+                g->inflight = false;
+                progressed = true;
+                finished_rounds++;
+            }
+        }
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->rounds += finished_rounds;
+        if (!progressed) S->cv_work.wait_for(lk, std::chrono::microseconds(50));
     }
 }
 
@@ -508,13 +657,11 @@ extern "C" int sc_server_submit(sc_server *S, const sc_params *p, const sc_run_c
     int rc = sc_check_params(p, true);
     if (rc) return rc;
     if (p->device != S->dev) return sc_fail(SC_ERR_ARG, "request device %d, server device %d", p->device, S->dev);
-    if (p->ndim == 1) {
-        if (p->time_integrator < 1 || p->time_integrator > 4)
-            return sc_fail(SC_ERR_ARG, "time_integrator should be between 1 and 4");
-        for (int k = 0; k < 2; k++)
-            if (p->mthbc[k] < 1 || p->mthbc[k] > 3)
-                return sc_fail(SC_ERR_UNSUPPORTED, "served 1D solves need mthbc 1, 2 or 3");
-    }
+    if (p->time_integrator < 1 || p->time_integrator > 4)
+        return sc_fail(SC_ERR_ARG, "time_integrator should be between 1 and 4");
+    for (int k = 0; k < 2 * p->ndim; k++)
+        if (p->mthbc[k] < 1 || p->mthbc[k] > 3)
+            return sc_fail(SC_ERR_UNSUPPORTED, "served solves need mthbc 1, 2 or 3");
     auto r = std::make_unique<Req>();
     r->p = *p;
     if (r->p.ndim == 1) {
@@ -527,7 +674,7 @@ extern "C" int sc_server_submit(sc_server *S, const sc_params *p, const sc_run_c
     r->dt0 = dt_initial;
     const size_t n = grid_doubles(r->p);
     r->q.assign(q, q + n);
-    r->bytes = (p->ndim == 1 ? 4 * n : sc_solver_doubles(&r->p)) * sizeof(double);
+    r->bytes = 4 * n * sizeof(double);  // 3 q buffers + the SSP(10,4) register
     memset(&r->run, 0, sizeof r->run);
     std::lock_guard<std::mutex> lk(S->mu);
     if (!S->fatal.empty()) return sc_fail(SC_ERR_HIP, "server stopped: %s", S->fatal.c_str());
@@ -622,7 +769,7 @@ extern "C" int sc_server_get_stats(sc_server *S, sc_server_stats *st)
 }
 
 // Outstanding requests are abandoned: the scheduler ends after its current
-// round and 2D workers run to the end of their solve.
+// round.
 extern "C" int sc_server_destroy(sc_server *S)
 {
     if (!S) return SC_OK;
@@ -632,13 +779,10 @@ extern "C" int sc_server_destroy(sc_server *S)
         S->cv_work.notify_all();
     }
     if (S->sched.joinable()) S->sched.join();
-    {
-        std::unique_lock<std::mutex> lk(S->mu);
-        S->cv_done.wait(lk, [&] { return S->workers == 0; });
-    }
     int prev = 0;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(S->dev);
+    for (auto &g : S->groups2d) g->run.destroy();
     for (auto &g : S->groups) {
         if (g->st) (void)hipStreamSynchronize(g->st);
         free_group_buffers(g.get());

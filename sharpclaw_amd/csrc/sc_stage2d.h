@@ -1,0 +1,164 @@
+// The fused 2D stage on one tile, shared by the single-grid stage kernel
+// (sc_kernels.hip) and the batched 2D kernels (sc_batch2d.hip).
+#pragma once
+#include "sc_stage.h"
+
+namespace sc {
+
+// ------------------------------------------------------ 2D fused stage
+
+// A block of 256 threads updates the TX x TY tile b (row-major over the
+// a.ntx x a.nty tiles).  q of the tile plus 3-cell halos along x
+// and y is staged in LDS (the corner blocks are not needed: every stencil
+// is one-directional).  x-interfaces, x in-cell problems, y-interfaces and
+// y in-cell problems run in turn on the same staged tile; the interface
+// results pass through LDS to the cells on either side.
+// CFL: the reference also sweeps the ghost rows j = 0, ny+1 and columns
+// i = 0, nx+1 (flux2.f90:36,68); for boundary options 1-3 their states are
+// copies (wall: with the tangential momentum negated) of the adjacent
+// interior row/column, whose wave speeds they therefore repeat exactly, so
+// the interior sweeps give the same maximum.
+template <int P, int CHAR, class T, int TX, int TY>
+__device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst<T> &c, int b)
+{
+    constexpr int M = Shape<P>::M, W = Shape<P>::W;
+    constexpr int NT = 256;
+    static_assert(TX * TY <= NT, "one cell per thread");
+    constexpr int LXT = TX + 6, LYT = TY + 6;
+    constexpr int NIX = (TX + 1) * TY, NIY = TX * (TY + 1);
+    constexpr int NI = NIX > NIY ? NIX : NIY;
+    __shared__ T sq[M][LYT][LXT];
+    __shared__ T sL[M][NI], sR[M][NI], sAm[M][NI], sAp[M][NI];
+    const int t = threadIdx.x;
+    const int nx = a.nx, ny = a.ny;
+
+    const int i0 = 1 + (b % a.ntx) * TX, j0 = 1 + (b / a.ntx) * TY;
+
+    for (int k = t; k < LYT * LXT; k += NT) {
+        int ly = k / LXT, lx = k - ly * LXT;
+        bool cx = lx < 3 || lx >= TX + 3, cy = ly < 3 || ly >= TY + 3;
+        if (cx && cy) continue;
+        int i = min(max(i0 - 3 + lx, -2), nx + 3);
+        int j = min(max(j0 - 3 + ly, -2), ny + 3);
+        bool negx = false, negy = false;
+        if (j >= 1 && j <= ny) {
+            i = bc_src(i, nx, a.bc[0], a.bc[1], negx);
+        } else {
+            i = min(max(i, 1), nx);
+            j = bc_src(j, ny, a.bc[2], a.bc[3], negy);
+        }
+        size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
+        size_t plane = (size_t)a.LX * a.LY;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            double val = a.qin[g + m * plane];
+            if ((m == 1 && negx) || (m == 2 && negy)) val = -val;
+            sq[m][ly][lx] = T(val);
+        }
+    }
+    __syncthreads();
+
+    T smx = T(0), smy = T(0);
+    T wave[M * W], s[W];
+    // x-interfaces
+    for (int k = t; k < NIX; k += NT) {
+        int r = k / (TX + 1), ci = k - r * (TX + 1);
+        T v[6][M], lf[M], rt[M], am[M], ap[M];
+#pragma unroll
+        for (int kk = 0; kk < 6; kk++)
+#pragma unroll
+            for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + 3][ci + kk];
+        reconstruct_iface<P, 1, CHAR, T>(c, v, lf, rt);
+        riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
+        if (i0 + ci <= nx + 1 && j0 + r <= ny) {
+#pragma unroll
+            for (int w = 0; w < W; w++) smx = fmax(smx, fabs(s[w]));
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            sL[m][k] = lf[m];
+            sR[m][k] = rt[m];
+            sAm[m][k] = am[m];
+            sAp[m][k] = ap[m];
+        }
+    }
+    __syncthreads();
+    const int cx = t % TX, cy = t / TX;
+    const bool owns_cell = t < TX * TY;
+    double dqx[M];
+    if (owns_cell) {
+        int kl = cy * (TX + 1) + cx, kr = kl + 1;
+        T ql_[M], qr_[M], am2[M], ap2[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            ql_[m] = sR[m][kl];
+            qr_[m] = sL[m][kr];
+        }
+        riemann<P, 1, T>(c, ql_, qr_, wave, s, am2, ap2);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            T X = ((sAm[m][kr] + sAp[m][kl]) + am2[m]) + ap2[m];
+            dqx[m] = -(a.dtdx * (double)X);
+        }
+    }
+    __syncthreads();
+    // y-interfaces
+    for (int k = t; k < NIY; k += NT) {
+        int r = k / TX, ci = k - r * TX;
+        T v[6][M], lf[M], rt[M], am[M], ap[M];
+#pragma unroll
+        for (int kk = 0; kk < 6; kk++)
+#pragma unroll
+            for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + kk][ci + 3];
+        reconstruct_iface<P, 2, CHAR, T>(c, v, lf, rt);
+        riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
+        if (i0 + ci <= nx && j0 + r <= ny + 1) {
+#pragma unroll
+            for (int w = 0; w < W; w++) smy = fmax(smy, fabs(s[w]));
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            sL[m][k] = lf[m];
+            sR[m][k] = rt[m];
+            sAm[m][k] = am[m];
+            sAp[m][k] = ap[m];
+        }
+    }
+    __syncthreads();
+    if (owns_cell) {
+        const int i = i0 + cx, j = j0 + cy;
+        int kb = cy * TX + cx, kt = kb + TX;
+        T ql_[M], qr_[M], am2[M], ap2[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            ql_[m] = sR[m][kb];
+            qr_[m] = sL[m][kt];
+        }
+        riemann<P, 2, T>(c, ql_, qr_, wave, s, am2, ap2);
+        if (i <= nx && j <= ny) {
+            const bool rq = op_reads_qold(a.op);
+            size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
+            size_t plane = (size_t)a.LX * a.LY;
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                T Y = ((sAm[m][kt] + sAp[m][kb]) + am2[m]) + ap2[m];
+                double r = dqx[m] + -(a.dtdy * (double)Y);
+                size_t gm = g + m * plane;
+                double q = a.qin[gm];
+                double qo = rq ? a.qold[gm] : 0.0;
+                double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
+                double out, qbo = 0.0;
+                combine(a.op, qo, q, r, qbi, out, qbo);
+                a.qout[gm] = out;
+                if (a.op == OP_SSP104_5) a.qbout[gm] = qbo;
+            }
+        }
+    }
+    __shared__ double red[2][NT / 64];
+    if (a.smax) {
+        merge_smax<T, NT / 64>(a.smax, smx, red[0]);
+        merge_smax<T, NT / 64>(a.smax + 1, smy, red[1]);
+    }
+}
+
+}  // namespace sc

@@ -161,3 +161,39 @@ def test_server_rejects_bad_requests():
         with pytest.raises(ScError):
             srv.submit(p, ctl, O.new_grid(p), 0.0, 0.01, 1e-4)
         assert srv.stats()["submitted"] == 0
+
+
+def test_server_2d_equals_solver_advance_bitwise():
+    """2D requests advance in device-resident rounds (stage launches over
+    many grids, time-loop decisions on the device); the result equals the
+    host-driven sc_advance bit for bit, for every integrator, including
+    rejected steps and the SSP(10,4) register."""
+    rng = np.random.default_rng(23)
+    cases = []
+    for k, (phys, ti, cfl) in enumerate([("euler", 3, (0.45, 0.5)), ("euler", 4, (2.45, 2.5)),
+                                         ("shallow_water", 1, (0.45, 0.5)), ("shallow_water", 2, (0.9, 1.0)),
+                                         ("euler", 3, (0.45, 0.5))]):
+        n = (int(rng.integers(33, 80)), int(rng.integers(33, 80)))
+        prec = SC_FP32 if k == 4 else SC_FP64
+        p = make_params(phys, n, time_integrator=ti, mthbc=(1, 3, 1, 3), precision=prec)
+        q = O.new_grid(p)
+        import _states
+        interior(p, q)[:] = _states.random_cells(p, n, rng, smooth=True, amp=0.3)
+        ctl = ScRunCtl(dt_max=1e99, cfl_max=cfl[1], cfl_desired=cfl[0], max_steps=100000, dt_variable=1)
+        cases.append((p, q, ctl))
+    from sharpclaw_amd import Solver
+    with Server(hbm_bytes=256 << 20, max_active=16) as srv:
+        ids = [srv.submit(p, ctl, q, 0.0, 0.03, 0.03) for p, q, ctl in cases]
+        srv.drain(120e3)
+        got = [srv.result(rid) for rid in ids]
+    rejected = 0
+    for (p, q, ctl), (gq, gs) in zip(cases, got):
+        with Solver(p) as s:
+            s.set_q(q)
+            dt, st = s.advance(ctl, 0.0, 0.03, 0.03)
+            rq = s.get_q()
+        assert gs.run.rejected == st.rejected
+        rejected += st.rejected
+        assert (gs.run.steps_taken, gs.run.dt_last, gs.run.t, gs.run.info) == (st.steps_taken, dt, st.t, st.info)
+        assert np.all(np.isfinite(interior(p, rq))) and np.array_equal(gq, rq, equal_nan=True)
+    assert rejected > 0
