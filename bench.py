@@ -382,6 +382,14 @@ def main():
         tile, (tx, ty) = s.get_tile()
         log(f"bench: fused stage kernel {ms_launch * 1e3:.1f} us per launch (tile {tx}x{ty})")
         sums = s.sums()
+        # PCIe-inclusive: the same kind of loop with q uploaded before and
+        # downloaded after it, as a host-array caller sees it
+        ctl_p = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=20, dt_variable=1)
+        w1 = time.perf_counter()
+        s.set_q(q0)
+        _, pst = s.advance(ctl_p, 0.0, 1e9, 1e-4)
+        s.get_q()
+        pcie_value = cells * pst.stage_evals / (time.perf_counter() - w1)
 
     from sharpclaw_amd import copy_bench
     copy_bytes = 1 << 30
@@ -421,6 +429,7 @@ def main():
         "copy_kernel_gbs": copy_gbs,
         "frac_of_copy_bw": achieved / copy_gbs,
         "pct_hbm_peak": 100.0 * achieved / HBM_PEAK_GBS,
+        "pcie_inclusive_value": pcie_value,
         "conservation_sums": [float(x) for x in sums],
     }
     if not args.no_cpu:
