@@ -55,6 +55,7 @@ struct Batch2DRunner {
     int ti = 3;
     int bc[4] = {1, 1, 1, 1};
     hipStream_t st = nullptr;
+    bool owns_st = true;
     hipEvent_t ev = nullptr;
     int cap = 0;
     Grid2D *d_grids = nullptr;
@@ -62,16 +63,30 @@ struct Batch2DRunner {
     unsigned long long *d_smax = nullptr;
     Grid2D *h_grids = nullptr;      // pinned
     SolveState *h_state = nullptr;  // pinned
+    // a captured round (continue_round), valid for (g_n, g_K, g_tiles, g_base)
+    hipGraphExec_t g_exec = nullptr;
+    int g_n = 0, g_K = 0;
+    long long g_tiles = 0;
+    double *g_base = nullptr;
 
-    int init(const sc_params &p);
+    // stream: the caller's stream to work on, or NULL for one of its own
+    int init(const sc_params &p, hipStream_t stream = nullptr);
+    void set_tile(int t);
     int reserve(int n);
     // h_grids[0..n) and h_state[0..n) filled by the caller (tile0 is set
-    // here); states come back into h_state when the round's event fires.
+    // here): uploaded, then one round; states come back into h_state when
+    // the round's event fires.
     int launch_round(int n, int K, double *base);
+    // One more round from the device states left by the previous round,
+    // replayed from a captured graph when the caller allows it.
+    int continue_round(int n, int K, double *base, bool graph);
     int wait();
     void destroy();
-};
 
+  private:
+    long long assign_tiles(int n);
+    hipError_t enqueue_steps(int n, int K, long long tiles, double *base);
+};
 // Iterations per round so that a round is a few milliseconds of work.
 int round_iterations(int stages, long long tiles);
 

@@ -34,8 +34,13 @@ __global__ __launch_bounds__(256) void stage2d_batch_kernel(Batch2DArgs A, PhysC
     const Grid2D &G = A.grids[lo];
     const SolveState &s = A.state[lo];
     if (!s.go) return;
-    const int local = blk - G.tile0;
-    if (local >= G.ntx * G.nty) return;
+    const int ntiles = G.ntx * G.nty;
+    int local = blk - G.tile0;
+    if (local >= ntiles) return;
+    // XCD-aware order within the grid (its first tile is a multiple of 8,
+    // so block b runs on XCD b mod 8): each XCD gets a contiguous band of
+    // the grid's tiles and keeps the halo rows it re-reads in its L2
+    if ((ntiles & 7) == 0) local = (local & 7) * (ntiles >> 3) + (local >> 3);
     const Plan plan = stage_plan(A.ti);
     int in = s.cur, out = other_buf(in, s.old);
     for (int k = 0; k < A.stage; k++) {
@@ -210,7 +215,7 @@ int round_iterations(int stages, long long tiles)
     return (int)std::min(64LL, std::max(1LL, k));
 }
 
-int Batch2DRunner::init(const sc_params &p)
+int Batch2DRunner::init(const sc_params &p, hipStream_t stream)
 {
     sel.phys = p.physics == SC_PHYS_EULER ? EU2 : SW2;
     sel.chr = p.char_decomp;
@@ -219,10 +224,20 @@ int Batch2DRunner::init(const sc_params &p)
     cf = sc_make_const<float>(&p);
     ti = p.time_integrator;
     for (int k = 0; k < 4; k++) bc[k] = p.mthbc[k];
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+    owns_st = stream == nullptr;
+    st = stream;
+    if ((owns_st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) ||
         hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
         return sc_fail(SC_ERR_HIP, "2D batch: stream/event creation failed");
     return SC_OK;
+}
+
+void Batch2DRunner::set_tile(int t)
+{
+    if (t == tile) return;
+    if (g_exec) (void)hipGraphExecDestroy(g_exec);
+    g_exec = nullptr;
+    tile = t;
 }
 
 int Batch2DRunner::reserve(int n)
@@ -230,6 +245,8 @@ int Batch2DRunner::reserve(int n)
     if (n <= cap) return SC_OK;
     int c = std::max(n, 2 * cap);
     if (st) (void)hipStreamSynchronize(st);
+    if (g_exec) (void)hipGraphExecDestroy(g_exec);
+    g_exec = nullptr;
     if (d_grids) (void)hipFree(d_grids);
     if (d_state) (void)hipFree(d_state);
     if (d_smax) (void)hipFree(d_smax);
@@ -251,16 +268,22 @@ int Batch2DRunner::reserve(int n)
     return SC_OK;
 }
 
-int Batch2DRunner::launch_round(int n, int K, double *base)
+// First tiles at multiples of 8 (the blocks in between exit at once).
+long long Batch2DRunner::assign_tiles(int n)
 {
-    int rc = reserve(n);
-    if (rc) return rc;
     long long tiles = 0;
     for (int i = 0; i < n; i++) {
+        tiles = (tiles + 7) & ~7LL;
         h_grids[i].tile0 = (int)tiles;
         tiles += (long long)h_grids[i].ntx * h_grids[i].nty;
     }
-    if (tiles > 0x7fffffffLL) return sc_fail(SC_ERR_ARG, "2D batch: too many tiles");
+    return tiles;
+}
+
+// ctl(pre), then K x [stages, ctl(post, pre unless last)], then the states
+// back to the pinned staging.
+hipError_t Batch2DRunner::enqueue_steps(int n, int K, long long tiles, double *base)
+{
     Batch2DArgs A;
     memset(&A, 0, sizeof A);
     A.grids = d_grids;
@@ -271,9 +294,7 @@ int Batch2DRunner::launch_round(int n, int K, double *base)
     for (int k = 0; k < 4; k++) A.bc[k] = bc[k];
     A.ti = ti;
     const int nst = stages_of_integrator(ti);
-    hipError_t e = hipMemcpyAsync(d_grids, h_grids, n * sizeof(Grid2D), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_state, h_state, n * sizeof(SolveState), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = launch_ctl2d(A, 0, 1, st);
+    hipError_t e = launch_ctl2d(A, 0, 1, st);
     for (int it = 0; it < K && e == hipSuccess; it++) {
         for (int k = 0; k < nst && e == hipSuccess; k++) {
             A.stage = k;
@@ -282,6 +303,56 @@ int Batch2DRunner::launch_round(int n, int K, double *base)
         if (e == hipSuccess) e = launch_ctl2d(A, 1, it + 1 < K, st);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(h_state, d_state, n * sizeof(SolveState), hipMemcpyDeviceToHost, st);
+    return e;
+}
+
+int Batch2DRunner::launch_round(int n, int K, double *base)
+{
+    int rc = reserve(n);
+    if (rc) return rc;
+    long long tiles = assign_tiles(n);
+    if (tiles > 0x7fffffffLL) return sc_fail(SC_ERR_ARG, "2D batch: too many tiles");
+    hipError_t e = hipMemcpyAsync(d_grids, h_grids, n * sizeof(Grid2D), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_state, h_state, n * sizeof(SolveState), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = enqueue_steps(n, K, tiles, base);
+    if (e == hipSuccess) e = hipEventRecord(ev, st);
+    if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "2D batch round: %s", hipGetErrorString(e));
+    return SC_OK;
+}
+
+int Batch2DRunner::continue_round(int n, int K, double *base, bool graph)
+{
+    long long tiles = 0;
+    for (int i = 0; i < n; i++) tiles = ((tiles + 7) & ~7LL) + (long long)h_grids[i].ntx * h_grids[i].nty;
+    hipError_t e = hipSuccess;
+    if (!graph) {
+        e = enqueue_steps(n, K, tiles, base);
+    } else {
+        if (g_exec && (g_n != n || g_K != K || g_tiles != tiles || g_base != base)) {
+            (void)hipGraphExecDestroy(g_exec);
+            g_exec = nullptr;
+        }
+        if (!g_exec) {
+            hipGraph_t gr = nullptr;
+            e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e == hipSuccess) {
+                hipError_t e2 = enqueue_steps(n, K, tiles, base);
+                e = hipStreamEndCapture(st, &gr);
+                if (e == hipSuccess) e = e2;
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&g_exec, gr, nullptr, nullptr, 0);
+            if (gr) (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) {
+                g_exec = nullptr;
+                return sc_fail(SC_ERR_HIP, "2D round capture: %s", hipGetErrorString(e));
+            }
+            g_n = n;
+            g_K = K;
+            g_tiles = tiles;
+            g_base = base;
+        }
+        e = hipGraphLaunch(g_exec, st);
+    }
     if (e == hipSuccess) e = hipEventRecord(ev, st);
     if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "2D batch round: %s", hipGetErrorString(e));
     return SC_OK;
@@ -297,13 +368,15 @@ int Batch2DRunner::wait()
 void Batch2DRunner::destroy()
 {
     if (st) (void)hipStreamSynchronize(st);
+    if (g_exec) (void)hipGraphExecDestroy(g_exec);
+    g_exec = nullptr;
     if (d_grids) (void)hipFree(d_grids);
     if (d_state) (void)hipFree(d_state);
     if (d_smax) (void)hipFree(d_smax);
     if (h_grids) (void)hipHostFree(h_grids);
     if (h_state) (void)hipHostFree(h_state);
     if (ev) (void)hipEventDestroy(ev);
-    if (st) (void)hipStreamDestroy(st);
+    if (st && owns_st) (void)hipStreamDestroy(st);
     d_grids = nullptr;
     d_state = nullptr;
     d_smax = nullptr;

@@ -19,6 +19,8 @@
 #include <tuple>
 
 #include "../../include/sharpclaw_amd.h"
+#include "sc_batch2d.h"
+#include "sc_batch_host.h"
 #include "sc_internal.h"
 #include "sc_kernels.h"
 
@@ -137,6 +139,10 @@ struct sc_solver {
     int cur, old;      // buffer indices of q and qold
     bool have_q;
     int tile2d;        // 2D tile shape index; -1: tune at first use
+    double *mem;       // one allocation: buf[0..2], qb, dq
+    int loop_mode;     // 2D sc_advance: -1 by grid size, 0 host-driven, 1 device-resident
+    bool use_graph;    // device loop: replay rounds from a captured graph
+    Batch2DRunner *dr; // 2D device-resident loop (created at first use)
     hipEvent_t ev0, ev1;
 };
 
@@ -181,9 +187,12 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     s->n = grid_n(&s->p);
     s->nblk_sums = 256;
     bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
-    for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&s->buf[k], s->n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&s->qb, s->n * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&s->dq, s->n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->mem, 5 * s->n * sizeof(double)) == hipSuccess;
+    if (ok) {
+        for (int k = 0; k < 3; k++) s->buf[k] = s->mem + k * s->n;
+        s->qb = s->mem + 3 * s->n;
+        s->dq = s->mem + 4 * s->n;
+    }
     ok = ok && hipMalloc(&s->partial, (size_t)s->nblk_sums * s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->sums, s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->slots, NSLOT * 2 * sizeof(unsigned long long)) == hipSuccess;
@@ -204,6 +213,10 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     s->cur = 0;
     s->old = 0;
     s->tile2d = -1;
+    s->loop_mode = -1;
+    if (const char *e = getenv("SC_HOST_LOOP")) s->loop_mode = atoi(e) != 0 ? 0 : 1;
+    s->use_graph = false;
+    if (const char *e = getenv("SC_GRAPH")) s->use_graph = atoi(e) != 0;
     if (const char *e = getenv("SC_TILE")) {
         int k = atoi(e);
         if (k >= 0 && k < NTILE2D) s->tile2d = k;
@@ -217,9 +230,11 @@ extern "C" int sc_destroy(sc_solver *s)
     if (!s) return SC_OK;
     DevGuard g(s->dev);
     if (s->st) hipStreamSynchronize(s->st);
-    for (int k = 0; k < 3; k++) if (s->buf[k]) hipFree(s->buf[k]);
-    if (s->qb) hipFree(s->qb);
-    if (s->dq) hipFree(s->dq);
+    if (s->dr) {
+        s->dr->destroy();
+        delete s->dr;
+    }
+    if (s->mem) hipFree(s->mem);
     if (s->partial) hipFree(s->partial);
     if (s->sums) hipFree(s->sums);
     if (s->slots) hipFree(s->slots);
@@ -486,6 +501,79 @@ static int stages_of(int ti)
     return n[ti];
 }
 
+// The sharpclaw() loop of a 2D grid kept on the device (sc_batch2d.h): the
+// CFL, dt update and accept/reject of every step are taken by a control
+// kernel, and the host reads the loop state back once per round of K steps
+// instead of the CFL once per step.  With use_graph, rounds after the first
+// replay one captured hipGraph (measured slower on ROCm 7 for this pattern:
+// 67.2 ms against 62.6 ms eager for 200 steps of 1024^2 Euler, so it is off
+// by default).  Same arithmetic as the host-driven loop below.
+static int advance2d_device(sc_solver *s, const sc_run_ctl *c, double tstart, double tend, double *dt_io,
+                            sc_run_stats *st)
+{
+    if (!s->dr) {
+        s->dr = new Batch2DRunner();
+        int rc = s->dr->init(s->p, s->st);
+        if (!rc) rc = s->dr->reserve(1);
+        if (rc) {
+            s->dr->destroy();
+            delete s->dr;
+            s->dr = nullptr;
+            return rc;
+        }
+    }
+    Batch2DRunner &R = *s->dr;
+    R.set_tile(s->tile2d);
+    int tx, ty;
+    tile2d_shape(s->tile2d, &tx, &ty);
+    Grid2D &G = R.h_grids[0];
+    memset(&G, 0, sizeof G);
+    G.nx = s->p.nx[0];
+    G.ny = s->p.nx[1];
+    G.LX = G.nx + 2 * s->p.mbc;
+    G.LY = G.ny + 2 * s->p.mbc;
+    G.ntx = (G.nx + tx - 1) / tx;
+    G.nty = (G.ny + ty - 1) / ty;
+    G.dx = s->p.dx[0];
+    G.dy = s->p.dx[1];
+    G.tstart = tstart;
+    G.tend = tend;
+    G.ctl = *c;
+    int info0 = 0;
+    loop_limits(*c, tstart, tend, *dt_io, &G.maxn, &info0);
+    R.h_state[0] = initial_state(tstart, *dt_io, s->cur, info0);
+    const int K = round_iterations(stages_of(s->p.time_integrator), (long long)G.ntx * G.nty);
+    HIPCHK(hipEventRecord(s->ev0, s->st));
+    int rc = R.launch_round(1, K, s->mem);
+    if (!rc) rc = R.wait();
+    while (!rc && !R.h_state[0].done) {
+        rc = R.continue_round(1, K, s->mem, s->use_graph);
+        if (!rc) rc = R.wait();
+    }
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(s->ev1, s->st));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    const SolveState &S = R.h_state[0];
+    s->cur = S.cur;
+    s->old = S.old;
+    memset(st, 0, sizeof *st);
+    st->gpu_ms = ms;
+    st->t = S.t;
+    st->cfl_maxused = S.cflmax;
+    st->cfl_last = S.cfl;
+    st->dt_minused = S.dtmin;
+    st->dt_maxused = S.dtmax;
+    st->dt_last = S.dt;
+    st->steps_taken = S.steps;
+    st->info = S.info;
+    st->rejected = S.rejected;
+    st->stage_evals = S.stages;
+    *dt_io = S.dt;
+    return SC_OK;
+}
+
 // src/sharpclaw.f90:35-235 (qold refreshed at every new step)
 extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, double tend, double *dt_io,
                           sc_run_stats *st)
@@ -495,6 +583,16 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
     DevGuard g(s->dev);
     int trc = tune_tile(s);
     if (trc) return trc;
+    if (s->p.ndim == 2) {
+        // The device loop saves the per-step CFL readback (~5-15 us); on
+        // grids whose stages take far longer than that the host loop is as
+        // fast (1024^2 Euler: 61.8 ms host, 62.6 ms device for 200 steps).
+        int tx, ty;
+        tile2d_shape(s->tile2d, &tx, &ty);
+        long long tiles = (long long)((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty);
+        bool device = s->loop_mode == 1 || (s->loop_mode < 0 && tiles < 2048);
+        if (device) return advance2d_device(s, c, tstart, tend, dt_io, st);
+    }
     memset(st, 0, sizeof *st);
     double t = tstart, dt = *dt_io, told = tstart, cflmax = 0.0, cfl = 0.0;
     double dtmin = dt, dtmax = dt;
