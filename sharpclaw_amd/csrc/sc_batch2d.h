@@ -51,7 +51,7 @@ struct Batch2DRunner {
     PhysSel sel;
     PhysConst<double> cd;
     PhysConst<float> cf;
-    int tile = 1;       // tile shape (sc_kernels.h tile table)
+    int tile = 0;       // tile shape (sc_kernels.h tile table)
     int ti = 3;
     int bc[4] = {1, 1, 1, 1};
     hipStream_t st = nullptr;

@@ -168,9 +168,9 @@ template <int P, int CHAR, class T>
 static hipError_t go2b_t(int tile, const Batch2DArgs &A, int ntiles, const PhysConst<T> &c, hipStream_t st)
 {
     switch (tile) {
-    case 0: return go2b<P, CHAR, T, 16, 16>(A, ntiles, c, st);
-    case 2: return go2b<P, CHAR, T, 15, 16>(A, ntiles, c, st);
-    case 3: return go2b<P, CHAR, T, 32, 7>(A, ntiles, c, st);
+    case 1: return go2b<P, CHAR, T, 15, 16>(A, ntiles, c, st);
+    case 2: return go2b<P, CHAR, T, 32, 7>(A, ntiles, c, st);
+    case 3: return go2b<P, CHAR, T, 14, 17>(A, ntiles, c, st);
     default: return go2b<P, CHAR, T, 16, 15>(A, ntiles, c, st);
     }
 }

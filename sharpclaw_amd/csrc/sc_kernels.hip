@@ -292,13 +292,13 @@ hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const P
     return hipErrorInvalidValue;
 }
 
-// Tile shapes of the 2D stage kernel (256 threads each).  16x16 runs 272
-// interfaces per sweep (a second, mostly idle pass); the others fit each
-// sweep in one pass at the price of 240 or 224 cells per tile.
+// Tile shapes of the 2D stage kernel (256 threads each): a thread per cell
+// plus spare threads for the tile's edge interfaces, TX*TY + max(TX, TY)
+// <= 256, so every sweep is one pass.
 struct TileShape {
     int tx, ty;
 };
-static const TileShape kTiles[NTILE2D] = {{16, 16}, {16, 15}, {15, 16}, {32, 7}};
+static const TileShape kTiles[NTILE2D] = {{16, 15}, {15, 16}, {32, 7}, {14, 17}};
 
 template <int P, int CHAR, class T, int TX, int TY>
 static hipError_t go2t(const StageArgs &a0, const PhysConst<T> &c, hipStream_t st)
@@ -315,10 +315,10 @@ template <int P, int CHAR, class T>
 static hipError_t go2(int tile, const StageArgs &a, const PhysConst<T> &c, hipStream_t st)
 {
     switch (tile) {
-    case 1: return go2t<P, CHAR, T, 16, 15>(a, c, st);
-    case 2: return go2t<P, CHAR, T, 15, 16>(a, c, st);
-    case 3: return go2t<P, CHAR, T, 32, 7>(a, c, st);
-    default: return go2t<P, CHAR, T, 16, 16>(a, c, st);
+    case 1: return go2t<P, CHAR, T, 15, 16>(a, c, st);
+    case 2: return go2t<P, CHAR, T, 32, 7>(a, c, st);
+    case 3: return go2t<P, CHAR, T, 14, 17>(a, c, st);
+    default: return go2t<P, CHAR, T, 16, 15>(a, c, st);
     }
 }
 

@@ -8,11 +8,16 @@ namespace sc {
 // ------------------------------------------------------ 2D fused stage
 
 // A block of 256 threads updates the TX x TY tile b (row-major over the
-// a.ntx x a.nty tiles).  q of the tile plus 3-cell halos along x
-// and y is staged in LDS (the corner blocks are not needed: every stencil
-// is one-directional).  x-interfaces, x in-cell problems, y-interfaces and
-// y in-cell problems run in turn on the same staged tile; the interface
-// results pass through LDS to the cells on either side.
+// a.ntx x a.nty tiles).  q of the tile plus 3-cell halos along x and y is
+// staged in LDS (the corner blocks are not needed: every stencil is
+// one-directional).  The thread of cell (cx, cy) solves the x-interface on
+// its right and, later, the y-interface above it; the TX*TY-th and later
+// threads solve the left-edge x-interfaces and the bottom-edge
+// y-interfaces.  A cell's in-cell problem and dq then need from its
+// neighbour only the interface's right state and apdq, which pass through
+// LDS; its own interface's left state and amdq stay in registers.  (Two
+// exchange arrays instead of four keep the LDS per block low enough for
+// four blocks per CU.)
 // CFL: the reference also sweeps the ghost rows j = 0, ny+1 and columns
 // i = 0, nx+1 (flux2.f90:36,68); for boundary options 1-3 their states are
 // copies (wall: with the tangential momentum negated) of the adjacent
@@ -23,15 +28,14 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
 {
     constexpr int M = Shape<P>::M, W = Shape<P>::W;
     constexpr int NT = 256;
-    static_assert(TX * TY <= NT, "one cell per thread");
+    static_assert(TX * TY + (TX > TY ? TX : TY) <= NT, "a thread per cell plus the edge interfaces");
     constexpr int LXT = TX + 6, LYT = TY + 6;
     constexpr int NIX = (TX + 1) * TY, NIY = TX * (TY + 1);
     constexpr int NI = NIX > NIY ? NIX : NIY;
     __shared__ T sq[M][LYT][LXT];
-    __shared__ T sL[M][NI], sR[M][NI], sAm[M][NI], sAp[M][NI];
+    __shared__ T sR[M][NI], sAp[M][NI];
     const int t = threadIdx.x;
     const int nx = a.nx, ny = a.ny;
-
     const int i0 = 1 + (b % a.ntx) * TX, j0 = 1 + (b / a.ntx) * TY;
 
     for (int k = t; k < LYT * LXT; k += NT) {
@@ -58,90 +62,105 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
     }
     __syncthreads();
 
+    const bool owns_cell = t < TX * TY;
+    const int e = t - TX * TY;  // edge-interface index of the spare threads
+    const int cx = owns_cell ? t % TX : 0, cy = owns_cell ? t / TX : 0;
     T smx = T(0), smy = T(0);
     T wave[M * W], s[W];
-    // x-interfaces
-    for (int k = t; k < NIX; k += NT) {
-        int r = k / (TX + 1), ci = k - r * (TX + 1);
-        T v[6][M], lf[M], rt[M], am[M], ap[M];
-#pragma unroll
-        for (int kk = 0; kk < 6; kk++)
-#pragma unroll
-            for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + 3][ci + kk];
-        reconstruct_iface<P, 1, CHAR, T>(c, v, lf, rt);
-        riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
-        if (i0 + ci <= nx + 1 && j0 + r <= ny) {
-#pragma unroll
-            for (int w = 0; w < W; w++) smx = fmax(smx, fabs(s[w]));
+    T lf[M], am[M];  // own interface: left state, amdq
+    double dqx[M];
+
+    // x-interfaces: interface ci of row r lies between tile cells ci-1, ci
+    {
+        int r = -1, ci = 0;
+        if (owns_cell) {
+            r = cy;
+            ci = cx + 1;
+        } else if (e < TY) {
+            r = e;
+            ci = 0;
         }
+        if (r >= 0) {
+            T v[6][M], rt[M], ap[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            sL[m][k] = lf[m];
-            sR[m][k] = rt[m];
-            sAm[m][k] = am[m];
-            sAp[m][k] = ap[m];
+            for (int kk = 0; kk < 6; kk++)
+#pragma unroll
+                for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + 3][ci + kk];
+            reconstruct_iface<P, 1, CHAR, T>(c, v, lf, rt);
+            riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
+            if (i0 + ci <= nx + 1 && j0 + r <= ny) {
+#pragma unroll
+                for (int w = 0; w < W; w++) smx = fmax(smx, fabs(s[w]));
+            }
+            const int k = r * (TX + 1) + ci;
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                sR[m][k] = rt[m];
+                sAp[m][k] = ap[m];
+            }
         }
     }
     __syncthreads();
-    const int cx = t % TX, cy = t / TX;
-    const bool owns_cell = t < TX * TY;
-    double dqx[M];
     if (owns_cell) {
-        int kl = cy * (TX + 1) + cx, kr = kl + 1;
-        T ql_[M], qr_[M], am2[M], ap2[M];
+        // in-cell problem of cell (cx, cy): ql = right state of the interface
+        // on its left, qr = left state of its own interface
+        const int kl = cy * (TX + 1) + cx;
+        T ql_[M], am2[M], ap2[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) ql_[m] = sR[m][kl];
+        riemann<P, 1, T>(c, ql_, lf, wave, s, am2, ap2);
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            ql_[m] = sR[m][kl];
-            qr_[m] = sL[m][kr];
-        }
-        riemann<P, 1, T>(c, ql_, qr_, wave, s, am2, ap2);
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            T X = ((sAm[m][kr] + sAp[m][kl]) + am2[m]) + ap2[m];
+            T X = ((am[m] + sAp[m][kl]) + am2[m]) + ap2[m];
             dqx[m] = -(a.dtdx * (double)X);
         }
     }
     __syncthreads();
-    // y-interfaces
-    for (int k = t; k < NIY; k += NT) {
-        int r = k / TX, ci = k - r * TX;
-        T v[6][M], lf[M], rt[M], am[M], ap[M];
-#pragma unroll
-        for (int kk = 0; kk < 6; kk++)
-#pragma unroll
-            for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + kk][ci + 3];
-        reconstruct_iface<P, 2, CHAR, T>(c, v, lf, rt);
-        riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
-        if (i0 + ci <= nx && j0 + r <= ny + 1) {
-#pragma unroll
-            for (int w = 0; w < W; w++) smy = fmax(smy, fabs(s[w]));
+    // y-interfaces: interface r of column ci lies between tile rows r-1, r
+    {
+        int r = -1, ci = 0;
+        if (owns_cell) {
+            r = cy + 1;
+            ci = cx;
+        } else if (e < TX) {
+            r = 0;
+            ci = e;
         }
+        if (r >= 0) {
+            T v[6][M], rt[M], ap[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            sL[m][k] = lf[m];
-            sR[m][k] = rt[m];
-            sAm[m][k] = am[m];
-            sAp[m][k] = ap[m];
+            for (int kk = 0; kk < 6; kk++)
+#pragma unroll
+                for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + kk][ci + 3];
+            reconstruct_iface<P, 2, CHAR, T>(c, v, lf, rt);
+            riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
+            if (i0 + ci <= nx && j0 + r <= ny + 1) {
+#pragma unroll
+                for (int w = 0; w < W; w++) smy = fmax(smy, fabs(s[w]));
+            }
+            const int k = r * TX + ci;
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                sR[m][k] = rt[m];
+                sAp[m][k] = ap[m];
+            }
         }
     }
     __syncthreads();
     if (owns_cell) {
         const int i = i0 + cx, j = j0 + cy;
-        int kb = cy * TX + cx, kt = kb + TX;
-        T ql_[M], qr_[M], am2[M], ap2[M];
+        const int kb = cy * TX + cx;
+        T ql_[M], am2[M], ap2[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            ql_[m] = sR[m][kb];
-            qr_[m] = sL[m][kt];
-        }
-        riemann<P, 2, T>(c, ql_, qr_, wave, s, am2, ap2);
+        for (int m = 0; m < M; m++) ql_[m] = sR[m][kb];
+        riemann<P, 2, T>(c, ql_, lf, wave, s, am2, ap2);
         if (i <= nx && j <= ny) {
             const bool rq = op_reads_qold(a.op);
             size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
             size_t plane = (size_t)a.LX * a.LY;
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                T Y = ((sAm[m][kt] + sAp[m][kb]) + am2[m]) + ap2[m];
+                T Y = ((am[m] + sAp[m][kb]) + am2[m]) + ap2[m];
                 double r = dqx[m] + -(a.dtdy * (double)Y);
                 size_t gm = g + m * plane;
                 double q = a.qin[gm];
