@@ -43,12 +43,16 @@ __device__ __forceinline__ double weno_side(double im, double a2, double a1, dou
 // small jumps (tt ~ jump^4, s ~ jump^8) and gives 0/0 in flat regions.  The
 // weights are homogeneous of degree 0 in (jumps, sqrt(eps)) and the
 // correction of degree 1, so the jumps are scaled by 1/M (M = max |jump|),
-// eps by 1/M^2, the weights formed as alpha_k = gamma_k / (eps + beta_k)^2
-// (w1 = s1/sum, w3 = s3/sum of the reference), and the correction scaled
-// back by M.  eps/M^2 is clamped to [1e-15, 1e15]; below the clamp the
-// reference weights differ only where the correction itself is below
-// float32 resolution.  Jumps below 1e-30 (where 1/M would overflow float32
-// near its denormal range) give no correction: its size is below M.
+// eps by 1/M^2, and the correction scaled back by M.  After scaling one of
+// the four jumps has magnitude 1, so the largest indicator is >= 3.9 (the
+// minimum over the other jump of 13(1-x)^2 + 3(1-3x)^2); with eps/M^2
+// clamped to [1e-15, 1e9] (every beta <= 100) each (eps + beta)^2 lies in
+// [1e-30, 1.1e18], so the reference's products and sum (s1, s2, s3 and t0
+// of reconstruct.f90:124-129) stay below 7e36 and the sum above 1.5e-29:
+// one reciprocal per side besides 1/M.  Below the lower clamp the weights
+// differ from the reference's only where the jumps agree to float32
+// resolution; above the upper one by beta/eps <= 1e-7 relative.  Jumps
+// below 1e-30 (where 1/M would overflow) give no correction.
 __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, float a0, float am1)
 {
     float M = fmaxf(fmaxf(fabsf(a2), fabsf(a1)), fmaxf(fabsf(a0), fabsf(am1)));
@@ -58,19 +62,20 @@ __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, flo
     a1 *= r;
     a0 *= r;
     am1 *= r;
-    float eps = fminf(fmaxf(1.0e-36f * r * r, 1.0e-15f), 1.0e15f);
+    float eps = fminf(fmaxf(1.0e-36f * r * r, 1.0e-15f), 1.0e9f);
     float t1 = im * (a2 - a1);
     float t2 = im * (a1 - a0);
     float t3 = im * (a0 - am1);
-    float b1 = 13.0f * (t1 * t1) + 3.0f * dsq(a2 - 3.0f * a1);
-    float b2 = 13.0f * (t2 * t2) + 3.0f * dsq(a1 + a0);
-    float b3 = 13.0f * (t3 * t3) + 3.0f * dsq(3.0f * a0 - am1);
-    float al1 = frcp(dsq(eps + b1));
-    float al2 = 6.0f * frcp(dsq(eps + b2));
-    float al3 = 3.0f * frcp(dsq(eps + b3));
-    float rs = frcp(al1 + al2 + al3);
-    float w1 = al1 * rs, w3 = al3 * rs;
-    return ((w1 * (t2 - t1) + (0.5f * w3 - 0.25f) * (t3 - t2)) * (1.0f / 3.0f)) * M;
+    float tt1 = dsq(eps + (13.0f * (t1 * t1) + 3.0f * dsq(a2 - 3.0f * a1)));
+    float tt2 = dsq(eps + (13.0f * (t2 * t2) + 3.0f * dsq(a1 + a0)));
+    float tt3 = dsq(eps + (13.0f * (t3 * t3) + 3.0f * dsq(3.0f * a0 - am1)));
+    float s1 = tt2 * tt3;
+    float s2 = 6.0f * tt1 * tt3;
+    float s3 = 3.0f * tt1 * tt2;
+    float t0 = frcp(s1 + s2 + s3);
+    s1 = s1 * t0;
+    s3 = s3 * t0;
+    return ((s1 * (t2 - t1) + (0.5f * s3 - 0.25f) * (t3 - t2)) * (1.0f / 3.0f)) * M;
 }
 
 template <class T> __device__ __forceinline__ T weno_corr(T im, T a2, T a1, T a0, T am1);
