@@ -61,7 +61,9 @@ typedef struct sc_params {
     int mcapa;           /* must be 0 on the device path */
     int lim_type;        /* must be 2 (WENO5) on the device path */
     int char_decomp;     /* 0 component-wise weno5, 2 characteristic weno5_char */
-    int tfluct_solver;   /* must be 0 (swapped-Riemann in-cell path, flux1.f90:155-188) */
+    int tfluct_solver;   /* 0 in-cell problem through rp (flux1.f90:155-188); 1 total
+                            fluctuation (flux1.f90:136-153), 1D advection/acoustics/
+                            shallow water only: the applications with a tfluct */
     int time_integrator; /* 1 SSP22, 2 SSP42, 3 SSP33, 4 SSP(10,4) */
     int src_term;        /* must be 0 (default src: dq = 0 in the interior) */
     int mthbc[4];        /* 1 extrapolation, 2 periodic, 3 wall (bc1.f90, bc2.f90) */

@@ -19,7 +19,7 @@ program harness
     character(len=16) :: mode
     character(len=256) :: fin, fout
     integer :: ixy, mx, mq, mw, nb, mx2, charf, i
-    integer :: hdr(10)
+    integer :: hdr(11)
     double precision :: rhdr(6)
     double precision, allocatable :: ql(:,:), qr(:,:), wave(:,:,:), s(:,:)
     double precision, allocatable :: amdq(:,:), apdq(:,:), evl(:,:,:), evr(:,:,:)
@@ -77,7 +77,7 @@ program harness
         write(22) ql, qr
 
     case ('step')
-        ! header: ndim nx ny meqn mwaves char_decomp mthbc(4)
+        ! header: ndim nx ny meqn mwaves char_decomp mthbc(4) tfluct_solver
         read(21) hdr
         read(21) rhdr      ! dx dy xlower ylower dt t
         ndim = hdr(1)
@@ -94,7 +94,7 @@ program harness
             dx(2) = rhdr(2); xlower(2) = rhdr(4)
         endif
         dt = rhdr(5); t = rhdr(6)
-        mbc = 3; maux = 0; mcapa = 0; lim_type = 2; tfluct_solver = 0
+        mbc = 3; maux = 0; mcapa = 0; lim_type = 2; tfluct_solver = hdr(11)
         src_term = 0; multid_recon = 0
         allocate(mthlim(mwaves)); mthlim = 5
         maxnx = maxval(nx)

@@ -122,7 +122,8 @@ def step(p, q, dt, t=0.0):
     shape = q.shape
     with tempfile.TemporaryDirectory() as d:
         write_setprob(p, d)
-        hdr = _i32(p.ndim, p.nx[0], p.nx[1], p.meqn, p.mwaves, p.char_decomp, *list(p.mthbc))
+        hdr = _i32(p.ndim, p.nx[0], p.nx[1], p.meqn, p.mwaves, p.char_decomp, *list(p.mthbc),
+                   p.tfluct_solver)
         rh = np.array([p.dx[0], p.dx[1], p.xlower[0], p.xlower[1], dt, t], dtype="<f8").tobytes()
         with open(os.path.join(d, "in.bin"), "wb") as f:
             f.write(hdr + rh + _f64(q))
@@ -140,6 +141,7 @@ def clawdata_for(p, **kw):
     cd.meqn, cd.mwaves = p.meqn, p.mwaves
     cd.mthlim = [5] * p.mwaves
     cd.char_decomp = p.char_decomp
+    cd.tfluct_solver = p.tfluct_solver
     cd.time_integrator = p.time_integrator
     cd.xlower = [p.xlower[d] for d in range(p.ndim)]
     cd.xupper = [p.xlower[d] + p.dx[d] * p.nx[d] for d in range(p.ndim)]

@@ -713,10 +713,20 @@ void orc_weno5_char(int mx2, int meqn, int LD, const double *q, const double *ev
 
 /* --------------------------------------------------------------- flux1 */
 
-/* src/flux1.f90:2-190 for lim_type=2, char_decomp 0/2, tfluct_solver=0,
- * mcapa=0, on one slice q1d(1-mbc:mx+mbc, meqn).  In 1D dq1d is the step's
- * dq (interior zeroed by src); for ndim > 1 it is zeroed here
- * (flux1.f90:62).  Returns the slice CFL number (flux1.f90:126-133). */
+/* src/flux1.f90:2-190 for lim_type=2, char_decomp 0/2, mcapa=0, on one
+ * slice q1d(1-mbc:mx+mbc, meqn).  In 1D dq1d is the step's dq (interior
+ * zeroed by src); for ndim > 1 it is zeroed here (flux1.f90:62).  Returns
+ * the slice CFL number (flux1.f90:126-133).
+ *
+ * tfluct_solver=1 (flux1.f90:136-153) takes the total fluctuation adq(i) of
+ * the in-cell problem (left state ql(i), right state qr(i)) from the
+ * application's tfluct.  The three applications that ship one
+ * (apps/advection/1d/tfluct1.f:33-41, apps/acoustics/1d/homogeneous/
+ * tfluct.f:31-53, apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/
+ * tfluct1sw.f:43-147) evaluate their own rp's formulas on that problem and
+ * return adq = amdq + apdq; so adq here is the rp on the swapped states
+ * (the same states the tfluct_solver=0 branch builds), summed, and the
+ * update is dq -= dtdx*((apdq(i) + adq(i)) + amdq(i+1)). */
 double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double *dq1d, double dt)
 {
     const int mbc = p->mbc, meqn = p->meqn, mwaves = p->mwaves, L = mx + 2 * mbc;
@@ -755,9 +765,17 @@ double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double 
             A1(ql, i, m) = A1(qr, i, m);
         }
     orc_rp(p, ixy, mx, ql, qr, wave, s, amdq2, apdq2);
-    for (int m = 0; m < meqn; m++)
-        for (int i = 1; i <= mx; i++)
-            A1(dq1d, i, m) = A1(dq1d, i, m) - dtdx * (A1(amdq, i + 1, m) + A1(apdq, i, m) + A1(amdq2, i, m) + A1(apdq2, i, m));
+    if (p->tfluct_solver == 1) {
+        for (int m = 0; m < meqn; m++)
+            for (int i = 1; i <= mx; i++) {
+                double adq = A1(amdq2, i, m) + A1(apdq2, i, m);
+                A1(dq1d, i, m) = A1(dq1d, i, m) - dtdx * ((A1(apdq, i, m) + adq) + A1(amdq, i + 1, m));
+            }
+    } else {
+        for (int m = 0; m < meqn; m++)
+            for (int i = 1; i <= mx; i++)
+                A1(dq1d, i, m) = A1(dq1d, i, m) - dtdx * (A1(amdq, i + 1, m) + A1(apdq, i, m) + A1(amdq2, i, m) + A1(apdq2, i, m));
+    }
     free(ql); free(qr); free(amdq); free(apdq); free(amdq2); free(apdq2); free(s); free(wave);
     return cfl;
 }

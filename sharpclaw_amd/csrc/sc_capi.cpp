@@ -90,7 +90,14 @@ int sc_check_params(const sc_params *p, bool grid)
     if (p->lim_type != 2) return sc_fail(SC_ERR_UNSUPPORTED, "lim_type %d: only 2 (WENO5) on the device path", p->lim_type);
     if (p->char_decomp != 0 && p->char_decomp != 2)
         return sc_fail(SC_ERR_UNSUPPORTED, "char_decomp %d: device path implements 0 and 2", p->char_decomp);
-    if (p->tfluct_solver != 0) return sc_fail(SC_ERR_UNSUPPORTED, "tfluct_solver 1 is not supported on the device path");
+    if (p->tfluct_solver != 0 && p->tfluct_solver != 1)
+        return sc_fail(SC_ERR_ARG, "tfluct_solver must be 0 or 1 (got %d)", p->tfluct_solver);
+    // tfluct routines exist for advection, acoustics and 1D shallow water
+    // (apps/advection/1d/tfluct1.f, apps/acoustics/1d/homogeneous/tfluct.f,
+    // apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/tfluct1sw.f); the
+    // library's own tfluct (src/tfluct.f90) stops the run
+    if (p->tfluct_solver == 1 && (p->ndim != 1 || p->physics == SC_PHYS_EULER))
+        return sc_fail(SC_ERR_UNSUPPORTED, "tfluct_solver 1: the reference has a tfluct routine for 1D advection, acoustics and shallow water only");
     if (p->src_term != 0) return sc_fail(SC_ERR_UNSUPPORTED, "src_term != 0 is not supported on the device path");
     if (p->precision != SC_FP64 && p->precision != SC_FP32) return sc_fail(SC_ERR_ARG, "precision must be SC_FP64 or SC_FP32");
     if (!grid) return SC_OK;

@@ -33,5 +33,6 @@ template <class T> inline sc::PhysConst<T> sc_make_const(const sc_params *p)
         c.efix = p->ndim == 1;  // rp1sw.f:40 on, rpn2sw.f:48 off
     else
         c.efix = 0;
+    c.tfluct = p->tfluct_solver == 1;
     return c;
 }

@@ -34,6 +34,7 @@ template <class T> struct PhysConst {
     T cc, zz;         // /cparam/ sound speed and impedance
     T hzz;            // 1/(2 zz)
     int efix;
+    int tfluct;       // 1: in-cell total fluctuation (tfluct_solver = 1), 1D only
 };
 
 template <class T> __device__ __forceinline__ T dsq(T x) { return x * x; }

@@ -171,7 +171,8 @@ bool same_group(const sc_params &a, const sc_params &b)
     return block_class(a.nx[0]) == block_class(b.nx[0]) && a.physics == b.physics && a.meqn == b.meqn && a.char_decomp == b.char_decomp &&
            a.precision == b.precision && a.time_integrator == b.time_integrator &&
            a.mthbc[0] == b.mthbc[0] && a.mthbc[1] == b.mthbc[1] && a.gamma == b.gamma &&
-           a.grav == b.grav && a.u == b.u && a.rho == b.rho && a.bulk == b.bulk && a.efix == b.efix;
+           a.grav == b.grav && a.u == b.u && a.rho == b.rho && a.bulk == b.bulk && a.efix == b.efix &&
+           a.tfluct_solver == b.tfluct_solver;
 }
 
 }  // namespace

@@ -143,7 +143,11 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
         const bool rq = op_reads_qold(a.op);
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            T X = ((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m];
+            // flux1.f90:150-152 (tfluct: the in-cell problem's total
+            // fluctuation adq = amdq + apdq, tfluct*.f of the applications)
+            // or flux1.f90:183-186 (the same problem through rp, swapped)
+            T X = c.tfluct ? (ap[m] + (am2[m] + ap2[m])) + sx[M + m][t + 1]
+                           : ((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m];
             double r = -(a.dtdx * (double)X);
             size_t g = (size_t)(i + 2) + (size_t)m * a.LX;
             double q = a.qin[g];

@@ -133,9 +133,53 @@ def runs():
          dtlog=dtl, ctl=np.array([1e99, 0.5, 0.45, 50000, 1]), dt_initial=np.array(1e-4))
 
 
+def tfluct():
+    """tfluct_solver = 1 (flux1.f90:136-153) with the applications' own
+    tfluct routines linked in place of the library stub: one step per case,
+    and two full runs."""
+    rng = np.random.default_rng(20261203)
+    out = {}
+    k = 0
+    for name in ("advection", "acoustics", "shallow_water"):
+        for bc in [(1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3), (1, 3, 1, 1)]:
+            if name == "advection" and 3 in bc[:2]:
+                continue
+            for char in (0, 2):
+                p = make_params(name, (40,), char_decomp=char, mthbc=bc, tfluct_solver=1)
+                q = _states.grid(p, rng, smooth=True)
+                dt = 0.1 * p.dx[0]
+                qb, dq, cfl = R.step(p, q, dt)
+                out.update({f"c{k}_name": np.array(name), f"c{k}_q": q, f"c{k}_qbc": qb,
+                            f"c{k}_dq": dq, f"c{k}_cfl": np.array(cfl), f"c{k}_dt": np.array(dt),
+                            f"c{k}_bc": np.array(bc), f"c{k}_char": np.array(char)})
+                k += 1
+    out["n"] = np.array(k)
+    save("step_tfluct1d.npz", **out)
+    # 1D shallow-water dam break as run_sw1d_dambreak, total-fluctuation solver
+    p = make_params("shallow_water", (200,), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1),
+                    tfluct_solver=1)
+    cd = R.clawdata_for(p, nout=1, outstyle=1, tfinal=0.05, dt_initial=1e-4, cfl_max=0.5,
+                        cfl_desired=0.45)
+    fr, dtl, info = R.run(p, cd, ["3", "2.0 0.5 0.45"])
+    save("run_sw1d_dambreak_tfluct.npz", t=np.array([f[0] for f in fr]),
+         q=np.stack([f[1] for f in fr]), dtlog=dtl, ctl=np.array([1e99, 0.5, 0.45, 50000, 1]),
+         dt_initial=np.array(1e-4))
+    # 1D acoustics pulse as run_ac1d_pulse, total-fluctuation solver
+    p = make_params("acoustics", (200,), xlower=(-1.0,), xupper=(1.0,), char_decomp=2,
+                    time_integrator=4, mthbc=(3, 3, 1, 1), tfluct_solver=1)
+    cd = R.clawdata_for(p, nout=1, outstyle=1, tfinal=0.25, dt_initial=1e-3, cfl_max=2.5,
+                        cfl_desired=2.4)
+    fr, dtl, info = R.run(p, cd, ["4", "1.0 100.0 0.0"])
+    save("run_ac1d_pulse_tfluct.npz", t=np.array([f[0] for f in fr]),
+         q=np.stack([f[1] for f in fr]), dtlog=dtl, ctl=np.array([1e99, 2.5, 2.4, 50000, 1]),
+         dt_initial=np.array(1e-3))
+
+
+PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct}
+
 if __name__ == "__main__":
     if not R.ref_available():
         sys.exit("oracle/_ref is not built (make -C oracle ref)")
-    routines()
-    steps()
-    runs()
+    # python make_golden.py [part ...]   (default: every part)
+    for part in (sys.argv[1:] or list(PARTS)):
+        PARTS[part]()

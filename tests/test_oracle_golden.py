@@ -50,6 +50,30 @@ def test_step_bit_exact(name, ndim):
         assert cfl == float(d[f"c{k}_cfl"]), k
 
 
+def test_step_tfluct_bit_exact():
+    """tfluct_solver = 1 with the applications' tfluct routines (advection,
+    acoustics, 1D shallow water)."""
+    d = load("step_tfluct1d.npz")
+    names = set()
+    for k in range(int(d["n"])):
+        name = str(d[f"c{k}_name"])
+        names.add(name)
+        p = make_params(name, (40,), char_decomp=int(d[f"c{k}_char"]),
+                        mthbc=tuple(int(b) for b in d[f"c{k}_bc"]), tfluct_solver=1)
+        qb, dq, cfl = O.step(p, d[f"c{k}_q"], float(d[f"c{k}_dt"]))
+        assert np.array_equal(qb, d[f"c{k}_qbc"]), k
+        assert np.array_equal(interior(p, dq), interior(p, d[f"c{k}_dq"])), k
+        assert cfl == float(d[f"c{k}_cfl"]), k
+        if name != "advection":
+            # the swapped-Riemann path (tfluct_solver = 0) sums the same four
+            # fluctuations in another order: equal to rounding, not bitwise
+            p0 = make_params(name, (40,), char_decomp=int(d[f"c{k}_char"]),
+                             mthbc=tuple(int(b) for b in d[f"c{k}_bc"]))
+            dq0 = O.step(p0, d[f"c{k}_q"], float(d[f"c{k}_dt"]))[1]
+            assert np.allclose(interior(p, dq0), interior(p, dq), rtol=0, atol=1e-14)
+    assert names == {"advection", "acoustics", "shallow_water"}
+
+
 def _replay(p, fname, single_step=False):
     d = load(fname)
     c = d["ctl"]
@@ -106,3 +130,14 @@ def test_run_ac1d_pulse():
 def test_run_sw2d_pulse():
     _replay(make_params("shallow_water", (24, 24), char_decomp=2, time_integrator=3,
                         mthbc=(2, 2, 2, 2)), "run_sw2d_pulse.npz")
+
+
+def test_run_sw1d_dambreak_tfluct():
+    _replay(make_params("shallow_water", (200,), char_decomp=2, time_integrator=3,
+                        mthbc=(1, 1, 1, 1), tfluct_solver=1), "run_sw1d_dambreak_tfluct.npz")
+
+
+def test_run_ac1d_pulse_tfluct():
+    _replay(make_params("acoustics", (200,), xlower=(-1.0,), xupper=(1.0,), char_decomp=2,
+                        time_integrator=4, mthbc=(3, 3, 1, 1), tfluct_solver=1),
+            "run_ac1d_pulse_tfluct.npz")

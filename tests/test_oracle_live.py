@@ -54,3 +54,20 @@ def test_step_live(name, ndim):
             assert np.array_equal(a[0], b[0])
             assert np.array_equal(interior(p, a[1]), interior(p, b[1]))
             assert a[2] == b[2]
+
+
+@pytest.mark.parametrize("name", ["advection", "acoustics", "shallow_water"])
+def test_step_tfluct_live(name):
+    """tfluct_solver = 1: the reference links the application's tfluct in
+    place of its stub (src/tfluct.f90)."""
+    rng = np.random.default_rng(78)
+    for bc in ((1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3)):
+        if name == "advection" and bc[0] == 3:
+            continue
+        for char in (0, 2):
+            p = make_params(name, (57,), char_decomp=char, mthbc=bc, tfluct_solver=1)
+            q = _states.grid(p, rng, smooth=True, amp=0.6)
+            a = O.step(p, q, 0.07 * p.dx[0])
+            b = R.step(p, q, 0.07 * p.dx[0])
+            assert np.array_equal(interior(p, a[1]), interior(p, b[1]))
+            assert a[2] == b[2]
