@@ -158,7 +158,13 @@ def test_batch_and_server_tfluct():
         assert res[i].steps_taken == gst.steps_taken
         rq, _, rst = O.sharpclaw(p, ctl, qs[i], 0.0, 0.05, 1e-4)
         assert rst.steps_taken == gst.steps_taken
-        assert rel_linf(interior(p, gq), interior(p, rq)) <= 1e-10, i
+        # the dam break's near-still water amplifies rounding (1 ulp in q0
+        # moves the oracle's result by up to 6e-8 here): whole-run tolerance
+        qp = qs[i].copy(order="F")
+        interior(p, qp)[:] = np.nextafter(interior(p, qp), np.inf)
+        rq2, _, _ = O.sharpclaw(p, ctl, qp, 0.0, 0.05, 1e-4)
+        tol = max(TOL_RUN, 8.0 * rel_linf(interior(p, rq2), interior(p, rq)))
+        assert rel_linf(interior(p, gq), interior(p, rq)) <= tol, i
 
 
 def test_tfluct_rejected_without_a_reference_tfluct():
