@@ -36,10 +36,12 @@ def app_of(p):
     return APPS[(p.physics, p.ndim)]
 
 
-def ref_available(app=None):
+def ref_available(app=None, programs=("harness", "xsclaw")):
+    """Whether the reference build holds the given programs (harness_<app>
+    single-routine driver, xsclaw_<app> reference program) for app (all
+    apps when None)."""
     apps = [app] if app else list(APPS.values())
-    return all(os.path.exists(os.path.join(REF, f"harness_{a}")) and
-               os.path.exists(os.path.join(REF, f"xsclaw_{a}")) for a in apps)
+    return all(os.path.exists(os.path.join(REF, f"{prog}_{a}")) for a in apps for prog in programs)
 
 
 def write_setprob(p, d, beta=200.0):

@@ -23,6 +23,11 @@ def have_driver(app):
     return os.path.exists(os.path.join(FB, f"xsclaw_gpu_{app}"))
 
 
+def have_ref(app):
+    """The reference program with its own ASCII output routine."""
+    return os.path.exists(os.path.join(R.REF, f"xsclaw_{app}_fmt"))
+
+
 def setup_case(d, p, cd, qlines, name, beta=200.0):
     R.write_setprob(p, d, beta=beta)
     cd.write(os.path.join(d, "sharpclaw.data"))
@@ -60,7 +65,7 @@ def run_pair(p, cd, qlines, name, app, tmp):
 
 
 @pytest.mark.gpu
-@pytest.mark.skipif(not (have_driver("adv1d") and R.ref_available("adv1d")),
+@pytest.mark.skipif(not (have_driver("adv1d") and have_ref("adv1d")),
                     reason="driver or reference build missing")
 def test_driver_c1_advection():
     p = make_params("advection", (500,), char_decomp=0, time_integrator=3, mthbc=(2, 2, 2, 2))
@@ -80,7 +85,7 @@ def test_driver_c1_advection():
 
 
 @pytest.mark.gpu
-@pytest.mark.skipif(not (have_driver("eu2d") and R.ref_available("eu2d")),
+@pytest.mark.skipif(not (have_driver("eu2d") and have_ref("eu2d")),
                     reason="driver or reference build missing")
 def test_driver_euler2d_quadrants():
     p = make_params("euler", (32, 32), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
