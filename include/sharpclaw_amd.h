@@ -117,6 +117,29 @@ int sc_destroy(sc_solver *s);
 int sc_set_q(sc_solver *s, const double *q);
 int sc_get_q(sc_solver *s, double *q);
 
+/* Frame output, replacing output(meqn,mbc,nx,xlower,dx,q,t,iframe,aux,maux)
+ * (main.f90:172,233; src/1d/out1.f90:32-104, src/2d/out2.f90:26-111) on the
+ * data side: sc_output_begin sets interior values with |q| < 1e-99 to zero
+ * in the solver's q itself, as the reference's writers do before printing
+ * (out1.f90:63-68, out2.f90:60-67), fills the ghost cells, snapshots the
+ * grid on the device and starts its copy to pinned host memory on a second
+ * stream; it returns without waiting, and the solver may be advanced
+ * meanwhile.  sc_output_wait blocks until that copy is done and gives the
+ * grid in the sc_get_q layout.  One frame may be in flight per solver
+ * (SC_ERR_STATE otherwise).  Writing fort.q/fort.t stays with the caller. */
+int sc_output_begin(sc_solver *s);
+int sc_output_wait(sc_solver *s, double *q);
+
+/* Binary restart file: a header (grid shape, physics, t, the next dt) and
+ * q in the sc_get_q layout with an FNV-1a checksum, written to path.tmp and
+ * renamed.  sc_read_restart checks the header against the solver and the
+ * checksum, uploads q and returns t and the next dt; continuing from it
+ * reproduces the uninterrupted run bit for bit.  No counterpart in the
+ * reference (it has no restart); for long runs that must survive a
+ * process ending. */
+int sc_write_restart(sc_solver *s, const char *path, double t, double dt_next);
+int sc_read_restart(sc_solver *s, const char *path, double *t, double *dt_next);
+
 /* qold = q (sharpclaw.f90:134) and q = qold (sharpclaw.f90:201).  Both are
  * buffer renamings on the device; no data moves. */
 int sc_save_qold(sc_solver *s);

@@ -12,7 +12,7 @@ import subprocess
 
 import numpy as np
 
-from sharpclaw_amd._abi import ScParams, ScRunCtl, ScRunStats, grid_shape
+from sharpclaw_amd._abi import ScParams, ScRunCtl, ScRunStats, grid_shape, interior
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", os.environ.get("ORACLE_LIB", "liboracle.so"))
@@ -145,6 +145,15 @@ def sharpclaw(p, ctl, q, tstart, tend, dt):
     st = ScRunStats()
     lib().orc_sharpclaw(C.byref(p), C.byref(ctl), q, tstart, tend, C.byref(d), C.byref(st))
     return q, d.value, st
+
+
+def flush_small(p, q):
+    """The frame writers' in-place reset (src/1d/out1.f90:63-68,
+    src/2d/out2.f90:60-67): interior values with |q| < 1e-99 become +0."""
+    q = _f(q).copy(order="F")
+    qi = interior(p, q)
+    qi[np.abs(qi) < 1e-99] = 0.0
+    return q
 
 
 def sums(p, q):

@@ -38,6 +38,10 @@ def lib():
         "sc_destroy": ([H], C.c_int),
         "sc_set_q": ([H, dp], C.c_int),
         "sc_get_q": ([H, dp], C.c_int),
+        "sc_output_begin": ([H], C.c_int),
+        "sc_output_wait": ([H, dp], C.c_int),
+        "sc_write_restart": ([H, C.c_char_p, C.c_double, C.c_double], C.c_int),
+        "sc_read_restart": ([H, C.c_char_p, dref, dref], C.c_int),
         "sc_save_qold": ([H], C.c_int),
         "sc_restore_qold": ([H], C.c_int),
         "sc_step": ([H, C.c_double, C.c_double, C.c_void_p, dref], C.c_int),

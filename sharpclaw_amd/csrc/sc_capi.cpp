@@ -17,6 +17,8 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <vector>
+#include <cstdint>
 
 #include "../../include/sharpclaw_amd.h"
 #include "sc_batch2d.h"
@@ -151,6 +153,13 @@ struct sc_solver {
     bool use_graph;    // device loop: replay rounds from a captured graph
     Batch2DRunner *dr; // 2D device-resident loop (created at first use)
     hipEvent_t ev0, ev1;
+    // frame output (created at first sc_output_begin): device snapshot,
+    // pinned host copy, copy stream and its events
+    double *snap;
+    double *hsnap;
+    hipStream_t cst;
+    hipEvent_t ev_snap, ev_out;
+    bool out_pending;
 };
 
 struct DevGuard {
@@ -248,6 +257,12 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->hslot) hipHostFree(s->hslot);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->cst) hipStreamSynchronize(s->cst);
+    if (s->snap) hipFree(s->snap);
+    if (s->hsnap) hipHostFree(s->hsnap);
+    if (s->ev_snap) hipEventDestroy(s->ev_snap);
+    if (s->ev_out) hipEventDestroy(s->ev_out);
+    if (s->cst) hipStreamDestroy(s->cst);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
     return SC_OK;
@@ -284,6 +299,135 @@ extern "C" int sc_get_q(sc_solver *s, double *q)
                      s->p.nx[0] + 6, s->p.ndim == 2 ? s->p.nx[1] + 6 : 1, s->p.mthbc, s->st));
     HIPCHK(hipMemcpyAsync(q, s->buf[s->cur], s->n * sizeof(double), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
+    return SC_OK;
+}
+
+// ----------------------------------------------------------- frame output
+
+extern "C" int sc_output_begin(sc_solver *s)
+{
+    if (!s) return sc_fail(SC_ERR_ARG, "NULL solver");
+    if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
+    if (s->out_pending) return sc_fail(SC_ERR_STATE, "the previous frame has not been collected (sc_output_wait)");
+    DevGuard g(s->dev);
+    if (!s->snap) {
+        bool ok = hipMalloc(&s->snap, s->n * sizeof(double)) == hipSuccess;
+        ok = ok && hipHostMalloc(&s->hsnap, s->n * sizeof(double), hipHostMallocDefault) == hipSuccess;
+        ok = ok && hipStreamCreateWithFlags(&s->cst, hipStreamNonBlocking) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&s->ev_snap, hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming) == hipSuccess;
+        if (!ok) return sc_fail(SC_ERR_HIP, "frame output buffers: %s", hipGetErrorString(hipGetLastError()));
+    }
+    const int LX = s->p.nx[0] + 6, LY = s->p.ndim == 2 ? s->p.nx[1] + 6 : 1;
+    double *q = s->buf[s->cur];
+    HIPCHK(launch_flush_small(s->p.ndim, s->p.meqn, q, s->p.nx[0], s->p.nx[1], LX, LY, s->st));
+    HIPCHK(launch_bc(s->p.ndim, s->p.meqn, q, s->p.nx[0], s->p.nx[1], LX, LY, s->p.mthbc, s->st));
+    HIPCHK(hipMemcpyAsync(s->snap, q, s->n * sizeof(double), hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipEventRecord(s->ev_snap, s->st));
+    HIPCHK(hipStreamWaitEvent(s->cst, s->ev_snap, 0));
+    HIPCHK(hipMemcpyAsync(s->hsnap, s->snap, s->n * sizeof(double), hipMemcpyDeviceToHost, s->cst));
+    HIPCHK(hipEventRecord(s->ev_out, s->cst));
+    s->out_pending = true;
+    return SC_OK;
+}
+
+extern "C" int sc_output_wait(sc_solver *s, double *q)
+{
+    if (!s || !q) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (!s->out_pending) return sc_fail(SC_ERR_STATE, "no frame in flight (sc_output_begin)");
+    DevGuard g(s->dev);
+    HIPCHK(hipEventSynchronize(s->ev_out));
+    memcpy(q, s->hsnap, s->n * sizeof(double));
+    s->out_pending = false;
+    return SC_OK;
+}
+
+// ---------------------------------------------------------------- restart
+
+namespace {
+
+struct RestartHeader {
+    char magic[8];  // "SCRSTRT1"
+    int32_t ndim, nx, ny, meqn, mbc, physics;
+    double t, dt_next;
+    uint64_t n;         // doubles that follow (the sc_get_q layout)
+    uint64_t checksum;  // FNV-1a 64 of those bytes
+};
+
+uint64_t fnv1a(const void *data, size_t bytes)
+{
+    const unsigned char *b = (const unsigned char *)data;
+    uint64_t h = 1469598103934665603ull;
+    for (size_t k = 0; k < bytes; k++) {
+        h ^= b[k];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+}  // namespace
+
+extern "C" int sc_write_restart(sc_solver *s, const char *path, double t, double dt_next)
+{
+    if (!s || !path) return sc_fail(SC_ERR_ARG, "NULL argument");
+    std::vector<double> q(s->n);
+    int rc = sc_get_q(s, q.data());
+    if (rc) return rc;
+    RestartHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, "SCRSTRT1", 8);
+    h.ndim = s->p.ndim;
+    h.nx = s->p.nx[0];
+    h.ny = s->p.ndim == 2 ? s->p.nx[1] : 1;
+    h.meqn = s->p.meqn;
+    h.mbc = s->p.mbc;
+    h.physics = s->p.physics;
+    h.t = t;
+    h.dt_next = dt_next;
+    h.n = s->n;
+    h.checksum = fnv1a(q.data(), s->n * sizeof(double));
+    std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return sc_fail(SC_ERR_ARG, "cannot open %s for writing", tmp.c_str());
+    bool ok = fwrite(&h, sizeof h, 1, f) == 1 && fwrite(q.data(), sizeof(double), s->n, f) == s->n;
+    ok = (fclose(f) == 0) && ok;
+    // the file appears complete or not at all
+    ok = ok && rename(tmp.c_str(), path) == 0;
+    if (!ok) {
+        remove(tmp.c_str());
+        return sc_fail(SC_ERR_ARG, "writing %s failed", path);
+    }
+    return SC_OK;
+}
+
+extern "C" int sc_read_restart(sc_solver *s, const char *path, double *t, double *dt_next)
+{
+    if (!s || !path || !t || !dt_next) return sc_fail(SC_ERR_ARG, "NULL argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return sc_fail(SC_ERR_ARG, "cannot open %s", path);
+    RestartHeader h;
+    std::vector<double> q(s->n);
+    bool ok = fread(&h, sizeof h, 1, f) == 1;
+    int ny = s->p.ndim == 2 ? s->p.nx[1] : 1;
+    if (ok && memcmp(h.magic, "SCRSTRT1", 8) != 0) {
+        fclose(f);
+        return sc_fail(SC_ERR_ARG, "%s is not a restart file", path);
+    }
+    if (ok && (h.ndim != s->p.ndim || h.nx != s->p.nx[0] || h.ny != ny || h.meqn != s->p.meqn ||
+               h.mbc != s->p.mbc || h.physics != s->p.physics || h.n != s->n)) {
+        fclose(f);
+        return sc_fail(SC_ERR_ARG, "%s holds a %dD grid %dx%d, meqn %d, physics %d: not this solver's", path,
+                       h.ndim, h.nx, h.ny, h.meqn, h.physics);
+    }
+    ok = ok && fread(q.data(), sizeof(double), s->n, f) == s->n;
+    fclose(f);
+    if (!ok) return sc_fail(SC_ERR_ARG, "%s is truncated", path);
+    if (fnv1a(q.data(), s->n * sizeof(double)) != h.checksum)
+        return sc_fail(SC_ERR_ARG, "%s: checksum mismatch", path);
+    int rc = sc_set_q(s, q.data());
+    if (rc) return rc;
+    *t = h.t;
+    *dt_next = h.dt_next;
     return SC_OK;
 }
 

@@ -67,5 +67,8 @@ hipError_t launch_weno(int meqn, int chr, int fp32, int mx2, const double *q, co
 hipError_t launch_sums(int ndim, int meqn, const double *q, int nx, int ny, int LX, int LY,
                        double *partial, int nblk, double *out, hipStream_t st);
 hipError_t launch_copy(double *dst, const double *src, size_t n, hipStream_t st);
+// |q| < 1e-99 -> 0 over the interior, in place (the reference's frame writers)
+hipError_t launch_flush_small(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY,
+                              hipStream_t st);
 
 }  // namespace sc

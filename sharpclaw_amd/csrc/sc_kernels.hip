@@ -9,6 +9,8 @@
 // The maximum wave speed is reduced across each wavefront with shuffles,
 // across the block in LDS, and merged into a device word with one atomic
 // per block.
+#include <algorithm>
+
 #include "sc_kernels.h"
 #include "sc_stage.h"
 #include "sc_stage2d.h"
@@ -245,6 +247,24 @@ __global__ void sums_final(const double *partial, int nblk, double *out)
     }
 }
 
+// out1.f90:63-68, out2.f90:60-67: interior values with |q| < 1e-99 are
+// set to +0 in q itself (NaN compares false and is kept)
+__global__ void flush_small_kernel(int ndim, double *q, int nx, int ny, int LX, int LY, int meqn)
+{
+    const long long ncell = (long long)nx * (ndim == 2 ? ny : 1);
+    const long long total = ncell * meqn;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        int m = (int)(k / ncell);
+        long long c = k - (long long)m * ncell;
+        int i = (int)(c % nx) + 1;
+        int j = ndim == 2 ? (int)(c / nx) + 1 : 1;
+        size_t g = (size_t)(i + 2) + (ndim == 2 ? (size_t)LX * (j + 2) + (size_t)m * LX * LY
+                                                  : (size_t)m * LX);
+        if (fabs(q[g]) < 1.0e-99) q[g] = 0.0;
+    }
+}
+
 __global__ void copy_kernel(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n2)
 {
     for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n2;
@@ -474,6 +494,15 @@ hipError_t launch_sums(int ndim, int meqn, const double *q, int nx, int ny, int 
     hipLaunchKernelGGL(sums_partial, dim3(nblk, meqn), dim3(256), 0, st, ndim, q, nx, ny, LX, LY,
                        partial, nblk);
     hipLaunchKernelGGL(sums_final, dim3(meqn), dim3(64), 0, st, partial, nblk, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_flush_small(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY,
+                              hipStream_t st)
+{
+    long long total = (long long)nx * (ndim == 2 ? ny : 1) * meqn;
+    int nblk = (int)std::min<long long>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(flush_small_kernel, dim3(nblk), dim3(256), 0, st, ndim, q, nx, ny, LX, LY, meqn);
     return hipGetLastError();
 }
 

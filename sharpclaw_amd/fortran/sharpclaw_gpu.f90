@@ -63,6 +63,15 @@ module sharpclaw_gpu
             type(c_ptr), value :: solver
             real(c_double), intent(inout) :: q(*)
         end function
+        integer(c_int) function sc_output_begin(solver) bind(C, name="sc_output_begin")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: solver
+        end function
+        integer(c_int) function sc_output_wait(solver, q) bind(C, name="sc_output_wait")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: solver
+            real(c_double), intent(inout) :: q(*)
+        end function
         integer(c_int) function sc_save_qold(solver) bind(C, name="sc_save_qold")
             import :: c_int, c_ptr
             type(c_ptr), value :: solver

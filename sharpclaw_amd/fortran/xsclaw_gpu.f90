@@ -173,6 +173,11 @@ program xsclaw_gpu
     ctl%max_steps = max_steps
     ctl%dt_variable = dt_variable
 
+    ! frames go through sc_output_begin/wait: the |q| < 1e-99 flush of the
+    ! reference's writers (out1.f90:63-68) then happens in the device q the
+    ! run continues from, as it does in the reference's q
+    call sc_check(sc_output_begin(solver), 'sc_output_begin')
+    call sc_check(sc_output_wait(solver, q), 'sc_output_wait')
     call write_frame(t0, 0)
     write(6,601) 0, t0
     open(10, file='fort.info', status='unknown', form='formatted')
@@ -198,7 +203,8 @@ program xsclaw_gpu
         dt_initial = dt_last
         iframe = n / nstepout
         if (iframe*nstepout == n) then
-            call sc_check(sc_get_q(solver, q), 'sc_get_q')
+            call sc_check(sc_output_begin(solver), 'sc_output_begin')
+            call sc_check(sc_output_wait(solver, q), 'sc_output_wait')
             call write_frame(tend, iframe)
             write(6,601) iframe, tend
             write(10,1010) tend, info, dt_minused, dt_maxused, dt_last, cfl_maxused, &

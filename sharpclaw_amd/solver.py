@@ -13,6 +13,7 @@
 Arrays are numpy float64 in Fortran order: q(1-mbc:nx+mbc[,1-mbc:ny+mbc], meqn).
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -64,6 +65,27 @@ class Solver:
         q = np.zeros(self.shape, order="F")
         check(lib().sc_get_q(self._h, q))
         return q
+
+    def output_begin(self):
+        """Start a frame: |q| < 1e-99 -> 0 in q itself (the reference's
+        writers), ghosts filled, device snapshot copied to the host
+        asynchronously; the solver may be advanced before output_wait()."""
+        check(lib().sc_output_begin(self._h))
+
+    def output_wait(self):
+        """The grid snapshotted by the last output_begin()."""
+        q = np.zeros(self.shape, order="F")
+        check(lib().sc_output_wait(self._h, q))
+        return q
+
+    def write_restart(self, path, t, dt_next):
+        check(lib().sc_write_restart(self._h, os.fsencode(path), t, dt_next))
+
+    def read_restart(self, path):
+        """Upload q from a restart file; returns (t, dt_next)."""
+        t, d = C.c_double(), C.c_double()
+        check(lib().sc_read_restart(self._h, os.fsencode(path), C.byref(t), C.byref(d)))
+        return t.value, d.value
 
     def save_qold(self):
         check(lib().sc_save_qold(self._h))

@@ -175,7 +175,18 @@ def tfluct():
          dt_initial=np.array(1e-3))
 
 
-PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct}
+def flush():
+    """A run whose frames reset values below 1e-99 (out1.f90:63-68): a narrow
+    Gaussian (tails down to 1e-218) advected through four frames."""
+    p = make_params("advection", (200,), char_decomp=0, time_integrator=3, mthbc=(2, 2, 2, 2))
+    cd = R.clawdata_for(p, nout=4, outstyle=1, tfinal=0.2, dt_initial=1e-3, cfl_max=0.55,
+                        cfl_desired=0.5)
+    fr, dtl, info = R.run(p, cd, ["1", "2000.0 0.5 2.0 2.0"])
+    save("run_adv1d_flush.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, ctl=np.array([1e99, 0.55, 0.5, 50000, 1]), dt_initial=np.array(1e-3))
+
+
+PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct, "flush": flush}
 
 if __name__ == "__main__":
     if not R.ref_available():

@@ -85,6 +85,29 @@ def test_driver_c1_advection():
 
 
 @pytest.mark.gpu
+@pytest.mark.skipif(not (have_driver("adv1d") and have_ref("adv1d")),
+                    reason="driver or reference build missing")
+def test_driver_frames_flush_tiny_values():
+    """A narrow Gaussian whose tails fall to 1e-215: every frame resets
+    |q| < 1e-99 in q (out1.f90:63-68) on the device the run continues from;
+    frame 0 is byte-identical, later frames agree to the printed digits."""
+    p = make_params("advection", (200,), char_decomp=0, time_integrator=3, mthbc=(2, 2, 2, 2))
+    cd = R.clawdata_for(p, nout=4, outstyle=1, tfinal=0.2, dt_initial=1e-3, cfl_max=0.55,
+                        cfl_desired=0.5)
+    with tempfile.TemporaryDirectory() as tmp:
+        dg, dr = run_pair(p, cd, ["1", "2000.0 0.5 2.0 2.0"], "advection", "adv1d", tmp)
+        assert open(os.path.join(dg, "fort.q0000")).read() == open(os.path.join(dr, "fort.q0000")).read()
+        a0 = read_q(os.path.join(dg, "fort.q0000"))
+        assert 0 < np.sum(a0 == 0.0) < 200
+        for k in range(1, 5):
+            f = f"fort.q{k:04d}"
+            a, b = read_q(os.path.join(dg, f)), read_q(os.path.join(dr, f))
+            assert a.shape == b.shape
+            assert np.max(np.abs(a - b)) <= 5e-8 * np.max(np.abs(b)), f
+        assert open(os.path.join(dg, "fort.info")).read() == open(os.path.join(dr, "fort.info")).read()
+
+
+@pytest.mark.gpu
 @pytest.mark.skipif(not (have_driver("eu2d") and have_ref("eu2d")),
                     reason="driver or reference build missing")
 def test_driver_euler2d_quadrants():

@@ -81,20 +81,25 @@ def _replay(p, fname, single_step=False):
                    max_steps=int(c[3]), dt_variable=int(c[4]))
     q = O.new_grid(p)
     interior(p, q)[:] = d["q"][0]
+    # every written frame resets |q| < 1e-99 in q (out1.f90:63-68)
+    q = O.flush_small(p, q)
     dt = float(d["dt_initial"])
     t = float(d["t"][0])
     if single_step:
-        # outstyle 3: one step per sharpclaw call (tend < tstart), main.f90:145
+        # outstyle 3: one step per sharpclaw call (tend < tstart), main.f90:145;
+        # frames are written every nstepout = nstop calls here
         nsteps = int(c[3])
         for _ in range(nsteps):
             q, dt, st = O.sharpclaw(p, ctl, q, t, t - 1.0, dt)
             t = st.t
+        q = O.flush_small(p, q)
         assert t == float(d["t"][-1])
         assert np.array_equal(interior(p, q), d["q"][-1])
         return
     nsteps = 0
     for k in range(1, len(d["t"])):
         q, dt, st = O.sharpclaw(p, ctl, q, t, float(d["t"][k]), dt)
+        q = O.flush_small(p, q)
         t = st.t
         nsteps += st.steps_taken - st.rejected
         assert t == float(d["t"][k])
@@ -141,3 +146,25 @@ def test_run_ac1d_pulse_tfluct():
     _replay(make_params("acoustics", (200,), xlower=(-1.0,), xupper=(1.0,), char_decomp=2,
                         time_integrator=4, mthbc=(3, 3, 1, 1), tfluct_solver=1),
             "run_ac1d_pulse_tfluct.npz")
+
+
+def test_flush_small_matches_the_reference_writer():
+    """The reference's frame writer resets |q| < 1e-99 in q (out1.f90:63-68):
+    frame 0 of a narrow Gaussian (tails to 1e-215) holds zeros exactly where
+    the initial data is below 1e-99, and flush_small puts them there."""
+    d = load("run_adv1d_flush.npz")
+    p = make_params("advection", (200,), char_decomp=0, time_integrator=3, mthbc=(2, 2, 2, 2))
+    x = (np.arange(1, 201) - 0.5) / 200.0
+    q = O.new_grid(p)
+    interior(p, q)[:, 0] = np.exp(-2000.0 * (x - 0.5) ** 2)
+    f = interior(p, O.flush_small(p, q))
+    assert np.array_equal(f == 0.0, d["q"][0] == 0.0)
+    assert 0 < np.sum(d["q"][0] == 0.0) < 200
+    g = np.array([1e-100, -1e-100, 1e-99, -0.0, np.nan, np.inf, 5e-324, -2e-99])
+    p1 = make_params("advection", (len(g),))
+    q1 = O.new_grid(p1)
+    interior(p1, q1)[:, 0] = g
+    h = interior(p1, O.flush_small(p1, q1))[:, 0]
+    want = np.array([0.0, 0.0, 1e-99, 0.0, np.nan, np.inf, 0.0, -2e-99])
+    assert np.array_equal(h, want, equal_nan=True)
+    assert not np.signbit(h[[0, 1, 3, 6]]).any()
