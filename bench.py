@@ -73,9 +73,9 @@ def valu_from_profile(tx, ty, ms_launch):
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        ks = json.load(f)["kernels"]
+        ks = json.load(f).get("kernels", {})
     k = ks.get(f"stage2d_kernel<4, 2, double, {tx}, {ty}>")
-    if k is None:
+    if k is None or "fp64_flop_per_launch" not in k:
         return None
     tf = k["fp64_flop_per_launch"] / (ms_launch * 1e-3) / 1e12
     return {"bound": "valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
