@@ -59,8 +59,10 @@ typedef struct sc_params {
     int mbc;             /* must be 3 (reconstruct.f90:85,158 hard-code it) */
     int maux;            /* must be 0 on the device path */
     int mcapa;           /* must be 0 on the device path */
-    int lim_type;        /* must be 2 (WENO5) on the device path */
-    int char_decomp;     /* 0 component-wise weno5, 2 characteristic weno5_char */
+    int lim_type;        /* 2 WENO5; 1 second-order TVD (1D, single grid) */
+    int char_decomp;     /* lim_type 2: 0 weno5, 1 weno5_wave (1D), 2 weno5_char,
+                            3 weno5_trans (1D); lim_type 1: 1 tvd2_wave, 2 tvd2_char
+                            (reconstruct.f90:77-763, dispatch flux1.f90:65-117) */
     int tfluct_solver;   /* 0 in-cell problem through rp (flux1.f90:155-188); 1 total
                             fluctuation (flux1.f90:136-153), 1D advection/acoustics/
                             shallow water only: the applications with a tfluct */
@@ -77,6 +79,9 @@ typedef struct sc_params {
     double rho, bulk;    /* acoustics (/cparam/: cc = sqrt(bulk/rho), zz = cc*rho) */
     int precision;       /* SC_FP64 or SC_FP32 */
     int device;          /* HIP device ordinal */
+    int mthlim[4];       /* limiter per wave family (sharpclaw.data mthlim), read
+                            when lim_type = 1: 1 minmod, 2 superbee, 3 van Leer,
+                            4 MC, 5 Cada-Torrilhon (reconstruct.f90:544-571) */
 } sc_params;
 
 /* Time-step control of src/sharpclaw.f90:2-243 (read from sharpclaw.data). */

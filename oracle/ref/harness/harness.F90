@@ -19,7 +19,7 @@ program harness
     character(len=16) :: mode
     character(len=256) :: fin, fout
     integer :: ixy, mx, mq, mw, nb, mx2, charf, i
-    integer :: hdr(11)
+    integer :: hdr(16)
     double precision :: rhdr(6)
     double precision, allocatable :: ql(:,:), qr(:,:), wave(:,:,:), s(:,:)
     double precision, allocatable :: amdq(:,:), apdq(:,:), evl(:,:,:), evr(:,:,:)
@@ -78,6 +78,7 @@ program harness
 
     case ('step')
         ! header: ndim nx ny meqn mwaves char_decomp mthbc(4) tfluct_solver
+        !         lim_type mthlim(4)
         read(21) hdr
         read(21) rhdr      ! dx dy xlower ylower dt t
         ndim = hdr(1)
@@ -94,9 +95,12 @@ program harness
             dx(2) = rhdr(2); xlower(2) = rhdr(4)
         endif
         dt = rhdr(5); t = rhdr(6)
-        mbc = 3; maux = 0; mcapa = 0; lim_type = 2; tfluct_solver = hdr(11)
+        mbc = 3; maux = 0; mcapa = 0; lim_type = hdr(12); tfluct_solver = hdr(11)
         src_term = 0; multid_recon = 0
-        allocate(mthlim(mwaves)); mthlim = 5
+        allocate(mthlim(mwaves))
+        do i = 1, mwaves
+            mthlim(i) = hdr(12 + min(i, 4))
+        enddo
         maxnx = maxval(nx)
         call alloc_recon_workspace(maxnx, mbc, meqn, mwaves, lim_type, char_decomp)
         allocate(g%s(1-mbc:maxnx+mbc, mwaves), g%wave(1-mbc:maxnx+mbc, meqn, mwaves))

@@ -125,7 +125,7 @@ def step(p, q, dt, t=0.0):
     with tempfile.TemporaryDirectory() as d:
         write_setprob(p, d)
         hdr = _i32(p.ndim, p.nx[0], p.nx[1], p.meqn, p.mwaves, p.char_decomp, *list(p.mthbc),
-                   p.tfluct_solver)
+                   p.tfluct_solver, p.lim_type, *list(p.mthlim))
         rh = np.array([p.dx[0], p.dx[1], p.xlower[0], p.xlower[1], dt, t], dtype="<f8").tobytes()
         with open(os.path.join(d, "in.bin"), "wb") as f:
             f.write(hdr + rh + _f64(q))
@@ -141,7 +141,8 @@ def clawdata_for(p, **kw):
     cd.ndim = p.ndim
     cd.nx = [p.nx[0], p.nx[1]][: p.ndim]
     cd.meqn, cd.mwaves = p.meqn, p.mwaves
-    cd.mthlim = [5] * p.mwaves
+    cd.mthlim = [p.mthlim[min(k, 3)] for k in range(p.mwaves)]
+    cd.lim_type = p.lim_type
     cd.char_decomp = p.char_decomp
     cd.tfluct_solver = p.tfluct_solver
     cd.time_integrator = p.time_integrator

@@ -711,10 +711,227 @@ void orc_weno5_char(int mx2, int meqn, int LD, const double *q, const double *ev
     free(hh);
 }
 
+/* "1.e-14" in reconstruct.f90:404,623 is a default-real literal */
+#define TOL14 ((double)1.0e-14f)
+
+/* The limiter functions of reconstruct.f90:544-571 (tvd2), 627-654
+ * (tvd2_char), 726-753 (tvd2_wave): 1 minmod, 2 superbee, 3 van Leer,
+ * 4 monotonized centred, 5 Cada & Torrilhon. */
+static double limiter(int mthlim, double r)
+{
+    switch (mthlim) {
+    case 1: return fmax(0.0, fmin(1.0, r));
+    case 2: return fmax(fmax(0.0, fmin(1.0, 2.0 * r)), fmin(2.0, r));
+    case 3: return (r + fabs(r)) / (1.0 + fabs(r));
+    case 4: {
+        double c = (1.0 + r) / 2.0;
+        return fmax(0.0, fmin(fmin(c, 2.0), 2.0 * r));
+    }
+    default: {
+        double beta = 2.0, xgamma = 2.0, alpha = 1.0 / 3.0;
+        double pp = (2.0 + r) / 3.0;
+        double amax = fmax(fmax(-alpha * r, 0.0), fmin(fmin(beta * r, pp), xgamma));
+        return fmax(0.0, fmin(pp, amax));
+    }
+    }
+}
+
+/* src/reconstruct.f90:244-347 (weno5_trans): the jumps are projected with
+ * the left eigenvectors of their own interface (gg), carried to the
+ * interface being reconstructed through the right eigenvectors of theirs
+ * and its left ones (hh), reconstructed per field and mapped back with its
+ * right eigenvectors.  hh(3,:), which the reference reads unset, is taken
+ * as 0: it only reaches qr(2), ql(3) (ghost interface 0). */
+void orc_weno5_trans(int mx2, int meqn, int LD, const double *q, const double *evl, const double *evr,
+                     double *ql, double *qr)
+{
+    const int mbc = 3;
+#define EV(a, i, r, c) (a)[(size_t)((i) - 1) + (size_t)LD * ((size_t)(r) + (size_t)meqn * (size_t)(c))]
+    const size_t S = (size_t)mx2 + 3;
+    double *dq = (double *)calloc(S * (size_t)meqn, sizeof(double));
+    double *gg = (double *)calloc(S * (size_t)meqn, sizeof(double));
+    double *hh = (double *)calloc(S * 5, sizeof(double));
+    double *u = (double *)calloc(S * (size_t)meqn * 2, sizeof(double));
+#define DQ(i, m) dq[(size_t)(i) + (size_t)(m) * S]
+#define GG(i, m) gg[(size_t)(i) + (size_t)(m) * S]
+#define HH(i, k) hh[(size_t)(i) + (size_t)((k) + 2) * S]
+#define UU(i, m, s) u[(size_t)(i) + S * ((size_t)(m) + (size_t)meqn * (size_t)(s))]
+    for (int m = 0; m < meqn; m++)
+        for (int i = 2; i <= mx2; i++) DQ(i, m) = Q2(q, i, m) - Q2(q, i - 1, m);
+    for (int mw = 0; mw < meqn; mw++)
+        for (int i = 2; i <= mx2; i++) {
+            double acc = 0.0;
+            for (int m = 0; m < meqn; m++) acc = acc + EV(evl, i, mw, m) * DQ(i, m);
+            GG(i, mw) = acc;
+        }
+    for (int mw = 0; mw < meqn; mw++) {
+        for (int m1 = -2; m1 <= 2; m1++)
+            for (int i = mbc + 1; i <= mx2 - 2; i++) {
+                double acc = 0.0;
+                for (int m = 0; m < meqn; m++)
+                    acc = acc + EV(evl, i, mw, m) * GG(i + m1, mw) * EV(evr, i + m1, m, mw);
+                HH(i, m1) = acc;
+            }
+        for (int i = mbc; i <= mx2 - mbc + 1; i++) {
+            UU(i, mw, 0) = weno_side(1.0, HH(i, -2), HH(i, -1), HH(i, 0), HH(i, 1));
+            UU(i, mw, 1) = weno_side(-1.0, HH(i, 2), HH(i, 1), HH(i, 0), HH(i, -1));
+        }
+    }
+    for (int s = 0; s < 2; s++)
+        for (int m = 0; m < meqn; m++)
+            for (int i = mbc; i <= mx2 - mbc + 1; i++) {
+                double uh = (-Q2(q, i - 2, m) + 7.0 * (Q2(q, i - 1, m) + Q2(q, i, m)) - Q2(q, i + 1, m)) / 12.0;
+                for (int mw = 0; mw < meqn; mw++) uh = uh + EV(evr, i, m, mw) * UU(i, mw, s);
+                if (s == 0)
+                    Q2(qr, i - 1, m) = uh;
+                else
+                    Q2(ql, i, m) = uh;
+            }
+#undef DQ
+#undef GG
+#undef HH
+#undef UU
+#undef EV
+    free(dq); free(gg); free(hh); free(u);
+}
+
+/* src/reconstruct.f90:350-428 (weno5_wave): WENO on the projections of the
+ * neighbouring waves onto this interface's wave, per wave family.  wave is
+ * the rp decomposition of the cell-average jumps (flux1.f90:98-99), in the
+ * slice layout with leading dimension LD.  The reference loops i = 2..mx2
+ * and reads q and wave outside their bounds at both ends; those interfaces
+ * (ghost interfaces -1, 0, mx+2, mx+3 of the slice) feed nothing the step
+ * uses, so this loops i = 4..mx2-2 and leaves the others unset. */
+void orc_weno5_wave(int mx2, int meqn, int mwaves, int LD, const double *q, const double *wave,
+                    double *ql, double *qr)
+{
+#define WV(i, m, w) wave[(size_t)((i) - 1) + (size_t)LD * ((size_t)(m) + (size_t)meqn * (size_t)(w))]
+    for (int i = 4; i <= mx2 - 2; i++) {
+        for (int m = 0; m < meqn; m++) {
+            Q2(qr, i - 1, m) = (-Q2(q, i - 2, m) + 7.0 * (Q2(q, i - 1, m) + Q2(q, i, m)) - Q2(q, i + 1, m)) / 12.0;
+            Q2(ql, i, m) = Q2(qr, i - 1, m);
+        }
+        for (int mw = 0; mw < mwaves; mw++) {
+            double u[2], wnorm2 = 0.0;
+            for (int m1 = 0; m1 < 2; m1++) {
+                int im = m1 == 0 ? 1 : -1, ione = im, inone = -im, intwo = -2 * im;
+                wnorm2 = WV(i, 0, mw) * WV(i, 0, mw);
+                double theta1 = WV(i + intwo, 0, mw) * WV(i, 0, mw);
+                double theta2 = WV(i + inone, 0, mw) * WV(i, 0, mw);
+                double theta3 = WV(i + ione, 0, mw) * WV(i, 0, mw);
+                for (int m = 1; m < meqn; m++) {
+                    wnorm2 = wnorm2 + WV(i, m, mw) * WV(i, m, mw);
+                    theta1 = theta1 + WV(i + intwo, m, mw) * WV(i, m, mw);
+                    theta2 = theta2 + WV(i + inone, m, mw) * WV(i, m, mw);
+                    theta3 = theta3 + WV(i + ione, m, mw) * WV(i, m, mw);
+                }
+                if (wnorm2 > TOL14) {
+                    u[m1] = weno_side((double)im, theta1, theta2, wnorm2, theta3);
+                    wnorm2 = 1.0 / wnorm2;
+                } else {
+                    u[m1] = 0.0;
+                    wnorm2 = 0.0;
+                }
+            }
+            for (int m = 0; m < meqn; m++) {
+                Q2(qr, i - 1, m) = Q2(qr, i - 1, m) + u[0] * WV(i, m, mw) * wnorm2;
+                Q2(ql, i, m) = Q2(ql, i, m) + u[1] * WV(i, m, mw) * wnorm2;
+            }
+        }
+    }
+#undef WV
+}
+
+/* src/reconstruct.f90:586-682 (tvd2_char): characteristic-wise limited
+ * slopes from the jumps on either side, projected with the eigenvectors of
+ * the interface between them. */
+void orc_tvd2_char(int mx2, int meqn, int LD, const double *q, const double *evl, const double *evr,
+                   const int *mthlim, double *ql, double *qr)
+{
+#define EV(a, i, r, c) (a)[(size_t)((i) - 1) + (size_t)LD * ((size_t)(r) + (size_t)meqn * (size_t)(c))]
+    const size_t S = (size_t)mx2 + 3;
+    double *dq = (double *)calloc(S * (size_t)meqn, sizeof(double));
+    double *hh = (double *)calloc(S * 3, sizeof(double));
+    double *u = (double *)calloc(S * (size_t)meqn * 2, sizeof(double));
+#define DQ(i, m) dq[(size_t)(i) + (size_t)(m) * S]
+#define HH(i, k) hh[(size_t)(i) + (size_t)((k) + 1) * S]
+#define UU(i, m, s) u[(size_t)(i) + S * ((size_t)(m) + (size_t)meqn * (size_t)(s))]
+    for (int m = 0; m < meqn; m++)
+        for (int i = 2; i <= mx2; i++) DQ(i, m) = Q2(q, i, m) - Q2(q, i - 1, m);
+    for (int m = 0; m < meqn; m++) {
+        for (int m1 = -1; m1 <= 1; m1++)
+            for (int i = 3; i <= mx2 - 1; i++) {
+                double acc = 0.0;
+                for (int mm = 0; mm < meqn; mm++) acc = acc + EV(evl, i, m, mm) * DQ(i + m1, mm);
+                HH(i, m1) = acc;
+            }
+        for (int m1 = 1; m1 <= 2; m1++) {
+            double im = m1 == 1 ? 1.0 : -1.0;
+            for (int i = 3; i <= mx2 - 1; i++) {
+                double r = fabs(HH(i, m1 - 2)) > TOL14 ? HH(i, m1 - 1) / HH(i, m1 - 2) : 0.0;
+                double sl = limiter(mthlim[m], r);
+                UU(i, m, m1 - 1) = im * 0.5 * sl * HH(i, m1 - 2);
+            }
+        }
+    }
+    for (int m = 0; m < meqn; m++)
+        for (int i = 3; i <= mx2 - 1; i++) {
+            double a = Q2(q, i - 1, m), b = Q2(q, i, m);
+            for (int mm = 0; mm < meqn; mm++) {
+                a = a + EV(evr, i, m, mm) * UU(i, mm, 0);
+                b = b + EV(evr, i, m, mm) * UU(i, mm, 1);
+            }
+            Q2(qr, i - 1, m) = a;
+            Q2(ql, i, m) = b;
+        }
+#undef DQ
+#undef HH
+#undef UU
+#undef EV
+    free(dq); free(hh); free(u);
+}
+
+/* src/reconstruct.f90:685-763 (tvd2_wave): each wave of the cell-average
+ * jumps, limited by its projection onto the upwind neighbour's (s > 0: the
+ * left one), is added to the left state and taken from the right state of
+ * its interface. */
+void orc_tvd2_wave(int mx2, int meqn, int mwaves, int LD, const double *q, const double *wave,
+                   const double *s, const int *mthlim, double *ql, double *qr)
+{
+#define WV(i, m, w) wave[(size_t)((i) - 1) + (size_t)LD * ((size_t)(m) + (size_t)meqn * (size_t)(w))]
+#define SP(i, w) s[(size_t)((i) - 1) + (size_t)LD * (size_t)(w)]
+    for (int i = 2; i <= mx2; i++)
+        for (int m = 0; m < meqn; m++) {
+            Q2(qr, i - 1, m) = Q2(q, i - 1, m);
+            Q2(ql, i, m) = Q2(q, i, m);
+        }
+    for (int mw = 0; mw < mwaves; mw++) {
+        double dotr = 0.0;
+        for (int i = 2; i <= mx2 - 2; i++) {
+            double wnorm2 = 0.0, dotl = dotr;
+            dotr = 0.0;
+            for (int m = 0; m < meqn; m++) {
+                wnorm2 = wnorm2 + WV(i, m, mw) * WV(i, m, mw);
+                dotr = dotr + WV(i, m, mw) * WV(i + 1, m, mw);
+            }
+            if (wnorm2 == 0.0) continue;
+            double r = SP(i, mw) > 0.0 ? dotl / wnorm2 : dotr / wnorm2;
+            double uu = 0.5 * limiter(mthlim[mw], r);
+            for (int m = 0; m < meqn; m++) {
+                Q2(qr, i - 1, m) = Q2(qr, i - 1, m) + WV(i, m, mw) * uu;
+                Q2(ql, i, m) = Q2(ql, i, m) - WV(i, m, mw) * uu;
+            }
+        }
+    }
+#undef WV
+#undef SP
+}
+
 /* --------------------------------------------------------------- flux1 */
 
-/* src/flux1.f90:2-190 for lim_type=2, char_decomp 0/2, mcapa=0, on one
- * slice q1d(1-mbc:mx+mbc, meqn).  In 1D dq1d is the step's dq (interior
+/* src/flux1.f90:2-190 for mcapa=0 and the reconstructions lim_type 2 with
+ * char_decomp 0-3 and lim_type 1 with char_decomp 1-2, on one slice
+ * q1d(1-mbc:mx+mbc, meqn).  In 1D dq1d is the step's dq (interior
  * zeroed by src); for ndim > 1 it is zeroed here (flux1.f90:62).  Returns
  * the slice CFL number (flux1.f90:126-133).
  *
@@ -741,13 +958,26 @@ double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double 
     double *apdq2 = (double *)calloc(n1, sizeof(double));
     double *s = (double *)calloc((size_t)L * mwaves, sizeof(double));
     double *wave = (double *)calloc(n1 * mwaves, sizeof(double));
-    if (p->char_decomp == 2) {
+    /* flux1.f90:65-117 */
+    if (p->char_decomp == 2 || p->char_decomp == 3) {
         double *evl = (double *)calloc(n1 * meqn, sizeof(double));
         double *evr = (double *)calloc(n1 * meqn, sizeof(double));
         orc_evec(p, ixy, mx, q1d, evl, evr);
-        orc_weno5_char(L, meqn, L, q1d, evl, evr, ql, qr);
+        if (p->lim_type == 1)
+            orc_tvd2_char(L, meqn, L, q1d, evl, evr, p->mthlim, ql, qr);
+        else if (p->char_decomp == 2)
+            orc_weno5_char(L, meqn, L, q1d, evl, evr, ql, qr);
+        else
+            orc_weno5_trans(L, meqn, L, q1d, evl, evr, ql, qr);
         free(evl);
         free(evr);
+    } else if (p->char_decomp == 1) {
+        /* waves of the cell-average jumps: rp with ql = qr = q1d */
+        orc_rp(p, ixy, mx, q1d, q1d, wave, s, amdq, apdq);
+        if (p->lim_type == 1)
+            orc_tvd2_wave(L, meqn, mwaves, L, q1d, wave, s, p->mthlim, ql, qr);
+        else
+            orc_weno5_wave(L, meqn, mwaves, L, q1d, wave, ql, qr);
     } else {
         orc_weno5(L, meqn, L, q1d, ql, qr);
     }

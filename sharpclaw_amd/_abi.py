@@ -55,6 +55,7 @@ class ScParams(C.Structure):
         ("bulk", C.c_double),
         ("precision", C.c_int),
         ("device", C.c_int),
+        ("mthlim", C.c_int * 4),
     ]
 
 
@@ -156,7 +157,8 @@ SHAPES = {
 
 def make_params(physics, nx, xlower=(0.0, 0.0), xupper=(1.0, 1.0), *, char_decomp=2,
                 time_integrator=3, mthbc=(1, 1, 1, 1), gamma=1.4, grav=9.81, u=1.0, rho=1.0,
-                bulk=4.0, precision=SC_FP64, efix=-1, device=0, tfluct_solver=0):
+                bulk=4.0, precision=SC_FP64, efix=-1, device=0, tfluct_solver=0, lim_type=2,
+                mthlim=(5, 5, 5, 5)):
     """Build an ScParams the way main.f90:61-122 fills ClawParams.
 
     dx = (xupper - xlower) / nx as main.f90:116 (float(nx) is exact here).
@@ -175,7 +177,7 @@ def make_params(physics, nx, xlower=(0.0, 0.0), xupper=(1.0, 1.0), *, char_decom
     p.mbc = 3
     p.maux = 0
     p.mcapa = 0
-    p.lim_type = 2
+    p.lim_type = lim_type
     p.char_decomp = char_decomp
     p.tfluct_solver = tfluct_solver
     p.time_integrator = time_integrator
@@ -198,6 +200,8 @@ def make_params(physics, nx, xlower=(0.0, 0.0), xupper=(1.0, 1.0), *, char_decom
     p.bulk = bulk
     p.precision = precision
     p.device = device
+    for k in range(4):
+        p.mthlim[k] = mthlim[k] if k < len(mthlim) else mthlim[-1]
     return p
 
 

@@ -25,6 +25,7 @@ module sharpclaw_gpu
         integer(c_int) :: physics, efix
         real(c_double) :: gamma, grav, u, rho, bulk
         integer(c_int) :: precision, device
+        integer(c_int) :: mthlim(4)
     end type sc_params
 
     type, bind(C) :: sc_run_ctl

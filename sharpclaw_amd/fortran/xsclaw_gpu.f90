@@ -164,6 +164,8 @@ program xsclaw_gpu
     prm%bulk = bulk_c
     prm%precision = precision
     prm%device = device
+    prm%mthlim = 0
+    prm%mthlim(1:min(4, mwaves)) = mthlim(1:min(4, mwaves))
     call sc_check(sc_create(prm, solver), 'sc_create')
     call sc_check(sc_set_q(solver, q), 'sc_set_q')
 

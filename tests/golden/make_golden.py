@@ -186,7 +186,64 @@ def flush():
          dtlog=dtl, ctl=np.array([1e99, 0.55, 0.5, 50000, 1]), dt_initial=np.array(1e-3))
 
 
-PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct, "flush": flush}
+RECON = [(2, 1), (2, 3), (1, 1), (1, 2)]   # (lim_type, char_decomp) beyond weno5/weno5_char
+
+
+def recon():
+    """The other reconstructions of flux1.f90:65-117: weno5_wave, weno5_trans,
+    tvd2_wave, tvd2_char (every limiter), one step per case in 1D, and a
+    run of each but tvd2_char."""
+    rng = np.random.default_rng(20261204)
+    out = {}
+    k = 0
+    for name in ("advection", "acoustics", "euler", "shallow_water"):
+        for lim, char in RECON:
+            for mth in ((1, 2, 3, 4, 5) if lim == 1 else (5,)):
+                bcs = [(1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3), (1, 3, 1, 1)]
+                if name == "advection":
+                    bcs = bcs[:2]
+                # every boundary kind with one limiter, one kind each with the others
+                for bc in (bcs if mth == 5 else [bcs[mth % len(bcs)]]):
+                    p = make_params(name, (40,), char_decomp=char, mthbc=bc, lim_type=lim,
+                                    mthlim=(mth,) * 4)
+                    q = _states.grid(p, rng, smooth=True)
+                    dt = 0.1 * p.dx[0]
+                    qb, dq, cfl = R.step(p, q, dt)
+                    out.update({f"c{k}_name": np.array(name), f"c{k}_q": q, f"c{k}_qbc": qb,
+                                f"c{k}_dq": dq, f"c{k}_cfl": np.array(cfl), f"c{k}_dt": np.array(dt),
+                                f"c{k}_bc": np.array(bc), f"c{k}_char": np.array(char),
+                                f"c{k}_lim": np.array(lim), f"c{k}_mthlim": np.array(mth)})
+                    k += 1
+    out["n"] = np.array(k)
+    save("step_recon1d.npz", **out)
+    runs_ = [
+        # (no tvd2_char run: the reference's tvd2_char writes up to three
+        # elements past its workspace (reconstruct.f90:598-606 index dq, hh,
+        # u to mx2-1 = maxnx+5 while alloc_recon_workspace sizes them to
+        # maxnx+mbc), and the program aborts in free() at its end)
+        # Shu-Osher, weno5_trans, SSP(10,4)
+        ("run_eu1d_shuosher_trans.npz", "euler", (400,), (-5.0,), (5.0,), 3, 2, 5, 4,
+         (1, 1, 1, 1), 0.36, 1e-3, 2.5, 2.45, ["2", "3.857143 2.629369 10.333333 -4.0"]),
+        # acoustics pulse between walls, weno5_wave
+        ("run_ac1d_pulse_wave.npz", "acoustics", (200,), (-1.0,), (1.0,), 1, 2, 5, 4,
+         (3, 3, 1, 1), 0.25, 1e-3, 2.5, 2.4, ["4", "1.0 100.0 0.0"]),
+        # advection, tvd2_wave with superbee, periodic, SSPRK3
+        ("run_adv1d_tvd2wave.npz", "advection", (200,), (0.0,), (1.0,), 1, 1, 2, 3,
+         (2, 2, 2, 2), 0.5, 1e-3, 0.55, 0.5, ["1", "200.0 0.3 0.6 0.8"]),
+    ]
+    for (fname, name, nx, xl, xu, char, lim, mth, ti, bc, tf, dt0, cmax, cdes, lines) in runs_:
+        p = make_params(name, nx, xlower=xl, xupper=xu, char_decomp=char, lim_type=lim,
+                        mthlim=(mth,) * 4, time_integrator=ti, mthbc=bc)
+        cd = R.clawdata_for(p, nout=1, outstyle=1, tfinal=tf, dt_initial=dt0, cfl_max=cmax,
+                            cfl_desired=cdes)
+        fr, dtl, info = R.run(p, cd, lines)
+        save(fname, t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]), dtlog=dtl,
+             ctl=np.array([1e99, cmax, cdes, 50000, 1]), dt_initial=np.array(dt0),
+             params=np.array([char, lim, mth, ti] + list(bc)), xl=np.array(xl), xu=np.array(xu))
+
+
+PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct, "flush": flush,
+         "recon": recon}
 
 if __name__ == "__main__":
     if not R.ref_available():

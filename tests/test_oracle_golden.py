@@ -168,3 +168,36 @@ def test_flush_small_matches_the_reference_writer():
     want = np.array([0.0, 0.0, 1e-99, 0.0, np.nan, np.inf, 0.0, -2e-99])
     assert np.array_equal(h, want, equal_nan=True)
     assert not np.signbit(h[[0, 1, 3, 6]]).any()
+
+
+def test_step_recon_bit_exact():
+    """weno5_wave, weno5_trans, tvd2_wave, tvd2_char with every limiter
+    (flux1.f90:65-117, reconstruct.f90:244-763)."""
+    d = load("step_recon1d.npz")
+    seen = set()
+    for k in range(int(d["n"])):
+        name, lim, char = str(d[f"c{k}_name"]), int(d[f"c{k}_lim"]), int(d[f"c{k}_char"])
+        mth = int(d[f"c{k}_mthlim"])
+        seen.add((lim, char, mth))
+        p = make_params(name, (40,), char_decomp=char, lim_type=lim, mthlim=(mth,) * 4,
+                        mthbc=tuple(int(b) for b in d[f"c{k}_bc"]))
+        qb, dq, cfl = O.step(p, d[f"c{k}_q"], float(d[f"c{k}_dt"]))
+        assert np.array_equal(qb, d[f"c{k}_qbc"]), k
+        assert np.array_equal(interior(p, dq), interior(p, d[f"c{k}_dq"])), (k, name, lim, char, mth)
+        assert cfl == float(d[f"c{k}_cfl"]), k
+    assert {(l, c) for l, c, _ in seen} == {(2, 1), (2, 3), (1, 1), (1, 2)}
+    assert {m for l, _, m in seen if l == 1} == {1, 2, 3, 4, 5}
+
+
+def _recon_params(d, name, nx):
+    c = [int(v) for v in d["params"]]
+    return make_params(name, nx, xlower=tuple(float(v) for v in d["xl"]),
+                       xupper=tuple(float(v) for v in d["xu"]), char_decomp=c[0], lim_type=c[1],
+                       mthlim=(c[2],) * 4, time_integrator=c[3], mthbc=tuple(c[4:8]))
+
+
+@pytest.mark.parametrize("fname,name,nx", [("run_eu1d_shuosher_trans.npz", "euler", (400,)),
+                                           ("run_ac1d_pulse_wave.npz", "acoustics", (200,)),
+                                           ("run_adv1d_tvd2wave.npz", "advection", (200,))])
+def test_run_recon(fname, name, nx):
+    _replay(_recon_params(load(fname), name, nx), fname)

@@ -71,3 +71,20 @@ def test_step_tfluct_live(name):
             b = R.step(p, q, 0.07 * p.dx[0])
             assert np.array_equal(interior(p, a[1]), interior(p, b[1]))
             assert a[2] == b[2]
+
+
+@pytest.mark.parametrize("name", ["advection", "acoustics", "euler", "shallow_water"])
+@pytest.mark.parametrize("lim,char", [(2, 1), (2, 3), (1, 1), (1, 2)])
+def test_step_recon_live(name, lim, char):
+    rng = np.random.default_rng(79)
+    for mth in ((1, 2, 3, 4, 5) if lim == 1 else (5,)):
+        for bc in ((1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3)):
+            if name == "advection" and bc[0] == 3:
+                continue
+            p = make_params(name, (57,), char_decomp=char, mthbc=bc, lim_type=lim,
+                            mthlim=(mth,) * 4)
+            q = _states.grid(p, rng, smooth=True, amp=0.6)
+            a = O.step(p, q, 0.07 * p.dx[0])
+            b = R.step(p, q, 0.07 * p.dx[0])
+            assert np.array_equal(interior(p, a[1]), interior(p, b[1])), (mth, bc)
+            assert a[2] == b[2]
