@@ -161,6 +161,9 @@ extern "C" int sc_batch_create(const sc_params *params, int nsolves, const int *
     int rc = sc_check_params(params, false);
     if (rc) return rc;
     if (params->ndim != 1) return sc_fail(SC_ERR_UNSUPPORTED, "batched solves are 1D grids");
+    if (params->lim_type != 2 || (params->char_decomp != 0 && params->char_decomp != 2))
+        return sc_fail(SC_ERR_UNSUPPORTED, "batched solves reconstruct with weno5 or weno5_char (lim_type 2, "
+                                           "char_decomp 0 or 2); the others run on single grids");
     if (params->time_integrator < 1 || params->time_integrator > 4)
         return sc_fail(SC_ERR_ARG, "time_integrator should be between 1 and 4");
     for (int k = 0; k < 2; k++) {

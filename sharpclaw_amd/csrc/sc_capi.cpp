@@ -89,9 +89,26 @@ int sc_check_params(const sc_params *p, bool grid)
     if (p->mbc != 3) return sc_fail(SC_ERR_ARG, "mbc must be 3 for WENO5 (reconstruct.f90:85)");
     if (p->maux != 0) return sc_fail(SC_ERR_UNSUPPORTED, "maux > 0 is not supported on the device path");
     if (p->mcapa != 0) return sc_fail(SC_ERR_UNSUPPORTED, "mcapa > 0 is not supported on the device path");
-    if (p->lim_type != 2) return sc_fail(SC_ERR_UNSUPPORTED, "lim_type %d: only 2 (WENO5) on the device path", p->lim_type);
-    if (p->char_decomp != 0 && p->char_decomp != 2)
-        return sc_fail(SC_ERR_UNSUPPORTED, "char_decomp %d: device path implements 0 and 2", p->char_decomp);
+    // reconstructions (flux1.f90:65-117)
+    if (p->lim_type == 0)
+        return sc_fail(SC_ERR_UNSUPPORTED, "lim_type 0: q2qlqr_poly leaves the interfaces within k of the boundary "
+                                           "unset (polyrecon.f90 loops i = k+1..mx-k-1)");
+    if (p->lim_type != 1 && p->lim_type != 2) return sc_fail(SC_ERR_ARG, "lim_type must be 0, 1 or 2 (got %d)", p->lim_type);
+    if (p->lim_type == 2 && (p->char_decomp < 0 || p->char_decomp > 3))
+        return sc_fail(SC_ERR_ARG, "char_decomp must be 0..3 with lim_type 2 (flux1.f90:111-114)");
+    if (p->lim_type == 1 && p->char_decomp == 0)
+        return sc_fail(SC_ERR_UNSUPPORTED, "lim_type 1 char_decomp 0: tvd2 reads its first slope before setting it "
+                                           "and divides by zero jumps unguarded (reconstruct.f90:540-542)");
+    if (p->lim_type == 1 && p->char_decomp != 1 && p->char_decomp != 2)
+        return sc_fail(SC_ERR_ARG, "char_decomp must be 1 or 2 with lim_type 1 (flux1.f90:76-91)");
+    if ((p->lim_type == 1 || p->char_decomp == 1 || p->char_decomp == 3) && p->ndim != 1)
+        return sc_fail(SC_ERR_UNSUPPORTED, "lim_type %d char_decomp %d: implemented for 1D grids", p->lim_type,
+                       p->char_decomp);
+    if (p->lim_type == 1)
+        for (int k = 0; k < p->mwaves && k < 4; k++)
+            if (p->mthlim[k] < 1 || p->mthlim[k] > 5)
+                return sc_fail(SC_ERR_ARG, "mthlim(%d) = %d: limiters are 1..5 (reconstruct.f90:626-654)", k + 1,
+                               p->mthlim[k]);
     if (p->tfluct_solver != 0 && p->tfluct_solver != 1)
         return sc_fail(SC_ERR_ARG, "tfluct_solver must be 0 or 1 (got %d)", p->tfluct_solver);
     // tfluct routines exist for advection, acoustics and 1D shallow water
@@ -196,6 +213,7 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     }
     s->sel.phys = sc_phys_id(params);
     s->sel.chr = params->char_decomp;
+    s->sel.lim = params->lim_type;
     s->sel.fp32 = params->precision == SC_FP32;
     s->cd = sc_make_const<double>(params);
     s->cf = sc_make_const<float>(params);
@@ -1042,7 +1060,7 @@ extern "C" int sc_flux1(const sc_params *params, int ixy, int mx, const double *
         return sc_fail(SC_ERR_ARG, "bad argument");
     DevGuard g(params->device);
     const int M = params->meqn, L = mx + 2 * params->mbc;
-    PhysSel sel{sc_phys_id(params), params->char_decomp, params->precision == SC_FP32};
+    PhysSel sel{sc_phys_id(params), params->char_decomp, params->precision == SC_FP32, params->lim_type};
     DevBuf dq, ddq;
     HIPCHK(dq.alloc((size_t)L * M));
     HIPCHK(ddq.alloc((size_t)L * M));

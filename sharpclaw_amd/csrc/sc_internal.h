@@ -34,5 +34,6 @@ template <class T> inline sc::PhysConst<T> sc_make_const(const sc_params *p)
     else
         c.efix = 0;
     c.tfluct = p->tfluct_solver == 1;
+    for (int k = 0; k < 4; k++) c.mthlim[k] = p->mthlim[k];
     return c;
 }

@@ -39,9 +39,10 @@ struct StageArgs {
 
 // physics selector of the launchers (sc_physics.h PhysId)
 struct PhysSel {
-    int phys;   // PhysId
-    int chr;    // 0 or 2
-    int fp32;   // 0 float64, 1 float32 arithmetic
+    int phys;     // PhysId
+    int chr;      // char_decomp
+    int fp32;     // 0 float64, 1 float32 arithmetic
+    int lim = 2;  // lim_type
 };
 
 hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const PhysConst<double> &cd,

@@ -20,15 +20,18 @@ namespace sc {
 // ------------------------------------------------------ 1D fused stage
 
 // One block of NT threads solves NT interfaces i = c0..c0+NT-1 and updates
-// the NT-1 cells c0..c0+NT-2 (stage1d_chunk).
+// the NT-1 cells c0..c0+NT-2 (stage1d_chunk).  CHAR: reconstruction code
+// (RK_*, sc_stage.h).
 template <int P, int IXY, int CHAR, class T, int NT>
 __global__ __launch_bounds__(NT) void stage1d_kernel(StageArgs a, PhysConst<T> c)
 {
     constexpr int M = Shape<P>::M;
+    constexpr int NB = nbr_words<P, IXY, CHAR, T>();
     __shared__ T sq[M][NT + 5];
     __shared__ T sx[2 * M][NT];
+    __shared__ T sw[NB > 0 ? NB : 1][NT + 4];
     __shared__ double red[NT / 64];
-    T smax = stage1d_chunk<P, IXY, CHAR, T, NT>(a, c, 1 + blockIdx.x * (NT - 1), sq, sx);
+    T smax = stage1d_chunk<P, IXY, CHAR, T, NT>(a, c, 1 + blockIdx.x * (NT - 1), sq, sx, &sw[0][0]);
     if (a.smax) merge_smax<T, NT / 64>(a.smax, smax, red);
 }
 
@@ -295,6 +298,17 @@ template <int P, int IXY>
 static hipError_t go1c(const PhysSel &s, const StageArgs &a, const PhysConst<double> &cd,
                        const PhysConst<float> &cf, hipStream_t st)
 {
+    if constexpr (P == ADV1 || P == AC1 || P == EU1 || P == SW1) {
+        // the other reconstructions of flux1.f90:65-117 (1D grids)
+        if (s.lim == 1) {
+            if (s.chr == 1) return go1p<P, IXY, RK_TVD_WAVE>(s, a, cd, cf, st);
+            if (s.chr == 2) return go1p<P, IXY, RK_TVD_CHAR>(s, a, cd, cf, st);
+            return hipErrorInvalidValue;
+        }
+        if (s.chr == 1) return go1p<P, IXY, RK_WENO_WAVE>(s, a, cd, cf, st);
+        if (s.chr == 3) return go1p<P, IXY, RK_WENO_TRANS>(s, a, cd, cf, st);
+    }
+    if (s.lim != 2 || (s.chr != 0 && s.chr != 2)) return hipErrorInvalidValue;
     return s.chr == 2 ? go1p<P, IXY, 2>(s, a, cd, cf, st) : go1p<P, IXY, 0>(s, a, cd, cf, st);
 }
 

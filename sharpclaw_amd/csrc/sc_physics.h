@@ -35,6 +35,7 @@ template <class T> struct PhysConst {
     T hzz;            // 1/(2 zz)
     int efix;
     int tfluct;       // 1: in-cell total fluctuation (tfluct_solver = 1), 1D only
+    int mthlim[4];    // TVD limiter per wave family (lim_type = 1)
 };
 
 template <class T> __device__ __forceinline__ T dsq(T x) { return x * x; }

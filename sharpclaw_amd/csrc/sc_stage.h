@@ -1,6 +1,8 @@
 // Building blocks shared by the fused stage kernels (sc_kernels.hip) and
 // the batched persistent kernel (sc_batch.hip).
 #pragma once
+#include <type_traits>
+
 #include "sc_kernels.h"
 #include "sc_weno.h"
 
@@ -80,6 +82,249 @@ __device__ __forceinline__ void merge_smax(unsigned long long *slot, T v, double
     }
 }
 
+// ------------------------------------------- 1D reconstruction variants
+//
+// Reconstruction codes of the 1D stage (flux1.f90:65-117):
+//   0 weno5, 2 weno5_char (lim_type 2, char_decomp 0/2; sc_weno.h)
+//   1 weno5_wave, 3 weno5_trans (lim_type 2, char_decomp 1/3)
+//   11 tvd2_wave, 12 tvd2_char (lim_type 1, char_decomp 1/2)
+// weno5_wave, weno5_trans and tvd2_wave use data of the neighbouring
+// interfaces (waves of the cell-average jumps, or projected jumps and
+// eigenvectors); each block computes those once per interface into LDS.
+
+constexpr int RK_WENO = 0, RK_WENO_WAVE = 1, RK_WENO_CHAR = 2, RK_WENO_TRANS = 3;
+constexpr int RK_TVD_WAVE = 11, RK_TVD_CHAR = 12;
+
+// "1.e-14" in reconstruct.f90:404,623 is a default-real literal
+template <class T> __device__ __forceinline__ T tol14() { return T((double)1.0e-14f); }
+
+// limiter functions of reconstruct.f90:626-654, 726-753 (the same five in
+// tvd2): 1 minmod, 2 superbee, 3 van Leer, 4 monotonized centred,
+// 5 Cada & Torrilhon
+template <class T> __device__ __forceinline__ T tvd_limiter(int kind, T r)
+{
+    switch (kind) {
+    case 1: return fmax(T(0), fmin(T(1), r));
+    case 2: return fmax(fmax(T(0), fmin(T(1), T(2) * r)), fmin(T(2), r));
+    case 3: {
+        T ar = fabs(r);
+        return fdiv(r + ar, T(1) + ar);
+    }
+    case 4: return fmax(T(0), fmin(fmin((T(1) + r) * T(0.5), T(2)), T(2) * r));
+    default: {
+        T pp = (T(2) + r) * T(1.0 / 3.0);
+        T amax = fmax(fmax(-(T(1.0 / 3.0) * r), T(0)), fmin(fmin(T(2) * r, pp), T(2)));
+        return fmax(T(0), fmin(pp, amax));
+    }
+    }
+}
+
+// words of T in an eigenvector object (stored to LDS for weno5_trans)
+template <int P, int IXY, class T> constexpr int eig_words()
+{
+    return std::is_empty<Eig<P, IXY, T>>::value ? 0 : (int)(sizeof(Eig<P, IXY, T>) / sizeof(T));
+}
+
+// per-interface words of neighbour data in LDS for reconstruction RK
+template <int P, int IXY, int RK, class T> constexpr int nbr_words()
+{
+    constexpr int M = Shape<P>::M, W = Shape<P>::W;
+    return (RK == RK_WENO_WAVE || RK == RK_TVD_WAVE) ? M * W + W
+           : RK == RK_WENO_TRANS                      ? M + eig_words<P, IXY, T>()
+                                                      : 0;
+}
+
+// Neighbour data of interface j (cells j-1, j = ql, qr) into column k of
+// sw (rows of NW words): the rp waves and speeds of the cell-average jump
+// (flux1.f90:98-99, 84-85: rp with ql = qr = q1d), or for weno5_trans the
+// jump projected with the interface's own left eigenvectors (gg,
+// reconstruct.f90:269-277) and the eigenvector object itself.
+template <int P, int IXY, int RK, class T>
+__device__ __forceinline__ void nbr_data(const PhysConst<T> &c, const T *ql, const T *qr, T *sw, int NW,
+                                         int k)
+{
+    constexpr int M = Shape<P>::M, W = Shape<P>::W;
+    if constexpr (RK == RK_WENO_WAVE || RK == RK_TVD_WAVE) {
+        T wave[M * W], s[W], am[M], ap[M];
+        riemann<P, IXY, T>(c, ql, qr, wave, s, am, ap);
+#pragma unroll
+        for (int e = 0; e < M * W; e++) sw[e * NW + k] = wave[e];
+#pragma unroll
+        for (int w = 0; w < W; w++) sw[(M * W + w) * NW + k] = s[w];
+    } else if constexpr (RK == RK_WENO_TRANS) {
+        Eig<P, IXY, T> e;
+        e.init(c, ql, qr);
+        T d[M], g[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) d[m] = qr[m] - ql[m];
+        e.to_char(d, g);
+#pragma unroll
+        for (int m = 0; m < M; m++) sw[m * NW + k] = g[m];
+        constexpr int NE = eig_words<P, IXY, T>();
+        if constexpr (NE > 0) {
+            T ew[NE];
+            __builtin_memcpy(ew, &e, sizeof(ew));
+#pragma unroll
+            for (int f = 0; f < NE; f++) sw[(M + f) * NW + k] = ew[f];
+        }
+    }
+}
+
+template <int P, int IXY, class T>
+__device__ __forceinline__ Eig<P, IXY, T> eig_at(const T *sw, int NW, int k)
+{
+    constexpr int M = Shape<P>::M, NE = eig_words<P, IXY, T>();
+    Eig<P, IXY, T> e;
+    if constexpr (NE > 0) {
+        T ew[NE];
+#pragma unroll
+        for (int f = 0; f < NE; f++) ew[f] = sw[(M + f) * NW + k];
+        __builtin_memcpy(&e, ew, sizeof(ew));
+    }
+    return e;
+}
+
+// Both sides of interface i-1/2 (lf = qr(i-1), rt = ql(i)) for the
+// reconstructions other than weno5 / weno5_char.  v: cells i-3..i+2;
+// sw column kc holds interface i (kc-2..kc+2: i-2..i+2).
+template <int P, int IXY, int RK, class T>
+__device__ __forceinline__ void reconstruct_other(const PhysConst<T> &c, const T (&v)[6][Shape<P>::M],
+                                                  const T *sw, int NW, int kc, T (&lf)[Shape<P>::M],
+                                                  T (&rt)[Shape<P>::M])
+{
+    constexpr int M = Shape<P>::M, W = Shape<P>::W;
+    if constexpr (RK == RK_WENO_WAVE || RK == RK_WENO_TRANS) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            T central = (-v[1][m] + T(7) * (v[2][m] + v[3][m]) - v[4][m]) * (T(1) / T(12));
+            lf[m] = central;
+            rt[m] = central;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            lf[m] = v[2][m];
+            rt[m] = v[3][m];
+        }
+    }
+    if constexpr (RK == RK_WENO_WAVE) {
+        // reconstruct.f90:364-426: WENO on the projections of the waves of
+        // interfaces i-2..i+2 onto the wave of interface i, per family
+#pragma unroll
+        for (int mw = 0; mw < W; mw++) {
+            T wi[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) wi[m] = sw[(m + M * mw) * NW + kc];
+            T dd[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                T acc = sw[(M * mw) * NW + kc - 2 + k] * wi[0];
+#pragma unroll
+                for (int m = 1; m < M; m++) acc = acc + sw[(m + M * mw) * NW + kc - 2 + k] * wi[m];
+                dd[k] = acc;
+            }
+            T u1 = T(0), u2 = T(0), sc = T(0);
+            if (dd[2] > tol14<T>()) {
+                u1 = weno_corr<T>(T(1), dd[0], dd[1], dd[2], dd[3]);
+                u2 = weno_corr<T>(T(-1), dd[4], dd[3], dd[2], dd[1]);
+                sc = frcp(dd[2]);
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                lf[m] = lf[m] + u1 * wi[m] * sc;
+                rt[m] = rt[m] + u2 * wi[m] * sc;
+            }
+        }
+    } else if constexpr (RK == RK_WENO_TRANS) {
+        // reconstruct.f90:269-345: jumps projected with their own
+        // interface's left eigenvectors (gg), carried to interface i by
+        // L(i) R(i+k), WENO per field, back with R(i)
+        Eig<P, IXY, T> own = eig_at<P, IXY, T>(sw, NW, kc);
+        T U1[M], U2[M];
+#pragma unroll
+        for (int mw = 0; mw < M; mw++) {
+            T hh[5];
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                Eig<P, IXY, T> ek = eig_at<P, IXY, T>(sw, NW, kc - 2 + k);
+                T e[M], col[M], w[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) e[m] = m == mw ? T(1) : T(0);
+                ek.from_char(e, col);
+                own.to_char(col, w);
+                hh[k] = sw[mw * NW + kc - 2 + k] * w[mw];
+            }
+            U1[mw] = weno_corr<T>(T(1), hh[0], hh[1], hh[2], hh[3]);
+            U2[mw] = weno_corr<T>(T(-1), hh[4], hh[3], hh[2], hh[1]);
+        }
+        T o1[M], o2[M];
+        own.from_char(U1, o1);
+        own.from_char(U2, o2);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            lf[m] = lf[m] + o1[m];
+            rt[m] = rt[m] + o2[m];
+        }
+    } else if constexpr (RK == RK_TVD_WAVE) {
+        // reconstruct.f90:698-760: each wave of interface i, limited by its
+        // projection onto the upwind neighbour's wave, added to the left
+        // state and taken from the right state
+#pragma unroll
+        for (int mw = 0; mw < W; mw++) {
+            T wnorm2 = T(0), dotl = T(0), dotr = T(0), wi[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                wi[m] = sw[(m + M * mw) * NW + kc];
+                T wl = sw[(m + M * mw) * NW + kc - 1], wr = sw[(m + M * mw) * NW + kc + 1];
+                wnorm2 = wnorm2 + wi[m] * wi[m];
+                dotl = dotl + wl * wi[m];
+                dotr = dotr + wi[m] * wr;
+            }
+            if (wnorm2 != T(0)) {
+                T s = sw[(M * W + mw) * NW + kc];
+                T r = fdiv(s > T(0) ? dotl : dotr, wnorm2);
+                T uu = T(0.5) * tvd_limiter<T>(c.mthlim[mw], r);
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    lf[m] = lf[m] + wi[m] * uu;
+                    rt[m] = rt[m] - wi[m] * uu;
+                }
+            }
+        }
+    } else if constexpr (RK == RK_TVD_CHAR) {
+        // reconstruct.f90:606-680: jumps on either side projected with the
+        // eigenvectors of interface i, limited per field, back with R(i)
+        Eig<P, IXY, T> e;
+        e.init(c, v[2], v[3]);
+        T dm[M], d0[M], dp[M], hm[M], h0[M], hp[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            dm[m] = v[2][m] - v[1][m];
+            d0[m] = v[3][m] - v[2][m];
+            dp[m] = v[4][m] - v[3][m];
+        }
+        e.to_char(dm, hm);
+        e.to_char(d0, h0);
+        e.to_char(dp, hp);
+        T U1[M], U2[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            T r1 = fabs(hm[m]) > tol14<T>() ? fdiv(h0[m], hm[m]) : T(0);
+            T r2 = fabs(h0[m]) > tol14<T>() ? fdiv(hp[m], h0[m]) : T(0);
+            U1[m] = T(0.5) * tvd_limiter<T>(c.mthlim[m], r1) * hm[m];
+            U2[m] = T(-0.5) * tvd_limiter<T>(c.mthlim[m], r2) * h0[m];
+        }
+        T o1[M], o2[M];
+        e.from_char(U1, o1);
+        e.from_char(U2, o2);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            lf[m] = lf[m] + o1[m];
+            rt[m] = rt[m] + o2[m];
+        }
+    }
+}
+
 // ------------------------------------------------------ 1D fused stage
 
 // One chunk of a 1D stage for a block of NT threads: interfaces
@@ -89,9 +334,11 @@ __device__ __forceinline__ void merge_smax(unsigned long long *slot, T v, double
 // interface results pass to the neighbouring thread through sx.  Returns
 // this thread's max |s| over interfaces 1..n+1 (flux1.f90:126-133).
 // Leaves with a barrier, so sq/sx can be reused by the next chunk.
+// CHAR is the reconstruction code (RK_*); the codes with neighbour data
+// need sw, nbr_words() rows of NT+4 words.
 template <int P, int IXY, int CHAR, class T, int NT>
 __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T> &c, int c0,
-                                           T (*sq)[NT + 5], T (*sx)[NT])
+                                           T (*sq)[NT + 5], T (*sx)[NT], T *sw = nullptr)
 {
     constexpr int M = Shape<P>::M, W = Shape<P>::W;
     constexpr int NC = NT - 1;
@@ -113,6 +360,21 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
     }
     __syncthreads();
 
+    constexpr int NW = NT + 4;
+    if constexpr (nbr_words<P, IXY, CHAR, T>() > 0) {
+        // interfaces c0-2..c0+NT+1: interface c0-2+k has cells sq[k], sq[k+1]
+        for (int k = t; k < NW; k += NT) {
+            T ql[M], qr[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                ql[m] = sq[m][k];
+                qr[m] = sq[m][k + 1];
+            }
+            nbr_data<P, IXY, CHAR, T>(c, ql, qr, sw, NW, k);
+        }
+        __syncthreads();
+    }
+
     const int i = c0 + t;  // interface i-1/2
     T v[6][M];
 #pragma unroll
@@ -120,7 +382,10 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
 #pragma unroll
         for (int m = 0; m < M; m++) v[k][m] = sq[m][t + k];
     T lf[M], rt[M], wave[M * W], s[W], am[M], ap[M];
-    reconstruct_iface<P, IXY, CHAR, T>(c, v, lf, rt);
+    if constexpr (CHAR == RK_WENO || CHAR == RK_WENO_CHAR)
+        reconstruct_iface<P, IXY, CHAR, T>(c, v, lf, rt);
+    else
+        reconstruct_other<P, IXY, CHAR, T>(c, v, sw, NW, t + 2, lf, rt);
     riemann<P, IXY, T>(c, lf, rt, wave, s, am, ap);
     T smax = T(0);
     if (i <= n + 1) {
