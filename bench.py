@@ -219,6 +219,40 @@ def cpu_baseline_batch(mx, hl, hr, x0, order, k):
                       f"threads, {wall:.1f} s wall"}
 
 
+RECON_1D = [("weno5", 2, 0), ("weno5_wave", 2, 1), ("weno5_char", 2, 2), ("weno5_trans", 2, 3),
+            ("tvd2_wave", 1, 1), ("tvd2_char", 1, 2)]
+
+
+def recon_bench(n=1 << 20, stages=60):
+    """The fused 1D stage per reconstruction (flux1.f90:65-117) on one 1D
+    Euler grid of n cells (smooth entropy wave, SSPRK3 stage mix, float64):
+    HIP-event time per stage launch, cell-updates/s, and the HBM rate of the
+    algorithmic bytes (state read + written, q^n read in two of three
+    stages: 3 x 8 x (2 + 3 + 3)/3 bytes per cell)."""
+    from sharpclaw_amd import Solver, interior, make_params
+    out = {"workload": f"1D Euler {n} cells, smooth entropy wave, SSPRK3 stage mix, float64, "
+                       "MC limiter for the TVD variants"}
+    x = -5.0 + (np.arange(1, n + 1) - 0.5) * (10.0 / n)
+    for name, lim, char in RECON_1D:
+        p = make_params("euler", (n,), xlower=(-5.0,), xupper=(5.0,), char_decomp=char,
+                        lim_type=lim, mthlim=(4, 4, 4, 4), time_integrator=3)
+        q = np.zeros((n + 6, 3), order="F")
+        qi = interior(p, q)
+        qi[:, 0] = 1.0 + 0.2 * np.sin(5.0 * x)
+        qi[:, 1] = 0.3 * qi[:, 0]
+        qi[:, 2] = 2.5 + 0.5 * qi[:, 1] ** 2 / qi[:, 0]
+        with Solver(p) as s:
+            s.set_q(q)
+            s.time_stage(3, 1e-6)
+            ms = s.time_stage(stages, 1e-6)
+        gbs = 3 * 8 * (2 + 3 + 3) / 3.0 * n / (ms * 1e-3) / 1e9
+        out[name] = {"lim_type": lim, "char_decomp": char, "us_per_stage": ms * 1e3,
+                     "value": n / (ms * 1e-3), "unit": "cell-updates/s per RK stage",
+                     "alg_gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+        log(f"bench: 1D {name}: {ms * 1e3:.1f} us per stage, {n / (ms * 1e-3):.3e} cell-updates/s")
+    return out
+
+
 def _serve_pass(srv, mix, ctl, paced):
     """Submit `mix` to `srv` (at the arrival times when paced, else all at
     once), drain, and return (latencies ms, cell-stages, per-kind latencies,
@@ -358,6 +392,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--requests", type=int, default=300, help="requests in the served workload")
     ap.add_argument("--rate", type=float, default=100.0, help="request arrival rate (1/s)")
+    ap.add_argument("--no-recon", action="store_true",
+                    help="skip the 1D reconstruction variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -455,6 +491,8 @@ def main():
         log(f"bench: cpu baseline {out['cpu_baseline']['value']:.3e} cell-stage/s")
     else:
         out["cpu_baseline"] = None
+    if not args.no_recon:
+        out["reconstructions_1d"] = recon_bench()
     if not args.no_batch:
         log(f"bench: batched workload ({args.batch} dam breaks)")
         out["batched"] = batched_dambreaks(args.batch, args.seed, cpu=not args.no_cpu)
