@@ -52,12 +52,30 @@ def lib():
         L.orc_sharpclaw.argtypes = [P, C.POINTER(ScRunCtl), dp, C.c_double, C.c_double,
                                     C.POINTER(C.c_double), C.POINTER(ScRunStats)]
         L.orc_sums.argtypes = [P, dp, dp]
+        L.orc_set_aux.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
 
 def _f(a):
     return np.asfortranarray(a, dtype=np.float64)
+
+
+_aux_ref = None
+
+
+def _set_aux(p, aux):
+    """The grid's aux array (1-mbc:mx+mbc, maux) for the next call (1D runs
+    with maux > 0: capacity).  Kept referenced while in use."""
+    global _aux_ref
+    if p.maux > 0:
+        if aux is None:
+            raise ValueError("maux > 0 needs the aux array")
+        _aux_ref = _f(aux)
+        lib().orc_set_aux(_aux_ref.ctypes.data_as(C.c_void_p))
+    else:
+        _aux_ref = None
+        lib().orc_set_aux(None)
 
 
 def rp(p, ixy, mx, ql, qr):
@@ -97,7 +115,8 @@ def weno5_char(q, evl, evr):
     return ql, qr
 
 
-def flux1(p, ixy, mx, q1d, dt, dq1d=None):
+def flux1(p, ixy, mx, q1d, dt, dq1d=None, aux=None):
+    _set_aux(p, aux)
     q1d = _f(q1d)
     dq = np.zeros_like(q1d, order="F") if dq1d is None else _f(dq1d).copy(order="F")
     cfl = lib().orc_flux1(C.byref(p), ixy, mx, q1d, dq, dt)
@@ -119,16 +138,18 @@ def bc(p, q):
     return q
 
 
-def step(p, q, dt, t=0.0):
+def step(p, q, dt, t=0.0, aux=None):
     """step(): returns (q with ghosts filled, dq, cfl)."""
+    _set_aux(p, aux)
     q = _f(q).copy(order="F")
     dq = np.zeros_like(q, order="F")
     cfl = lib().orc_step(C.byref(p), q, dq, dt, t)
     return q, dq, cfl
 
 
-def ts(p, q, dt, told=0.0):
+def ts(p, q, dt, told=0.0, aux=None):
     """One SSP time step from q (qold = q); returns (q_new, cfl, nevals)."""
+    _set_aux(p, aux)
     q = _f(q).copy(order="F")
     qold = q.copy(order="F")
     r1 = np.zeros_like(q, order="F")
@@ -138,8 +159,9 @@ def ts(p, q, dt, told=0.0):
     return q, cfl, n.value
 
 
-def sharpclaw(p, ctl, q, tstart, tend, dt):
+def sharpclaw(p, ctl, q, tstart, tend, dt, aux=None):
     """The time loop of one output interval; returns (q, dt_last, stats)."""
+    _set_aux(p, aux)
     q = _f(q).copy(order="F")
     d = C.c_double(dt)
     st = ScRunStats()

@@ -19,7 +19,7 @@ program harness
     character(len=16) :: mode
     character(len=256) :: fin, fout
     integer :: ixy, mx, mq, mw, nb, mx2, charf, i
-    integer :: hdr(16)
+    integer :: hdr(18)
     double precision :: rhdr(6)
     double precision, allocatable :: ql(:,:), qr(:,:), wave(:,:,:), s(:,:)
     double precision, allocatable :: amdq(:,:), apdq(:,:), evl(:,:,:), evr(:,:,:)
@@ -78,7 +78,7 @@ program harness
 
     case ('step')
         ! header: ndim nx ny meqn mwaves char_decomp mthbc(4) tfluct_solver
-        !         lim_type mthlim(4)
+        !         lim_type mthlim(4) maux mcapa; q, then aux when maux > 0
         read(21) hdr
         read(21) rhdr      ! dx dy xlower ylower dt t
         ndim = hdr(1)
@@ -95,7 +95,7 @@ program harness
             dx(2) = rhdr(2); xlower(2) = rhdr(4)
         endif
         dt = rhdr(5); t = rhdr(6)
-        mbc = 3; maux = 0; mcapa = 0; lim_type = hdr(12); tfluct_solver = hdr(11)
+        mbc = 3; maux = hdr(17); mcapa = hdr(18); lim_type = hdr(12); tfluct_solver = hdr(11)
         src_term = 0; multid_recon = 0
         allocate(mthlim(mwaves))
         do i = 1, mwaves
@@ -115,13 +115,14 @@ program harness
         allocate(g%aux2(1-mbc:maxnx+mbc, maux))
         allocate(qs(1-mbc:nx(1)+mbc, 1-mbc:nx(2)+mbc, meqn))
         allocate(dqs(1-mbc:nx(1)+mbc, 1-mbc:nx(2)+mbc, meqn))
-        allocate(auxs(1-mbc:nx(1)+mbc, 1-mbc:nx(2)+mbc, 1))
+        allocate(auxs(1-mbc:nx(1)+mbc, 1-mbc:nx(2)+mbc, max(1, maux)))
 #else
         allocate(qs(1-mbc:nx(1)+mbc, meqn), dqs(1-mbc:nx(1)+mbc, meqn))
-        allocate(auxs(1-mbc:nx(1)+mbc, 1))
+        allocate(auxs(1-mbc:nx(1)+mbc, max(1, maux)))
 #endif
         read(21) qs
         dqs = 0.d0; auxs = 0.d0
+        if (maux > 0) read(21) auxs
         call step(qs, g, dqs, auxs, dt, cfl, t, rp, src, tfluct)
         write(22) cfl
         write(22) qs, dqs

@@ -1,0 +1,14 @@
+! Project hook substituted for src/setaux.f90 in the 1D oracle build: the
+! aux array aux(1-mbc:mx+mbc, maux) is read, column-major float64, from
+! aux.bin in the working directory (written by the tests).  main.f90:165
+! calls it only when maux > 0.
+subroutine setaux(maxmx,mbc,nx,xlower,dx,maux,aux)
+    implicit none
+    integer :: maxmx, mbc, maux
+    integer :: nx(1)
+    double precision :: xlower(1), dx(1)
+    double precision :: aux(1-mbc:nx(1)+mbc, maux)
+    open(unit=52, file='aux.bin', access='stream', form='unformatted', status='old')
+    read(52) aux
+    close(52)
+end subroutine setaux

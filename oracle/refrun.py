@@ -33,6 +33,9 @@ APPS = {
 
 
 def app_of(p):
+    # src_term 1: the radial source of 1dRadialSymm-qwave (src1.f90) linked
+    if p.src_term == 1 and p.physics == SC_PHYS_SHALLOW_WATER and p.ndim == 1:
+        return "sw1dr"
     return APPS[(p.physics, p.ndim)]
 
 
@@ -40,7 +43,7 @@ def ref_available(app=None, programs=("harness", "xsclaw")):
     """Whether the reference build holds the given programs (harness_<app>
     single-routine driver, xsclaw_<app> reference program) for app (all
     apps when None)."""
-    apps = [app] if app else list(APPS.values())
+    apps = [app] if app else list(APPS.values()) + ["sw1dr"]
     return all(os.path.exists(os.path.join(REF, f"{prog}_{a}")) for a in apps for prog in programs)
 
 
@@ -119,16 +122,19 @@ def weno(p, q, evl=None, evr=None):
     return b[:n].reshape((mx2, meqn), order="F"), b[n:2 * n].reshape((mx2, meqn), order="F")
 
 
-def step(p, q, dt, t=0.0):
-    """Reference step(): returns (q after bc, dq, cfl)."""
+def step(p, q, dt, t=0.0, aux=None):
+    """Reference step(): returns (q after bc, dq, cfl).  aux: the grid's
+    aux array (1-mbc:nx+mbc, maux) when p.maux > 0."""
     shape = q.shape
     with tempfile.TemporaryDirectory() as d:
         write_setprob(p, d)
         hdr = _i32(p.ndim, p.nx[0], p.nx[1], p.meqn, p.mwaves, p.char_decomp, *list(p.mthbc),
-                   p.tfluct_solver, p.lim_type, *list(p.mthlim))
+                   p.tfluct_solver, p.lim_type, *list(p.mthlim), p.maux, p.mcapa)
         rh = np.array([p.dx[0], p.dx[1], p.xlower[0], p.xlower[1], dt, t], dtype="<f8").tobytes()
         with open(os.path.join(d, "in.bin"), "wb") as f:
             f.write(hdr + rh + _f64(q))
+            if p.maux > 0:
+                f.write(_f64(aux))
         _run(f"harness_{app_of(p)}", ["step", "in.bin", "out.bin"], d)
         b = np.fromfile(os.path.join(d, "out.bin"), dtype="<f8")
     n = int(np.prod(shape))
@@ -145,6 +151,7 @@ def clawdata_for(p, **kw):
     cd.lim_type = p.lim_type
     cd.char_decomp = p.char_decomp
     cd.tfluct_solver = p.tfluct_solver
+    cd.maux, cd.mcapa, cd.src_term = p.maux, p.mcapa, p.src_term
     cd.time_integrator = p.time_integrator
     cd.xlower = [p.xlower[d] for d in range(p.ndim)]
     cd.xupper = [p.xlower[d] + p.dx[d] * p.nx[d] for d in range(p.ndim)]
@@ -168,11 +175,15 @@ def read_frame(path):
     return t, q.copy(order="F")
 
 
-def run(p, cd, qinit_lines, beta=200.0):
+def run(p, cd, qinit_lines, beta=200.0, aux=None):
     """Run the reference program.  Returns (frames [(t, q_interior)], dtlog
-    (N x 2: told, dt of each new step), fort.info text)."""
+    (N x 2: told, dt of each new step), fort.info text).  aux: the aux array
+    setaux gives the program (aux.bin) when p.maux > 0."""
     with tempfile.TemporaryDirectory() as d:
         write_setprob(p, d, beta=beta)
+        if aux is not None:
+            with open(os.path.join(d, "aux.bin"), "wb") as f:
+                f.write(_f64(aux))
         cd.write(os.path.join(d, "sharpclaw.data"))
         with open(os.path.join(d, "qinit.data"), "w") as f:
             f.write("\n".join(qinit_lines) + "\n")

@@ -927,6 +927,12 @@ void orc_tvd2_wave(int mx2, int meqn, int mwaves, int LD, const double *q, const
 #undef SP
 }
 
+/* The grid's aux array aux(1-mbc:mx+mbc, maux) of 1D runs with maux > 0
+ * (capacity, flux1.f90:57-58).  The oracle is test infrastructure called
+ * from one thread: the array is set once, before the calls that use it. */
+static const double *g_aux = NULL;
+void orc_set_aux(const double *aux) { g_aux = aux; }
+
 /* --------------------------------------------------------------- flux1 */
 
 /* src/flux1.f90:2-190 for mcapa=0 and the reconstructions lim_type 2 with
@@ -948,7 +954,12 @@ double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double 
 {
     const int mbc = p->mbc, meqn = p->meqn, mwaves = p->mwaves, L = mx + 2 * mbc;
     const size_t n1 = (size_t)L * meqn;
-    double dtdx = dt / p->dx[ixy - 1];
+    /* flux1.f90:56-61: dtdx(i) = dt/(dx*aux(i,mcapa)) with a capacity */
+    double *dtdx = (double *)malloc((size_t)L * sizeof(double));
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        dtdx[i + mbc - 1] = (p->mcapa > 0 && g_aux) ? dt / (p->dx[ixy - 1] * A1(g_aux, i, p->mcapa - 1))
+                                                   : dt / p->dx[ixy - 1];
+#define DTDX(i) dtdx[(i) + mbc - 1]
     if (p->ndim > 1) memset(dq1d, 0, n1 * sizeof(double));
     double *ql = (double *)calloc(n1, sizeof(double));
     double *qr = (double *)calloc(n1, sizeof(double));
@@ -985,7 +996,7 @@ double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double 
     double cfl = 0.0;
     for (int mw = 0; mw < mwaves; mw++)
         for (int i = 1; i <= mx + 1; i++) {
-            double a = dtdx * S2(s, i, mw), b = -dtdx * S2(s, i, mw);
+            double a = DTDX(i) * S2(s, i, mw), b = -DTDX(i - 1) * S2(s, i, mw);
             cfl = fmax(cfl, fmax(a, b));
         }
     /* flux1.f90:163-168: qr(i-1) <- ql(i), ql(i) <- qr(i) */
@@ -999,13 +1010,15 @@ double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double 
         for (int m = 0; m < meqn; m++)
             for (int i = 1; i <= mx; i++) {
                 double adq = A1(amdq2, i, m) + A1(apdq2, i, m);
-                A1(dq1d, i, m) = A1(dq1d, i, m) - dtdx * ((A1(apdq, i, m) + adq) + A1(amdq, i + 1, m));
+                A1(dq1d, i, m) = A1(dq1d, i, m) - DTDX(i) * ((A1(apdq, i, m) + adq) + A1(amdq, i + 1, m));
             }
     } else {
         for (int m = 0; m < meqn; m++)
             for (int i = 1; i <= mx; i++)
-                A1(dq1d, i, m) = A1(dq1d, i, m) - dtdx * (A1(amdq, i + 1, m) + A1(apdq, i, m) + A1(amdq2, i, m) + A1(apdq2, i, m));
+                A1(dq1d, i, m) = A1(dq1d, i, m) - DTDX(i) * (A1(amdq, i + 1, m) + A1(apdq, i, m) + A1(amdq2, i, m) + A1(apdq2, i, m));
     }
+#undef DTDX
+    free(dtdx);
     free(ql); free(qr); free(amdq); free(apdq); free(amdq2); free(apdq2); free(s); free(wave);
     return cfl;
 }
@@ -1143,7 +1156,8 @@ double orc_flux2(const sc_params *p, const double *q, double *dq, double dt, int
     return cfl;
 }
 
-/* step1d.f90 / step2d.f90: bc, src (dq = 0 in the interior), flux.
+/* step1d.f90 / step2d.f90: bc, src (dq = 0 in the interior, or in 1D with
+ * src_term 1 the radial shallow-water source), flux.
  * Returns the CFL number, or -1 for an invalid boundary option. */
 double orc_step(const sc_params *p, double *q, double *dq, double dt, double t)
 {
@@ -1152,8 +1166,18 @@ double orc_step(const sc_params *p, double *q, double *dq, double dt, double t)
     if (orc_bc(p, q)) return -1.0;
     if (p->ndim == 1) {
         const int mx = p->nx[0], L = mx + 2 * mbc;
-        for (int m = 0; m < meqn; m++)
-            for (int i = 1; i <= mx; i++) A1(dq, i, m) = 0.0;
+        if (p->src_term == 1) {
+            /* apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/src1.f90:24-28 */
+            for (int i = 1; i <= mx; i++) {
+                double rad_pos = p->xlower[0] + (i - 0.5) * p->dx[0];
+                A1(dq, i, 0) = -1.0 * dt * A1(q, i, 1) / rad_pos;
+                A1(dq, i, 1) = -1.0 * dt * (A1(q, i, 1) * A1(q, i, 1)) / A1(q, i, 0) / rad_pos;
+            }
+        } else {
+            /* src/1d/src1.f90 */
+            for (int m = 0; m < meqn; m++)
+                for (int i = 1; i <= mx; i++) A1(dq, i, m) = 0.0;
+        }
         return orc_flux1(p, 1, mx, q, dq, dt);
     }
     const int nx = p->nx[0], ny = p->nx[1], LX = nx + 2 * mbc, LY = ny + 2 * mbc;

@@ -158,7 +158,7 @@ SHAPES = {
 def make_params(physics, nx, xlower=(0.0, 0.0), xupper=(1.0, 1.0), *, char_decomp=2,
                 time_integrator=3, mthbc=(1, 1, 1, 1), gamma=1.4, grav=9.81, u=1.0, rho=1.0,
                 bulk=4.0, precision=SC_FP64, efix=-1, device=0, tfluct_solver=0, lim_type=2,
-                mthlim=(5, 5, 5, 5)):
+                mthlim=(5, 5, 5, 5), maux=0, mcapa=0, src_term=0):
     """Build an ScParams the way main.f90:61-122 fills ClawParams.
 
     dx = (xupper - xlower) / nx as main.f90:116 (float(nx) is exact here).
@@ -175,13 +175,13 @@ def make_params(physics, nx, xlower=(0.0, 0.0), xupper=(1.0, 1.0), *, char_decom
     p.meqn = meqn
     p.mwaves = mwaves
     p.mbc = 3
-    p.maux = 0
-    p.mcapa = 0
+    p.maux = maux
+    p.mcapa = mcapa
     p.lim_type = lim_type
     p.char_decomp = char_decomp
     p.tfluct_solver = tfluct_solver
     p.time_integrator = time_integrator
-    p.src_term = 0
+    p.src_term = src_term
     for k in range(4):
         p.mthbc[k] = mthbc[k] if k < len(mthbc) else 1
     for d in range(2):

@@ -242,8 +242,80 @@ def recon():
              params=np.array([char, lim, mth, ti] + list(bc)), xl=np.array(xl), xu=np.array(xu))
 
 
+def kappa_aux(mx, maux, mcapa, xl, xu):
+    """A smooth capacity profile in column mcapa of aux(1-mbc:mx+mbc, maux)."""
+    dx = (xu - xl) / mx
+    x = xl + (np.arange(-2, mx + 4) - 0.5) * dx
+    aux = np.zeros((mx + 6, maux), order="F")
+    for m in range(maux):
+        aux[:, m] = 1.0 + 0.1 * m
+    aux[:, mcapa - 1] = 1.0 + 0.5 * np.sin(2.0 * np.pi * (x - xl) / (xu - xl)) ** 2
+    return aux
+
+
+def aux():
+    """Capacity (mcapa > 0: dtdx(i) = dt/(dx*aux(i,mcapa)), flux1.f90:56-61)
+    and the radial shallow-water source of 1dRadialSymm-qwave (src1.f90),
+    one step per case and a run of each."""
+    rng = np.random.default_rng(20261205)
+    out = {}
+    k = 0
+    for name in ("advection", "acoustics", "euler", "shallow_water"):
+        for char in (0, 2):
+            for bc in [(1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3)]:
+                if name == "advection" and bc[0] == 3:
+                    continue
+                for maux, mcapa in ((1, 1), (3, 2)):
+                    p = make_params(name, (40,), char_decomp=char, mthbc=bc, maux=maux, mcapa=mcapa)
+                    q = _states.grid(p, rng, smooth=True)
+                    a = rng.uniform(0.5, 2.0, (46, maux))
+                    dt = 0.05 * p.dx[0]
+                    qb, dq, cfl = R.step(p, q, dt, aux=a)
+                    out.update({f"c{k}_name": np.array(name), f"c{k}_q": q, f"c{k}_aux": a,
+                                f"c{k}_qbc": qb, f"c{k}_dq": dq, f"c{k}_cfl": np.array(cfl),
+                                f"c{k}_dt": np.array(dt), f"c{k}_bc": np.array(bc),
+                                f"c{k}_char": np.array(char), f"c{k}_maux": np.array(maux),
+                                f"c{k}_mcapa": np.array(mcapa), f"c{k}_src": np.array(0),
+                                f"c{k}_tfluct": np.array(0)})
+                    k += 1
+    for char in (0, 2):
+        for bc in [(3, 1, 1, 1), (1, 1, 1, 1)]:
+            for tf in (0, 1):
+                p = make_params("shallow_water", (40,), xlower=(0.0,), xupper=(2.5,), char_decomp=char,
+                                mthbc=bc, src_term=1, tfluct_solver=tf)
+                q = _states.grid(p, rng, smooth=True)
+                dt = 0.05 * p.dx[0]
+                qb, dq, cfl = R.step(p, q, dt)
+                out.update({f"c{k}_name": np.array("shallow_water"), f"c{k}_q": q,
+                            f"c{k}_aux": np.zeros((46, 1)), f"c{k}_qbc": qb, f"c{k}_dq": dq,
+                            f"c{k}_cfl": np.array(cfl), f"c{k}_dt": np.array(dt),
+                            f"c{k}_bc": np.array(bc), f"c{k}_char": np.array(char),
+                            f"c{k}_maux": np.array(0), f"c{k}_mcapa": np.array(0),
+                            f"c{k}_src": np.array(1), f"c{k}_tfluct": np.array(tf)})
+                k += 1
+    out["n"] = np.array(k)
+    save("step_aux1d.npz", **out)
+    # radial dam break (1dRadialSymm-qwave: rp1sw + src1, wall at r = 0), SSP(10,4)
+    p = make_params("shallow_water", (400,), xlower=(0.0,), xupper=(2.5,), char_decomp=2,
+                    time_integrator=4, mthbc=(3, 1, 1, 1), src_term=1)
+    cd = R.clawdata_for(p, nout=2, outstyle=1, tfinal=0.3, dt_initial=1e-4, cfl_max=1.0,
+                        cfl_desired=0.9)
+    fr, dtl, info = R.run(p, cd, ["3", "2.0 0.5 0.5"])
+    save("run_sw1d_radial_src.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, ctl=np.array([1e99, 1.0, 0.9, 50000, 1]), dt_initial=np.array(1e-4))
+    # Shu-Osher on a capacity-weighted grid (maux 2, mcapa 2), SSPRK3
+    p = make_params("euler", (400,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                    time_integrator=3, mthbc=(1, 1, 1, 1), maux=2, mcapa=2)
+    a = kappa_aux(400, 2, 2, -5.0, 5.0)
+    cd = R.clawdata_for(p, nout=2, outstyle=1, tfinal=0.2, dt_initial=1e-3, cfl_max=0.5,
+                        cfl_desired=0.45)
+    fr, dtl, info = R.run(p, cd, ["2", "3.857143 2.629369 10.333333 -4.0"], aux=a)
+    save("run_eu1d_capacity.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, ctl=np.array([1e99, 0.5, 0.45, 50000, 1]), dt_initial=np.array(1e-3), aux=a)
+
+
 PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct, "flush": flush,
-         "recon": recon}
+         "recon": recon, "aux": aux}
 
 if __name__ == "__main__":
     if not R.ref_available():

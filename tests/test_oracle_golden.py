@@ -74,7 +74,7 @@ def test_step_tfluct_bit_exact():
     assert names == {"advection", "acoustics", "shallow_water"}
 
 
-def _replay(p, fname, single_step=False):
+def _replay(p, fname, single_step=False, aux=None):
     d = load(fname)
     c = d["ctl"]
     ctl = ScRunCtl(dt_max=float(c[0]), cfl_max=float(c[1]), cfl_desired=float(c[2]),
@@ -90,7 +90,7 @@ def _replay(p, fname, single_step=False):
         # frames are written every nstepout = nstop calls here
         nsteps = int(c[3])
         for _ in range(nsteps):
-            q, dt, st = O.sharpclaw(p, ctl, q, t, t - 1.0, dt)
+            q, dt, st = O.sharpclaw(p, ctl, q, t, t - 1.0, dt, aux=aux)
             t = st.t
         q = O.flush_small(p, q)
         assert t == float(d["t"][-1])
@@ -98,7 +98,7 @@ def _replay(p, fname, single_step=False):
         return
     nsteps = 0
     for k in range(1, len(d["t"])):
-        q, dt, st = O.sharpclaw(p, ctl, q, t, float(d["t"][k]), dt)
+        q, dt, st = O.sharpclaw(p, ctl, q, t, float(d["t"][k]), dt, aux=aux)
         q = O.flush_small(p, q)
         t = st.t
         nsteps += st.steps_taken - st.rejected
@@ -201,3 +201,48 @@ def _recon_params(d, name, nx):
                                            ("run_adv1d_tvd2wave.npz", "advection", (200,))])
 def test_run_recon(fname, name, nx):
     _replay(_recon_params(load(fname), name, nx), fname)
+
+
+def test_step_aux_bit_exact():
+    """Capacity (mcapa > 0, flux1.f90:56-61: dtdx(i) = dt/(dx*aux(i,mcapa)),
+    also in the CFL) and the radial shallow-water source of
+    1dRadialSymm-qwave (src1.f90:24-28), with and without tfluct."""
+    d = load("step_aux1d.npz")
+    kinds = set()
+    for k in range(int(d["n"])):
+        name = str(d[f"c{k}_name"])
+        src, maux, mcapa = int(d[f"c{k}_src"]), int(d[f"c{k}_maux"]), int(d[f"c{k}_mcapa"])
+        kinds.add((src, mcapa > 0))
+        kw = dict(xlower=(0.0,), xupper=(2.5,)) if src else {}
+        p = make_params(name, (40,), char_decomp=int(d[f"c{k}_char"]),
+                        mthbc=tuple(int(b) for b in d[f"c{k}_bc"]), maux=maux, mcapa=mcapa,
+                        src_term=src, tfluct_solver=int(d[f"c{k}_tfluct"]), **kw)
+        qb, dq, cfl = O.step(p, d[f"c{k}_q"], float(d[f"c{k}_dt"]),
+                             aux=d[f"c{k}_aux"] if maux else None)
+        assert np.array_equal(qb, d[f"c{k}_qbc"]), k
+        assert np.array_equal(interior(p, dq), interior(p, d[f"c{k}_dq"])), (k, name)
+        assert cfl == float(d[f"c{k}_cfl"]), k
+    assert kinds == {(0, True), (1, False)}
+
+
+def test_run_sw1d_radial_src():
+    _replay(make_params("shallow_water", (400,), xlower=(0.0,), xupper=(2.5,), char_decomp=2,
+                        time_integrator=4, mthbc=(3, 1, 1, 1), src_term=1),
+            "run_sw1d_radial_src.npz")
+
+
+def test_run_eu1d_capacity():
+    p = make_params("euler", (400,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                    time_integrator=3, mthbc=(1, 1, 1, 1), maux=2, mcapa=2)
+    d = load("run_eu1d_capacity.npz")
+    _replay(p, "run_eu1d_capacity.npz", aux=d["aux"])
+    # the capacity changes the run
+    p0 = make_params("euler", (400,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                     time_integrator=3, mthbc=(1, 1, 1, 1))
+    q = O.new_grid(p0)
+    interior(p0, q)[:] = d["q"][0]
+    c = d["ctl"]
+    ctl = ScRunCtl(dt_max=float(c[0]), cfl_max=float(c[1]), cfl_desired=float(c[2]),
+                   max_steps=int(c[3]), dt_variable=int(c[4]))
+    q0, _, _ = O.sharpclaw(p0, ctl, q, 0.0, float(d["t"][1]), float(d["dt_initial"]))
+    assert np.max(np.abs(interior(p0, q0) - d["q"][1])) > 1e-3

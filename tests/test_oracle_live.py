@@ -88,3 +88,32 @@ def test_step_recon_live(name, lim, char):
             b = R.step(p, q, 0.07 * p.dx[0])
             assert np.array_equal(interior(p, a[1]), interior(p, b[1])), (mth, bc)
             assert a[2] == b[2]
+
+
+@pytest.mark.parametrize("name", ["advection", "acoustics", "euler", "shallow_water"])
+def test_step_capacity_live(name):
+    rng = np.random.default_rng(80)
+    for bc in ((1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3)):
+        if name == "advection" and bc[0] == 3:
+            continue
+        for char in (0, 2):
+            p = make_params(name, (57,), char_decomp=char, mthbc=bc, maux=2, mcapa=1)
+            q = _states.grid(p, rng, smooth=True, amp=0.6)
+            aux = rng.uniform(0.5, 2.0, (63, 2))
+            a = O.step(p, q, 0.05 * p.dx[0], aux=aux)
+            b = R.step(p, q, 0.05 * p.dx[0], aux=aux)
+            assert np.array_equal(interior(p, a[1]), interior(p, b[1]))
+            assert a[2] == b[2]
+
+
+def test_step_radial_source_live():
+    rng = np.random.default_rng(81)
+    for char in (0, 2):
+        for tf in (0, 1):
+            p = make_params("shallow_water", (57,), xlower=(0.0,), xupper=(2.5,), char_decomp=char,
+                            mthbc=(3, 1, 1, 1), src_term=1, tfluct_solver=tf)
+            q = _states.grid(p, rng, smooth=True, amp=0.6)
+            a = O.step(p, q, 0.05 * p.dx[0])
+            b = R.step(p, q, 0.05 * p.dx[0])
+            assert np.array_equal(interior(p, a[1]), interior(p, b[1]))
+            assert a[2] == b[2]
