@@ -57,8 +57,8 @@ typedef struct sc_params {
     int meqn;
     int mwaves;
     int mbc;             /* must be 3 (reconstruct.f90:85,158 hard-code it) */
-    int maux;            /* must be 0 on the device path */
-    int mcapa;           /* must be 0 on the device path */
+    int maux;            /* aux arrays per cell (1D grids; sc_set_aux) */
+    int mcapa;           /* capacity column of aux, 1-based; 0 none (flux1.f90:56-61) */
     int lim_type;        /* 2 WENO5; 1 second-order TVD (1D, single grid) */
     int char_decomp;     /* lim_type 2: 0 weno5, 1 weno5_wave (1D), 2 weno5_char,
                             3 weno5_trans (1D); lim_type 1: 1 tvd2_wave, 2 tvd2_char
@@ -67,7 +67,9 @@ typedef struct sc_params {
                             fluctuation (flux1.f90:136-153), 1D advection/acoustics/
                             shallow water only: the applications with a tfluct */
     int time_integrator; /* 1 SSP22, 2 SSP42, 3 SSP33, 4 SSP(10,4) */
-    int src_term;        /* must be 0 (default src: dq = 0 in the interior) */
+    int src_term;        /* source: 0 none (src1.f90/src2.f90: dq = 0); 1 the radial
+                            shallow-water source of apps/shallowWater/2dRadialDam/
+                            1dRadialSymm-qwave/src1.f90 (1D shallow water) */
     int mthbc[4];        /* 1 extrapolation, 2 periodic, 3 wall (bc1.f90, bc2.f90) */
     double xlower[2];
     double dx[2];
@@ -121,6 +123,12 @@ int sc_destroy(sc_solver *s);
 /* Host <-> device copies of the whole q array, ghosts included. */
 int sc_set_q(sc_solver *s, const double *q);
 int sc_get_q(sc_solver *s, double *q);
+
+/* The aux array aux(1-mbc:mx+mbc, maux) of a 1D grid, column-major as
+ * setaux(maxmx,mbc,nx,xlower,dx,maux,aux) fills it (main.f90:164-166).
+ * The device keeps the capacity column mcapa (flux1.f90:56-61); the device
+ * Riemann solvers take no aux.  Needed before stepping when mcapa > 0. */
+int sc_set_aux(sc_solver *s, const double *aux);
 
 /* Frame output, replacing output(meqn,mbc,nx,xlower,dx,q,t,iframe,aux,maux)
  * (main.f90:172,233; src/1d/out1.f90:32-104, src/2d/out2.f90:26-111) on the

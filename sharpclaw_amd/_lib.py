@@ -38,6 +38,7 @@ def lib():
         "sc_destroy": ([H], C.c_int),
         "sc_set_q": ([H, dp], C.c_int),
         "sc_get_q": ([H, dp], C.c_int),
+        "sc_set_aux": ([H, dp], C.c_int),
         "sc_output_begin": ([H], C.c_int),
         "sc_output_wait": ([H, dp], C.c_int),
         "sc_write_restart": ([H, C.c_char_p, C.c_double, C.c_double], C.c_int),

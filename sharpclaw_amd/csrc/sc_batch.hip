@@ -161,6 +161,8 @@ extern "C" int sc_batch_create(const sc_params *params, int nsolves, const int *
     int rc = sc_check_params(params, false);
     if (rc) return rc;
     if (params->ndim != 1) return sc_fail(SC_ERR_UNSUPPORTED, "batched solves are 1D grids");
+    if (params->maux != 0 || params->src_term != 0)
+        return sc_fail(SC_ERR_UNSUPPORTED, "batched solves take no aux arrays and no source term");
     if (params->lim_type != 2 || (params->char_decomp != 0 && params->char_decomp != 2))
         return sc_fail(SC_ERR_UNSUPPORTED, "batched solves reconstruct with weno5 or weno5_char (lim_type 2, "
                                            "char_decomp 0 or 2); the others run on single grids");

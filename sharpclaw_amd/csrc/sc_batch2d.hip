@@ -66,6 +66,11 @@ __global__ __launch_bounds__(256) void stage2d_batch_kernel(Batch2DArgs A, PhysC
     a.op = plan.op[A.stage];
     a.ntx = G.ntx;
     a.nty = G.nty;
+    a.capa = nullptr;
+    a.src = 0;
+    a.dt = s.dt;
+    a.xlo = 0.0;
+    a.dxc = G.dx;
     stage2d_tile<P, CHAR, T, TX, TY>(a, c, local);
 }
 

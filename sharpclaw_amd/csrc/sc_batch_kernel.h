@@ -129,6 +129,11 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
         a.ntx = a.nty = 1;
         a.qbin = buf[3];
         a.qbout = buf[3];
+        a.capa = nullptr;  // batched solves have no aux and no source
+        a.src = 0;
+        a.dt = 0.0;
+        a.xlo = 0.0;
+        a.dxc = S.dx;
 
         int lim = 0x7fffffff;
         if (A.step_limit > 0) lim = A.step_limit;

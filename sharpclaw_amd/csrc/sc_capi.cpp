@@ -87,8 +87,12 @@ int sc_check_params(const sc_params *p, bool grid)
         return sc_fail(SC_ERR_ARG, "physics %d in %dD needs meqn=%d mwaves=%d (got %d, %d)", p->physics,
                     p->ndim, shape_m(id), shape_w(id), p->meqn, p->mwaves);
     if (p->mbc != 3) return sc_fail(SC_ERR_ARG, "mbc must be 3 for WENO5 (reconstruct.f90:85)");
-    if (p->maux != 0) return sc_fail(SC_ERR_UNSUPPORTED, "maux > 0 is not supported on the device path");
-    if (p->mcapa != 0) return sc_fail(SC_ERR_UNSUPPORTED, "mcapa > 0 is not supported on the device path");
+    // aux arrays (main.f90:96-98): the device physics take no aux in their
+    // Riemann solvers; a capacity column (flux1.f90:56-61) is used on 1D grids
+    if (p->maux < 0 || p->mcapa < 0 || p->mcapa > p->maux)
+        return sc_fail(SC_ERR_ARG, "mcapa (%d) must index aux(:,1..maux) (maux = %d)", p->mcapa, p->maux);
+    if (p->maux > 0 && p->ndim != 1)
+        return sc_fail(SC_ERR_UNSUPPORTED, "aux arrays (maux > 0) are implemented for 1D grids");
     // reconstructions (flux1.f90:65-117)
     if (p->lim_type == 0)
         return sc_fail(SC_ERR_UNSUPPORTED, "lim_type 0: q2qlqr_poly leaves the interfaces within k of the boundary "
@@ -117,7 +121,13 @@ int sc_check_params(const sc_params *p, bool grid)
     // library's own tfluct (src/tfluct.f90) stops the run
     if (p->tfluct_solver == 1 && (p->ndim != 1 || p->physics == SC_PHYS_EULER))
         return sc_fail(SC_ERR_UNSUPPORTED, "tfluct_solver 1: the reference has a tfluct routine for 1D advection, acoustics and shallow water only");
-    if (p->src_term != 0) return sc_fail(SC_ERR_UNSUPPORTED, "src_term != 0 is not supported on the device path");
+    // source terms: 0 none (src/1d/src1.f90, src/2d/src2.f90); 1 the radial
+    // shallow-water source of apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/src1.f90
+    if (p->src_term == 1 && !(p->physics == SC_PHYS_SHALLOW_WATER && p->ndim == 1))
+        return sc_fail(SC_ERR_UNSUPPORTED, "src_term 1 (radial shallow-water source) is for 1D shallow water");
+    if (p->src_term != 0 && p->src_term != 1)
+        return sc_fail(SC_ERR_UNSUPPORTED, "src_term %d: device sources are 0 (none) and 1 (radial shallow water)",
+                       p->src_term);
     if (p->precision != SC_FP64 && p->precision != SC_FP32) return sc_fail(SC_ERR_ARG, "precision must be SC_FP64 or SC_FP32");
     if (!grid) return SC_OK;
     if (p->time_integrator < 1 || p->time_integrator > 4)
@@ -170,6 +180,7 @@ struct sc_solver {
     bool use_graph;    // device loop: replay rounds from a captured graph
     Batch2DRunner *dr; // 2D device-resident loop (created at first use)
     hipEvent_t ev0, ev1;
+    double *capa;      // capacity aux(:,mcapa) in the q layout (1D, mcapa > 0)
     // frame output (created at first sc_output_begin): device snapshot,
     // pinned host copy, copy stream and its events
     double *snap;
@@ -276,6 +287,7 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->cst) hipStreamSynchronize(s->cst);
+    if (s->capa) hipFree(s->capa);
     if (s->snap) hipFree(s->snap);
     if (s->hsnap) hipHostFree(s->hsnap);
     if (s->ev_snap) hipEventDestroy(s->ev_snap);
@@ -449,6 +461,23 @@ extern "C" int sc_read_restart(sc_solver *s, const char *path, double *t, double
     return SC_OK;
 }
 
+// ------------------------------------------------------------------- aux
+
+extern "C" int sc_set_aux(sc_solver *s, const double *aux)
+{
+    if (!s || !aux) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (s->p.maux < 1) return sc_fail(SC_ERR_STATE, "maux = 0: the solver has no aux arrays");
+    if (s->p.mcapa < 1) return SC_OK;  // no device physics reads aux otherwise
+    DevGuard g(s->dev);
+    const size_t LX = (size_t)s->p.nx[0] + 2 * s->p.mbc;
+    if (!s->capa && hipMalloc(&s->capa, LX * sizeof(double)) != hipSuccess)
+        return sc_fail(SC_ERR_HIP, "aux: %s", hipGetErrorString(hipGetLastError()));
+    HIPCHK(hipMemcpyAsync(s->capa, aux + (size_t)(s->p.mcapa - 1) * LX, LX * sizeof(double), hipMemcpyHostToDevice,
+                          s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return SC_OK;
+}
+
 extern "C" int sc_save_qold(sc_solver *s)
 {
     if (!s) return sc_fail(SC_ERR_ARG, "NULL solver");
@@ -472,6 +501,10 @@ static StageArgs base_args(const sc_solver *s)
     a.LX = s->p.nx[0] + 2 * s->p.mbc;
     a.LY = s->p.ndim == 2 ? s->p.nx[1] + 2 * s->p.mbc : 1;
     for (int k = 0; k < 4; k++) a.bc[k] = s->p.mthbc[k];
+    a.capa = s->p.mcapa > 0 ? s->capa : nullptr;
+    a.src = s->p.src_term;
+    a.xlo = s->p.xlower[0];
+    a.dxc = s->p.dx[0];
     return a;
 }
 
@@ -584,6 +617,7 @@ extern "C" int sc_step(sc_solver *s, double dt, double t, double *dq, double *cf
     (void)t;
     if (!s || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
+    if (s->p.mcapa > 0 && !s->capa) return sc_fail(SC_ERR_STATE, "mcapa > 0: no aux uploaded (sc_set_aux)");
     DevGuard g(s->dev);
     StageArgs a = base_args(s);
     double *q = s->buf[s->cur];
@@ -593,6 +627,7 @@ extern "C" int sc_step(sc_solver *s, double dt, double t, double *dq, double *cf
     a.qout = s->dq;
     a.op = OP_DQ;
     a.dtdx = dt / s->p.dx[0];
+    a.dt = dt;
     a.dtdy = dt / s->p.dx[1];
     a.smax = next_slot(s);
     HIPCHK(run_stage(s, a));
@@ -630,6 +665,7 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
 {
     StageArgs a = base_args(s);
     a.dtdx = dt / s->p.dx[0];
+    a.dt = dt;
     a.dtdy = dt / s->p.dx[1];
     a.qold = s->buf[s->old];
     a.qbin = s->qb;
@@ -655,6 +691,7 @@ extern "C" int sc_ts(sc_solver *s, double dt, double told, double *cfl)
     (void)told;
     if (!s || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
+    if (s->p.mcapa > 0 && !s->capa) return sc_fail(SC_ERR_STATE, "mcapa > 0: no aux uploaded (sc_set_aux)");
     DevGuard g(s->dev);
     int rc = tune_tile(s);
     if (rc) return rc;
@@ -749,6 +786,7 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
 {
     if (!s || !c || !dt_io || !st) return sc_fail(SC_ERR_ARG, "NULL argument");
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
+    if (s->p.mcapa > 0 && !s->capa) return sc_fail(SC_ERR_STATE, "mcapa > 0: no aux uploaded (sc_set_aux)");
     DevGuard g(s->dev);
     int trc = tune_tile(s);
     if (trc) return trc;
@@ -862,11 +900,13 @@ extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stag
 {
     if (!s || !ms_per_stage || n < 1) return sc_fail(SC_ERR_ARG, "bad argument");
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
+    if (s->p.mcapa > 0 && !s->capa) return sc_fail(SC_ERR_STATE, "mcapa > 0: no aux uploaded (sc_set_aux)");
     DevGuard g(s->dev);
     int trc = tune_tile(s);
     if (trc) return trc;
     StageArgs a = base_args(s);
     a.dtdx = dt / s->p.dx[0];
+    a.dt = dt;
     a.dtdy = dt / s->p.dx[1];
     const int q0 = s->cur;
     a.qold = s->buf[q0];
@@ -1058,6 +1098,8 @@ extern "C" int sc_flux1(const sc_params *params, int ixy, int mx, const double *
     if (rc) return rc;
     if (mx < 1 || (ixy != 1 && ixy != 2) || (ixy == 2 && params->ndim != 2) || !q1d || !dq1d || !cfl)
         return sc_fail(SC_ERR_ARG, "bad argument");
+    if (params->mcapa > 0)
+        return sc_fail(SC_ERR_UNSUPPORTED, "sc_flux1 takes no aux array: capacity runs go through sc_set_aux");
     DevGuard g(params->device);
     const int M = params->meqn, L = mx + 2 * params->mbc;
     PhysSel sel{sc_phys_id(params), params->char_decomp, params->precision == SC_FP32, params->lim_type};

@@ -35,6 +35,12 @@ struct StageArgs {
     double dtdx, dtdy;
     int op;
     int ntx, nty;        // tiles per direction (2D)
+    // 1D only: capacity aux(:,mcapa) in the q layout (NULL: none), and the
+    // source term (0 none, 1 radial shallow water) with the stage's dt and
+    // the cell-centre coordinates xlo + (i - 1/2) dxc it needs
+    const double *capa;
+    int src;
+    double dt, xlo, dxc;
 };
 
 // physics selector of the launchers (sc_physics.h PhysId)

@@ -658,6 +658,8 @@ extern "C" int sc_server_submit(sc_server *S, const sc_params *p, const sc_run_c
     int rc = sc_check_params(p, true);
     if (rc) return rc;
     if (p->device != S->dev) return sc_fail(SC_ERR_ARG, "request device %d, server device %d", p->device, S->dev);
+    if (p->maux != 0 || p->src_term != 0)
+        return sc_fail(SC_ERR_UNSUPPORTED, "served solves take no aux arrays and no source term");
     if (p->lim_type != 2 || (p->char_decomp != 0 && p->char_decomp != 2))
         return sc_fail(SC_ERR_UNSUPPORTED, "served solves reconstruct with weno5 or weno5_char (lim_type 2, "
                                            "char_decomp 0 or 2); the others run on single grids");

@@ -389,8 +389,16 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
     riemann<P, IXY, T>(c, lf, rt, wave, s, am, ap);
     T smax = T(0);
     if (i <= n + 1) {
+        if (a.capa) {
+            // flux1.f90:128-132 with dtdx(i) = dt/(dx*capa(i)): s > 0 is
+            // scaled by the right cell's capacity, s < 0 by the left one's
+            const T kr = T(1) / T(a.capa[i + 2]), kl = T(1) / T(a.capa[i + 1]);
 #pragma unroll
-        for (int w = 0; w < W; w++) smax = fmax(smax, fabs(s[w]));
+            for (int w = 0; w < W; w++) smax = fmax(smax, fmax(s[w] * kr, -(s[w] * kl)));
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; w++) smax = fmax(smax, fabs(s[w]));
+        }
     }
 #pragma unroll
     for (int m = 0; m < M; m++) {
@@ -406,6 +414,21 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
         for (int m = 0; m < M; m++) qrc[m] = sx[m][t + 1];
         riemann<P, IXY, T>(c, rt, qrc, wave, s, am2, ap2);
         const bool rq = op_reads_qold(a.op);
+        // dtdx(i) = dt/(dx*capa(i)) with a capacity (flux1.f90:56-61)
+        const double dtdx = a.capa ? a.dtdx / a.capa[i + 2] : a.dtdx;
+        // src(q,dq,...) before flux1 (step1d.f90): dq starts from the source
+        double sv[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) sv[m] = 0.0;
+        if constexpr (P == SW1) {
+            if (a.src == 1) {
+                // apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/src1.f90:24-28
+                const double h = a.qin[(size_t)(i + 2)], hu = a.qin[(size_t)(i + 2) + a.LX];
+                const double rad = a.xlo + (i - 0.5) * a.dxc;
+                sv[0] = -1.0 * a.dt * hu / rad;
+                sv[1] = -1.0 * a.dt * (hu * hu) / h / rad;
+            }
+        }
 #pragma unroll
         for (int m = 0; m < M; m++) {
             // flux1.f90:150-152 (tfluct: the in-cell problem's total
@@ -413,7 +436,7 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
             // or flux1.f90:183-186 (the same problem through rp, swapped)
             T X = c.tfluct ? (ap[m] + (am2[m] + ap2[m])) + sx[M + m][t + 1]
                            : ((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m];
-            double r = -(a.dtdx * (double)X);
+            double r = sv[m] - dtdx * (double)X;
             size_t g = (size_t)(i + 2) + (size_t)m * a.LX;
             double q = a.qin[g];
             double qo = rq ? a.qold[g] : 0.0;

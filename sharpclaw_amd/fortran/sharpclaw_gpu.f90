@@ -64,6 +64,11 @@ module sharpclaw_gpu
             type(c_ptr), value :: solver
             real(c_double), intent(inout) :: q(*)
         end function
+        integer(c_int) function sc_set_aux(solver, aux) bind(C, name="sc_set_aux")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: solver
+            real(c_double), intent(in) :: aux(*)
+        end function
         integer(c_int) function sc_output_begin(solver) bind(C, name="sc_output_begin")
             import :: c_int, c_ptr
             type(c_ptr), value :: solver

@@ -9,7 +9,10 @@
 !
 ! Device options live in gpu.data (so sharpclaw.data stays as the reference
 ! reads it):  physics id (1 advection, 2 acoustics, 3 Euler, 4 shallow
-! water), precision (0 float64, 1 float32), device ordinal, efix (-1 default).
+! water), precision (0 float64, 1 float32), device ordinal, efix (-1 default),
+! and optionally a source id (0 none; 1 the radial shallow-water source of
+! 1dRadialSymm-qwave/src1.f90), which the reference selects by linking that
+! app's src (its sharpclaw.data src_term is read and ignored, main.f90:96).
 ! The physics constants are taken from the common blocks the user's setprob
 ! fills (/param/ gamma,gamma1; /comrp/ u or g; /cparam/ rho,bulk,cc,zz).
 module xsclaw_data
@@ -46,7 +49,7 @@ program xsclaw_gpu
     type(sc_params) :: prm
     type(sc_run_ctl) :: ctl
     type(c_ptr) :: solver
-    integer :: md, i, n, iframe, info, physics, precision, device, efix
+    integer :: md, i, n, iframe, info, physics, precision, device, efix, source, ios
     integer :: steps_taken
     double precision :: tstart, tend, dtout
     double precision :: cfl_maxused, cfl_last, dt_minused, dt_maxused, dt_last
@@ -124,6 +127,9 @@ program xsclaw_gpu
     read(56,*) precision
     read(56,*) device
     read(56,*) efix
+    source = 0
+    read(56,*,iostat=ios) source
+    if (ios /= 0) source = 0
     close(56)
 
     call setprob
@@ -151,7 +157,7 @@ program xsclaw_gpu
     prm%char_decomp = char_decomp
     prm%tfluct_solver = tfluct_solver
     prm%time_integrator = time_integrator
-    prm%src_term = src_term
+    prm%src_term = source
     prm%mthbc = mthbc
     prm%xlower = xlower
     prm%dx = dx
@@ -167,6 +173,7 @@ program xsclaw_gpu
     prm%mthlim = 0
     prm%mthlim(1:min(4, mwaves)) = mthlim(1:min(4, mwaves))
     call sc_check(sc_create(prm, solver), 'sc_create')
+    if (maux > 0) call sc_check(sc_set_aux(solver, aux), 'sc_set_aux')
     call sc_check(sc_set_q(solver, q), 'sc_set_q')
 
     ctl%dt_max = dt_max

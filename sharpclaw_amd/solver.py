@@ -66,6 +66,14 @@ class Solver:
         check(lib().sc_get_q(self._h, q))
         return q
 
+    def set_aux(self, aux):
+        """aux(1-mbc:mx+mbc, maux) of the grid (setaux's array); the device
+        keeps the capacity column mcapa."""
+        aux = _f(aux)
+        if self.p.maux > 0 and aux.shape != (self.shape[0], self.p.maux):
+            raise ValueError(f"aux has shape {aux.shape}, expected {(self.shape[0], self.p.maux)}")
+        check(lib().sc_set_aux(self._h, aux))
+
     def output_begin(self):
         """Start a frame: |q| < 1e-99 -> 0 in q itself (the reference's
         writers), ghosts filled, device snapshot copied to the host
