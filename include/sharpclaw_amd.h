@@ -84,6 +84,10 @@ typedef struct sc_params {
     int mthlim[4];       /* limiter per wave family (sharpclaw.data mthlim), read
                             when lim_type = 1: 1 minmod, 2 superbee, 3 van Leer,
                             4 MC, 5 Cada-Torrilhon (reconstruct.f90:544-571) */
+    double tperiod;      /* 2D advection (velocities in aux(:,:,1:2)): before each
+                            new time step the velocities are aux0 * cos(2 pi
+                            (t + dt/2) / tperiod), aux0 the sc_set_aux array
+                            (apps/advection/2d/swirl/b4step.f90:26-41); 0: constant */
 } sc_params;
 
 /* Time-step control of src/sharpclaw.f90:2-243 (read from sharpclaw.data). */

@@ -53,6 +53,7 @@ def lib():
                                     C.POINTER(C.c_double), C.POINTER(ScRunStats)]
         L.orc_sums.argtypes = [P, dp, dp]
         L.orc_set_aux.argtypes = [C.c_void_p]
+        L.orc_swirl_setaux.argtypes = [P, dp]
         _lib = L
     return _lib
 
@@ -176,6 +177,15 @@ def flush_small(p, q):
     qi = interior(p, q)
     qi[np.abs(qi) < 1e-99] = 0.0
     return q
+
+
+def swirl_setaux(p, maux=2):
+    """apps/advection/2d/swirl/setaux.f: edge velocities of the swirl
+    stream function, aux(1-mbc:nx+mbc, 1-mbc:ny+mbc, maux) (columns past 2
+    zero)."""
+    aux = np.zeros((p.nx[0] + 2 * p.mbc, p.nx[1] + 2 * p.mbc, maux), order="F")
+    lib().orc_swirl_setaux(C.byref(p), aux)
+    return aux
 
 
 def sums(p, q):

@@ -28,6 +28,7 @@ APPS = {
     (SC_PHYS_EULER, 1): "eu1d",
     (SC_PHYS_SHALLOW_WATER, 1): "sw1d",
     (SC_PHYS_EULER, 2): "eu2d",
+    (SC_PHYS_ADVECTION, 2): "adv2d",
     (SC_PHYS_SHALLOW_WATER, 2): "sw2d",
 }
 
@@ -50,7 +51,9 @@ def ref_available(app=None, programs=("harness", "xsclaw")):
 def write_setprob(p, d, beta=200.0):
     """setprob.data in the format each app's setprob reads."""
     path = os.path.join(d, "setprob.data")
-    if p.physics == SC_PHYS_ADVECTION:       # apps/advection/1d/setprob.f: u, beta, ic
+    if p.physics == SC_PHYS_ADVECTION and p.ndim == 2:   # swirl/setprob.f: tperiod
+        write_values(path, [float(p.tperiod)])
+    elif p.physics == SC_PHYS_ADVECTION:     # apps/advection/1d/setprob.f: u, beta, ic
         write_values(path, [float(p.u), float(beta), 1])
     elif p.physics == SC_PHYS_ACOUSTICS:     # acoustics/1d/homogeneous/setprob.f: rho, K, beta
         write_values(path, [float(p.rho), float(p.bulk), float(beta)])

@@ -68,7 +68,34 @@ static phys_t mkphys(const sc_params *p)
 #define W3(a, i, m, w) (a)[(size_t)((i) + mbc - 1) + (size_t)L * ((size_t)(m) + (size_t)meqn * (size_t)(w))]
 #define S2(a, i, w) (a)[(size_t)((i) + mbc - 1) + (size_t)(w) * (size_t)L]
 
+/* The grid's aux array of runs with maux > 0: aux(1-mbc:mx+mbc, maux) in
+ * 1D (capacity, flux1.f90:57-58), aux(1-mbc:nx+mbc, 1-mbc:ny+mbc, maux) in
+ * 2D (advection velocities).  g_aux_slice is the aux(:,j,:) or aux(i,:,:)
+ * slice of the sweep in progress (flux2.f90:37-40,70-73).  The oracle is
+ * test infrastructure called from one thread: the array is set before the
+ * calls that use it. */
+static const double *g_aux = NULL;
+static const double *g_aux_slice = NULL;
+void orc_set_aux(const double *aux) { g_aux = aux; }
+
 /* ------------------------------------------------------------------ rp */
+
+/* apps/advection/2d/swirl/rpn2ad1.f:45-50 on one slice: the speed of
+ * interface i is auxl(i,ixy), the slice's aux (flux1.f90:122-123 pass aux
+ * as auxl; the swapped in-cell call of flux1.f90:170-179 sets auxl(i) =
+ * aux(i) as well) */
+static void rp_advection_aux(int mbc, int mx, int L, int meqn, int ixy, const double *aux, const double *ql,
+                             const double *qr, double *wave, double *s, double *amdq, double *apdq)
+{
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double u = A1(aux, i, ixy - 1);
+        double w = A1(ql, i, 0) - A1(qr, i - 1, 0);
+        W3(wave, i, 0, 0) = w;
+        S2(s, i, 0) = u;
+        A1(amdq, i, 0) = fmin(u, 0.0) * w;
+        A1(apdq, i, 0) = fmax(u, 0.0) * w;
+    }
+}
 
 /* apps/advection/1d/rp1.f:35-43 */
 static void rp_advection(const phys_t *f, int mbc, int mx, int L, int meqn, const double *ql,
@@ -460,7 +487,13 @@ int orc_rp(const sc_params *p, int ixy, int mx, const double *ql, const double *
     int mbc = p->mbc, meqn = p->meqn, mwaves = p->mwaves, L = mx + 2 * p->mbc;
     switch (p->physics) {
     case SC_PHYS_ADVECTION:
-        rp_advection(&f, mbc, mx, L, meqn, ql, qr, wave, s, amdq, apdq);
+        if (p->ndim == 2) {
+            const double *aux = g_aux_slice ? g_aux_slice : g_aux;
+            if (!aux) return SC_ERR_ARG;
+            rp_advection_aux(mbc, mx, L, meqn, ixy, aux, ql, qr, wave, s, amdq, apdq);
+        } else {
+            rp_advection(&f, mbc, mx, L, meqn, ql, qr, wave, s, amdq, apdq);
+        }
         return 0;
     case SC_PHYS_ACOUSTICS:
         rp_acoustics(&f, mbc, mx, L, meqn, ql, qr, wave, s, amdq, apdq);
@@ -927,12 +960,6 @@ void orc_tvd2_wave(int mx2, int meqn, int mwaves, int LD, const double *q, const
 #undef SP
 }
 
-/* The grid's aux array aux(1-mbc:mx+mbc, maux) of 1D runs with maux > 0
- * (capacity, flux1.f90:57-58).  The oracle is test infrastructure called
- * from one thread: the array is set once, before the calls that use it. */
-static const double *g_aux = NULL;
-void orc_set_aux(const double *aux) { g_aux = aux; }
-
 /* --------------------------------------------------------------- flux1 */
 
 /* src/flux1.f90:2-190 for mcapa=0 and the reconstructions lim_type 2 with
@@ -957,7 +984,7 @@ double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double 
     /* flux1.f90:56-61: dtdx(i) = dt/(dx*aux(i,mcapa)) with a capacity */
     double *dtdx = (double *)malloc((size_t)L * sizeof(double));
     for (int i = 1 - mbc; i <= mx + mbc; i++)
-        dtdx[i + mbc - 1] = (p->mcapa > 0 && g_aux) ? dt / (p->dx[ixy - 1] * A1(g_aux, i, p->mcapa - 1))
+        dtdx[i + mbc - 1] = (p->ndim == 1 && p->mcapa > 0 && g_aux) ? dt / (p->dx[ixy - 1] * A1(g_aux, i, p->mcapa - 1))
                                                    : dt / p->dx[ixy - 1];
 #define DTDX(i) dtdx[(i) + mbc - 1]
     if (p->ndim > 1) memset(dq1d, 0, n1 * sizeof(double));
@@ -1132,11 +1159,16 @@ double orc_flux2(const sc_params *p, const double *q, double *dq, double dt, int
     int Lmax = LX > LY ? LX : LY;
     double *q1d = (double *)malloc((size_t)Lmax * meqn * sizeof(double));
     double *dq1d = (double *)malloc((size_t)Lmax * meqn * sizeof(double));
+    const int maux = p->maux > 0 && g_aux ? p->maux : 0;
+    double *a1d = (double *)malloc((size_t)Lmax * (maux > 0 ? maux : 1) * sizeof(double));
 #define G(a, i, j, m) (a)[(size_t)((i) + mbc - 1) + (size_t)LX * ((size_t)((j) + mbc - 1) + (size_t)LY * (size_t)(m))]
     double cfl = 0.0;
     for (int j = 1 - g; j <= ny + g; j++) {
         for (int m = 0; m < meqn; m++)
             for (int i = 1 - mbc; i <= nx + mbc; i++) q1d[(size_t)(i + mbc - 1) + (size_t)m * LX] = G(q, i, j, m);
+        for (int m = 0; m < maux; m++)
+            for (int i = 1 - mbc; i <= nx + mbc; i++) a1d[(size_t)(i + mbc - 1) + (size_t)m * LX] = G(g_aux, i, j, m);
+        g_aux_slice = maux ? a1d : NULL;
         double c = orc_flux1(p, 1, nx, q1d, dq1d, dt);
         cfl = fmax(cfl, c);
         for (int m = 0; m < meqn; m++)
@@ -1145,6 +1177,9 @@ double orc_flux2(const sc_params *p, const double *q, double *dq, double dt, int
     for (int i = 1 - g; i <= nx + g; i++) {
         for (int m = 0; m < meqn; m++)
             for (int j = 1 - mbc; j <= ny + mbc; j++) q1d[(size_t)(j + mbc - 1) + (size_t)m * LY] = G(q, i, j, m);
+        for (int m = 0; m < maux; m++)
+            for (int j = 1 - mbc; j <= ny + mbc; j++) a1d[(size_t)(j + mbc - 1) + (size_t)m * LY] = G(g_aux, i, j, m);
+        g_aux_slice = maux ? a1d : NULL;
         double c = orc_flux1(p, 2, ny, q1d, dq1d, dt);
         cfl = fmax(cfl, c);
         for (int m = 0; m < meqn; m++)
@@ -1153,6 +1188,8 @@ double orc_flux2(const sc_params *p, const double *q, double *dq, double dt, int
 #undef G
     free(q1d);
     free(dq1d);
+    free(a1d);
+    g_aux_slice = NULL;
     return cfl;
 }
 
@@ -1272,6 +1309,16 @@ int orc_sharpclaw(const sc_params *p, const sc_run_ctl *c, double *q, double tst
     double dtmin = dt, dtmax = dt;
     int maxn = c->max_steps, info = 0, steps_taken = 0, retake = 0;
     memset(st, 0, sizeof(*st));
+    /* 2D advection with tperiod != 0: b4step scales the velocities of aux0
+     * (the array set with orc_set_aux) by vt each new step */
+    const double *aux0 = g_aux;
+    double *auxw = NULL;
+    size_t naux = 0;
+    if (p->physics == SC_PHYS_ADVECTION && p->ndim == 2 && p->tperiod != 0.0 && aux0) {
+        naux = (size_t)(p->nx[0] + 2 * p->mbc) * (size_t)(p->nx[1] + 2 * p->mbc) * (size_t)p->maux;
+        auxw = (double *)malloc(naux * sizeof(double));
+        g_aux = auxw;
+    }
     if (c->dt_variable == 0) {
         if (tend < tstart) {
             maxn = 1;
@@ -1291,6 +1338,13 @@ int orc_sharpclaw(const sc_params *p, const sc_run_ctl *c, double *q, double tst
         if (!retake) {
             told = t;
             if (told + dt > tend && tstart < tend) dt = tend - told;
+            if (auxw) {
+                /* apps/advection/2d/swirl/b4step.f90:26-41 (setprob.f: pi =
+                 * 4 atan(1), pi2 = 2 pi), called at sharpclaw.f90:129 */
+                const double pi = 4.0 * atan(1.0), pi2 = 2.0 * pi;
+                double vt = cos(pi2 * (told + dt / 2.0) / p->tperiod);
+                for (size_t k = 0; k < naux; k++) auxw[k] = vt * aux0[k];
+            }
             memcpy(qold, q, n * sizeof(double));
         }
         retake = 0;
@@ -1336,7 +1390,34 @@ done:
     free(qold);
     free(r1);
     free(r2);
+    if (auxw) {
+        g_aux = aux0;
+        free(auxw);
+    }
     return info;
+}
+
+/* apps/advection/2d/swirl/setaux.f:27-37 with psi.f:14 and setprob.f's pi:
+ * edge velocities from the stream function psi = sin(pi x)^2 sin(pi y)^2 / pi
+ * at the lower-left corners, aux(1-mbc:nx+mbc, 1-mbc:ny+mbc, 2). */
+static double swirl_psi(double pi, double x, double y)
+{
+    double sx = sin(pi * x), sy = sin(pi * y);
+    return ((sx * sx) * (sy * sy)) / pi;
+}
+
+void orc_swirl_setaux(const sc_params *p, double *aux)
+{
+    const int mbc = p->mbc, nx = p->nx[0], ny = p->nx[1], LX = nx + 2 * mbc, LY = ny + 2 * mbc;
+    const double pi = 4.0 * atan(1.0);
+    for (int i = 1 - mbc; i <= nx + mbc; i++)
+        for (int j = 1 - mbc; j <= ny + mbc; j++) {
+            double xll = p->xlower[0] + (i - 1) * p->dx[0];
+            double yll = p->xlower[1] + (j - 1) * p->dx[1];
+            size_t g = (size_t)(i + mbc - 1) + (size_t)LX * (size_t)(j + mbc - 1);
+            aux[g] = -(swirl_psi(pi, xll, yll + p->dx[1]) - swirl_psi(pi, xll, yll)) / p->dx[1];
+            aux[g + (size_t)LX * LY] = (swirl_psi(pi, xll + p->dx[0], yll) - swirl_psi(pi, xll, yll)) / p->dx[0];
+        }
 }
 
 /* conservation sums: (sum over interior cells of q(.,m)) * dx [* dy] */

@@ -56,6 +56,7 @@ class ScParams(C.Structure):
         ("precision", C.c_int),
         ("device", C.c_int),
         ("mthlim", C.c_int * 4),
+        ("tperiod", C.c_double),
     ]
 
 
@@ -150,6 +151,7 @@ SHAPES = {
     (SC_PHYS_ACOUSTICS, 1): (2, 2),
     (SC_PHYS_EULER, 1): (3, 3),
     (SC_PHYS_EULER, 2): (4, 3),
+    (SC_PHYS_ADVECTION, 2): (1, 1),
     (SC_PHYS_SHALLOW_WATER, 1): (2, 2),
     (SC_PHYS_SHALLOW_WATER, 2): (3, 3),
 }
@@ -158,7 +160,7 @@ SHAPES = {
 def make_params(physics, nx, xlower=(0.0, 0.0), xupper=(1.0, 1.0), *, char_decomp=2,
                 time_integrator=3, mthbc=(1, 1, 1, 1), gamma=1.4, grav=9.81, u=1.0, rho=1.0,
                 bulk=4.0, precision=SC_FP64, efix=-1, device=0, tfluct_solver=0, lim_type=2,
-                mthlim=(5, 5, 5, 5), maux=0, mcapa=0, src_term=0):
+                mthlim=(5, 5, 5, 5), maux=0, mcapa=0, src_term=0, tperiod=0.0):
     """Build an ScParams the way main.f90:61-122 fills ClawParams.
 
     dx = (xupper - xlower) / nx as main.f90:116 (float(nx) is exact here).
@@ -202,6 +204,7 @@ def make_params(physics, nx, xlower=(0.0, 0.0), xupper=(1.0, 1.0), *, char_decom
     p.device = device
     for k in range(4):
         p.mthlim[k] = mthlim[k] if k < len(mthlim) else mthlim[-1]
+    p.tperiod = tperiod
     return p
 
 

@@ -56,7 +56,7 @@ extern "C" int sc_version(void) { return 1; }
 int sc_phys_id(const sc_params *p)
 {
     switch (p->physics) {
-    case SC_PHYS_ADVECTION: return p->ndim == 1 ? ADV1 : -1;
+    case SC_PHYS_ADVECTION: return p->ndim == 1 ? ADV1 : ADV2;
     case SC_PHYS_ACOUSTICS: return p->ndim == 1 ? AC1 : -1;
     case SC_PHYS_EULER: return p->ndim == 1 ? EU1 : EU2;
     case SC_PHYS_SHALLOW_WATER: return p->ndim == 1 ? SW1 : SW2;
@@ -66,12 +66,12 @@ int sc_phys_id(const sc_params *p)
 
 static int shape_m(int id)
 {
-    static const int M[] = {1, 2, 3, 2, 4, 3};
+    static const int M[] = {1, 2, 3, 2, 4, 3, 1};
     return M[id];
 }
 static int shape_w(int id)
 {
-    static const int W[] = {1, 2, 3, 2, 3, 3};
+    static const int W[] = {1, 2, 3, 2, 3, 3, 1};
     return W[id];
 }
 
@@ -91,8 +91,15 @@ int sc_check_params(const sc_params *p, bool grid)
     // Riemann solvers; a capacity column (flux1.f90:56-61) is used on 1D grids
     if (p->maux < 0 || p->mcapa < 0 || p->mcapa > p->maux)
         return sc_fail(SC_ERR_ARG, "mcapa (%d) must index aux(:,1..maux) (maux = %d)", p->mcapa, p->maux);
-    if (p->maux > 0 && p->ndim != 1)
-        return sc_fail(SC_ERR_UNSUPPORTED, "aux arrays (maux > 0) are implemented for 1D grids");
+    if (id == ADV2) {
+        // apps/advection/2d/swirl: velocities in aux(:,:,1:2), char_decomp 0
+        if (p->maux < 2 || p->mcapa != 0)
+            return sc_fail(SC_ERR_ARG, "2D advection takes its edge velocities from aux(:,:,1:2): maux >= 2, mcapa = 0");
+        if (p->char_decomp != 0 || p->lim_type != 2 || p->tfluct_solver != 0)
+            return sc_fail(SC_ERR_UNSUPPORTED, "2D advection: weno5 (char_decomp 0, lim_type 2, tfluct_solver 0)");
+    } else if (p->maux > 0 && p->ndim != 1) {
+        return sc_fail(SC_ERR_UNSUPPORTED, "aux arrays (maux > 0) in 2D are implemented for advection");
+    }
     // reconstructions (flux1.f90:65-117)
     if (p->lim_type == 0)
         return sc_fail(SC_ERR_UNSUPPORTED, "lim_type 0: q2qlqr_poly leaves the interfaces within k of the boundary "
@@ -181,6 +188,10 @@ struct sc_solver {
     Batch2DRunner *dr; // 2D device-resident loop (created at first use)
     hipEvent_t ev0, ev1;
     double *capa;      // capacity aux(:,mcapa) in the q layout (1D, mcapa > 0)
+    double *aux2;      // 2D advection: edge velocities aux(:,:,1:2)
+    double auxg[2];    // their max |.| over the ghost rows (x) / columns (y) swept for the CFL
+    double vscale;     // b4step's factor on the velocities for the current step
+    bool vt_stale;     // a new step starts: vscale is due
     // frame output (created at first sc_output_begin): device snapshot,
     // pinned host copy, copy stream and its events
     double *snap;
@@ -223,6 +234,8 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
         s->p.dx[1] = 1.0;
     }
     s->sel.phys = sc_phys_id(params);
+    s->vscale = 1.0;  // the velocities as uploaded until the first step
+    s->vt_stale = true;
     s->sel.chr = params->char_decomp;
     s->sel.lim = params->lim_type;
     s->sel.fp32 = params->precision == SC_FP32;
@@ -288,6 +301,7 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->cst) hipStreamSynchronize(s->cst);
     if (s->capa) hipFree(s->capa);
+    if (s->aux2) hipFree(s->aux2);
     if (s->snap) hipFree(s->snap);
     if (s->hsnap) hipHostFree(s->hsnap);
     if (s->ev_snap) hipEventDestroy(s->ev_snap);
@@ -467,8 +481,24 @@ extern "C" int sc_set_aux(sc_solver *s, const double *aux)
 {
     if (!s || !aux) return sc_fail(SC_ERR_ARG, "NULL argument");
     if (s->p.maux < 1) return sc_fail(SC_ERR_STATE, "maux = 0: the solver has no aux arrays");
-    if (s->p.mcapa < 1) return SC_OK;  // no device physics reads aux otherwise
     DevGuard g(s->dev);
+    if (s->sel.phys == ADV2) {
+        // edge velocities aux(:,:,1) (x) and aux(:,:,2) (y), rpn2ad1.f:47
+        const int nx = s->p.nx[0], ny = s->p.nx[1];
+        const size_t LX = (size_t)nx + 6, LY = (size_t)ny + 6, plane = LX * LY;
+        if (!s->aux2 && hipMalloc(&s->aux2, 2 * plane * sizeof(double)) != hipSuccess)
+            return sc_fail(SC_ERR_HIP, "aux: %s", hipGetErrorString(hipGetLastError()));
+        HIPCHK(hipMemcpyAsync(s->aux2, aux, 2 * plane * sizeof(double), hipMemcpyHostToDevice, s->st));
+        auto A = [&](int i, int j, int d) { return aux[(size_t)(i + 2) + LX * (size_t)(j + 2) + (size_t)d * plane]; };
+        double gx = 0.0, gy = 0.0;
+        for (int i = 1; i <= nx + 1; i++) gx = std::fmax(gx, std::fmax(std::fabs(A(i, 0, 0)), std::fabs(A(i, ny + 1, 0))));
+        for (int j = 1; j <= ny + 1; j++) gy = std::fmax(gy, std::fmax(std::fabs(A(0, j, 1)), std::fabs(A(nx + 1, j, 1))));
+        s->auxg[0] = gx;
+        s->auxg[1] = gy;
+        HIPCHK(hipStreamSynchronize(s->st));
+        return SC_OK;
+    }
+    if (s->p.mcapa < 1) return SC_OK;  // no device physics reads aux otherwise
     const size_t LX = (size_t)s->p.nx[0] + 2 * s->p.mbc;
     if (!s->capa && hipMalloc(&s->capa, LX * sizeof(double)) != hipSuccess)
         return sc_fail(SC_ERR_HIP, "aux: %s", hipGetErrorString(hipGetLastError()));
@@ -481,6 +511,7 @@ extern "C" int sc_set_aux(sc_solver *s, const double *aux)
 extern "C" int sc_save_qold(sc_solver *s)
 {
     if (!s) return sc_fail(SC_ERR_ARG, "NULL solver");
+    s->vt_stale = true;
     s->old = s->cur;
     return SC_OK;
 }
@@ -503,6 +534,8 @@ static StageArgs base_args(const sc_solver *s)
     for (int k = 0; k < 4; k++) a.bc[k] = s->p.mthbc[k];
     a.capa = s->p.mcapa > 0 ? s->capa : nullptr;
     a.src = s->p.src_term;
+    a.aux = s->aux2;
+    a.vscale = s->vscale;
     a.xlo = s->p.xlower[0];
     a.dxc = s->p.dx[0];
     return a;
@@ -604,6 +637,11 @@ static int read_cfl(sc_solver *s, unsigned long long *w, double dt, double *cfl)
     double sx, sy;
     memcpy(&sx, &s->hslot[0], 8);
     memcpy(&sy, &s->hslot[1], 8);
+    if (s->sel.phys == ADV2) {
+        // the ghost rows / columns the reference also sweeps (flux2.f90:36,68)
+        sx = std::fmax(sx, std::fabs(s->vscale) * s->auxg[0]);
+        sy = std::fmax(sy, std::fabs(s->vscale) * s->auxg[1]);
+    }
     // cfl = max over interfaces of dtdx*|s| (flux1.f90:126-133), max of the
     // x and y sweeps in 2D (flux2.f90:47,80); dtdx*|s| is monotone in |s|
     double c = (dt / s->p.dx[0]) * sx;
@@ -617,7 +655,8 @@ extern "C" int sc_step(sc_solver *s, double dt, double t, double *dq, double *cf
     (void)t;
     if (!s || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
-    if (s->p.mcapa > 0 && !s->capa) return sc_fail(SC_ERR_STATE, "mcapa > 0: no aux uploaded (sc_set_aux)");
+    if ((s->p.mcapa > 0 && !s->capa) || (s->sel.phys == ADV2 && !s->aux2))
+        return sc_fail(SC_ERR_STATE, "no aux uploaded (sc_set_aux)");
     DevGuard g(s->dev);
     StageArgs a = base_args(s);
     double *q = s->buf[s->cur];
@@ -686,15 +725,30 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
     return SC_OK;
 }
 
+// apps/advection/2d/swirl/b4step.f90:26-41 (called by sharpclaw.f90:129 at
+// each new step, not on a retake): velocities = aux0 * cos(pi2 (t + dt/2) /
+// tperiod), pi2 = 2 pi with pi = 4 atan(1) (swirl/setprob.f)
+static void b4step_factor(sc_solver *s, double told, double dt)
+{
+    if (s->sel.phys != ADV2 || s->p.tperiod == 0.0) return;
+    const double pi = 4.0 * std::atan(1.0), pi2 = 2.0 * pi;
+    s->vscale = std::cos(pi2 * (told + dt / 2.0) / s->p.tperiod);
+}
+
 extern "C" int sc_ts(sc_solver *s, double dt, double told, double *cfl)
 {
-    (void)told;
     if (!s || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
-    if (s->p.mcapa > 0 && !s->capa) return sc_fail(SC_ERR_STATE, "mcapa > 0: no aux uploaded (sc_set_aux)");
+    if ((s->p.mcapa > 0 && !s->capa) || (s->sel.phys == ADV2 && !s->aux2))
+        return sc_fail(SC_ERR_STATE, "no aux uploaded (sc_set_aux)");
     DevGuard g(s->dev);
     int rc = tune_tile(s);
     if (rc) return rc;
+    if (s->vt_stale) {
+        // first attempt of a step (sc_save_qold): the reference's b4step
+        b4step_factor(s, told, dt);
+        s->vt_stale = false;
+    }
     unsigned long long *slot = nullptr;
     rc = enqueue_ts(s, dt, &slot);
     if (rc) return rc;
@@ -786,7 +840,8 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
 {
     if (!s || !c || !dt_io || !st) return sc_fail(SC_ERR_ARG, "NULL argument");
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
-    if (s->p.mcapa > 0 && !s->capa) return sc_fail(SC_ERR_STATE, "mcapa > 0: no aux uploaded (sc_set_aux)");
+    if ((s->p.mcapa > 0 && !s->capa) || (s->sel.phys == ADV2 && !s->aux2))
+        return sc_fail(SC_ERR_STATE, "no aux uploaded (sc_set_aux)");
     DevGuard g(s->dev);
     int trc = tune_tile(s);
     if (trc) return trc;
@@ -797,7 +852,8 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
         int tx, ty;
         tile2d_shape(s->tile2d, &tx, &ty);
         long long tiles = (long long)((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty);
-        bool device = s->loop_mode == 1 || (s->loop_mode < 0 && tiles < 2048);
+        // 2D advection steps its velocities per step on the host (b4step)
+        bool device = s->sel.phys != ADV2 && (s->loop_mode == 1 || (s->loop_mode < 0 && tiles < 2048));
         if (device) return advance2d_device(s, c, tstart, tend, dt_io, st);
     }
     memset(st, 0, sizeof *st);
@@ -825,6 +881,7 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
         if (!retake) {
             told = t;
             if (told + dt > tend && tstart < tend) dt = tend - told;
+            b4step_factor(s, told, dt);
             s->old = s->cur;  // qold = q
         }
         retake = false;
@@ -900,7 +957,8 @@ extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stag
 {
     if (!s || !ms_per_stage || n < 1) return sc_fail(SC_ERR_ARG, "bad argument");
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
-    if (s->p.mcapa > 0 && !s->capa) return sc_fail(SC_ERR_STATE, "mcapa > 0: no aux uploaded (sc_set_aux)");
+    if ((s->p.mcapa > 0 && !s->capa) || (s->sel.phys == ADV2 && !s->aux2))
+        return sc_fail(SC_ERR_STATE, "no aux uploaded (sc_set_aux)");
     DevGuard g(s->dev);
     int trc = tune_tile(s);
     if (trc) return trc;

@@ -41,6 +41,10 @@ struct StageArgs {
     const double *capa;
     int src;
     double dt, xlo, dxc;
+    // 2D advection: edge velocities aux(:,:,1:2) in the q layout (plane
+    // LX*LY each) and the step's factor on them (b4step)
+    const double *aux;
+    double vscale;
 };
 
 // physics selector of the launchers (sc_physics.h PhysId)

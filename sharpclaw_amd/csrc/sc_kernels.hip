@@ -372,6 +372,9 @@ hipError_t launch_stage2d(const PhysSel &s, int tile, const StageArgs &a, const 
     switch (s.phys) {
     case EU2: return go2c<EU2>(s, tile, a, cd, cf, st);
     case SW2: return go2c<SW2>(s, tile, a, cd, cf, st);
+    case ADV2:
+        // component-wise only (the scalar's characteristic form is the same)
+        return s.fp32 ? go2<ADV2, 0, float>(tile, a, cf, st) : go2<ADV2, 0, double>(tile, a, cd, st);
     }
     return hipErrorInvalidValue;
 }

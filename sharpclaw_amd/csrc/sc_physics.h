@@ -16,7 +16,10 @@
 
 namespace sc {
 
-enum PhysId { ADV1 = 0, AC1 = 1, EU1 = 2, SW1 = 3, EU2 = 4, SW2 = 5 };
+// ADV2: 2D advection with the edge velocities in aux (apps/advection/2d/
+// swirl/rpn2ad1.f); the stage kernel puts the interface's velocity in
+// PhysConst::u before each Riemann solve
+enum PhysId { ADV1 = 0, AC1 = 1, EU1 = 2, SW1 = 3, EU2 = 4, SW2 = 5, ADV2 = 6 };
 
 template <int P> struct Shape;
 template <> struct Shape<ADV1> { static constexpr int M = 1, W = 1, DIM = 1; };
@@ -25,6 +28,7 @@ template <> struct Shape<EU1> { static constexpr int M = 3, W = 3, DIM = 1; };
 template <> struct Shape<SW1> { static constexpr int M = 2, W = 2, DIM = 1; };
 template <> struct Shape<EU2> { static constexpr int M = 4, W = 3, DIM = 2; };
 template <> struct Shape<SW2> { static constexpr int M = 3, W = 3, DIM = 2; };
+template <> struct Shape<ADV2> { static constexpr int M = 1, W = 1, DIM = 2; };
 
 // physical constants in the compute precision
 template <class T> struct PhysConst {
@@ -405,7 +409,8 @@ template <int P, int IXY, class T>
 __device__ __forceinline__ void riemann(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
                                         T *s, T *amdq, T *apdq)
 {
-    if constexpr (P == ADV1) rp_adv(c, lf, rt, wave, s, amdq, apdq);
+    // rpn2ad1.f:45-50 is rp1.f's formula with the speed auxl(i,ixy)
+    if constexpr (P == ADV1 || P == ADV2) rp_adv(c, lf, rt, wave, s, amdq, apdq);
     else if constexpr (P == AC1) rp_ac(c, lf, rt, wave, s, amdq, apdq);
     else if constexpr (P == EU1) rp_eu1(c, lf, rt, wave, s, amdq, apdq);
     else if constexpr (P == SW1) rp_sw1(c, lf, rt, wave, s, amdq, apdq);
@@ -425,7 +430,7 @@ __device__ __forceinline__ void eigvec(const PhysConst<T> &c, const T *qa, const
 {
     constexpr int M = Shape<P>::M;
     const T h = T(0.5);
-    if constexpr (P == ADV1) {  // apps/advection/1d/evec.f:74-77
+    if constexpr (P == ADV1 || P == ADV2) {  // apps/advection/1d/evec.f:74-77
         evl[0] = T(1);
         evr[0] = T(1);
     } else if constexpr (P == AC1) {  // apps/acoustics/1d/homogeneous/evec.f:32-42
@@ -533,6 +538,8 @@ template <int IXY, class T> struct Eig<ADV1, IXY, T> {
     __device__ __forceinline__ void to_char(const T *d, T *w) const { w[0] = d[0]; }
     __device__ __forceinline__ void from_char(const T *c, T *o) const { o[0] = c[0]; }
 };
+
+template <int IXY, class T> struct Eig<ADV2, IXY, T> : Eig<ADV1, IXY, T> {};
 
 template <int IXY, class T> struct Eig<AC1, IXY, T> {
     T zz, z;

@@ -7,6 +7,22 @@ namespace sc {
 
 // ------------------------------------------------------ 2D fused stage
 
+// 2D advection (rpn2ad1.f:45-50): the speed of the x- (d = 0) or y- (d = 1)
+// problem of grid cell (i, j) -- its left / bottom edge velocity times the
+// step's factor, i.e. auxl(i,ixy) of the slice.  Indices past the ghost
+// cells (cells of a partial tile beyond the grid, whose results are not
+// used) are clamped.
+template <class T>
+__device__ __forceinline__ PhysConst<T> adv_const(const PhysConst<T> &c, const StageArgs &a, int i, int j, int d)
+{
+    PhysConst<T> cu = c;
+    i = min(i, a.nx + 3);
+    j = min(j, a.ny + 3);
+    cu.u = T(a.vscale * a.aux[(size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2) +
+                             (size_t)d * (size_t)a.LX * (size_t)a.LY]);
+    return cu;
+}
+
 // A block of 256 threads updates the TX x TY tile b (row-major over the
 // a.ntx x a.nty tiles).  q of the tile plus 3-cell halos along x and y is
 // staged in LDS (the corner blocks are not needed: every stencil is
@@ -22,7 +38,9 @@ namespace sc {
 // i = 0, nx+1 (flux2.f90:36,68); for boundary options 1-3 their states are
 // copies (wall: with the tangential momentum negated) of the adjacent
 // interior row/column, whose wave speeds they therefore repeat exactly, so
-// the interior sweeps give the same maximum.
+// the interior sweeps give the same maximum.  (For 2D advection the speeds
+// are the aux velocities, which the ghost rows do not repeat: the host adds
+// their maximum, which does not depend on q.)
 template <int P, int CHAR, class T, int TX, int TY>
 __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst<T> &c, int b)
 {
@@ -87,7 +105,10 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
 #pragma unroll
                 for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + 3][ci + kk];
             reconstruct_iface<P, 1, CHAR, T>(c, v, lf, rt);
-            riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
+            if constexpr (P == ADV2)
+                riemann<P, 1, T>(adv_const(c, a, i0 + ci, j0 + r, 0), lf, rt, wave, s, am, ap);
+            else
+                riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
             if (i0 + ci <= nx + 1 && j0 + r <= ny) {
 #pragma unroll
                 for (int w = 0; w < W; w++) smx = fmax(smx, fabs(s[w]));
@@ -108,7 +129,10 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
         T ql_[M], am2[M], ap2[M];
 #pragma unroll
         for (int m = 0; m < M; m++) ql_[m] = sR[m][kl];
-        riemann<P, 1, T>(c, ql_, lf, wave, s, am2, ap2);
+        if constexpr (P == ADV2)
+            riemann<P, 1, T>(adv_const(c, a, i0 + cx, j0 + cy, 0), ql_, lf, wave, s, am2, ap2);
+        else
+            riemann<P, 1, T>(c, ql_, lf, wave, s, am2, ap2);
 #pragma unroll
         for (int m = 0; m < M; m++) {
             T X = ((am[m] + sAp[m][kl]) + am2[m]) + ap2[m];
@@ -133,7 +157,10 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
 #pragma unroll
                 for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + kk][ci + 3];
             reconstruct_iface<P, 2, CHAR, T>(c, v, lf, rt);
-            riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
+            if constexpr (P == ADV2)
+                riemann<P, 2, T>(adv_const(c, a, i0 + ci, j0 + r, 1), lf, rt, wave, s, am, ap);
+            else
+                riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
             if (i0 + ci <= nx && j0 + r <= ny + 1) {
 #pragma unroll
                 for (int w = 0; w < W; w++) smy = fmax(smy, fabs(s[w]));
@@ -153,7 +180,10 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
         T ql_[M], am2[M], ap2[M];
 #pragma unroll
         for (int m = 0; m < M; m++) ql_[m] = sR[m][kb];
-        riemann<P, 2, T>(c, ql_, lf, wave, s, am2, ap2);
+        if constexpr (P == ADV2)
+            riemann<P, 2, T>(adv_const(c, a, i0 + cx, j0 + cy, 1), ql_, lf, wave, s, am2, ap2);
+        else
+            riemann<P, 2, T>(c, ql_, lf, wave, s, am2, ap2);
         if (i <= nx && j <= ny) {
             const bool rq = op_reads_qold(a.op);
             size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);

@@ -107,7 +107,7 @@ __device__ __forceinline__ void reconstruct_iface(const PhysConst<T> &c, const T
         lf[m] = central;
         rt[m] = central;
     }
-    if constexpr (CHAR == 0 || P == ADV1) {
+    if constexpr (CHAR == 0 || P == ADV1 || P == ADV2) {
         // component-wise (reconstruct.f90:94-140); with the advection
         // eigenvectors L = R = 1 the characteristic form is the same
 #pragma unroll

@@ -26,6 +26,7 @@ module sharpclaw_gpu
         real(c_double) :: gamma, grav, u, rho, bulk
         integer(c_int) :: precision, device
         integer(c_int) :: mthlim(4)
+        real(c_double) :: tperiod
     end type sc_params
 
     type, bind(C) :: sc_run_ctl

@@ -170,6 +170,7 @@ program xsclaw_gpu
     prm%bulk = bulk_c
     prm%precision = precision
     prm%device = device
+    prm%tperiod = 0.d0
     prm%mthlim = 0
     prm%mthlim(1:min(4, mwaves)) = mthlim(1:min(4, mwaves))
     call sc_check(sc_create(prm, solver), 'sc_create')

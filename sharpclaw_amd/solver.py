@@ -70,8 +70,9 @@ class Solver:
         """aux(1-mbc:mx+mbc, maux) of the grid (setaux's array); the device
         keeps the capacity column mcapa."""
         aux = _f(aux)
-        if self.p.maux > 0 and aux.shape != (self.shape[0], self.p.maux):
-            raise ValueError(f"aux has shape {aux.shape}, expected {(self.shape[0], self.p.maux)}")
+        want = tuple(self.shape[:-1]) + (self.p.maux,)
+        if self.p.maux > 0 and aux.shape != want:
+            raise ValueError(f"aux has shape {aux.shape}, expected {want}")
         check(lib().sc_set_aux(self._h, aux))
 
     def output_begin(self):

@@ -19,6 +19,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import _states  # noqa: E402
+from oracle import oracle as O  # noqa: E402  (input data only: the swirl aux arrays)
 from oracle import refrun as R  # noqa: E402
 from sharpclaw_amd import interior, make_params  # noqa: E402
 
@@ -314,8 +315,41 @@ def aux():
          dtlog=dtl, ctl=np.array([1e99, 0.5, 0.45, 50000, 1]), dt_initial=np.array(1e-3), aux=a)
 
 
+def swirl():
+    """2D advection with the swirl app's edge velocities in aux (rpn2ad1.f,
+    setaux.f/psi.f) and its time-dependent b4step: steps on random data and
+    two whole runs of the app itself (tperiod 0 and 1.5)."""
+    rng = np.random.default_rng(20261206)
+    out = {}
+    k = 0
+    for shape in ((24, 24), (19, 23)):
+        for bc in [(1, 1, 1, 1), (2, 2, 2, 2), (1, 2, 2, 1)]:
+            if (bc[0] == 2) != (bc[1] == 2) or (bc[2] == 2) != (bc[3] == 2):
+                bc = (2, 2, 1, 1)
+            p = make_params("advection", shape, char_decomp=0, mthbc=bc, maux=2)
+            q = _states.grid(p, rng, smooth=True)
+            a = O.swirl_setaux(p)
+            dt = 0.02 * p.dx[0]
+            qb, dq, cfl = R.step(p, q, dt, aux=a)
+            out.update({f"c{k}_q": q, f"c{k}_aux": a, f"c{k}_qbc": qb, f"c{k}_dq": dq,
+                        f"c{k}_cfl": np.array(cfl), f"c{k}_dt": np.array(dt),
+                        f"c{k}_bc": np.array(bc), f"c{k}_shape": np.array(shape)})
+            k += 1
+    out["n"] = np.array(k)
+    save("step_swirl2d.npz", **out)
+    for T, fname in ((0.0, "run_swirl2d.npz"), (1.5, "run_swirl2d_tperiod.npz")):
+        p = make_params("advection", (50, 50), char_decomp=0, mthbc=(1, 1, 1, 1), maux=2,
+                        time_integrator=4, tperiod=T)
+        cd = R.clawdata_for(p, nout=3, outstyle=1, tfinal=1.5, dt_initial=0.025, cfl_max=2.5,
+                            cfl_desired=2.45)
+        fr, dtl, info = R.run(p, cd, [])
+        save(fname, t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+             ctl=np.array([1e99, 2.5, 2.45, 50000, 1]), dt_initial=np.array(0.025),
+             tperiod=np.array(T))
+
+
 PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct, "flush": flush,
-         "recon": recon, "aux": aux}
+         "recon": recon, "aux": aux, "swirl": swirl}
 
 if __name__ == "__main__":
     if not R.ref_available():

@@ -246,3 +246,41 @@ def test_run_eu1d_capacity():
                    max_steps=int(c[3]), dt_variable=int(c[4]))
     q0, _, _ = O.sharpclaw(p0, ctl, q, 0.0, float(d["t"][1]), float(d["dt_initial"]))
     assert np.max(np.abs(interior(p0, q0) - d["q"][1])) > 1e-3
+
+
+def test_step_swirl_bit_exact():
+    """2D advection, edge velocities aux(:,:,1:2) of the swirl app
+    (rpn2ad1.f, sliced per sweep as flux2.f90:37-40,70-73)."""
+    d = load("step_swirl2d.npz")
+    for k in range(int(d["n"])):
+        shape = tuple(int(v) for v in d[f"c{k}_shape"])
+        p = make_params("advection", shape, char_decomp=0, mthbc=tuple(int(b) for b in d[f"c{k}_bc"]),
+                        maux=2)
+        # the swirl app's setaux restated (setaux.f + psi.f)
+        assert np.array_equal(O.swirl_setaux(p), d[f"c{k}_aux"])
+        qb, dq, cfl = O.step(p, d[f"c{k}_q"], float(d[f"c{k}_dt"]), aux=d[f"c{k}_aux"])
+        assert np.array_equal(qb, d[f"c{k}_qbc"]), k
+        assert np.array_equal(interior(p, dq), interior(p, d[f"c{k}_dq"])), k
+        assert cfl == float(d[f"c{k}_cfl"]), k
+
+
+@pytest.mark.parametrize("fname", ["run_swirl2d.npz", "run_swirl2d_tperiod.npz"])
+def test_run_swirl(fname):
+    """The swirl app's own run: setaux from psi, b4step scaling the velocities
+    by cos(2 pi (t + dt/2) / tperiod) every new step (b4step.f90:26-41)."""
+    d = load(fname)
+    p = make_params("advection", (50, 50), char_decomp=0, mthbc=(1, 1, 1, 1), maux=2,
+                    time_integrator=4, tperiod=float(d["tperiod"]))
+    c = d["ctl"]
+    ctl = ScRunCtl(dt_max=float(c[0]), cfl_max=float(c[1]), cfl_desired=float(c[2]),
+                   max_steps=int(c[3]), dt_variable=int(c[4]))
+    aux = O.swirl_setaux(p)
+    q = O.new_grid(p)
+    interior(p, q)[:] = d["q"][0]
+    t, dt = float(d["t"][0]), float(d["dt_initial"])
+    for k in range(1, len(d["t"])):
+        q, dt, st = O.sharpclaw(p, ctl, q, t, float(d["t"][k]), dt, aux=aux)
+        q = O.flush_small(p, q)
+        t = st.t
+        assert t == float(d["t"][k])
+        assert np.array_equal(interior(p, q), d["q"][k]), k
