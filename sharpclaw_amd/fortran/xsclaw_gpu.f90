@@ -12,7 +12,10 @@
 ! water), precision (0 float64, 1 float32), device ordinal, efix (-1 default),
 ! and optionally a source id (0 none; 1 the radial shallow-water source of
 ! 1dRadialSymm-qwave/src1.f90), which the reference selects by linking that
-! app's src (its sharpclaw.data src_term is read and ignored, main.f90:96).
+! app's src (its sharpclaw.data src_term is read and ignored, main.f90:96),
+! and the period of the swirl app's b4step velocity factor (2D advection;
+! the host b4step's changes to aux do not reach the device, which applies
+! that factor itself).
 ! The physics constants are taken from the common blocks the user's setprob
 ! fills (/param/ gamma,gamma1; /comrp/ u or g; /cparam/ rho,bulk,cc,zz).
 module xsclaw_data
@@ -50,6 +53,7 @@ program xsclaw_gpu
     type(sc_run_ctl) :: ctl
     type(c_ptr) :: solver
     integer :: md, i, n, iframe, info, physics, precision, device, efix, source, ios
+    double precision :: tperiod
     integer :: steps_taken
     double precision :: tstart, tend, dtout
     double precision :: cfl_maxused, cfl_last, dt_minused, dt_maxused, dt_last
@@ -128,8 +132,11 @@ program xsclaw_gpu
     read(56,*) device
     read(56,*) efix
     source = 0
+    tperiod = 0.d0
     read(56,*,iostat=ios) source
     if (ios /= 0) source = 0
+    if (ios == 0) read(56,*,iostat=ios) tperiod
+    if (ios /= 0) tperiod = 0.d0
     close(56)
 
     call setprob
@@ -170,7 +177,7 @@ program xsclaw_gpu
     prm%bulk = bulk_c
     prm%precision = precision
     prm%device = device
-    prm%tperiod = 0.d0
+    prm%tperiod = tperiod
     prm%mthlim = 0
     prm%mthlim(1:min(4, mwaves)) = mthlim(1:min(4, mwaves))
     call sc_check(sc_create(prm, solver), 'sc_create')
