@@ -118,8 +118,8 @@ __device__ __forceinline__ void rp_eu1(const PhysConst<T> &c, const T *lf, const
     const T g = c.gamma, g1 = c.gamma1, h = T(0.5);
     T rl = lf[0], ml = lf[1], el = lf[2];
     T rr = rt[0], mr = rt[1], er = rt[2];
-    T irl = frcp(rl), irr = frcp(rr);
     T isl = frsq(rl), isr = frsq(rr);  // 1/sqrt(rho)
+    T irl = isl * isl, irr = isr * isr;  // 1/rho
     T pl = g1 * (el - h * (ml * ml) * irl);
     T pr = g1 * (er - h * (mr * mr) * irr);
     T irs = frcp(rl * isl + rr * isr);  // 1/(sqrt(rhol) + sqrt(rhor))
@@ -201,8 +201,8 @@ __device__ __forceinline__ void rp_eu2(const PhysConst<T> &c, const T *lf, const
     const T g = c.gamma, g1 = c.gamma1, h = T(0.5);
     T rl = lf[0], ul = lf[mu], vl = lf[mv], el = lf[3];
     T rr = rt[0], ur = rt[mu], vr = rt[mv], er = rt[3];
-    T irl = frcp(rl), irr = frcp(rr);
-    T isl = frsq(rl), isr = frsq(rr);
+    T isl = frsq(rl), isr = frsq(rr);  // 1/sqrt(rho)
+    T irl = isl * isl, irr = isr * isr;
     T pl = g1 * (el - h * (ul * ul + vl * vl) * irl);
     T pr = g1 * (er - h * (ur * ur + vr * vr) * irr);
     T irs = frcp(rl * isl + rr * isr);
@@ -284,8 +284,8 @@ __device__ __forceinline__ void rp_sw1(const PhysConst<T> &c, const T *lf, const
 {
     const T gr = c.grav, h = T(0.5);
     T hl = lf[0], hul = lf[1], hr = rt[0], hur = rt[1];
-    T ihl = frcp(hl), ihr = frcp(hr);
     T isl = frsq(hl), isr = frsq(hr);
+    T ihl = isl * isl, ihr = isr * isr;
     T ubar = (hul * isl + hur * isr) * frcp(hl * isl + hr * isr);
     T cbar = fsqrt(h * gr * (hl + hr));
     T icb = frcp(cbar);
@@ -403,6 +403,72 @@ __device__ __forceinline__ void rp_sw2(const PhysConst<T> &c, const T *lf, const
         for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + sfract * wave[6 + m];
     } while (0);
     apdq_total<3, 3>(wave, s, amdq, apdq);
+}
+
+// ------------------------------------------- in-cell total fluctuation
+//
+// The update uses the in-cell problem (left ql(i), right qr(i)) only
+// through amdq2(i) + apdq2(i) (flux1.f90:184-186).  For the Roe solvers of
+// Euler and shallow water that sum is sum_w s_w wave_w = A_roe (qr - ql)
+// = f(qr(i)) - f(ql(i)) (Roe's property; the entropy fix moves flux between
+// amdq and apdq and keeps the sum: apdq = df - amdq, rp1eu.f / rpn2sw.f),
+// the total fluctuation flux1.f90:146-148 names as the conservative one.
+// It is formed here from the normal fluxes of the two states, in float64
+// also when T is float.  The result agrees with the swapped Riemann solve to
+// rounding; where a state is unphysical (negative density or pressure) the
+// Roe solve gives NaN and the flux difference a finite number.
+
+// normal flux f(q) of direction IXY in float64
+template <int P, int IXY, class T>
+__device__ __forceinline__ void normal_flux(const PhysConst<T> &c, const T *q, double *f)
+{
+    constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
+    if constexpr (P == EU2) {
+        const double r = q[0], mn = q[mu], mt = q[mv], E = q[3];
+        const double ir = frcp(r);
+        const double un = mn * ir;
+        const double p = double(c.gamma1) * (E - 0.5 * (mn * mn + mt * mt) * ir);
+        f[0] = mn;
+        f[mu] = mn * un + p;
+        f[mv] = mt * un;
+        f[3] = (E + p) * un;
+    } else if constexpr (P == SW2) {
+        const double h = q[0], hn = q[mu], ht = q[mv];
+        const double un = hn * frcp(h);
+        f[0] = hn;
+        f[mu] = hn * un + 0.5 * double(c.grav) * (h * h);
+        f[mv] = ht * un;
+    } else if constexpr (P == EU1) {
+        const double r = q[0], m = q[1], E = q[2];
+        const double ir = frcp(r);
+        const double u = m * ir;
+        const double p = double(c.gamma1) * (E - 0.5 * (m * m) * ir);
+        f[0] = m;
+        f[1] = m * u + p;
+        f[2] = (E + p) * u;
+    } else {
+        static_assert(P == SW1, "normal_flux: Euler and shallow water only");
+        const double h = q[0], hu = q[1];
+        f[0] = hu;
+        f[1] = hu * (hu * frcp(h)) + 0.5 * double(c.grav) * (h * h);
+    }
+}
+
+// physics whose in-cell total fluctuation is taken from normal_flux
+template <int P> constexpr bool incell_by_flux() { return P == EU2 || P == SW2; }
+
+// adq = amdq2 + apdq2 of the in-cell problem (left ql, right qr) as
+// f(qr) - f(ql)
+template <int P, int IXY, class T>
+__device__ __forceinline__ void incell_flux_diff(const PhysConst<T> &c, const T *ql, const T *qr,
+                                                 double *adq)
+{
+    constexpr int M = Shape<P>::M;
+    double fl[M], fr[M];
+    normal_flux<P, IXY, T>(c, ql, fl);
+    normal_flux<P, IXY, T>(c, qr, fr);
+#pragma unroll
+    for (int m = 0; m < M; m++) adq[m] = fr[m] - fl[m];
 }
 
 template <int P, int IXY, class T>
@@ -567,8 +633,8 @@ __device__ __forceinline__ void roe_euler(const PhysConst<T> &c, const T *qa, co
 {
     const T g1 = c.gamma1, h = T(0.5);
     T rl = qa[0], rr = qb[0];
-    T irl = frcp(rl), irr = frcp(rr);
-    T isl = frsq(rl), isr = frsq(rr);
+    T isl = frsq(rl), isr = frsq(rr);  // 1/sqrt(rho)
+    T irl = isl * isl, irr = isr * isr;
     T ul = qa[MU], ur = qb[MU];
     T vl = MV >= 0 ? qa[MV] : T(0), vr = MV >= 0 ? qb[MV] : T(0);
     T pl = g1 * (qa[ME] - h * (ul * ul + vl * vl) * irl);

@@ -225,8 +225,7 @@ __device__ __forceinline__ void reconstruct_other(const PhysConst<T> &c, const T
             }
             T u1 = T(0), u2 = T(0), sc = T(0);
             if (dd[2] > tol14<T>()) {
-                u1 = weno_corr<T>(T(1), dd[0], dd[1], dd[2], dd[3]);
-                u2 = weno_corr<T>(T(-1), dd[4], dd[3], dd[2], dd[1]);
+                weno_pair(dd[0], dd[1], dd[2], dd[3], dd[4], u1, u2);
                 sc = frcp(dd[2]);
             }
 #pragma unroll
@@ -254,8 +253,7 @@ __device__ __forceinline__ void reconstruct_other(const PhysConst<T> &c, const T
                 own.to_char(col, w);
                 hh[k] = sw[mw * NW + kc - 2 + k] * w[mw];
             }
-            U1[mw] = weno_corr<T>(T(1), hh[0], hh[1], hh[2], hh[3]);
-            U2[mw] = weno_corr<T>(T(-1), hh[4], hh[3], hh[2], hh[1]);
+            weno_pair(hh[0], hh[1], hh[2], hh[3], hh[4], U1[mw], U2[mw]);
         }
         T o1[M], o2[M];
         own.from_char(U1, o1);

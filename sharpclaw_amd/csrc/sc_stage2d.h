@@ -126,17 +126,28 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
         // in-cell problem of cell (cx, cy): ql = right state of the interface
         // on its left, qr = left state of its own interface
         const int kl = cy * (TX + 1) + cx;
-        T ql_[M], am2[M], ap2[M];
+        T ql_[M];
 #pragma unroll
         for (int m = 0; m < M; m++) ql_[m] = sR[m][kl];
-        if constexpr (P == ADV2)
-            riemann<P, 1, T>(adv_const(c, a, i0 + cx, j0 + cy, 0), ql_, lf, wave, s, am2, ap2);
-        else
-            riemann<P, 1, T>(c, ql_, lf, wave, s, am2, ap2);
+        if constexpr (incell_by_flux<P>()) {
+            double adq[M];
+            incell_flux_diff<P, 1, T>(c, ql_, lf, adq);
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            T X = ((am[m] + sAp[m][kl]) + am2[m]) + ap2[m];
-            dqx[m] = -(a.dtdx * (double)X);
+            for (int m = 0; m < M; m++) {
+                double X = (double)(am[m] + sAp[m][kl]) + adq[m];
+                dqx[m] = -(a.dtdx * X);
+            }
+        } else {
+            T am2[M], ap2[M];
+            if constexpr (P == ADV2)
+                riemann<P, 1, T>(adv_const(c, a, i0 + cx, j0 + cy, 0), ql_, lf, wave, s, am2, ap2);
+            else
+                riemann<P, 1, T>(c, ql_, lf, wave, s, am2, ap2);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                T X = ((am[m] + sAp[m][kl]) + am2[m]) + ap2[m];
+                dqx[m] = -(a.dtdx * (double)X);
+            }
         }
     }
     __syncthreads();
@@ -177,21 +188,32 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
     if (owns_cell) {
         const int i = i0 + cx, j = j0 + cy;
         const int kb = cy * TX + cx;
-        T ql_[M], am2[M], ap2[M];
+        T ql_[M];
 #pragma unroll
         for (int m = 0; m < M; m++) ql_[m] = sR[m][kb];
-        if constexpr (P == ADV2)
-            riemann<P, 2, T>(adv_const(c, a, i0 + cx, j0 + cy, 1), ql_, lf, wave, s, am2, ap2);
-        else
-            riemann<P, 2, T>(c, ql_, lf, wave, s, am2, ap2);
+        // Y[m] = ((amdq(j+1) + apdq(j)) + amdq2(j)) + apdq2(j)
+        double Y[M];
+        if constexpr (incell_by_flux<P>()) {
+            double adq[M];
+            incell_flux_diff<P, 2, T>(c, ql_, lf, adq);
+#pragma unroll
+            for (int m = 0; m < M; m++) Y[m] = (double)(am[m] + sAp[m][kb]) + adq[m];
+        } else {
+            T am2[M], ap2[M];
+            if constexpr (P == ADV2)
+                riemann<P, 2, T>(adv_const(c, a, i0 + cx, j0 + cy, 1), ql_, lf, wave, s, am2, ap2);
+            else
+                riemann<P, 2, T>(c, ql_, lf, wave, s, am2, ap2);
+#pragma unroll
+            for (int m = 0; m < M; m++) Y[m] = (double)(((am[m] + sAp[m][kb]) + am2[m]) + ap2[m]);
+        }
         if (i <= nx && j <= ny) {
             const bool rq = op_reads_qold(a.op);
             size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
             size_t plane = (size_t)a.LX * a.LY;
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                T Y = ((am[m] + sAp[m][kb]) + am2[m]) + ap2[m];
-                double r = dqx[m] + -(a.dtdy * (double)Y);
+                double r = dqx[m] + -(a.dtdy * Y[m]);
                 size_t gm = g + m * plane;
                 double q = a.qin[gm];
                 double qo = rq ? a.qold[gm] : 0.0;

@@ -78,6 +78,48 @@ __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, flo
     return ((s1 * (t2 - t1) + (0.5f * s3 - 0.25f) * (t3 - t2)) * (1.0f / 3.0f)) * M;
 }
 
+// The weights and correction of weno_side from the squared differences
+// p1..p3 (= t1^2..t3^2), the second terms b1..b3 of the indicators and the
+// differences f21 = t2 - t1, f32 = t3 - t2.
+__device__ __forceinline__ double weno_w(double p1, double p2, double p3, double b1, double b2,
+                                         double b3, double f21, double f32)
+{
+    double tt1 = dsq(SC_EPWENO + (13.0 * p1 + 3.0 * dsq(b1)));
+    double tt2 = dsq(SC_EPWENO + (13.0 * p2 + 3.0 * dsq(b2)));
+    double tt3 = dsq(SC_EPWENO + (13.0 * p3 + 3.0 * dsq(b3)));
+    double s1 = tt2 * tt3;
+    double s2 = 6.0 * tt1 * tt3;
+    double s3 = 3.0 * tt1 * tt2;
+    double t0 = frcp(s1 + s2 + s3);
+    s1 = s1 * t0;
+    s3 = s3 * t0;
+    return (s1 * f21 + (0.5 * s3 - 0.25) * f32) * (1.0 / 3.0);
+}
+
+// Both sides of one field in float64 from its five jumps h0..h4 at the
+// interface: the left side (m1 = 1) is weno_side(1, h0, h1, h2, h3), the
+// right side (m1 = 2) weno_side(-1, h4, h3, h2, h1).  With e_k = h_k -
+// h_(k+1) the left side's t1, t2, t3 are e0, e1, e2 and the right side's
+// e3, e2, e1 (its im = -1 flips each sign), so the two share e1, e2, their
+// squares and t3 - t2.
+__device__ __forceinline__ void weno_pair(double h0, double h1, double h2, double h3, double h4,
+                                          double &cl, double &cr)
+{
+    const double e0 = h0 - h1, e1 = h1 - h2, e2 = h2 - h3, e3 = h3 - h4;
+    const double p0 = e0 * e0, p1 = e1 * e1, p2 = e2 * e2, p3 = e3 * e3;
+    const double f32 = e2 - e1;
+    cl = weno_w(p0, p1, p2, h0 - 3.0 * h1, h1 + h2, 3.0 * h2 - h3, e1 - e0, f32);
+    cr = weno_w(p3, p2, p1, h4 - 3.0 * h3, h3 + h2, 3.0 * h2 - h1, e2 - e3, -f32);
+}
+
+// both sides in float: weno_side_f32 scales each side by its own jumps
+__device__ __forceinline__ void weno_pair(float h0, float h1, float h2, float h3, float h4, float &cl,
+                                          float &cr)
+{
+    cl = weno_side_f32(1.0f, h0, h1, h2, h3);
+    cr = weno_side_f32(-1.0f, h4, h3, h2, h1);
+}
+
 template <class T> __device__ __forceinline__ T weno_corr(T im, T a2, T a1, T a0, T am1);
 template <> __device__ __forceinline__ double weno_corr<double>(double im, double a2, double a1,
                                                                 double a0, double am1)
@@ -112,8 +154,10 @@ __device__ __forceinline__ void reconstruct_iface(const PhysConst<T> &c, const T
         // eigenvectors L = R = 1 the characteristic form is the same
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            lf[m] = weno_corr<T>(T(1), d[0][m], d[1][m], d[2][m], d[3][m]) + lf[m];
-            rt[m] = weno_corr<T>(T(-1), d[4][m], d[3][m], d[2][m], d[1][m]) + rt[m];
+            T cl, cr;
+            weno_pair(d[0][m], d[1][m], d[2][m], d[3][m], d[4][m], cl, cr);
+            lf[m] = cl + lf[m];
+            rt[m] = cr + rt[m];
         }
     } else {
         // characteristic-wise (reconstruct.f90:178-237): project the five
@@ -131,10 +175,8 @@ __device__ __forceinline__ void reconstruct_iface(const PhysConst<T> &c, const T
         }
         T cl[M], cr[M], ol[M], orr[M];
 #pragma unroll
-        for (int p = 0; p < M; p++) {
-            cl[p] = weno_corr<T>(T(1), hh[p][0], hh[p][1], hh[p][2], hh[p][3]);
-            cr[p] = weno_corr<T>(T(-1), hh[p][4], hh[p][3], hh[p][2], hh[p][1]);
-        }
+        for (int p = 0; p < M; p++)
+            weno_pair(hh[p][0], hh[p][1], hh[p][2], hh[p][3], hh[p][4], cl[p], cr[p]);
         e.from_char(cl, ol);
         e.from_char(cr, orr);
 #pragma unroll
