@@ -24,7 +24,7 @@ struct Grid2D {
 struct Batch2DArgs {
     const Grid2D *grids;        // ascending tile0
     SolveState *state;
-    unsigned long long *smax;   // 2 CFL words per grid (x, y sweeps)
+    unsigned long long *smax;   // one CFL set (CFL_WORDS words, x and y sweeps) per grid
     int ngrids;
     double *base;
     int bc[4];

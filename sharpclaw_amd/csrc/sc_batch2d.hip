@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void stage2d_batch_kernel(Batch2DArgs A, PhysC
     a.qout = buf + out * n;
     a.qbin = buf + 3 * n;
     a.qbout = buf + 3 * n;
-    a.smax = A.stage == plan.n - 1 ? A.smax + 2 * lo : nullptr;
+    a.smax = A.stage == plan.n - 1 ? A.smax + (size_t)CFL_WORDS * lo : nullptr;
     a.nx = G.nx;
     a.ny = G.ny;
     a.LX = G.LX;
@@ -97,8 +97,8 @@ __global__ void ctl2d_kernel(Batch2DArgs A, int do_post, int do_pre)
     const int nst = stages_of_integrator(A.ti);
     if (do_post && s.go) {
         s.go = 0;
-        double sx = __longlong_as_double((long long)A.smax[2 * g]);
-        double sy = __longlong_as_double((long long)A.smax[2 * g + 1]);
+        double sx = __longlong_as_double((long long)cfl_word(A.smax + (size_t)CFL_WORDS * g, 0));
+        double sy = __longlong_as_double((long long)cfl_word(A.smax + (size_t)CFL_WORDS * g, 1));
         s.stages += nst;
         s.cfl = fmax((s.dt / G.dx) * sx, (s.dt / G.dy) * sy);
         const sc_run_ctl &ctl = G.ctl;
@@ -153,8 +153,10 @@ __global__ void ctl2d_kernel(Batch2DArgs A, int do_post, int do_pre)
             s.retake = 0;
             s.t = s.told + s.dt;
             s.go = 1;
-            A.smax[2 * g] = 0ull;
-            A.smax[2 * g + 1] = 0ull;
+            for (int k = 0; k < CFL_SUB; k++) {
+                A.smax[(size_t)CFL_WORDS * g + k * CFL_STRIDE] = 0ull;
+                A.smax[(size_t)CFL_WORDS * g + k * CFL_STRIDE + 1] = 0ull;
+            }
         }
     }
     A.state[g] = s;
@@ -265,7 +267,7 @@ int Batch2DRunner::reserve(int n)
     cap = 0;
     if (hipMalloc(&d_grids, c * sizeof(Grid2D)) != hipSuccess ||
         hipMalloc(&d_state, c * sizeof(SolveState)) != hipSuccess ||
-        hipMalloc(&d_smax, 2 * c * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&d_smax, (size_t)CFL_WORDS * c * sizeof(unsigned long long)) != hipSuccess ||
         hipHostMalloc(&h_grids, c * sizeof(Grid2D), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&h_state, c * sizeof(SolveState), hipHostMallocDefault) != hipSuccess)
         return sc_fail(SC_ERR_NOMEM, "2D batch: descriptor allocation failed");

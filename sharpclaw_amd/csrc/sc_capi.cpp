@@ -160,7 +160,7 @@ int sc_check_params(const sc_params *p, bool grid)
 
 // ------------------------------------------------------------- handle
 
-static constexpr int NSLOT = 1024;  // CFL words (2 each) before the ring is cleared
+static constexpr int NSLOT = 64;  // CFL sets (CFL_WORDS words each) before the ring is cleared
 
 struct sc_solver {
     sc_params p;
@@ -253,15 +253,15 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     }
     ok = ok && hipMalloc(&s->partial, (size_t)s->nblk_sums * s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->sums, s->p.meqn * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&s->slots, NSLOT * 2 * sizeof(unsigned long long)) == hipSuccess;
-    ok = ok && hipHostMalloc(&s->hslot, 2 * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipMalloc(&s->slots, NSLOT * CFL_WORDS * sizeof(unsigned long long)) == hipSuccess;
+    ok = ok && hipHostMalloc(&s->hslot, CFL_WORDS * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
     if (ok) {
         for (int k = 0; k < 3 && ok; k++)
             ok = hipMemsetAsync(s->buf[k], 0, s->n * sizeof(double), s->st) == hipSuccess;
         ok = ok && hipMemsetAsync(s->qb, 0, s->n * sizeof(double), s->st) == hipSuccess;
         ok = ok && hipMemsetAsync(s->dq, 0, s->n * sizeof(double), s->st) == hipSuccess;
-        ok = ok && hipMemsetAsync(s->slots, 0, NSLOT * 2 * sizeof(unsigned long long), s->st) == hipSuccess;
+        ok = ok && hipMemsetAsync(s->slots, 0, NSLOT * CFL_WORDS * sizeof(unsigned long long), s->st) == hipSuccess;
         ok = ok && hipStreamSynchronize(s->st) == hipSuccess;
     }
     if (!ok) {
@@ -620,23 +620,24 @@ extern "C" int sc_get_tile(sc_solver *s, int *tile, int *tx, int *ty)
     return SC_OK;
 }
 
-// next CFL word pair, cleared ring-wise
+// next CFL set, cleared ring-wise
 static unsigned long long *next_slot(sc_solver *s)
 {
     if (s->slot_ctr % NSLOT == 0 && s->slot_ctr > 0)
-        hipMemsetAsync(s->slots, 0, NSLOT * 2 * sizeof(unsigned long long), s->st);
-    unsigned long long *w = s->slots + 2 * (s->slot_ctr % NSLOT);
+        hipMemsetAsync(s->slots, 0, NSLOT * CFL_WORDS * sizeof(unsigned long long), s->st);
+    unsigned long long *w = s->slots + CFL_WORDS * (s->slot_ctr % NSLOT);
     s->slot_ctr++;
     return w;
 }
 
 static int read_cfl(sc_solver *s, unsigned long long *w, double dt, double *cfl)
 {
-    HIPCHK(hipMemcpyAsync(s->hslot, w, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpyAsync(s->hslot, w, CFL_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     double sx, sy;
-    memcpy(&sx, &s->hslot[0], 8);
-    memcpy(&sy, &s->hslot[1], 8);
+    const unsigned long long wx = cfl_word(s->hslot, 0), wy = cfl_word(s->hslot, 1);
+    memcpy(&sx, &wx, 8);
+    memcpy(&sy, &wy, 8);
     if (s->sel.phys == ADV2) {
         // the ghost rows / columns the reference also sweeps (flux2.f90:36,68)
         sx = std::fmax(sx, std::fabs(s->vscale) * s->auxg[0]);
@@ -995,7 +996,7 @@ extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stag
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     *ms_per_stage = ms / ((double)n * P.n);
-    HIPCHK(hipMemsetAsync(s->slots, 0, NSLOT * 2 * sizeof(unsigned long long), s->st));
+    HIPCHK(hipMemsetAsync(s->slots, 0, NSLOT * CFL_WORDS * sizeof(unsigned long long), s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     s->slot_ctr = 0;
     return SC_OK;
@@ -1165,10 +1166,10 @@ extern "C" int sc_flux1(const sc_params *params, int ixy, int mx, const double *
     HIPCHK(dq.alloc((size_t)L * M));
     HIPCHK(ddq.alloc((size_t)L * M));
     unsigned long long *slot = nullptr;
-    HIPCHK(hipMalloc(&slot, 2 * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&slot, CFL_WORDS * sizeof(unsigned long long)));
     UP(dq, q1d, (size_t)L * M);
     HIPCHK(hipMemset(ddq.p, 0, (size_t)L * M * sizeof(double)));
-    HIPCHK(hipMemset(slot, 0, 2 * sizeof(unsigned long long)));
+    HIPCHK(hipMemset(slot, 0, CFL_WORDS * sizeof(unsigned long long)));
     StageArgs a;
     memset(&a, 0, sizeof a);
     a.qin = dq.p;
@@ -1181,14 +1182,15 @@ extern "C" int sc_flux1(const sc_params *params, int ixy, int mx, const double *
     a.op = OP_DQ;
     a.dtdx = dt / params->dx[ixy - 1];
     hipError_t e = launch_stage1d(sel, ixy, a, sc_make_const<double>(params), sc_make_const<float>(params), 0);
-    unsigned long long w[2] = {0, 0};
+    std::vector<unsigned long long> w(CFL_WORDS, 0ull);
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(w, slot, sizeof w, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(w.data(), slot, CFL_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost);
     hipFree(slot);
     if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "flux1: %s", hipGetErrorString(e));
     DOWN(dq1d, ddq, (size_t)L * M);
     double sx;
-    memcpy(&sx, &w[0], 8);
+    const unsigned long long wx = cfl_word(w.data(), 0);
+    memcpy(&sx, &wx, 8);
     *cfl = a.dtdx * sx;
     return SC_OK;
 }

@@ -65,8 +65,29 @@ template <class T> __device__ __forceinline__ T wave_max(T v)
     return v;
 }
 
-// max|s| of the block merged into a device word with one atomic per block
-// (doubles >= 0 order like their bit patterns).  red: NW doubles of LDS.
+// CFL words.  Each workgroup merges its max |s| into a device word with one
+// atomicMax (doubles >= 0 order like their bit patterns).  Thousands of
+// workgroups per launch on one address queue up behind each other (tens of
+// microseconds per launch at 1024^2), so each word has CFL_SUB copies
+// CFL_STRIDE words (256 bytes) apart and workgroup b merges into copy
+// b mod CFL_SUB.  A CFL set holds the x word (offset 0) and the y word
+// (offset 1) with their copies, CFL_WORDS words; a reader takes the max over
+// the copies (cfl_word).
+constexpr int CFL_SUB = 32, CFL_STRIDE = 32, CFL_WORDS = CFL_SUB * CFL_STRIDE;
+
+// word w (0: x, 1: y) of a CFL set, as the bit pattern of a double >= 0
+__host__ __device__ inline unsigned long long cfl_word(const unsigned long long *set, int w)
+{
+    unsigned long long m = 0;
+    for (int k = 0; k < CFL_SUB; k++) {
+        unsigned long long v = set[k * CFL_STRIDE + w];
+        m = v > m ? v : m;
+    }
+    return m;
+}
+
+// max|s| of the block merged into word `slot` (x or y word of a CFL set)
+// with one atomic per block.  red: NW doubles of LDS.
 template <class T, int NW>
 __device__ __forceinline__ void merge_smax(unsigned long long *slot, T v, double *red)
 {
@@ -78,7 +99,8 @@ __device__ __forceinline__ void merge_smax(unsigned long long *slot, T v, double
         double m = red[0];
 #pragma unroll
         for (int k = 1; k < NW; k++) m = fmax(m, red[k]);
-        if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+        if (m > 0.0)
+            atomicMax(slot + (blockIdx.x % CFL_SUB) * CFL_STRIDE, (unsigned long long)__double_as_longlong(m));
     }
 }
 
