@@ -80,20 +80,26 @@ __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, flo
 
 // The weights and correction of weno_side from the squared differences
 // p1..p3 (= t1^2..t3^2), the second terms b1..b3 of the indicators and the
-// differences f21 = t2 - t1, f32 = t3 - t2.
+// differences f21 = t2 - t1, f32 = t3 - t2 (f15 = 1.5 f32, f12 = f32 / 12).
+// The reference's tt_k = (eps + 13 p_k + 3 b_k^2)^2 enter only through
+// ratios of products of two of them, so the common factor 9 is divided out:
+// g_k = tt_k / 9 = (eps/3 + (13/3) p_k + b_k^2)^2.  With u = g1 g2 and
+// D = g2 g3 + 6 g1 g3 + 3 u (the reference's s1 + s2 + s3 over 81) the
+// correction (s1 (t2-t1) + (s3/2 - 1/4)(t3-t2)) / 3 over (s1+s2+s3) is
+// (g2 g3 f21 + 1.5 u f32) / (3 D) - f32 / 12.
 __device__ __forceinline__ double weno_w(double p1, double p2, double p3, double b1, double b2,
-                                         double b3, double f21, double f32)
+                                         double b3, double f21, double f15, double f12)
 {
-    double tt1 = dsq(SC_EPWENO + (13.0 * p1 + 3.0 * dsq(b1)));
-    double tt2 = dsq(SC_EPWENO + (13.0 * p2 + 3.0 * dsq(b2)));
-    double tt3 = dsq(SC_EPWENO + (13.0 * p3 + 3.0 * dsq(b3)));
-    double s1 = tt2 * tt3;
-    double s2 = 6.0 * tt1 * tt3;
-    double s3 = 3.0 * tt1 * tt2;
-    double t0 = frcp(s1 + s2 + s3);
-    s1 = s1 * t0;
-    s3 = s3 * t0;
-    return (s1 * f21 + (0.5 * s3 - 0.25) * f32) * (1.0 / 3.0);
+    constexpr double c13 = 13.0 / 3.0, e3 = SC_EPWENO / 3.0;
+    const double g1 = dsq(e3 + fma(c13, p1, b1 * b1));
+    const double g2 = dsq(e3 + fma(c13, p2, b2 * b2));
+    const double g3 = dsq(e3 + fma(c13, p3, b3 * b3));
+    const double s1 = g2 * g3;
+    const double u = g1 * g2;
+    const double d = fma(3.0, u, fma(6.0 * g1, g3, s1));
+    const double t0 = frcp(d);
+    const double num = fma(u, f15, s1 * f21);
+    return fma(num * t0, 1.0 / 3.0, -f12);
 }
 
 // Both sides of one field in float64 from its five jumps h0..h4 at the
@@ -108,8 +114,9 @@ __device__ __forceinline__ void weno_pair(double h0, double h1, double h2, doubl
     const double e0 = h0 - h1, e1 = h1 - h2, e2 = h2 - h3, e3 = h3 - h4;
     const double p0 = e0 * e0, p1 = e1 * e1, p2 = e2 * e2, p3 = e3 * e3;
     const double f32 = e2 - e1;
-    cl = weno_w(p0, p1, p2, h0 - 3.0 * h1, h1 + h2, 3.0 * h2 - h3, e1 - e0, f32);
-    cr = weno_w(p3, p2, p1, h4 - 3.0 * h3, h3 + h2, 3.0 * h2 - h1, e2 - e3, -f32);
+    const double f15 = 1.5 * f32, f12 = f32 * (1.0 / 12.0);
+    cl = weno_w(p0, p1, p2, h0 - 3.0 * h1, h1 + h2, 3.0 * h2 - h3, e1 - e0, f15, f12);
+    cr = weno_w(p3, p2, p1, h4 - 3.0 * h3, h3 + h2, 3.0 * h2 - h1, e2 - e3, -f15, -f12);
 }
 
 // both sides in float: weno_side_f32 scales each side by its own jumps
