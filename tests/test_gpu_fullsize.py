@@ -1,0 +1,114 @@
+"""The fused 2D stage at the benchmark's full size (2D Euler 1024 x 1024,
+characteristic WENO5 + Roe, SSPRK3), checked through properties that do not
+depend on the grid size; the oracle itself runs the small cases
+(test_gpu_parity.py), at 1024^2 it would take minutes per step.
+
+  transpose   the problem with x and y exchanged (q(i,j) -> q(j,i), momenta
+              swapped) gives the transposed result: the y sweep of one is the
+              x sweep of the other, interface by interface, with the same
+              operations (bitwise in practice; asserted to 1e-13 so that a
+              different contraction choice of the compiler in the two
+              sweeps would not fail it, while any index or direction mistake
+              does)
+  periodic    with periodic boundaries sum q dx dy of a smooth state is
+              conserved to rounding (the fluctuations telescope)
+  float32     the float32 option stays within relative L1 1e-5 of the
+              float64 path
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_linf
+
+from sharpclaw_amd import SC_FP32, ScRunCtl, Solver, interior, make_params
+
+pytestmark = pytest.mark.gpu
+
+N = 1024
+
+
+def quadrants(p, seed=2024):
+    """Four constant states (rho, u, v, p) in U[0.1,1.5] x U[-1,1]^2 x
+    U[0.03,1.5], split at x = y = 0.5."""
+    rng = np.random.default_rng(seed)
+    st = np.stack([rng.uniform(0.1, 1.5, 4), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4),
+                   rng.uniform(0.03, 1.5, 4)], axis=1)
+    x = (np.arange(p.nx[0]) + 0.5) / p.nx[0]
+    y = (np.arange(p.nx[1]) + 0.5) / p.nx[1]
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    k = (X > 0.5).astype(int) + 2 * (Y > 0.5).astype(int)
+    rho, u, v, pr = (st[k, c] for c in range(4))
+    q = np.zeros((p.nx[0] + 6, p.nx[1] + 6, 4), order="F")
+    qi = interior(p, q)
+    qi[..., 0], qi[..., 1], qi[..., 2] = rho, rho * u, rho * v
+    qi[..., 3] = pr / 0.4 + 0.5 * rho * (u * u + v * v)
+    return q
+
+
+def transposed(q):
+    t = np.asfortranarray(np.transpose(q, (1, 0, 2)))
+    t[..., [1, 2]] = t[..., [2, 1]]
+    return np.asfortranarray(t)
+
+
+def run(p, q0, steps, tile=-1):
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=steps, dt_variable=1)
+    with Solver(p) as s:
+        s.set_tile(tile)
+        s.set_q(q0)
+        dt, st = s.advance(ctl, 0.0, 1e9, 1e-4)
+        return s.get_q(), dt, st, s.sums()
+
+
+def test_transpose_symmetry():
+    p = make_params("euler", (N, N), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    q0 = quadrants(p)
+    qa, dta, sta, _ = run(p, q0, 10)
+    qb, dtb, stb, _ = run(p, transposed(q0), 10)
+    assert sta.steps_taken == stb.steps_taken == 10 and abs(dta - dtb) <= 1e-13 * dta
+    assert np.all(np.isfinite(interior(p, qa)))
+    a, b = interior(p, transposed(qa)), interior(p, qb)
+    print("transpose: bitwise equal" if np.array_equal(a, b) else f"transpose: rel L-inf {rel_linf(a, b):.3e}")
+    assert rel_linf(a, b) <= 1e-13
+
+
+def smooth_periodic(p):
+    """A smooth periodic state (the four-quadrant data wrapped around a
+    periodic box drives the pressure negative within a few steps, in the
+    oracle as well)."""
+    x = (np.arange(p.nx[0]) + 0.5) / p.nx[0]
+    y = (np.arange(p.nx[1]) + 0.5) / p.nx[1]
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    tp = 2 * np.pi
+    rho = 1.0 + 0.5 * np.sin(tp * X) * np.cos(tp * Y)
+    u = 0.5 * np.sin(tp * Y)
+    v = 0.5 * np.cos(tp * X)
+    pr = 1.0 + 0.2 * np.cos(tp * (X + Y))
+    q = np.zeros((p.nx[0] + 6, p.nx[1] + 6, 4), order="F")
+    qi = interior(p, q)
+    qi[..., 0], qi[..., 1], qi[..., 2] = rho, rho * u, rho * v
+    qi[..., 3] = pr / 0.4 + 0.5 * rho * (u * u + v * v)
+    return q
+
+
+def test_periodic_conservation():
+    p = make_params("euler", (N, N), char_decomp=2, time_integrator=3, mthbc=(2, 2, 2, 2))
+    q0 = smooth_periodic(p)
+    s0 = interior(p, q0).sum(axis=(0, 1)) * p.dx[0] * p.dx[1]
+    _, _, st, s1 = run(p, q0, 10)
+    assert st.steps_taken == 10
+    scale = np.abs(interior(p, q0)).sum(axis=(0, 1)) * p.dx[0] * p.dx[1]
+    assert np.all(np.abs(s1 - s0) <= 1e-12 * scale), (s1 - s0) / scale
+
+
+def test_fp32_option_l1_fullsize():
+    p = make_params("euler", (N, N), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    q0 = quadrants(p)
+    q64, _, _, _ = run(p, q0, 20)
+    p32 = make_params("euler", (N, N), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1),
+                      precision=SC_FP32)
+    q32, _, _, _ = run(p32, q0, 20)
+    a, b = interior(p, q32), interior(p, q64)
+    err = np.sum(np.abs(a - b)) / np.sum(np.abs(b))
+    print("1024^2 float32 relative L1 vs float64:", err)
+    assert err <= 1e-5
