@@ -454,7 +454,8 @@ __device__ __forceinline__ void normal_flux(const PhysConst<T> &c, const T *q, d
     }
 }
 
-// physics whose in-cell total fluctuation is taken from normal_flux
+// 2D physics whose in-cell total fluctuation is taken from normal_flux (the
+// 1D stage decides per reconstruction, stage1d_chunk)
 template <int P> constexpr bool incell_by_flux() { return P == EU2 || P == SW2; }
 
 // adq = amdq2 + apdq2 of the in-cell problem (left ql, right qr) as

@@ -428,11 +428,21 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
     __syncthreads();
 
     if (t < NC && i <= n) {
-        // in-cell problem of cell i: left ql(i) = rt, right qr(i) from interface i+1/2
+        // in-cell problem of cell i: left ql(i) = rt, right qr(i) from interface i+1/2.
+        // Euler and shallow water with weno5 / weno5_char (the reconstructions of
+        // batches and the server) take its total fluctuation as f(qr) - f(ql)
+        // (incell_flux_diff); the other reconstructions keep the swapped Roe
+        // solve, so their results stay the reference's also where a state is
+        // unphysical (NaN at the Shu-Osher shock with weno5_wave)
+        constexpr bool BYFLUX = (P == EU1 || P == SW1) && (CHAR == RK_WENO || CHAR == RK_WENO_CHAR);
         T qrc[M], am2[M], ap2[M];
+        double adq[M];
 #pragma unroll
         for (int m = 0; m < M; m++) qrc[m] = sx[m][t + 1];
-        riemann<P, IXY, T>(c, rt, qrc, wave, s, am2, ap2);
+        if constexpr (BYFLUX)
+            incell_flux_diff<P, IXY, T>(c, rt, qrc, adq);
+        else
+            riemann<P, IXY, T>(c, rt, qrc, wave, s, am2, ap2);
         const bool rq = op_reads_qold(a.op);
         // dtdx(i) = dt/(dx*capa(i)) with a capacity (flux1.f90:56-61)
         const double dtdx = a.capa ? a.dtdx / a.capa[i + 2] : a.dtdx;
@@ -454,9 +464,14 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
             // flux1.f90:150-152 (tfluct: the in-cell problem's total
             // fluctuation adq = amdq + apdq, tfluct*.f of the applications)
             // or flux1.f90:183-186 (the same problem through rp, swapped)
-            T X = c.tfluct ? (ap[m] + (am2[m] + ap2[m])) + sx[M + m][t + 1]
-                           : ((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m];
-            double r = sv[m] - dtdx * (double)X;
+            double X;
+            if constexpr (BYFLUX)
+                X = c.tfluct ? ((double)ap[m] + adq[m]) + (double)sx[M + m][t + 1]
+                             : (double)(sx[M + m][t + 1] + ap[m]) + adq[m];
+            else
+                X = c.tfluct ? (double)((ap[m] + (am2[m] + ap2[m])) + sx[M + m][t + 1])
+                             : (double)(((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m]);
+            double r = sv[m] - dtdx * X;
             size_t g = (size_t)(i + 2) + (size_t)m * a.LX;
             double q = a.qin[g];
             double qo = rq ? a.qold[g] : 0.0;
