@@ -47,12 +47,13 @@ __device__ __forceinline__ double weno_side(double im, double a2, double a1, dou
 // the four jumps has magnitude 1, so the largest indicator is >= 3.9 (the
 // minimum over the other jump of 13(1-x)^2 + 3(1-3x)^2); with eps/M^2
 // clamped to [1e-15, 1e9] (every beta <= 100) each (eps + beta)^2 lies in
-// [1e-30, 1.1e18], so the reference's products and sum (s1, s2, s3 and t0
-// of reconstruct.f90:124-129) stay below 7e36 and the sum above 1.5e-29:
-// one reciprocal per side besides 1/M.  Below the lower clamp the weights
-// differ from the reference's only where the jumps agree to float32
-// resolution; above the upper one by beta/eps <= 1e-7 relative.  Jumps
-// below 1e-30 (where 1/M would overflow) give no correction.
+// [1e-30, 1.1e18].  The weights are formed as in weno_w, from g_k = (eps +
+// beta_k)^2 / 9 in [1.1e-31, 1.3e17]: the products g g stay below 2e34 and
+// their sum D above 1e-31, so one reciprocal per side besides 1/M.  Below
+// the lower clamp the weights differ from the reference's only where the
+// jumps agree to float32 resolution; above the upper one by beta/eps <=
+// 1e-7 relative.  Jumps below 1e-30 (where 1/M would overflow) give no
+// correction.
 __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, float a0, float am1)
 {
     float M = fmaxf(fmaxf(fabsf(a2), fabsf(a1)), fmaxf(fabsf(a0), fabsf(am1)));
@@ -62,20 +63,21 @@ __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, flo
     a1 *= r;
     a0 *= r;
     am1 *= r;
-    float eps = fminf(fmaxf(1.0e-36f * r * r, 1.0e-15f), 1.0e9f);
-    float t1 = im * (a2 - a1);
-    float t2 = im * (a1 - a0);
-    float t3 = im * (a0 - am1);
-    float tt1 = dsq(eps + (13.0f * (t1 * t1) + 3.0f * dsq(a2 - 3.0f * a1)));
-    float tt2 = dsq(eps + (13.0f * (t2 * t2) + 3.0f * dsq(a1 + a0)));
-    float tt3 = dsq(eps + (13.0f * (t3 * t3) + 3.0f * dsq(3.0f * a0 - am1)));
-    float s1 = tt2 * tt3;
-    float s2 = 6.0f * tt1 * tt3;
-    float s3 = 3.0f * tt1 * tt2;
-    float t0 = frcp(s1 + s2 + s3);
-    s1 = s1 * t0;
-    s3 = s3 * t0;
-    return ((s1 * (t2 - t1) + (0.5f * s3 - 0.25f) * (t3 - t2)) * (1.0f / 3.0f)) * M;
+    constexpr float c13 = 13.0f / 3.0f;
+    const float e3 = fminf(fmaxf((1.0e-36f / 3.0f) * r * r, 1.0e-15f / 3.0f), 1.0e9f / 3.0f);
+    const float t1 = im * (a2 - a1);
+    const float t2 = im * (a1 - a0);
+    const float t3 = im * (a0 - am1);
+    const float g1 = dsq(e3 + fmaf(c13, t1 * t1, dsq(a2 - 3.0f * a1)));
+    const float g2 = dsq(e3 + fmaf(c13, t2 * t2, dsq(a1 + a0)));
+    const float g3 = dsq(e3 + fmaf(c13, t3 * t3, dsq(3.0f * a0 - am1)));
+    const float s1 = g2 * g3;
+    const float u = g1 * g2;
+    const float d = fmaf(3.0f, u, fmaf(6.0f * g1, g3, s1));
+    const float t0 = frcp(d);
+    const float f32 = t3 - t2;
+    const float num = fmaf(u, 1.5f * f32, s1 * (t2 - t1));
+    return fmaf(num * t0, 1.0f / 3.0f, f32 * (-1.0f / 12.0f)) * M;
 }
 
 // The weights and correction of weno_side from the squared differences
