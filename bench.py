@@ -83,6 +83,38 @@ def valu_from_profile(tx, ty, ms_launch):
             "valu_lane_instr_per_cell": k["valu_lane_instr_per_cell"]}
 
 
+def fp32_option_run(args, q0, q64):
+    """The same run with float32 reconstruction and Riemann arithmetic
+    (float64 state, accumulation, dt/CFL): cell-updates/s, the stage time,
+    and the relative L1 of its result against the float64 run's (the float64
+    path matches the oracle to 1e-12 per step; the oracle itself would take
+    minutes per step at this size)."""
+    from sharpclaw_amd import SC_FP32, ScRunCtl, Solver, interior, make_params
+    n = args.n
+    p = make_params("euler", (n, n), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1),
+                    precision=SC_FP32)
+    with Solver(p) as s:
+        s.set_q(q0)
+        dt, t = 1e-4, 0.0
+        if args.warmup > 0:
+            ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=args.warmup,
+                           dt_variable=1)
+            dt, wst = s.advance(ctl, 0.0, 1e9, dt)
+            t = wst.t
+        ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=args.steps, dt_variable=1)
+        dt_out, rst = s.advance(ctl, t, 1e9, dt)
+        ms_launch = s.time_stage(50, dt_out)
+        q32 = s.get_q()
+    a, b = interior(p, q32), interior(p, q64)
+    l1 = float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+    value = n * n * rst.stage_evals / (rst.gpu_ms * 1e-3)
+    log(f"bench: fp32 option {value:.3e} cell-updates/s, stage {ms_launch * 1e3:.1f} us, "
+        f"relative L1 vs fp64 {l1:.2e}")
+    return {"value": value, "unit": "cell-updates/s", "us_per_launch": ms_launch * 1e3,
+            "l1_vs_fp64": l1, "steps": args.warmup + args.steps,
+            "note": "relative L1 over the interior after the same steps as the float64 run"}
+
+
 def cpu_baseline(n, st, steps):
     """The reference Fortran (oracle/_ref, flang -O3, serial) on a bounded
     sample of the same workload; the C restatement when it is not built."""
@@ -130,6 +162,7 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
     out = {"workload": f"{nsolves} 1D shallow-water dam breaks, mx log-uniform [64,4096] "
                        f"(sum {int(mx.sum())} cells), char WENO5 + Roe (efix), SSPRK3, to t=0.2",
            "seed": seed}
+    qend = {}
     for prec, name in ((SC_FP32, "fp32"), (SC_FP64, "fp64")):
         p = make_params("shallow_water", (8,), char_decomp=2, time_integrator=3, precision=prec)
         with Batch(p, mx, xl, xu) as b:
@@ -143,6 +176,7 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
             res, ms = b.run(ctl, 0.0, tend, dt0)
             block = b.block
             tuning = b.tuning()
+            qend[name] = [b.get_q(i)[3:-3] for i in range(nsolves)]
         bad = [i for i, r in enumerate(res) if r.info != 0 or abs(r.t - tend) > 1e-12]
         if bad:
             raise RuntimeError(f"batched solves {bad[:5]} did not reach t={tend}")
@@ -160,6 +194,12 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
         log(f"bench: batch {name}: {ms:.1f} ms, {cs / (ms * 1e-3):.3e} cell-stage/s, "
             f"p50 {out[name]['latency_ms']['p50']:.1f} ms p99 {out[name]['latency_ms']['p99']:.1f} ms, "
             f"block {block}")
+    # float32 results against the float64 ones (relative L1 per solve)
+    l1 = np.array([np.sum(np.abs(a - b)) / np.sum(np.abs(b)) for a, b in zip(qend["fp32"], qend["fp64"])])
+    out["fp32_l1_vs_fp64"] = {"max": float(l1.max()), "p50": float(np.median(l1)),
+                              "mean": float(l1.mean()),
+                              "note": "per solve, sum|q32 - q64| / sum|q64| over the interior at t=0.2"}
+    log(f"bench: batch fp32 relative L1 vs fp64: max {l1.max():.2e}, median {np.median(l1):.2e}")
     # float64 single-solve path on a stratified sample (quantiles of mx)
     order = np.argsort(mx)
     pick = [int(order[int(round(f * (nsolves - 1)))]) for f in np.linspace(0.05, 0.95, single_sample)]
@@ -180,13 +220,15 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
     log(f"bench: fp64 single-solve path {single:.3e} cell-stage/s; batched fp32 is "
         f"{out['fp32_batched_vs_fp64_single']:.1f}x")
     if cpu:
-        out["cpu_baseline"] = cpu_baseline_batch(mx, hl, hr, x0, order, cpu_sample)
+        out["cpu_baseline"] = cpu_baseline_batch(mx, hl, hr, x0, order, cpu_sample, qend["fp32"])
     return out
 
 
-def cpu_baseline_batch(mx, hl, hr, x0, order, k):
+def cpu_baseline_batch(mx, hl, hr, x0, order, k, q32=None):
     """Reference Fortran (flang -O3), one serial process per solve over the
-    host cores, on k solves at mx quantiles 2..90% (bounded time)."""
+    host cores, on k solves at mx quantiles 2..90% (bounded time).  With q32
+    (the batch's float32 results) also the relative L1 of those solves
+    against the reference's own final frames."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import refrun
     from sharpclaw_amd import make_params
@@ -201,22 +243,28 @@ def cpu_baseline_batch(mx, hl, hr, x0, order, k):
         cd = refrun.clawdata_for(p, nout=1, outstyle=1, tfinal=0.2, dt_initial=1e-5, cfl_max=0.5,
                                  cfl_desired=0.45, max_steps=1000000)
         t0 = time.perf_counter()
-        _, dtlog, _ = refrun.run(p, cd, ["3", f"{float(hl[i])!r} {float(hr[i])!r} {float(x0[i])!r}"])
-        return time.perf_counter() - t0, int(mx[i]) * 3 * len(dtlog)
+        frames, dtlog, _ = refrun.run(p, cd, ["3", f"{float(hl[i])!r} {float(hr[i])!r} {float(x0[i])!r}"])
+        return time.perf_counter() - t0, int(mx[i]) * 3 * len(dtlog), frames[-1][1]
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(cores) as ex:
         rs = list(ex.map(one, pick))
     wall = time.perf_counter() - t0
-    cs = sum(c for _, c in rs)
-    per_core = cs / sum(w for w, _ in rs)
+    cs = sum(r[1] for r in rs)
+    per_core = cs / sum(r[0] for r in rs)
     total = float(np.sum(9.0 * np.asarray(mx, dtype=np.float64) ** 2))
-    return {"value": cs / wall, "unit": "cell-updates/s per RK stage", "cores": cores,
-            "kind": "reference", "per_core_value": per_core,
-            "full_batch_estimate_s": total / (per_core * cores),
-            "sample": f"{k} dam breaks at mx quantiles 2..90% {sorted(int(mx[i]) for i in pick)}, "
-                      f"one serial oracle/_ref/xsclaw_sw1d process each (flang -O3) over {cores} "
-                      f"threads, {wall:.1f} s wall"}
+    out = {"value": cs / wall, "unit": "cell-updates/s per RK stage", "cores": cores,
+           "kind": "reference", "per_core_value": per_core,
+           "full_batch_estimate_s": total / (per_core * cores),
+           "sample": f"{k} dam breaks at mx quantiles 2..90% {sorted(int(mx[i]) for i in pick)}, "
+                     f"one serial oracle/_ref/xsclaw_sw1d process each (flang -O3) over {cores} "
+                     f"threads, {wall:.1f} s wall"}
+    if q32 is not None:
+        l1 = [float(np.sum(np.abs(q32[i] - r[2])) / np.sum(np.abs(r[2]))) for i, r in zip(pick, rs)]
+        out["fp32_l1_vs_reference"] = {"max": max(l1), "solves": len(l1),
+                                       "note": "batched float32 result vs the reference's t=0.2 "
+                                               "frame, relative L1 over the interior"}
+    return out
 
 
 RECON_1D = [("weno5", 2, 0), ("weno5_wave", 2, 1), ("weno5_char", 2, 2), ("weno5_trans", 2, 3),
@@ -430,7 +478,10 @@ def main():
             f"rejected {rst.rejected}, cfl_last {rst.cfl_last:.4f}")
         value = cells * rst.stage_evals / (rst.gpu_ms * 1e-3)
         # kernel-only: the fused stage launches of 50 SSPRK3 steps back to back
+        # (right after the timed loop, before any copy leaves the GPU idle;
+        # they write only free buffers, q stays the loop's result)
         ms_launch = s.time_stage(50, dt_out)
+        q_end = s.get_q()
         tile, (tx, ty) = s.get_tile()
         log(f"bench: fused stage kernel {ms_launch * 1e3:.1f} us per launch (tile {tx}x{ty})")
         sums = s.sums()
@@ -442,6 +493,8 @@ def main():
         _, pst = s.advance(ctl_p, 0.0, 1e9, 1e-4)
         s.get_q()
         pcie_value = cells * pst.stage_evals / (time.perf_counter() - w1)
+
+    fp32_option = None if args.fp32 else fp32_option_run(args, q0, q_end)
 
     from sharpclaw_amd import copy_bench
     copy_bytes = 1 << 30
@@ -482,6 +535,7 @@ def main():
         "frac_of_copy_bw": achieved / copy_gbs,
         "pct_hbm_peak": 100.0 * achieved / HBM_PEAK_GBS,
         "pcie_inclusive_value": pcie_value,
+        "fp32_option": fp32_option,
         "conservation_sums": [float(x) for x in sums],
     }
     if not args.no_cpu:
