@@ -71,6 +71,9 @@ __global__ __launch_bounds__(256) void stage2d_batch_kernel(Batch2DArgs A, PhysC
     a.dt = s.dt;
     a.xlo = 0.0;
     a.dxc = G.dx;
+    a.aux = nullptr;
+    a.vscale = 1.0;
+    a.full = 0;
     stage2d_tile<P, CHAR, T, TX, TY>(a, c, local);
 }
 

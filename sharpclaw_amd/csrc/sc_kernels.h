@@ -45,6 +45,9 @@ struct StageArgs {
     // LX*LY each) and the step's factor on them (b4step)
     const double *aux;
     double vscale;
+    // 2D: 1 = every tile takes the full path (no constant-tile shortcut);
+    // the tile-shape tuner times the full path
+    int full;
 };
 
 // physics selector of the launchers (sc_physics.h PhysId)

@@ -56,6 +56,21 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
     const int nx = a.nx, ny = a.ny;
     const int i0 = 1 + (b % a.ntx) * TX, j0 = 1 + (b / a.ntx) * TY;
 
+    // Each staged value is also compared with the tile's first interior cell
+    // (i0, j0): when every staged cell of the tile and its halos equals it,
+    // every x and y stencil of the tile is constant, its WENO5 corrections
+    // and all its fluctuations are exactly zero, and the stage update is the
+    // RK combination with dq = 0 (up to the sign of a zero).  The barrier
+    // after staging reduces the test over the block (__syncthreads_and), so
+    // the branch is block-uniform.
+    const size_t plane = (size_t)a.LX * a.LY;
+    T ref[M];
+    {
+        const size_t g0 = (size_t)(i0 + 2) + (size_t)a.LX * (size_t)(j0 + 2);
+#pragma unroll
+        for (int m = 0; m < M; m++) ref[m] = T(a.qin[g0 + m * plane]);
+    }
+    bool same = true;
     for (int k = t; k < LYT * LXT; k += NT) {
         int ly = k / LXT, lx = k - ly * LXT;
         bool cx = lx < 3 || lx >= TX + 3, cy = ly < 3 || ly >= TY + 3;
@@ -70,15 +85,66 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
             j = bc_src(j, ny, a.bc[2], a.bc[3], negy);
         }
         size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
-        size_t plane = (size_t)a.LX * a.LY;
 #pragma unroll
         for (int m = 0; m < M; m++) {
             double val = a.qin[g + m * plane];
             if ((m == 1 && negx) || (m == 2 && negy)) val = -val;
-            sq[m][ly][lx] = T(val);
+            const T tv = T(val);
+            sq[m][ly][lx] = tv;
+            same = same && tv == ref[m];
         }
     }
-    __syncthreads();
+    const bool flat = __syncthreads_and(same) != 0;
+    __shared__ double red[2][NT / 64];
+
+    // A constant tile: the speeds of one x and one y interface (thread 0,
+    // through the same reconstruction and Riemann solve as every other
+    // interface of the tile, so the CFL words get the same bits) and the
+    // RK combination with dq = 0.  2D advection's speeds vary with aux, so
+    // it always takes the full path.
+    if constexpr (P != ADV2) {
+        if (flat && !a.full) {
+            T cmx = T(0), cmy = T(0);
+            if (t == 0) {
+                T v[6][M], lf[M], rt[M], wave[M * W], s[W], am[M], ap[M];
+#pragma unroll
+                for (int k = 0; k < 6; k++)
+#pragma unroll
+                    for (int m = 0; m < M; m++) v[k][m] = sq[m][3][3];
+                reconstruct_iface<P, 1, CHAR, T>(c, v, lf, rt);
+                riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
+#pragma unroll
+                for (int w = 0; w < W; w++) cmx = fmax(cmx, fabs(s[w]));
+                reconstruct_iface<P, 2, CHAR, T>(c, v, lf, rt);
+                riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
+#pragma unroll
+                for (int w = 0; w < W; w++) cmy = fmax(cmy, fabs(s[w]));
+            }
+            if (t < TX * TY) {
+                const int i = i0 + t % TX, j = j0 + t / TX;
+                if (i <= nx && j <= ny) {
+                    const bool rq = op_reads_qold(a.op);
+                    size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
+#pragma unroll
+                    for (int m = 0; m < M; m++) {
+                        size_t gm = g + m * plane;
+                        double q = a.qin[gm];
+                        double qo = rq ? a.qold[gm] : 0.0;
+                        double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
+                        double out, qbo = 0.0;
+                        combine(a.op, qo, q, 0.0, qbi, out, qbo);
+                        a.qout[gm] = out;
+                        if (a.op == OP_SSP104_5) a.qbout[gm] = qbo;
+                    }
+                }
+            }
+            if (a.smax) {
+                merge_smax<T, NT / 64>(a.smax, cmx, red[0]);
+                merge_smax<T, NT / 64>(a.smax + 1, cmy, red[1]);
+            }
+            return;
+        }
+    }
 
     const bool owns_cell = t < TX * TY;
     const int e = t - TX * TY;  // edge-interface index of the spare threads
@@ -210,7 +276,6 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
         if (i <= nx && j <= ny) {
             const bool rq = op_reads_qold(a.op);
             size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
-            size_t plane = (size_t)a.LX * a.LY;
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 double r = dqx[m] + -(a.dtdy * Y[m]);
@@ -225,7 +290,6 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
             }
         }
     }
-    __shared__ double red[2][NT / 64];
     if (a.smax) {
         merge_smax<T, NT / 64>(a.smax, smx, red[0]);
         merge_smax<T, NT / 64>(a.smax + 1, smy, red[1]);
