@@ -112,3 +112,24 @@ def test_fp32_option_l1_fullsize():
     err = np.sum(np.abs(a - b)) / np.sum(np.abs(b))
     print("1024^2 float32 relative L1 vs float64:", err)
     assert err <= 1e-5
+
+
+def test_constant_state_constant_tiles():
+    """A uniform state: every tile is constant (the stage's shortcut), q stays
+    uniform and equal to q0 up to the rounding of the RK combinations
+    (0.75 q + 0.25 q is q only to an ulp), and dt follows the CFL of the
+    state's speeds max(|u| + c, |v| + c) (cfl_desired * dx / smax)."""
+    p = make_params("euler", (N, N), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    rho, u, v, pr = 0.8, 0.3, -0.6, 1.1
+    q0 = np.zeros((N + 6, N + 6, 4), order="F")
+    qi = interior(p, q0)
+    qi[..., 0], qi[..., 1], qi[..., 2] = rho, rho * u, rho * v
+    qi[..., 3] = pr / 0.4 + 0.5 * rho * (u * u + v * v)
+    q, dt, st, _ = run(p, q0, 3)
+    assert st.steps_taken == 3
+    qi1 = interior(p, q)
+    assert np.all(qi1 == qi1[0, 0, :])
+    assert rel_linf(qi1, interior(p, q0)) <= 1e-15
+    c = np.sqrt(1.4 * pr / rho)
+    want = 0.45 * p.dx[0] / max(abs(u) + c, abs(v) + c)
+    assert abs(dt - want) <= 1e-13 * want, (dt, want)
