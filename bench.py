@@ -370,7 +370,7 @@ def served_bench(n, rate, seed, cpu=True):
     to one Server; latency = completion - submission on the server clock.
     A second pass submits the same requests all at once: its makespan gives
     the server's saturated throughput."""
-    from sharpclaw_amd import SC_FP64, ScRunCtl, Server, Solver
+    from sharpclaw_amd import SC_FP32, SC_FP64, ScRunCtl, Server, Solver
     from sharpclaw_amd.workloads import SERVED_CTL, served_mix
     mix = served_mix(n, seed, rate)
     ctl = ScRunCtl(**SERVED_CTL)
@@ -386,6 +386,12 @@ def served_bench(n, rate, seed, cpu=True):
         st = srv.stats()
         slat, scs, _, sspan, _ = _serve_pass(srv, mix, ctl, paced=False)
         sst = srv.stats()
+        # the same requests, all float32 arithmetic, submitted at once
+        mix32 = served_mix(n, seed, rate)
+        for r in mix32:
+            r["params"].precision = SC_FP32
+            r["precision"] = SC_FP32
+        f_lat, f_cs, _, f_span, _ = _serve_pass(srv, mix32, ctl, paced=False)
     info = {"workload": f"{n} requests, Poisson arrivals at {rate:g}/s (seed {seed}): 1D acoustics "
                         f"mx 128..4096, 1D Euler mx 128..1999, 2D shallow water 128^2..1024^2, "
                         f"Gaussian bumps, float32/float64 at random, char WENO5 + SSPRK3",
@@ -400,11 +406,17 @@ def served_bench(n, rate, seed, cpu=True):
                           "latency_ms": {"p50": float(np.percentile(slat, 50)),
                                          "p99": float(np.percentile(slat, 99))},
                           "peak_active": sst["peak_active"],
-                          "sample": f"the same {n} requests submitted at once"}}
+                          "sample": f"the same {n} requests submitted at once"},
+            "saturated_fp32": {"value": f_cs / f_span, "makespan_s": f_span,
+                               "latency_ms": {"p50": float(np.percentile(f_lat, 50)),
+                                              "p99": float(np.percentile(f_lat, 99))},
+                               "sample": f"the same {n} requests, all float32 arithmetic, "
+                                         "submitted at once"}}
     log(f"bench: served {n} requests in {span_s:.2f} s, {info['value']:.3e} cell-stage/s, "
         f"p50 {info['latency_ms']['p50']:.1f} ms p99 {info['latency_ms']['p99']:.1f} ms, "
         f"{st['rounds']} rounds, peak {st['peak_active']} active")
     log(f"bench: saturated (all {n} at once): {sspan:.2f} s, {scs / sspan:.3e} cell-stage/s")
+    log(f"bench: saturated, all float32: {f_span:.2f} s, {f_cs / f_span:.3e} cell-stage/s")
     # the float64 single-solve path on a sample: each request alone, in turn
     sample = [r for r in served_mix(n, seed, rate)][:: max(1, n // 12)][:12]
     cs1 = ms1 = 0.0
@@ -421,8 +433,11 @@ def served_bench(n, rate, seed, cpu=True):
                                       "same mix, float64, one at a time through Solver.advance"}
     info["saturated_vs_fp64_single"] = (info["saturated"]["value"] /
                                         info["fp64_single_solve_path"]["value"])
+    info["saturated_fp32_vs_fp64_single"] = (info["saturated_fp32"]["value"] /
+                                             info["fp64_single_solve_path"]["value"])
     log(f"bench: fp64 single-solve path on the mix {info['fp64_single_solve_path']['value']:.3e} "
-        f"cell-stage/s; saturated server is {info['saturated_vs_fp64_single']:.2f}x")
+        f"cell-stage/s; saturated server is {info['saturated_vs_fp64_single']:.2f}x "
+        f"(all float32: {info['saturated_fp32_vs_fp64_single']:.2f}x)")
     if cpu:
         info["cpu_baseline"] = cpu_baseline_served(seed)
     return info
