@@ -296,6 +296,78 @@ def cpu_baseline_batch(mx, hl, hr, x0, order, k, q32=None):
     return out
 
 
+def shu_osher_q(p):
+    """Shu-Osher initial data: x < -4: rho = 3.857143, u = 2.629369,
+    p = 10.333333; else rho = 1 + 0.2 sin(5x), u = 0, p = 1."""
+    from sharpclaw_amd import grid_shape, interior
+    g1 = p.gamma - 1.0
+    x = p.xlower[0] + (np.arange(1, p.nx[0] + 1) - 0.5) * p.dx[0]
+    left = x < -4.0
+    rho = np.where(left, 3.857143, 1.0 + 0.2 * np.sin(5.0 * x))
+    u = np.where(left, 2.629369, 0.0)
+    pr = np.where(left, 10.333333, 1.0)
+    q = np.zeros(grid_shape(p), order="F")
+    qi = interior(p, q)
+    qi[:, 0], qi[:, 1], qi[:, 2] = rho, rho * u, pr / g1 + 0.5 * rho * u * u
+    return q
+
+
+def euler1d_bench(cpu=True):
+    """One 1D grid: Euler (gamma 1.4) Shu-Osher, 1600 cells on [-5, 5],
+    characteristic WENO5 + Roe (efix), SSP(10,4), CFL 2.45 / 2.5, to t = 1.8,
+    float64 and float32 arithmetic, the whole run through Solver.advance
+    (host-driven steps; the grid is 7 workgroups, so latency, not
+    bandwidth, sets its rate).  With cpu, the reference Fortran on one core
+    on the same run, and the float64 result against the reference's."""
+    from sharpclaw_amd import SC_FP32, SC_FP64, ScRunCtl, Solver, interior, make_params
+    n, tend = 1600, 1.8
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=2.5, cfl_desired=2.45, max_steps=50000, dt_variable=1)
+    out = {"workload": "1D Euler Shu-Osher, 1600 cells on [-5,5], char WENO5 + Roe, SSP(10,4), "
+                       "CFL 2.45/2.5, to t=1.8"}
+    qend = {}
+    for prec, name in ((SC_FP64, "fp64"), (SC_FP32, "fp32")):
+        p = make_params("euler", (n,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                        time_integrator=4, mthbc=(1, 1, 1, 1), precision=prec)
+        q0 = shu_osher_q(p)
+        with Solver(p) as s:
+            s.set_q(q0)
+            s.advance(ScRunCtl(dt_max=1e99, cfl_max=2.5, cfl_desired=2.45, max_steps=3, dt_variable=1),
+                      0.0, tend, 0.001)  # warm-up (kernel load)
+            s.set_q(q0)
+            w0 = time.perf_counter()
+            _, st = s.advance(ctl, 0.0, tend, 0.001)
+            wall = time.perf_counter() - w0
+            qend[name] = interior(p, s.get_q()).copy()
+        cs = n * st.stage_evals
+        out[name] = {"steps": st.steps_taken, "rejected": st.rejected, "wall_ms": wall * 1e3,
+                     "gpu_ms": st.gpu_ms, "value": cs / (st.gpu_ms * 1e-3),
+                     "unit": "cell-updates/s per RK stage"}
+        log(f"bench: 1D Euler {name}: {st.steps_taken} steps in {wall * 1e3:.1f} ms wall "
+            f"({cs / (st.gpu_ms * 1e-3):.3e} cell-stage/s)")
+    a, b = qend["fp32"], qend["fp64"]
+    out["fp32_l1_vs_fp64"] = float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+    if cpu:
+        from oracle import refrun
+        if refrun.ref_available("eu1d", programs=("xsclaw",)):
+            p = make_params("euler", (n,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
+                            time_integrator=4, mthbc=(1, 1, 1, 1))
+            cd = refrun.clawdata_for(p, nout=1, outstyle=1, tfinal=tend, dt_initial=0.001,
+                                     cfl_max=2.5, cfl_desired=2.45, max_steps=50000)
+            t0 = time.perf_counter()
+            frames, dtlog, _ = refrun.run(p, cd, ["2", "3.857143 2.629369 10.333333 -4.0"])
+            wall = time.perf_counter() - t0
+            ref = frames[-1][1]
+            out["cpu_baseline"] = {
+                "value": n * 10 * len(dtlog) / wall, "unit": "cell-updates/s per RK stage",
+                "cores": 1, "kind": "reference", "wall_ms": wall * 1e3,
+                "sample": "the whole run, oracle/_ref/xsclaw_eu1d (flang -O3, 1 thread), "
+                          "whole-process wall incl. input/qinit/output",
+                "fp64_rel_linf_vs_reference": float(np.max(np.abs(qend["fp64"] - ref)) / np.max(np.abs(ref))),
+                "fp64_rel_l1_vs_reference": float(np.sum(np.abs(qend["fp64"] - ref)) / np.sum(np.abs(ref)))}
+            log(f"bench: 1D Euler reference (1 core): {wall * 1e3:.0f} ms")
+    return out
+
+
 RECON_1D = [("weno5", 2, 0), ("weno5_wave", 2, 1), ("weno5_char", 2, 2), ("weno5_trans", 2, 3),
             ("tvd2_wave", 1, 1), ("tvd2_char", 1, 2)]
 
@@ -599,6 +671,7 @@ def main():
         out["cpu_baseline"] = None
     if not args.no_recon:
         out["reconstructions_1d"] = recon_bench()
+        out["euler1d"] = euler1d_bench(cpu=not args.no_cpu)
     if not args.no_batch:
         log(f"bench: batched workload ({args.batch} dam breaks)")
         out["batched"] = batched_dambreaks(args.batch, args.seed, cpu=not args.no_cpu)
