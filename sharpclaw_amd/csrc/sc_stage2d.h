@@ -1,6 +1,8 @@
 // The fused 2D stage on one tile, shared by the single-grid stage kernel
 // (sc_kernels.hip) and the batched 2D kernels (sc_batch2d.hip).
 #pragma once
+#include <type_traits>
+
 #include "sc_stage.h"
 
 namespace sc {
@@ -97,38 +99,47 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
     const bool flat = __syncthreads_and(same) != 0;
     __shared__ double red[2][NT / 64];
 
-    // A constant tile: the speeds of one x and one y interface (thread 0,
-    // through the same reconstruction and Riemann solve as every other
-    // interface of the tile, so the CFL words get the same bits) and the
-    // RK combination with dq = 0.  2D advection's speeds vary with aux, so
-    // it always takes the full path.
+    // A constant tile: the speeds of one x interface (thread 0) and one y
+    // interface (thread 64, another wave) for the CFL, and the RK combination
+    // with dq = 0.  A constant stencil's reconstruction is its central value
+    // (the WENO5 corrections are zeros, at most of the other sign, which the
+    // speeds' magnitudes do not see), and the Riemann solve is the one every
+    // interface of the tile takes, so the CFL words get the bits of the full
+    // path.  2D advection's speeds vary with aux, so it always takes the
+    // full path.
     if constexpr (P != ADV2) {
         if (flat && !a.full) {
             T cmx = T(0), cmy = T(0);
-            if (t == 0) {
-                T v[6][M], lf[M], rt[M], wave[M * W], s[W], am[M], ap[M];
+            if (t == 0 || t == 64) {
+                T cen[M], wave[M * W], s[W], am[M], ap[M];
 #pragma unroll
-                for (int k = 0; k < 6; k++)
+                for (int m = 0; m < M; m++) {
+                    const T v = sq[m][3][3];
+                    cen[m] = weno_central(v, v, v, v);
+                }
+                T sm = T(0);
+                if (t == 0)
+                    riemann<P, 1, T>(c, cen, cen, wave, s, am, ap);
+                else
+                    riemann<P, 2, T>(c, cen, cen, wave, s, am, ap);
 #pragma unroll
-                    for (int m = 0; m < M; m++) v[k][m] = sq[m][3][3];
-                reconstruct_iface<P, 1, CHAR, T>(c, v, lf, rt);
-                riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
-#pragma unroll
-                for (int w = 0; w < W; w++) cmx = fmax(cmx, fabs(s[w]));
-                reconstruct_iface<P, 2, CHAR, T>(c, v, lf, rt);
-                riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
-#pragma unroll
-                for (int w = 0; w < W; w++) cmy = fmax(cmy, fabs(s[w]));
+                for (int w = 0; w < W; w++) sm = fmax(sm, fabs(s[w]));
+                if (t == 0)
+                    cmx = sm;
+                else
+                    cmy = sm;
             }
             if (t < TX * TY) {
-                const int i = i0 + t % TX, j = j0 + t / TX;
+                const int cx0 = t % TX, cy0 = t / TX;
+                const int i = i0 + cx0, j = j0 + cy0;
                 if (i <= nx && j <= ny) {
                     const bool rq = op_reads_qold(a.op);
                     size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
 #pragma unroll
                     for (int m = 0; m < M; m++) {
                         size_t gm = g + m * plane;
-                        double q = a.qin[gm];
+                        // float64 tiles hold q itself
+                        double q = std::is_same<T, double>::value ? (double)sq[m][cy0 + 3][cx0 + 3] : a.qin[gm];
                         double qo = rq ? a.qold[gm] : 0.0;
                         double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
                         double out, qbo = 0.0;
@@ -280,7 +291,8 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
             for (int m = 0; m < M; m++) {
                 double r = dqx[m] + -(a.dtdy * Y[m]);
                 size_t gm = g + m * plane;
-                double q = a.qin[gm];
+                // float64 tiles hold q itself
+                double q = std::is_same<T, double>::value ? (double)sq[m][cy + 3][cx + 3] : a.qin[gm];
                 double qo = rq ? a.qold[gm] : 0.0;
                 double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
                 double out, qbo = 0.0;

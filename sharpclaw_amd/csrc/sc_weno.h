@@ -141,6 +141,13 @@ template <> __device__ __forceinline__ float weno_corr<float>(float im, float a2
     return weno_side_f32(im, a2, a1, a0, am1);
 }
 
+// The fourth-order central value (-q(i-2) + 7(q(i-1) + q(i)) - q(i+1)) / 12
+// both sides of interface i-1/2 start from (reconstruct.f90:171-176)
+template <class T> __device__ __forceinline__ T weno_central(T a, T b, T c, T d)
+{
+    return (-a + T(7) * (b + c) - d) * (T(1) / T(12));
+}
+
 // Both sides of interface i-1/2 from the six cells v[k][m] = q(i-3+k, m).
 template <int P, int IXY, int CHAR, class T>
 __device__ __forceinline__ void reconstruct_iface(const PhysConst<T> &c, const T (&v)[6][Shape<P>::M],
@@ -154,7 +161,7 @@ __device__ __forceinline__ void reconstruct_iface(const PhysConst<T> &c, const T
         for (int m = 0; m < M; m++) d[k][m] = v[k + 1][m] - v[k][m];
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        T central = (-v[1][m] + T(7) * (v[2][m] + v[3][m]) - v[4][m]) * (T(1) / T(12));
+        T central = weno_central(v[1][m], v[2][m], v[3][m], v[4][m]);
         lf[m] = central;
         rt[m] = central;
     }
