@@ -473,7 +473,8 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
                              : (double)(((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m]);
             double r = sv[m] - dtdx * X;
             size_t g = (size_t)(i + 2) + (size_t)m * a.LX;
-            double q = a.qin[g];
+            // float64 chunks hold q itself
+            double q = std::is_same<T, double>::value ? (double)sq[m][t + 3] : a.qin[g];
             double qo = rq ? a.qold[g] : 0.0;
             double qbi = a.op == OP_SSP104_10 ? a.qbin[g] : 0.0;
             double out, qbo = 0.0;
