@@ -123,6 +123,7 @@ struct sc_batch {
     PhysConst<double> cd;
     PhysConst<float> cf;
     int dev, ncu, nsolves, block;
+    int ends;                 // work-queue ends (BatchArgs::ends)
     hipStream_t st;
     std::vector<BatchSolve> solves;
     std::vector<SolveState> state;
@@ -130,7 +131,8 @@ struct sc_batch {
     std::vector<int> cur;     // buffer holding each solve's current q
     BatchSolve *d_solves;
     SolveState *d_state;
-    int *d_order, *d_next;
+    int *d_order;
+    unsigned long long *d_next;  // work-queue counters
     double *d_base;
     size_t total;             // doubles allocated
     double tick_ms;           // wall_clock64 period in ms
@@ -194,6 +196,12 @@ extern "C" int sc_batch_create(const sc_params *params, int nsolves, const int *
     b->dev = params->device;
     b->nsolves = nsolves;
     b->block = 0;  // tuned at the first run
+    // every 8th workgroup takes the smallest solves first (BatchArgs::ends);
+    // measured on 4096 dam breaks against all-largest-first: the same
+    // makespan, p50 latency 4.7x lower, p99 5% higher.  SC_BATCH_ENDS
+    // overrides it (1: largest first only).
+    b->ends = 8;
+    if (const char *e = getenv("SC_BATCH_ENDS")) b->ends = std::max(1, atoi(e));
     hipDeviceProp_t prop;
     (void)hipGetDeviceProperties(&prop, b->dev);
     b->ncu = prop.multiProcessorCount;
@@ -219,7 +227,7 @@ extern "C" int sc_batch_create(const sc_params *params, int nsolves, const int *
     ok = ok && hipMalloc(&b->d_solves, nsolves * sizeof(BatchSolve)) == hipSuccess;
     ok = ok && hipMalloc(&b->d_state, nsolves * sizeof(SolveState)) == hipSuccess;
     ok = ok && hipMalloc(&b->d_order, nsolves * sizeof(int)) == hipSuccess;
-    ok = ok && hipMalloc(&b->d_next, sizeof(int)) == hipSuccess;
+    ok = ok && hipMalloc(&b->d_next, sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMalloc(&b->d_base, b->total * sizeof(double)) == hipSuccess;
     ok = ok && hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess;
     ok = ok && hipMemcpy(b->d_order, b->order.data(), nsolves * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
@@ -280,6 +288,7 @@ static int run_once(sc_batch *b, int nt, double tstart, double dt0, int info0, i
     A.order = b->d_order;
     A.nsolves = b->nsolves;
     A.next = b->d_next;
+    A.ends = b->ends;
     A.base = b->d_base;
     A.meqn = b->p.meqn;
     A.bc[0] = b->p.mthbc[0];
@@ -289,7 +298,7 @@ static int run_once(sc_batch *b, int nt, double tstart, double dt0, int info0, i
     A.budget_chunks = 0;
     hipError_t e = hipMemcpyAsync(b->d_state, b->state.data(), b->nsolves * sizeof(SolveState),
                                   hipMemcpyHostToDevice, b->st);
-    if (e == hipSuccess) e = hipMemsetAsync(b->d_next, 0, sizeof(int), b->st);
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_next, 0, sizeof(unsigned long long), b->st);
     if (e == hipSuccess) e = hipEventRecord(b->ev0, b->st);
     if (e == hipSuccess) e = launch_batch(b->sel, nt, A, b->cd, b->cf, b->ncu, 0, b->st);
     if (e == hipSuccess) e = hipEventRecord(b->ev1, b->st);

@@ -48,7 +48,12 @@ struct BatchArgs {
     SolveState *state;
     const int *order;   // solves of this launch, largest first
     int nsolves;        // entries of order
-    int *next;          // work-queue counter
+    // work-queue counters: low 32 bits = entries taken from the front of
+    // order (largest first), high 32 bits = entries taken from the back
+    unsigned long long *next;
+    int ends;           // 1: every workgroup takes from the front; k > 1:
+                        // workgroups b with b % k == k - 1 take from the back
+                        // (smallest first)
     double *base;
     int meqn;
     int bc[2];
@@ -84,6 +89,22 @@ __device__ __forceinline__ int other_buf(int a, int b)
     return 0;
 }
 
+// Index into A.order of the next solve for this workgroup, A.nsolves when
+// the queue is empty.  One 64-bit atomic claims an entry from either end:
+// with f front and b back entries taken before it, the claim is valid while
+// f + b < nsolves, so the two ends never take the same entry and exactly
+// nsolves claims succeed.  With ends = k > 1 the largest solves still start
+// at once on most workgroups while every k-th one finishes the small solves
+// first, so completion latencies are low for most solves.
+__device__ __forceinline__ int fetch_solve(const BatchArgs &A)
+{
+    const bool back = A.ends > 1 && (int)(blockIdx.x % A.ends) == A.ends - 1;
+    const unsigned long long old = atomicAdd(A.next, back ? (1ull << 32) : 1ull);
+    const unsigned f = (unsigned)(old & 0xffffffffull), b = (unsigned)(old >> 32);
+    if ((unsigned long long)f + b >= (unsigned long long)A.nsolves) return A.nsolves;
+    return back ? A.nsolves - 1 - (int)b : (int)f;
+}
+
 template <int P, int CHAR, class T, int NT>
 __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
 {
@@ -102,7 +123,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
     // test on either side of the back-edge would be merged by jump threading
     // into a nested loop that the other lanes of the wave iterate without
     // lane 0 -- a livelock under exec masking.)
-    if (t == 0) sb = atomicAdd(A.next, 1);
+    if (t == 0) sb = fetch_solve(A);
     __syncthreads();
     int k = sb;
     __syncthreads();
@@ -217,7 +238,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
         }
         if (t == 0) {
             A.state[b] = s;
-            sb = atomicAdd(A.next, 1);
+            sb = fetch_solve(A);
         }
         __syncthreads();
         k = sb;

@@ -143,7 +143,7 @@ struct Group {
     BatchSolve *d_solves = nullptr;
     SolveState *d_state = nullptr;
     int *d_order = nullptr;
-    int *d_next = nullptr;
+    unsigned long long *d_next = nullptr;  // work-queue counters of a round
     BatchSolve *hs = nullptr;   // pinned staging of the descriptors
     SolveState *hst = nullptr;
     int *ho = nullptr;
@@ -261,7 +261,7 @@ Group *group_for(sc_server *S, const sc_params &p)
     }
     if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev, hipEventDisableTiming) != hipSuccess ||
-        hipMalloc(&g->d_next, sizeof(int)) != hipSuccess)
+        hipMalloc(&g->d_next, sizeof(unsigned long long)) != hipSuccess)
         return nullptr;
     S->groups.push_back(std::move(g));
     return S->groups.back().get();
@@ -437,6 +437,7 @@ int launch_round(sc_server *S, Group *g)
     A.order = g->d_order;
     A.nsolves = n;
     A.next = g->d_next;
+    A.ends = 1;
     A.base = S->arena_base;
     A.meqn = g->p.meqn;
     A.bc[0] = g->p.mthbc[0];
@@ -448,7 +449,7 @@ int launch_round(sc_server *S, Group *g)
     if (e == hipSuccess)
         e = hipMemcpyAsync(g->d_state, g->hst, n * sizeof(SolveState), hipMemcpyHostToDevice, g->st);
     if (e == hipSuccess) e = hipMemcpyAsync(g->d_order, g->ho, n * sizeof(int), hipMemcpyHostToDevice, g->st);
-    if (e == hipSuccess) e = hipMemsetAsync(g->d_next, 0, sizeof(int), g->st);
+    if (e == hipSuccess) e = hipMemsetAsync(g->d_next, 0, sizeof(unsigned long long), g->st);
     if (e == hipSuccess) e = launch_batch(g->sel, g->nt, A, g->cd, g->cf, S->ncu, 0, g->st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(g->hst, g->d_state, n * sizeof(SolveState), hipMemcpyDeviceToHost, g->st);
