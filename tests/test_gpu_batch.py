@@ -242,3 +242,25 @@ def test_batch_alone_vs_mixed_and_block_sizes_bitwise(fp32):
     assert got in (64, 128, 256, 512, 1024)
     for i in range(n):
         assert np.array_equal(tuned[i], mixed[i])
+
+
+@pytest.mark.parametrize("ends", ["1", "2", "3", "8"])
+def test_batch_queue_ends_bitwise(ends, monkeypatch):
+    """The two-ended work queue hands every solve to exactly one workgroup,
+    whichever share of workgroups takes the smallest solves first (every
+    ends-th one; 1 = largest first only): results equal the single-ended
+    run bit for bit, and every solve reaches t_end."""
+    mx, hl, hr, x0 = dambreak_batch(20, seed=11, mx_lo=16, mx_hi=600)
+    n = len(mx)
+    base = make_params("shallow_water", (8,), char_decomp=2, time_integrator=3)
+    xl, xu = np.zeros(n), np.ones(n)
+    qs = [dambreak_q(int(mx[i]), hl[i], hr[i], x0[i]) for i in range(n)]
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=100000, dt_variable=1)
+    monkeypatch.setenv("SC_BATCH_ENDS", "1")
+    ref, rr, _ = _run_batch(base, mx, xl, xu, qs, ctl, 0.02, 1e-3, block=64)
+    monkeypatch.setenv("SC_BATCH_ENDS", ends)
+    got, rg, _ = _run_batch(base, mx, xl, xu, qs, ctl, 0.02, 1e-3, block=64)
+    for i in range(n):
+        assert rg[i].info == 0 and rg[i].t == pytest.approx(0.02, abs=1e-14)
+        assert rg[i].steps_taken == rr[i].steps_taken
+        assert np.array_equal(got[i], ref[i]), i
