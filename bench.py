@@ -408,6 +408,35 @@ def recon_bench(n=1 << 20, stages=60):
     return out
 
 
+def recon2d_bench(n=1024, stages=12):
+    """The 2D stage per reconstruction on the headline grid (2D Euler n x n,
+    the smooth state of smooth_q, SSPRK3 stage mix, float64): weno5 and
+    weno5_char run the fused tile kernel; weno5_wave, weno5_trans and the
+    TVD variants run the slice sweeps (x sweeps write dq of the x direction,
+    y sweeps add theirs and combine, flux2.f90:36-92).  HIP-event time per
+    stage, cell-updates/s and the HBM rate of the algorithmic bytes
+    (4 x 8 x (2 + 3 + 3)/3 per cell; the sweeps also move dq of the x
+    direction through HBM, 2 x 32 bytes per cell more)."""
+    from sharpclaw_amd import Solver, make_params
+    out = {"workload": f"2D Euler {n}x{n}, smooth state, SSPRK3 stage mix, float64, MC limiter for "
+                       "the TVD variants"}
+    for name, lim, char in RECON_1D:
+        p = make_params("euler", (n, n), char_decomp=char, lim_type=lim, mthlim=(4, 4, 4, 4),
+                        time_integrator=3)
+        with Solver(p) as s:
+            s.set_q(smooth_q(p))
+            s.time_stage(2, 1e-6)
+            ms = s.time_stage(stages, 1e-6)
+        cells = n * n
+        gbs = 4 * 8 * (2 + 3 + 3) / 3.0 * cells / (ms * 1e-3) / 1e9
+        out[name] = {"lim_type": lim, "char_decomp": char, "us_per_stage": ms * 1e3,
+                     "path": "tile" if lim == 2 and char in (0, 2) else "slice sweeps",
+                     "value": cells / (ms * 1e-3), "unit": "cell-updates/s per RK stage",
+                     "alg_gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+        log(f"bench: 2D {name}: {ms * 1e3:.1f} us per stage, {cells / (ms * 1e-3):.3e} cell-updates/s")
+    return out
+
+
 def _serve_pass(srv, mix, ctl, paced):
     """Submit `mix` to `srv` (at the arrival times when paced, else all at
     once), drain, and return (latencies ms, cell-stages, per-kind latencies,
@@ -680,6 +709,7 @@ def main():
         out["cpu_baseline"] = None
     if not args.no_recon:
         out["reconstructions_1d"] = recon_bench()
+        out["reconstructions_2d"] = recon2d_bench()
         out["euler1d"] = euler1d_bench(cpu=not args.no_cpu)
     if not args.no_batch:
         log(f"bench: batched workload ({args.batch} dam breaks)")

@@ -112,11 +112,16 @@ int sc_check_params(const sc_params *p, bool grid)
                                            "and divides by zero jumps unguarded (reconstruct.f90:540-542)");
     if (p->lim_type == 1 && p->char_decomp != 1 && p->char_decomp != 2)
         return sc_fail(SC_ERR_ARG, "char_decomp must be 1 or 2 with lim_type 1 (flux1.f90:76-91)");
-    if ((p->lim_type == 1 || p->char_decomp == 1 || p->char_decomp == 3) && p->ndim != 1)
-        return sc_fail(SC_ERR_UNSUPPORTED, "lim_type %d char_decomp %d: implemented for 1D grids", p->lim_type,
-                       p->char_decomp);
+    // in 2D these run as x and y slice sweeps (flux2.f90), for Euler and
+    // shallow water
+    if ((p->lim_type == 1 || p->char_decomp == 1 || p->char_decomp == 3) && p->ndim == 2 && id != EU2 && id != SW2)
+        return sc_fail(SC_ERR_UNSUPPORTED, "lim_type %d char_decomp %d: implemented for 1D grids and 2D Euler / "
+                       "shallow water", p->lim_type, p->char_decomp);
+    // tvd2_wave limits wave families 1..mwaves, tvd2_char characteristic
+    // fields 1..meqn (reconstruct.f90:623, 726; with meqn > mwaves, 2D Euler,
+    // the reference reads mthlim past its mwaves entries)
     if (p->lim_type == 1)
-        for (int k = 0; k < p->mwaves && k < 4; k++)
+        for (int k = 0; k < (p->char_decomp == 2 ? p->meqn : p->mwaves) && k < 4; k++)
             if (p->mthlim[k] < 1 || p->mthlim[k] > 5)
                 return sc_fail(SC_ERR_ARG, "mthlim(%d) = %d: limiters are 1..5 (reconstruct.f90:626-654)", k + 1,
                                p->mthlim[k]);
@@ -173,6 +178,7 @@ struct sc_solver {
     double *buf[3];    // rotating q buffers
     double *qb;        // SSP(10,4) register
     double *dq;        // step() output
+    double *dqx;       // 2D slice sweeps: dq of the x sweeps (NULL on the tile path)
     double *partial;   // sums scratch
     double *sums;      // sums result
     int nblk_sums;
@@ -182,7 +188,7 @@ struct sc_solver {
     int cur, old;      // buffer indices of q and qold
     bool have_q;
     int tile2d;        // 2D tile shape index; -1: tune at first use
-    double *mem;       // one allocation: buf[0..2], qb, dq
+    double *mem;       // one allocation: buf[0..2], qb, dq[, dqx]
     int loop_mode;     // 2D sc_advance: -1 by grid size, 0 host-driven, 1 device-resident
     bool use_graph;    // device loop: replay rounds from a captured graph
     Batch2DRunner *dr; // 2D device-resident loop (created at first use)
@@ -244,12 +250,15 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     s->dev = params->device;
     s->n = grid_n(&s->p);
     s->nblk_sums = 256;
+    const bool sweeps = s->p.ndim == 2 && sweep2d_path(s->sel);
+    const int narr = sweeps ? 6 : 5;
     bool ok = hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) == hipSuccess;
-    ok = ok && hipMalloc(&s->mem, 5 * s->n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&s->mem, narr * s->n * sizeof(double)) == hipSuccess;
     if (ok) {
         for (int k = 0; k < 3; k++) s->buf[k] = s->mem + k * s->n;
         s->qb = s->mem + 3 * s->n;
         s->dq = s->mem + 4 * s->n;
+        s->dqx = sweeps ? s->mem + 5 * s->n : nullptr;
     }
     ok = ok && hipMalloc(&s->partial, (size_t)s->nblk_sums * s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->sums, s->p.meqn * sizeof(double)) == hipSuccess;
@@ -261,12 +270,13 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
             ok = hipMemsetAsync(s->buf[k], 0, s->n * sizeof(double), s->st) == hipSuccess;
         ok = ok && hipMemsetAsync(s->qb, 0, s->n * sizeof(double), s->st) == hipSuccess;
         ok = ok && hipMemsetAsync(s->dq, 0, s->n * sizeof(double), s->st) == hipSuccess;
+        if (s->dqx) ok = ok && hipMemsetAsync(s->dqx, 0, s->n * sizeof(double), s->st) == hipSuccess;
         ok = ok && hipMemsetAsync(s->slots, 0, NSLOT * CFL_WORDS * sizeof(unsigned long long), s->st) == hipSuccess;
         ok = ok && hipStreamSynchronize(s->st) == hipSuccess;
     }
     if (!ok) {
         sc_destroy(s);
-        return sc_fail(SC_ERR_NOMEM, "device allocation of %zu doubles x 5 failed", grid_n(params));
+        return sc_fail(SC_ERR_NOMEM, "device allocation of %zu doubles x %d failed", grid_n(params), narr);
     }
     s->cur = 0;
     s->old = 0;
@@ -544,6 +554,7 @@ static StageArgs base_args(const sc_solver *s)
 static hipError_t run_stage(sc_solver *s, const StageArgs &a)
 {
     if (s->p.ndim == 1) return launch_stage1d(s->sel, 1, a, s->cd, s->cf, s->st);
+    if (s->dqx) return launch_sweep2d(s->sel, a, s->dqx, s->cd, s->cf, s->st);
     return launch_stage2d(s->sel, s->tile2d < 0 ? 0 : s->tile2d, a, s->cd, s->cf, s->st);
 }
 
@@ -557,7 +568,7 @@ static std::map<std::tuple<int, int, int, int, int>, int> g_tune;
 
 static int tune_tile(sc_solver *s)
 {
-    if (s->p.ndim != 2 || s->tile2d >= 0) return SC_OK;
+    if (s->p.ndim != 2 || s->tile2d >= 0 || s->dqx) return SC_OK;
     // one tuning per size class (powers of two per direction), so grids of
     // many sizes (served requests) do not each pay for a tuning
     auto lg = [](int n) { int k = 0; while ((2 << k) <= n) k++; return k; };
@@ -855,7 +866,8 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
         tile2d_shape(s->tile2d, &tx, &ty);
         long long tiles = (long long)((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty);
         // 2D advection steps its velocities per step on the host (b4step)
-        bool device = s->sel.phys != ADV2 && (s->loop_mode == 1 || (s->loop_mode < 0 && tiles < 2048));
+        // (the slice sweeps have no device-loop kernel)
+        bool device = s->sel.phys != ADV2 && !s->dqx && (s->loop_mode == 1 || (s->loop_mode < 0 && tiles < 2048));
         if (device) return advance2d_device(s, c, tstart, tend, dt_io, st);
     }
     memset(st, 0, sizeof *st);
@@ -1160,9 +1172,12 @@ extern "C" int sc_flux1(const sc_params *params, int ixy, int mx, const double *
         return sc_fail(SC_ERR_ARG, "bad argument");
     if (params->mcapa > 0)
         return sc_fail(SC_ERR_UNSUPPORTED, "sc_flux1 takes no aux array: capacity runs go through sc_set_aux");
+    PhysSel sel{sc_phys_id(params), params->char_decomp, params->precision == SC_FP32, params->lim_type};
+    if (params->ndim == 2 && sweep2d_path(sel))
+        return sc_fail(SC_ERR_UNSUPPORTED, "sc_flux1 on 2D slices reconstructs with weno5 / weno5_char; the "
+                                           "other reconstructions run whole 2D stages (sc_step, sc_ts, sc_advance)");
     DevGuard g(params->device);
     const int M = params->meqn, L = mx + 2 * params->mbc;
-    PhysSel sel{sc_phys_id(params), params->char_decomp, params->precision == SC_FP32, params->lim_type};
     DevBuf dq, ddq;
     HIPCHK(dq.alloc((size_t)L * M));
     HIPCHK(ddq.alloc((size_t)L * M));

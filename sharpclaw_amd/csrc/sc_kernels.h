@@ -60,6 +60,11 @@ struct PhysSel {
 
 hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const PhysConst<double> &cd,
                           const PhysConst<float> &cf, hipStream_t st);
+// 2D stage by x and y slice sweeps (the reconstructions the tile kernel does
+// not stage: lim_type 1, char_decomp 1 or 3); dqx: q-sized scratch
+bool sweep2d_path(const PhysSel &s);
+hipError_t launch_sweep2d(const PhysSel &s, const StageArgs &a, double *dqx, const PhysConst<double> &cd,
+                          const PhysConst<float> &cf, hipStream_t st);
 // 2D tile shapes selectable per launch (index into a fixed table)
 constexpr int NTILE2D = 4;
 hipError_t launch_stage2d(const PhysSel &s, int tile, const StageArgs &a, const PhysConst<double> &cd,

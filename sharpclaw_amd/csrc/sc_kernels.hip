@@ -35,6 +35,32 @@ __global__ __launch_bounds__(NT) void stage1d_kernel(StageArgs a, PhysConst<T> c
     if (a.smax) merge_smax<T, NT / 64>(a.smax, smax, red);
 }
 
+// ------------------------------------------------- 2D stage by slices
+
+// x sweeps (IXY = 1) over rows 0..ny+1 or y sweeps (IXY = 2) over columns
+// 0..nx+1 of a 2D stage (sweep2d_chunk), one block per (chunk, slice).
+// Workgroups are dealt to the 8 XCDs round-robin; the block index is
+// remapped so that each XCD takes a contiguous band of (chunk, slice)
+// pairs, slice fastest: neighbouring columns of a y sweep, which share
+// every cache line they load and store, then meet in one XCD's L2.
+template <int P, int IXY, int CHAR, class T, int NT>
+__global__ __launch_bounds__(NT) void sweep2d_kernel(StageArgs a, PhysConst<T> c, double *dqx)
+{
+    constexpr int M = Shape<P>::M;
+    constexpr int NB = nbr_words<P, IXY, CHAR, T>();
+    __shared__ T sq[M][NT + 5];
+    __shared__ T sx[2 * M][NT];
+    __shared__ T sw[NB > 0 ? NB : 1][NT + 4];
+    __shared__ double red[NT / 64];
+    const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7;
+    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int b = xcd < r8 ? xcd * (q8 + 1) + k : r8 * (q8 + 1) + (xcd - r8) * q8 + k;
+    const int nsl = IXY == 1 ? a.ny + 2 : a.nx + 2;
+    const int c0 = 1 + (b / nsl) * (NT - 1), sl = b % nsl;
+    T smax = sweep2d_chunk<P, IXY, CHAR, T, NT>(a, c, dqx, c0, sl, sq, sx, &sw[0][0]);
+    if (a.smax) merge_smax<T, NT / 64>(a.smax + (IXY - 1), smax, red);
+}
+
 // ------------------------------------------------------ 2D fused stage
 
 template <int P, int CHAR, class T, int TX, int TY>
@@ -323,6 +349,54 @@ hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const P
     case EU2: return ixy == 1 ? go1c<EU2, 1>(s, a, cd, cf, st) : go1c<EU2, 2>(s, a, cd, cf, st);
     case SW2: return ixy == 1 ? go1c<SW2, 1>(s, a, cd, cf, st) : go1c<SW2, 2>(s, a, cd, cf, st);
     }
+    return hipErrorInvalidValue;
+}
+
+// the slice sweeps run 128 threads a block: a 1024-cell line is 9 chunks
+constexpr int NTS = 128;
+
+template <int P, int CHAR, class T>
+static hipError_t gos(const StageArgs &a, const PhysConst<T> &c, double *dqx, hipStream_t st)
+{
+    // blocks: chunks x slices (sweep2d_kernel maps the index to both)
+    dim3 gx(((a.nx + NTS - 2) / (NTS - 1)) * (a.ny + 2)), gy(((a.ny + NTS - 2) / (NTS - 1)) * (a.nx + 2));
+    hipLaunchKernelGGL((sweep2d_kernel<P, 1, CHAR, T, NTS>), gx, dim3(NTS), 0, st, a, c, dqx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((sweep2d_kernel<P, 2, CHAR, T, NTS>), gy, dim3(NTS), 0, st, a, c, dqx);
+    return hipGetLastError();
+}
+
+template <int P, int CHAR>
+static hipError_t gosp(const PhysSel &s, const StageArgs &a, double *dqx, const PhysConst<double> &cd,
+                       const PhysConst<float> &cf, hipStream_t st)
+{
+    return s.fp32 ? gos<P, CHAR, float>(a, cf, dqx, st) : gos<P, CHAR, double>(a, cd, dqx, st);
+}
+
+template <int P>
+static hipError_t gosc(const PhysSel &s, const StageArgs &a, double *dqx, const PhysConst<double> &cd,
+                       const PhysConst<float> &cf, hipStream_t st)
+{
+    if (s.lim == 1) {
+        if (s.chr == 1) return gosp<P, RK_TVD_WAVE>(s, a, dqx, cd, cf, st);
+        if (s.chr == 2) return gosp<P, RK_TVD_CHAR>(s, a, dqx, cd, cf, st);
+        return hipErrorInvalidValue;
+    }
+    if (s.lim != 2) return hipErrorInvalidValue;
+    if (s.chr == 1) return gosp<P, RK_WENO_WAVE>(s, a, dqx, cd, cf, st);
+    if (s.chr == 3) return gosp<P, RK_WENO_TRANS>(s, a, dqx, cd, cf, st);
+    return hipErrorInvalidValue;  // weno5 / weno5_char: the tile kernel
+}
+
+bool sweep2d_path(const PhysSel &s) { return s.lim != 2 || s.chr == 1 || s.chr == 3; }
+
+hipError_t launch_sweep2d(const PhysSel &s, const StageArgs &a, double *dqx, const PhysConst<double> &cd,
+                          const PhysConst<float> &cf, hipStream_t st)
+{
+    if (a.nx < 1 || a.ny < 1 || !dqx) return hipErrorInvalidValue;
+    if (s.phys == EU2) return gosc<EU2>(s, a, dqx, cd, cf, st);
+    if (s.phys == SW2) return gosc<SW2>(s, a, dqx, cd, cf, st);
     return hipErrorInvalidValue;
 }
 

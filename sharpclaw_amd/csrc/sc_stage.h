@@ -70,7 +70,7 @@ template <class T> __device__ __forceinline__ T wave_max(T v)
 // workgroups per launch on one address queue up behind each other (tens of
 // microseconds per launch at 1024^2), so each word has CFL_SUB copies
 // CFL_STRIDE words (256 bytes) apart and workgroup b merges into copy
-// b mod CFL_SUB.  A CFL set holds the x word (offset 0) and the y word
+// b mod CFL_SUB (b = blockIdx.x + blockIdx.y).  A CFL set holds the x word (offset 0) and the y word
 // (offset 1) with their copies, CFL_WORDS words; a reader takes the max over
 // the copies (cfl_word).
 constexpr int CFL_SUB = 32, CFL_STRIDE = 32, CFL_WORDS = CFL_SUB * CFL_STRIDE;
@@ -100,7 +100,7 @@ __device__ __forceinline__ void merge_smax(unsigned long long *slot, T v, double
 #pragma unroll
         for (int k = 1; k < NW; k++) m = fmax(m, red[k]);
         if (m > 0.0)
-            atomicMax(slot + (blockIdx.x % CFL_SUB) * CFL_STRIDE, (unsigned long long)__double_as_longlong(m));
+            atomicMax(slot + ((blockIdx.x + blockIdx.y) % CFL_SUB) * CFL_STRIDE, (unsigned long long)__double_as_longlong(m));
     }
 }
 
@@ -347,36 +347,35 @@ __device__ __forceinline__ void reconstruct_other(const PhysConst<T> &c, const T
 
 // ------------------------------------------------------ 1D fused stage
 
-// One chunk of a 1D stage for a block of NT threads: interfaces
-// i = c0..c0+NT-1 (thread t owns interface c0+t) and the NT-1 cells
-// c0..c0+NT-2, whose updated stage values are written to a.qout.  The cells
-// c0-3..c0+NT+1 are staged in sq (with boundary ghost cells synthesised);
-// interface results pass to the neighbouring thread through sx.  Returns
-// this thread's max |s| over interfaces 1..n+1 (flux1.f90:126-133).
-// Leaves with a barrier, so sq/sx can be reused by the next chunk.
-// CHAR is the reconstruction code (RK_*); the codes with neighbour data
-// need sw, nbr_words() rows of NT+4 words.
-template <int P, int IXY, int CHAR, class T, int NT>
-__device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T> &c, int c0,
-                                           T (*sq)[NT + 5], T (*sx)[NT], T *sw = nullptr)
+// One chunk of a line of n cells (a 1D grid, or one row or column of a 2D
+// grid as flux2.f90 passes it to flux1) for a block of NT threads:
+// interfaces i = c0..c0+NT-1 (thread t owns interface c0+t) and the NT-1
+// cells c0..c0+NT-2.  load(j, v) gives the M values of line cell j =
+// -2..n+3 (ghost cells resolved by the caller); the cells c0-3..c0+NT+1 are
+// staged in sq and interface results pass to the neighbouring thread
+// through sx.  Each cell i <= n of the chunk is handed to emit(i, X, qc)
+// with its flux difference X = amdq(i+1) + apdq(i) + amdq2(i) + apdq2(i)
+// (flux1.f90:183-186; 150-152 with tfluct) and its staged values qc.
+// capa: the line's capacity aux(:,mcapa) in the line's q layout, or NULL.
+// Returns this thread's max |s| (max(s dtdx(i), -s dtdx(i-1)) / (dt/dx)
+// with a capacity) over interfaces 1..n+1 (flux1.f90:126-133).  Leaves
+// with a barrier, so sq/sx can be reused by the next chunk.  CHAR is the
+// reconstruction code (RK_*); the codes with neighbour data need sw,
+// nbr_words() rows of NT+4 words.
+template <int P, int IXY, int CHAR, class T, int NT, class Load, class Emit>
+__device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, const double *capa,
+                                        T (*sq)[NT + 5], T (*sx)[NT], T *sw, Load load, Emit emit)
 {
     constexpr int M = Shape<P>::M, W = Shape<P>::W;
     constexpr int NC = NT - 1;
     constexpr int NL = NT + 5;
-    constexpr int MN = IXY == 1 ? 1 : 2;  // normal momentum for a wall
     const int t = threadIdx.x;
-    const int n = a.nx;
 
     for (int k = t; k < NL; k += NT) {
-        int j = min(max(c0 - 3 + k, -2), n + 3);
-        bool neg;
-        int js = bc_src(j, n, a.bc[0], a.bc[1], neg);
+        T val[M];
+        load(min(max(c0 - 3 + k, -2), n + 3), val);
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            double val = a.qin[(size_t)(js + 2) + (size_t)m * a.LX];
-            if (m == MN && neg) val = -val;
-            sq[m][k] = T(val);
-        }
+        for (int m = 0; m < M; m++) sq[m][k] = val[m];
     }
     __syncthreads();
 
@@ -409,10 +408,10 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
     riemann<P, IXY, T>(c, lf, rt, wave, s, am, ap);
     T smax = T(0);
     if (i <= n + 1) {
-        if (a.capa) {
+        if (capa) {
             // flux1.f90:128-132 with dtdx(i) = dt/(dx*capa(i)): s > 0 is
             // scaled by the right cell's capacity, s < 0 by the left one's
-            const T kr = T(1) / T(a.capa[i + 2]), kl = T(1) / T(a.capa[i + 1]);
+            const T kr = T(1) / T(capa[i + 2]), kl = T(1) / T(capa[i + 1]);
 #pragma unroll
             for (int w = 0; w < W; w++) smax = fmax(smax, fmax(s[w] * kr, -(s[w] * kl)));
         } else {
@@ -443,6 +442,47 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
             incell_flux_diff<P, IXY, T>(c, rt, qrc, adq);
         else
             riemann<P, IXY, T>(c, rt, qrc, wave, s, am2, ap2);
+        double X[M];
+        T qc[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            // flux1.f90:150-152 (tfluct: the in-cell problem's total
+            // fluctuation adq = amdq + apdq, tfluct*.f of the applications)
+            // or flux1.f90:183-186 (the same problem through rp, swapped)
+            if constexpr (BYFLUX)
+                X[m] = c.tfluct ? ((double)ap[m] + adq[m]) + (double)sx[M + m][t + 1]
+                                : (double)(sx[M + m][t + 1] + ap[m]) + adq[m];
+            else
+                X[m] = c.tfluct ? (double)((ap[m] + (am2[m] + ap2[m])) + sx[M + m][t + 1])
+                                : (double)(((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m]);
+            qc[m] = sq[m][t + 3];
+        }
+        emit(i, X, qc);
+    }
+    __syncthreads();
+    return smax;
+}
+
+// One chunk of a 1D stage: line_chunk on the grid a.qin with the boundary
+// ghosts of bc1.f90, updated stage values written to a.qout.
+template <int P, int IXY, int CHAR, class T, int NT>
+__device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T> &c, int c0,
+                                           T (*sq)[NT + 5], T (*sx)[NT], T *sw = nullptr)
+{
+    constexpr int M = Shape<P>::M;
+    constexpr int MN = IXY == 1 ? 1 : 2;  // normal momentum for a wall
+    const int n = a.nx;
+    auto load = [&](int j, T (&v)[M]) {
+        bool neg;
+        const int js = bc_src(j, n, a.bc[0], a.bc[1], neg);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            double val = a.qin[(size_t)(js + 2) + (size_t)m * a.LX];
+            if (m == MN && neg) val = -val;
+            v[m] = T(val);
+        }
+    };
+    auto emit = [&](int i, const double (&X)[M], const T (&qc)[M]) {
         const bool rq = op_reads_qold(a.op);
         // dtdx(i) = dt/(dx*capa(i)) with a capacity (flux1.f90:56-61)
         const double dtdx = a.capa ? a.dtdx / a.capa[i + 2] : a.dtdx;
@@ -461,20 +501,10 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
         }
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            // flux1.f90:150-152 (tfluct: the in-cell problem's total
-            // fluctuation adq = amdq + apdq, tfluct*.f of the applications)
-            // or flux1.f90:183-186 (the same problem through rp, swapped)
-            double X;
-            if constexpr (BYFLUX)
-                X = c.tfluct ? ((double)ap[m] + adq[m]) + (double)sx[M + m][t + 1]
-                             : (double)(sx[M + m][t + 1] + ap[m]) + adq[m];
-            else
-                X = c.tfluct ? (double)((ap[m] + (am2[m] + ap2[m])) + sx[M + m][t + 1])
-                             : (double)(((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m]);
-            double r = sv[m] - dtdx * X;
+            double r = sv[m] - dtdx * X[m];
             size_t g = (size_t)(i + 2) + (size_t)m * a.LX;
             // float64 chunks hold q itself
-            double q = std::is_same<T, double>::value ? (double)sq[m][t + 3] : a.qin[g];
+            double q = std::is_same<T, double>::value ? (double)qc[m] : a.qin[g];
             double qo = rq ? a.qold[g] : 0.0;
             double qbi = a.op == OP_SSP104_10 ? a.qbin[g] : 0.0;
             double out, qbo = 0.0;
@@ -482,9 +512,70 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
             a.qout[g] = out;
             if (a.op == OP_SSP104_5) a.qbout[g] = qbo;
         }
-    }
-    __syncthreads();
-    return smax;
+    };
+    return line_chunk<P, IXY, CHAR, T, NT>(c, c0, n, a.capa, sq, sx, sw, load, emit);
+}
+
+// ------------------------------------------------- 2D stage by slices
+
+// The 2D stage as the reference's flux2 forms it (flux2.f90:36-92): x sweeps
+// over the rows j = 0..ny+1, then y sweeps over the columns i = 0..nx+1,
+// each a 1D line through line_chunk.  This is the path of the
+// reconstructions whose interfaces need data of their neighbouring
+// interfaces (weno5_wave, weno5_trans, tvd2_wave, tvd2_char), which the
+// tile kernel does not stage.  The block takes the chunk starting at cell
+// c0 of slice sl.  The x sweep (IXY = 1) writes
+// dqx = -(dtdx X) of the interior cells; the y sweep adds -(dtdy Y) and
+// combines the stage.  Ghost cells follow bc2.f90: x ghosts of every row
+// first, then the y ghosts of every column, so a corner cell takes both
+// maps (and both negations with walls).  The ghost slices j = 0, ny+1 and
+// i = 0, nx+1 feed only the CFL, as in the reference.
+template <int P, int IXY, int CHAR, class T, int NT>
+__device__ __forceinline__ T sweep2d_chunk(const StageArgs &a, const PhysConst<T> &c, double *dqx, int c0,
+                                           int sl, T (*sq)[NT + 5], T (*sx)[NT], T *sw)
+{
+    constexpr int M = Shape<P>::M;
+    // sl: row j (x sweeps) or column i (y sweeps)
+    const int n = IXY == 1 ? a.nx : a.ny;
+    const int nother = IXY == 1 ? a.ny : a.nx;
+    const size_t plane = (size_t)a.LX * (size_t)a.LY;
+    auto load = [&](int k, T (&v)[M]) {
+        const int i = IXY == 1 ? k : sl, j = IXY == 1 ? sl : k;
+        bool negx, negy;
+        const int is = bc_src(i, a.nx, a.bc[0], a.bc[1], negx);
+        const int js = bc_src(j, a.ny, a.bc[2], a.bc[3], negy);
+        const size_t g = (size_t)(is + 2) + (size_t)a.LX * (size_t)(js + 2);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            double val = a.qin[g + (size_t)m * plane];
+            if ((m == 1 && negx) || (m == 2 && negy)) val = -val;
+            v[m] = T(val);
+        }
+    };
+    auto emit = [&](int k, const double (&X)[M], const T (&qc)[M]) {
+        if (sl < 1 || sl > nother) return;
+        const int i = IXY == 1 ? k : sl, j = IXY == 1 ? sl : k;
+        const size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
+        if constexpr (IXY == 1) {
+#pragma unroll
+            for (int m = 0; m < M; m++) dqx[g + (size_t)m * plane] = -(a.dtdx * X[m]);
+        } else {
+            const bool rq = op_reads_qold(a.op);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const size_t gm = g + (size_t)m * plane;
+                double r = dqx[gm] + -(a.dtdy * X[m]);
+                double q = std::is_same<T, double>::value ? (double)qc[m] : a.qin[gm];
+                double qo = rq ? a.qold[gm] : 0.0;
+                double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
+                double out, qbo = 0.0;
+                combine(a.op, qo, q, r, qbi, out, qbo);
+                a.qout[gm] = out;
+                if (a.op == OP_SSP104_5) a.qbout[gm] = qbo;
+            }
+        }
+    };
+    return line_chunk<P, IXY, CHAR, T, NT>(c, c0, n, nullptr, sq, sx, sw, load, emit);
 }
 
 }  // namespace sc

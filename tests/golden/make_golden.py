@@ -33,7 +33,7 @@ def tag(name, ndim):
 
 def save(fname, **arrays):
     np.savez_compressed(os.path.join(HERE, fname), **arrays)
-    print("wrote", fname, {k: getattr(v, "shape", v) for k, v in arrays.items()})
+    print("wrote", fname, len(arrays), "arrays")
 
 
 def routines():
@@ -243,6 +243,64 @@ def recon():
              params=np.array([char, lim, mth, ti] + list(bc)), xl=np.array(xl), xu=np.array(xu))
 
 
+def recon2d():
+    """The reconstructions of recon() on 2D grids (flux2.f90 calls flux1 per
+    row and column): Euler and shallow water, one step per case, and a run
+    of each physics.  Square grids only: on a grid with nx != ny the
+    reference's rp and evec index the slice with the leading dimension of
+    the larger direction (maxnx) and mis-index.  No 2D Euler tvd2_char: with
+    meqn 4 > mwaves 3 it reads mthlim(4) past the end of mthlim.  Runs with
+    the shallow-water pulse and four Euler quadrants."""
+    rng = np.random.default_rng(20261207)
+    out = {}
+    k = 0
+    for name in ("euler", "shallow_water"):
+        for lim, char in RECON:
+            if name == "euler" and (lim, char) == (1, 2):
+                continue
+            for mth in ((1, 2, 3, 4, 5) if lim == 1 else (5,)):
+                bcs = [(1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3), (1, 3, 2, 2)]
+                for bc in (bcs if mth == 5 else [bcs[mth % len(bcs)]]):
+                    p = make_params(name, (12, 12), char_decomp=char, mthbc=bc, lim_type=lim,
+                                    mthlim=(mth,) * 4)
+                    q = _states.grid(p, rng, smooth=True)
+                    dt = 0.1 * p.dx[0]
+                    qb, dq, cfl = R.step(p, q, dt)
+                    out.update({f"c{k}_name": np.array(name), f"c{k}_q": q, f"c{k}_qbc": qb,
+                                f"c{k}_dq": dq, f"c{k}_cfl": np.array(cfl), f"c{k}_dt": np.array(dt),
+                                f"c{k}_bc": np.array(bc), f"c{k}_char": np.array(char),
+                                f"c{k}_lim": np.array(lim), f"c{k}_mthlim": np.array(mth)})
+                    k += 1
+    out["n"] = np.array(k)
+    save("step_recon2d.npz", **out)
+    rng = np.random.default_rng(8)
+    st = np.stack([rng.uniform(0.5, 1.5, 4), rng.uniform(-0.5, 0.5, 4), rng.uniform(-0.5, 0.5, 4),
+                   rng.uniform(0.5, 1.5, 4)], axis=1)
+    quad = ["1", "0.5 0.5"] + [" ".join(repr(float(v)) for v in row) for row in st]
+    runs_ = [
+        # shallow-water pulse as run_sw2d_pulse, weno5_wave, SSPRK3
+        ("run_sw2d_pulse_wave.npz", "shallow_water", (24, 24), 1, 2, 5, 3, (2, 2, 2, 2), 0.06,
+         1e-4, 0.5, 0.45, ["2", "0.3 60.0 0.4 0.55"]),
+        # (no tvd2_char run: the program aborts in free() after one, as in 1D)
+        # four quadrants (moderate jumps), weno5_trans, SSPRK3
+        ("run_eu2d_quad_trans.npz", "euler", (24, 24), 3, 2, 5, 3, (1, 1, 1, 1), 0.1, 1e-3, 0.5,
+         0.45, quad),
+        # four quadrants, tvd2_wave with van Leer, SSPRK3
+        ("run_eu2d_quad_tvd2wave.npz", "euler", (24, 24), 1, 1, 3, 3, (1, 1, 1, 1), 0.1, 1e-3,
+         0.5, 0.45, quad),
+    ]
+    for (fname, name, nx, char, lim, mth, ti, bc, tf, dt0, cmax, cdes, lines) in runs_:
+        xl, xu = (0.0, 0.0), (1.0, 1.0)
+        p = make_params(name, nx, xlower=xl, xupper=xu, char_decomp=char, lim_type=lim,
+                        mthlim=(mth,) * 4, time_integrator=ti, mthbc=bc)
+        cd = R.clawdata_for(p, nout=1, outstyle=1, tfinal=tf, dt_initial=dt0, cfl_max=cmax,
+                            cfl_desired=cdes)
+        fr, dtl, info = R.run(p, cd, lines)
+        save(fname, t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]), dtlog=dtl,
+             ctl=np.array([1e99, cmax, cdes, 50000, 1]), dt_initial=np.array(dt0),
+             params=np.array([char, lim, mth, ti] + list(bc)), xl=np.array(xl), xu=np.array(xu))
+
+
 def kappa_aux(mx, maux, mcapa, xl, xu):
     """A smooth capacity profile in column mcapa of aux(1-mbc:mx+mbc, maux)."""
     dx = (xu - xl) / mx
@@ -349,7 +407,7 @@ def swirl():
 
 
 PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct, "flush": flush,
-         "recon": recon, "aux": aux, "swirl": swirl}
+         "recon": recon, "recon2d": recon2d, "aux": aux, "swirl": swirl}
 
 if __name__ == "__main__":
     if not R.ref_available():

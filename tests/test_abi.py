@@ -54,7 +54,8 @@ def test_invalid_options_rejected_before_device_work():
         dict(physics="advection", nx=(10,), mthbc=(3, 3, 1, 1)),   # wall needs component 2
         dict(physics="euler", nx=(10,), mthbc=(2, 1, 1, 1)),       # periodic on one side only
         dict(physics="euler", nx=(10,), time_integrator=7),
-        dict(physics="euler", nx=(10, 10), char_decomp=1),         # weno5_wave: 1D grids only
+        dict(physics="advection", nx=(10, 10), maux=2, char_decomp=1),  # 2D advection: weno5 only
+        dict(physics="euler", nx=(10, 10), lim_type=1, char_decomp=2, mthlim=(1, 1, 1, 0)),  # 4 fields
         dict(physics="euler", nx=(10,), lim_type=1, char_decomp=0),  # tvd2: undefined first slope
         dict(physics="euler", nx=(10,), lim_type=1, char_decomp=2, mthlim=(9, 1, 1, 1)),
         dict(physics="euler", nx=(10,), lim_type=0),                 # q2qlqr_poly

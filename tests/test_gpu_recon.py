@@ -162,8 +162,7 @@ def test_recon_options_rejected_where_not_built():
            make_params("euler", (16,), lim_type=1, char_decomp=0),
            make_params("euler", (16,), lim_type=1, char_decomp=3),
            make_params("euler", (16,), lim_type=1, char_decomp=2, mthlim=(6, 1, 1, 1)),
-           make_params("euler", (16, 16), char_decomp=3),
-           make_params("shallow_water", (16, 16), lim_type=1, char_decomp=2),
+           make_params("advection", (16, 16), char_decomp=3, maux=2),
            make_params("euler", (16,), char_decomp=4)]
     for p in bad:
         with pytest.raises(ScError):

@@ -203,6 +203,34 @@ def test_run_recon(fname, name, nx):
     _replay(_recon_params(load(fname), name, nx), fname)
 
 
+def test_step_recon2d_bit_exact():
+    """The same reconstructions on 2D grids (flux2.f90 slices): Euler and
+    shallow water on square grids, every limiter."""
+    d = load("step_recon2d.npz")
+    seen = set()
+    for k in range(int(d["n"])):
+        name, lim, char = str(d[f"c{k}_name"]), int(d[f"c{k}_lim"]), int(d[f"c{k}_char"])
+        mth = int(d[f"c{k}_mthlim"])
+        seen.add((name, lim, char, mth))
+        p = make_params(name, (12, 12), char_decomp=char, lim_type=lim, mthlim=(mth,) * 4,
+                        mthbc=tuple(int(b) for b in d[f"c{k}_bc"]))
+        qb, dq, cfl = O.step(p, d[f"c{k}_q"], float(d[f"c{k}_dt"]))
+        assert np.array_equal(qb, d[f"c{k}_qbc"]), k
+        assert np.array_equal(interior(p, dq), interior(p, d[f"c{k}_dq"])), (k, name, lim, char, mth)
+        assert cfl == float(d[f"c{k}_cfl"]), k
+    assert {(n, l, c) for n, l, c, _ in seen} == {(n, l, c) for n in ("euler", "shallow_water")
+                                                  for l, c in ((2, 1), (2, 3), (1, 1), (1, 2))
+                                                  if (n, l, c) != ("euler", 1, 2)}
+    assert {m for _, l, _, m in seen if l == 1} == {1, 2, 3, 4, 5}
+
+
+@pytest.mark.parametrize("fname,name", [("run_sw2d_pulse_wave.npz", "shallow_water"),
+                                        ("run_eu2d_quad_trans.npz", "euler"),
+                                        ("run_eu2d_quad_tvd2wave.npz", "euler")])
+def test_run_recon2d(fname, name):
+    _replay(_recon_params(load(fname), name, (24, 24)), fname)
+
+
 def test_step_aux_bit_exact():
     """Capacity (mcapa > 0, flux1.f90:56-61: dtdx(i) = dt/(dx*aux(i,mcapa)),
     also in the CFL) and the radial shallow-water source of
