@@ -38,27 +38,30 @@ __global__ __launch_bounds__(NT) void stage1d_kernel(StageArgs a, PhysConst<T> c
 // ------------------------------------------------- 2D stage by slices
 
 // x sweeps (IXY = 1) over rows 0..ny+1 or y sweeps (IXY = 2) over columns
-// 0..nx+1 of a 2D stage (sweep2d_chunk), one block per (chunk, slice).
-// Workgroups are dealt to the 8 XCDs round-robin; the block index is
-// remapped so that each XCD takes a contiguous band of (chunk, slice)
-// pairs, slice fastest: neighbouring columns of a y sweep, which share
-// every cache line they load and store, then meet in one XCD's L2.
-template <int P, int IXY, int CHAR, class T, int NT>
-__global__ __launch_bounds__(NT) void sweep2d_kernel(StageArgs a, PhysConst<T> c, double *dqx)
+// 0..nx+1 of a 2D stage (sweep2d_chunk): one block per (chunk, group of NLN
+// slices), NT interfaces per slice.  The x sweep runs one row per block
+// (its loads are contiguous along the row); the y sweep runs NLN = 8
+// neighbouring columns per block, so each staged row of the chunk is a
+// contiguous 64-byte load and each update a contiguous store.  Workgroups
+// are dealt to the 8 XCDs round-robin; the block index is remapped so that
+// each XCD takes a contiguous band of (chunk, group) pairs, group fastest:
+// neighbouring groups, which share cache lines, meet in one XCD's L2.
+template <int P, int IXY, int CHAR, class T, int NT, int NLN>
+__global__ __launch_bounds__(NT * NLN) void sweep2d_kernel(StageArgs a, PhysConst<T> c, double *dqx)
 {
     constexpr int M = Shape<P>::M;
     constexpr int NB = nbr_words<P, IXY, CHAR, T>();
-    __shared__ T sq[M][NT + 5];
-    __shared__ T sx[2 * M][NT];
-    __shared__ T sw[NB > 0 ? NB : 1][NT + 4];
-    __shared__ double red[NT / 64];
+    __shared__ T sq[M * (NT + 5) * NLN];
+    __shared__ T sx[2 * M * NT * NLN];
+    __shared__ T sw[NB > 0 ? NB * (NT + 4) * NLN : 1];
+    __shared__ double red[NT * NLN / 64];
     const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7;
     const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
     const int b = xcd < r8 ? xcd * (q8 + 1) + k : r8 * (q8 + 1) + (xcd - r8) * q8 + k;
-    const int nsl = IXY == 1 ? a.ny + 2 : a.nx + 2;
-    const int c0 = 1 + (b / nsl) * (NT - 1), sl = b % nsl;
-    T smax = sweep2d_chunk<P, IXY, CHAR, T, NT>(a, c, dqx, c0, sl, sq, sx, &sw[0][0]);
-    if (a.smax) merge_smax<T, NT / 64>(a.smax + (IXY - 1), smax, red);
+    const int ngrp = ((IXY == 1 ? a.ny : a.nx) + 2 + NLN - 1) / NLN;
+    const int c0 = 1 + (b / ngrp) * (NT - 1), sl0 = (b % ngrp) * NLN;
+    T smax = sweep2d_chunk<P, IXY, CHAR, T, NT, NLN>(a, c, dqx, c0, sl0, sq, sx, sw);
+    if (a.smax) merge_smax<T, NT * NLN / 64>(a.smax + (IXY - 1), smax, red);
 }
 
 // ------------------------------------------------------ 2D fused stage
@@ -352,18 +355,22 @@ hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const P
     return hipErrorInvalidValue;
 }
 
-// the slice sweeps run 128 threads a block: a 1024-cell line is 9 chunks
-constexpr int NTS = 128;
+// slice sweeps: x rows of 128 interfaces (a 1024-cell row is 9 chunks), y
+// groups of 8 columns x 32 interfaces
+constexpr int NTX = 128, NTY = 32, NLY = 8;
 
 template <int P, int CHAR, class T>
 static hipError_t gos(const StageArgs &a, const PhysConst<T> &c, double *dqx, hipStream_t st)
 {
-    // blocks: chunks x slices (sweep2d_kernel maps the index to both)
-    dim3 gx(((a.nx + NTS - 2) / (NTS - 1)) * (a.ny + 2)), gy(((a.ny + NTS - 2) / (NTS - 1)) * (a.nx + 2));
-    hipLaunchKernelGGL((sweep2d_kernel<P, 1, CHAR, T, NTS>), gx, dim3(NTS), 0, st, a, c, dqx);
+    // blocks: chunks x slice groups (sweep2d_kernel maps the index to both)
+    const long long nbx = (long long)((a.nx + NTX - 2) / (NTX - 1)) * (a.ny + 2);
+    const long long nby = (long long)((a.ny + NTY - 2) / (NTY - 1)) * ((a.nx + 2 + NLY - 1) / NLY);
+    if (nbx > 0x7fffffffLL || nby > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((sweep2d_kernel<P, 1, CHAR, T, NTX, 1>), dim3((unsigned)nbx), dim3(NTX), 0, st, a, c, dqx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((sweep2d_kernel<P, 2, CHAR, T, NTS>), gy, dim3(NTS), 0, st, a, c, dqx);
+    hipLaunchKernelGGL((sweep2d_kernel<P, 2, CHAR, T, NTY, NLY>), dim3((unsigned)nby), dim3(NTY * NLY), 0, st, a,
+                       c, dqx);
     return hipGetLastError();
 }
 

@@ -347,64 +347,73 @@ __device__ __forceinline__ void reconstruct_other(const PhysConst<T> &c, const T
 
 // ------------------------------------------------------ 1D fused stage
 
-// One chunk of a line of n cells (a 1D grid, or one row or column of a 2D
-// grid as flux2.f90 passes it to flux1) for a block of NT threads:
-// interfaces i = c0..c0+NT-1 (thread t owns interface c0+t) and the NT-1
-// cells c0..c0+NT-2.  load(j, v) gives the M values of line cell j =
-// -2..n+3 (ghost cells resolved by the caller); the cells c0-3..c0+NT+1 are
-// staged in sq and interface results pass to the neighbouring thread
-// through sx.  Each cell i <= n of the chunk is handed to emit(i, X, qc)
-// with its flux difference X = amdq(i+1) + apdq(i) + amdq2(i) + apdq2(i)
-// (flux1.f90:183-186; 150-152 with tfluct) and its staged values qc.
-// capa: the line's capacity aux(:,mcapa) in the line's q layout, or NULL.
-// Returns this thread's max |s| (max(s dtdx(i), -s dtdx(i-1)) / (dt/dx)
-// with a capacity) over interfaces 1..n+1 (flux1.f90:126-133).  Leaves
-// with a barrier, so sq/sx can be reused by the next chunk.  CHAR is the
-// reconstruction code (RK_*); the codes with neighbour data need sw,
-// nbr_words() rows of NT+4 words.
-template <int P, int IXY, int CHAR, class T, int NT, class Load, class Emit>
-__device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, const double *capa,
-                                        T (*sq)[NT + 5], T (*sx)[NT], T *sw, Load load, Emit emit)
+// One chunk of NLN lines of n cells each (a 1D grid, or rows or columns of
+// a 2D grid as flux2.f90 passes them to flux1) for a block of NT * NLN
+// threads: interfaces i = c0..c0+NT-1 of every line (thread t = p NLN + l
+// owns interface c0+p of line l) and the NT-1 cells c0..c0+NT-2.
+// load(l, j, v) gives the M values of cell j = -2..n+3 of line l (ghost
+// cells resolved by the caller); the cells c0-3..c0+NT+1 are staged in sq
+// and interface results pass to the neighbouring thread of the line
+// through sx.  Both hold line l's word k at k NLN + l, so the NLN lines of
+// one staged row are adjacent (the loads of a row of NLN neighbouring
+// columns are contiguous).  Each cell i <= n of the chunk is handed to
+// emit(l, i, X, qc) with its flux difference X = amdq(i+1) + apdq(i) +
+// amdq2(i) + apdq2(i) (flux1.f90:183-186; 150-152 with tfluct) and its
+// staged values qc.  capa: a single line's capacity aux(:,mcapa) in the
+// line's q layout, or NULL.  Returns this thread's max |s| (max(s dtdx(i),
+// -s dtdx(i-1)) / (dt/dx) with a capacity) over interfaces 1..n+1
+// (flux1.f90:126-133).  Leaves with a barrier, so the LDS arrays can be
+// reused by the next chunk.  CHAR is the reconstruction code (RK_*); the
+// codes with neighbour data need sw, NLN blocks of nbr_words() rows of
+// NT+4 words.  sq holds M (NT+5) NLN words, sx 2M NT NLN.
+template <int P, int IXY, int CHAR, class T, int NT, int NLN, class Load, class Emit>
+__device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, const double *capa, T *sq,
+                                        T *sx, T *sw, Load load, Emit emit)
 {
     constexpr int M = Shape<P>::M, W = Shape<P>::W;
     constexpr int NC = NT - 1;
     constexpr int NL = NT + 5;
-    const int t = threadIdx.x;
+    constexpr int NB = nbr_words<P, IXY, CHAR, T>();
+    auto SQ = [&](int m, int k, int l) -> T & { return sq[(m * NL + k) * NLN + l]; };
+    auto SX = [&](int m, int k, int l) -> T & { return sx[(m * NT + k) * NLN + l]; };
 
-    for (int k = t; k < NL; k += NT) {
+    for (int e = threadIdx.x; e < NL * NLN; e += NT * NLN) {
+        const int k = e / NLN, l = e % NLN;
         T val[M];
-        load(min(max(c0 - 3 + k, -2), n + 3), val);
+        load(l, min(max(c0 - 3 + k, -2), n + 3), val);
 #pragma unroll
-        for (int m = 0; m < M; m++) sq[m][k] = val[m];
+        for (int m = 0; m < M; m++) SQ(m, k, l) = val[m];
     }
     __syncthreads();
 
     constexpr int NW = NT + 4;
-    if constexpr (nbr_words<P, IXY, CHAR, T>() > 0) {
-        // interfaces c0-2..c0+NT+1: interface c0-2+k has cells sq[k], sq[k+1]
-        for (int k = t; k < NW; k += NT) {
+    if constexpr (NB > 0) {
+        // interfaces c0-2..c0+NT+1: interface c0-2+k has cells k, k+1
+        for (int e = threadIdx.x; e < NW * NLN; e += NT * NLN) {
+            const int k = e / NLN, l = e % NLN;
             T ql[M], qr[M];
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                ql[m] = sq[m][k];
-                qr[m] = sq[m][k + 1];
+                ql[m] = SQ(m, k, l);
+                qr[m] = SQ(m, k + 1, l);
             }
-            nbr_data<P, IXY, CHAR, T>(c, ql, qr, sw, NW, k);
+            nbr_data<P, IXY, CHAR, T>(c, ql, qr, sw + l * NB * NW, NW, k);
         }
         __syncthreads();
     }
 
+    const int t = threadIdx.x / NLN, ln = threadIdx.x % NLN;
     const int i = c0 + t;  // interface i-1/2
     T v[6][M];
 #pragma unroll
     for (int k = 0; k < 6; k++)
 #pragma unroll
-        for (int m = 0; m < M; m++) v[k][m] = sq[m][t + k];
+        for (int m = 0; m < M; m++) v[k][m] = SQ(m, t + k, ln);
     T lf[M], rt[M], wave[M * W], s[W], am[M], ap[M];
     if constexpr (CHAR == RK_WENO || CHAR == RK_WENO_CHAR)
         reconstruct_iface<P, IXY, CHAR, T>(c, v, lf, rt);
     else
-        reconstruct_other<P, IXY, CHAR, T>(c, v, sw, NW, t + 2, lf, rt);
+        reconstruct_other<P, IXY, CHAR, T>(c, v, sw + ln * NB * NW, NW, t + 2, lf, rt);
     riemann<P, IXY, T>(c, lf, rt, wave, s, am, ap);
     T smax = T(0);
     if (i <= n + 1) {
@@ -421,8 +430,8 @@ __device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, co
     }
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        sx[m][t] = lf[m];
-        sx[M + m][t] = am[m];
+        SX(m, t, ln) = lf[m];
+        SX(M + m, t, ln) = am[m];
     }
     __syncthreads();
 
@@ -437,7 +446,7 @@ __device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, co
         T qrc[M], am2[M], ap2[M];
         double adq[M];
 #pragma unroll
-        for (int m = 0; m < M; m++) qrc[m] = sx[m][t + 1];
+        for (int m = 0; m < M; m++) qrc[m] = SX(m, t + 1, ln);
         if constexpr (BYFLUX)
             incell_flux_diff<P, IXY, T>(c, rt, qrc, adq);
         else
@@ -449,15 +458,15 @@ __device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, co
             // flux1.f90:150-152 (tfluct: the in-cell problem's total
             // fluctuation adq = amdq + apdq, tfluct*.f of the applications)
             // or flux1.f90:183-186 (the same problem through rp, swapped)
+            const T amn = SX(M + m, t + 1, ln);
             if constexpr (BYFLUX)
-                X[m] = c.tfluct ? ((double)ap[m] + adq[m]) + (double)sx[M + m][t + 1]
-                                : (double)(sx[M + m][t + 1] + ap[m]) + adq[m];
+                X[m] = c.tfluct ? ((double)ap[m] + adq[m]) + (double)amn : (double)(amn + ap[m]) + adq[m];
             else
-                X[m] = c.tfluct ? (double)((ap[m] + (am2[m] + ap2[m])) + sx[M + m][t + 1])
-                                : (double)(((sx[M + m][t + 1] + ap[m]) + am2[m]) + ap2[m]);
-            qc[m] = sq[m][t + 3];
+                X[m] = c.tfluct ? (double)((ap[m] + (am2[m] + ap2[m])) + amn)
+                                : (double)(((amn + ap[m]) + am2[m]) + ap2[m]);
+            qc[m] = SQ(m, t + 3, ln);
         }
-        emit(i, X, qc);
+        emit(ln, i, X, qc);
     }
     __syncthreads();
     return smax;
@@ -472,7 +481,7 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
     constexpr int M = Shape<P>::M;
     constexpr int MN = IXY == 1 ? 1 : 2;  // normal momentum for a wall
     const int n = a.nx;
-    auto load = [&](int j, T (&v)[M]) {
+    auto load = [&](int, int j, T (&v)[M]) {
         bool neg;
         const int js = bc_src(j, n, a.bc[0], a.bc[1], neg);
 #pragma unroll
@@ -482,7 +491,7 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
             v[m] = T(val);
         }
     };
-    auto emit = [&](int i, const double (&X)[M], const T (&qc)[M]) {
+    auto emit = [&](int, int i, const double (&X)[M], const T (&qc)[M]) {
         const bool rq = op_reads_qold(a.op);
         // dtdx(i) = dt/(dx*capa(i)) with a capacity (flux1.f90:56-61)
         const double dtdx = a.capa ? a.dtdx / a.capa[i + 2] : a.dtdx;
@@ -513,7 +522,7 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
             if (a.op == OP_SSP104_5) a.qbout[g] = qbo;
         }
     };
-    return line_chunk<P, IXY, CHAR, T, NT>(c, c0, n, a.capa, sq, sx, sw, load, emit);
+    return line_chunk<P, IXY, CHAR, T, NT, 1>(c, c0, n, a.capa, &sq[0][0], &sx[0][0], sw, load, emit);
 }
 
 // ------------------------------------------------- 2D stage by slices
@@ -524,22 +533,24 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
 // reconstructions whose interfaces need data of their neighbouring
 // interfaces (weno5_wave, weno5_trans, tvd2_wave, tvd2_char), which the
 // tile kernel does not stage.  The block takes the chunk starting at cell
-// c0 of slice sl.  The x sweep (IXY = 1) writes
+// c0 of the NLN slices sl0..sl0+NLN-1 (slices past the last, n_other + 1,
+// repeat its loads and write nothing).  The x sweep (IXY = 1) writes
 // dqx = -(dtdx X) of the interior cells; the y sweep adds -(dtdy Y) and
 // combines the stage.  Ghost cells follow bc2.f90: x ghosts of every row
 // first, then the y ghosts of every column, so a corner cell takes both
 // maps (and both negations with walls).  The ghost slices j = 0, ny+1 and
 // i = 0, nx+1 feed only the CFL, as in the reference.
-template <int P, int IXY, int CHAR, class T, int NT>
+template <int P, int IXY, int CHAR, class T, int NT, int NLN>
 __device__ __forceinline__ T sweep2d_chunk(const StageArgs &a, const PhysConst<T> &c, double *dqx, int c0,
-                                           int sl, T (*sq)[NT + 5], T (*sx)[NT], T *sw)
+                                           int sl0, T *sq, T *sx, T *sw)
 {
     constexpr int M = Shape<P>::M;
-    // sl: row j (x sweeps) or column i (y sweeps)
+    // slices: rows j (x sweeps) or columns i (y sweeps)
     const int n = IXY == 1 ? a.nx : a.ny;
     const int nother = IXY == 1 ? a.ny : a.nx;
     const size_t plane = (size_t)a.LX * (size_t)a.LY;
-    auto load = [&](int k, T (&v)[M]) {
+    auto load = [&](int l, int k, T (&v)[M]) {
+        const int sl = min(sl0 + l, nother + 1);
         const int i = IXY == 1 ? k : sl, j = IXY == 1 ? sl : k;
         bool negx, negy;
         const int is = bc_src(i, a.nx, a.bc[0], a.bc[1], negx);
@@ -552,7 +563,8 @@ __device__ __forceinline__ T sweep2d_chunk(const StageArgs &a, const PhysConst<T
             v[m] = T(val);
         }
     };
-    auto emit = [&](int k, const double (&X)[M], const T (&qc)[M]) {
+    auto emit = [&](int l, int k, const double (&X)[M], const T (&qc)[M]) {
+        const int sl = sl0 + l;
         if (sl < 1 || sl > nother) return;
         const int i = IXY == 1 ? k : sl, j = IXY == 1 ? sl : k;
         const size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
@@ -575,7 +587,7 @@ __device__ __forceinline__ T sweep2d_chunk(const StageArgs &a, const PhysConst<T
             }
         }
     };
-    return line_chunk<P, IXY, CHAR, T, NT>(c, c0, n, nullptr, sq, sx, sw, load, emit);
+    return line_chunk<P, IXY, CHAR, T, NT, NLN>(c, c0, n, nullptr, sq, sx, sw, load, emit);
 }
 
 }  // namespace sc
