@@ -130,6 +130,32 @@ def test_driver_euler2d_quadrants():
         assert abs(float(ta[0]) - float(tb[0])) <= 1e-14 and ta[1:] == tb[1:]
 
 
+@pytest.mark.gpu
+@pytest.mark.skipif(not (have_driver("eu2d") and have_ref("eu2d")),
+                    reason="driver or reference build missing")
+def test_driver_euler2d_quadrants_tvd2wave():
+    """lim_type 1 / char_decomp 1 (tvd2_wave, van Leer) read from
+    sharpclaw.data: the 2D slice-sweep path behind the driver."""
+    p = make_params("euler", (32, 32), char_decomp=1, lim_type=1, mthlim=(3, 3, 3, 3),
+                    time_integrator=3, mthbc=(1, 1, 1, 1))
+    rng = np.random.default_rng(8)
+    st = np.stack([rng.uniform(0.5, 1.5, 4), rng.uniform(-0.5, 0.5, 4), rng.uniform(-0.5, 0.5, 4),
+                   rng.uniform(0.5, 1.5, 4)], axis=1)
+    lines = ["1", "0.5 0.5"] + [" ".join(repr(float(v)) for v in row) for row in st]
+    cd = R.clawdata_for(p, nout=2, outstyle=1, tfinal=0.06, dt_initial=1e-3, cfl_max=0.5,
+                        cfl_desired=0.45)
+    with tempfile.TemporaryDirectory() as tmp:
+        dg, dr = run_pair(p, cd, lines, "euler", "eu2d", tmp)
+        assert open(os.path.join(dg, "fort.q0000")).read() == open(os.path.join(dr, "fort.q0000")).read()
+        for f in ("fort.q0001", "fort.q0002"):
+            a, b = read_q(os.path.join(dg, f)), read_q(os.path.join(dr, f))
+            assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b)), f
+        for f in ("fort.t0001", "fort.t0002"):
+            ta = open(os.path.join(dg, f)).read().split()
+            tb = open(os.path.join(dr, f)).read().split()
+            assert abs(float(ta[0]) - float(tb[0])) <= 1e-14 and ta[1:] == tb[1:], f
+
+
 @pytest.mark.skipif(not have_driver("eu1d"), reason="driver not built")
 def test_driver_reports_errors_like_the_reference():
     """A failing library call stops the driver with the library's message

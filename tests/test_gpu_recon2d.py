@@ -17,6 +17,7 @@ import _states
 from conftest import rel_linf
 from oracle import oracle as O
 from sharpclaw_amd import SC_FP32, ScError, ScRunCtl, Server, Solver, interior, make_params
+from sharpclaw_amd import solver as S
 from test_gpu_parity import rel_l1, run_tolerance
 
 pytestmark = pytest.mark.gpu
@@ -158,11 +159,38 @@ def test_fp32_recon2d_l1(lim, char):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("env", [{"SC_HOST_LOOP": "0"}, {"SC_HOST_LOOP": "0", "SC_GRAPH": "1"}])
+def test_recon2d_ignores_device_loop_request(env, monkeypatch):
+    """The slice sweeps have no device-resident loop: a request for one (or
+    for its graph replay) runs the host loop, with the same result bit for
+    bit."""
+    p = make_params("shallow_water", (40, 40), char_decomp=3, time_integrator=3, mthbc=(1, 1, 3, 3))
+    q0 = bump(p, 0.2, 0.1)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=12, dt_variable=1)
+    res = []
+    for e in ({}, env):
+        for k in ("SC_HOST_LOOP", "SC_GRAPH"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in e.items():
+            monkeypatch.setenv(k, v)
+        with Solver(p) as s:
+            s.set_q(q0)
+            dt, st = s.advance(ctl, 0.0, 10.0, 1e-3)
+            res.append((s.get_q(), dt, st.steps_taken, st.rejected))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1:] == res[1][1:]
+
+
 def test_recon2d_rejected_where_not_built():
-    """2D advection keeps weno5; the server keeps weno5 / weno5_char."""
+    """2D advection keeps weno5; the server and sc_flux1 on 2D slices keep
+    weno5 / weno5_char."""
     for lim, char in RECON:
         with pytest.raises(ScError):
             Solver(make_params("advection", (16, 16), lim_type=lim, char_decomp=char, maux=2))
+        p = make_params("euler", (16, 16), lim_type=lim, char_decomp=char)
+        q1d = _states.random_cells(p, (22,), np.random.default_rng(3), smooth=True)
+        with pytest.raises(ScError):
+            S.flux1(p, 2, 16, q1d, 1e-3)
     with Server(hbm_bytes=64 << 20, max_active=2) as srv:
         pr = make_params("shallow_water", (32, 32), lim_type=1, char_decomp=1)
         ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=10, dt_variable=1)
