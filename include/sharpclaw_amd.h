@@ -59,10 +59,13 @@ typedef struct sc_params {
     int mbc;             /* must be 3 (reconstruct.f90:85,158 hard-code it) */
     int maux;            /* aux arrays per cell (1D grids; sc_set_aux) */
     int mcapa;           /* capacity column of aux, 1-based; 0 none (flux1.f90:56-61) */
-    int lim_type;        /* 2 WENO5; 1 second-order TVD (1D, single grid) */
-    int char_decomp;     /* lim_type 2: 0 weno5, 1 weno5_wave (1D), 2 weno5_char,
-                            3 weno5_trans (1D); lim_type 1: 1 tvd2_wave, 2 tvd2_char
-                            (reconstruct.f90:77-763, dispatch flux1.f90:65-117) */
+    int lim_type;        /* 2 WENO5; 1 second-order TVD (single grids: 1D, 2D Euler
+                            and shallow water) */
+    int char_decomp;     /* lim_type 2: 0 weno5, 1 weno5_wave, 2 weno5_char,
+                            3 weno5_trans; lim_type 1: 1 tvd2_wave, 2 tvd2_char
+                            (reconstruct.f90:77-763, dispatch flux1.f90:65-117).
+                            1 and 3, and lim_type 1, on single grids: 1D, and 2D
+                            Euler / shallow water (x and y slice sweeps) */
     int tfluct_solver;   /* 0 in-cell problem through rp (flux1.f90:155-188); 1 total
                             fluctuation (flux1.f90:136-153), 1D advection/acoustics/
                             shallow water only: the applications with a tfluct */
@@ -81,7 +84,8 @@ typedef struct sc_params {
     double rho, bulk;    /* acoustics (/cparam/: cc = sqrt(bulk/rho), zz = cc*rho) */
     int precision;       /* SC_FP64 or SC_FP32 */
     int device;          /* HIP device ordinal */
-    int mthlim[4];       /* limiter per wave family (sharpclaw.data mthlim), read
+    int mthlim[4];       /* limiter per wave family (sharpclaw.data mthlim; per
+                            characteristic field 1..meqn for tvd2_char), read
                             when lim_type = 1: 1 minmod, 2 superbee, 3 van Leer,
                             4 MC, 5 Cada-Torrilhon (reconstruct.f90:544-571) */
     double tperiod;      /* 2D advection (velocities in aux(:,:,1:2)): before each
