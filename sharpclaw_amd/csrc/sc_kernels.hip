@@ -27,11 +27,13 @@ __global__ __launch_bounds__(NT) void stage1d_kernel(StageArgs a, PhysConst<T> c
 {
     constexpr int M = Shape<P>::M;
     constexpr int NB = nbr_words<P, IXY, CHAR, T>();
+    constexpr bool AL = sx_in_sw<P, IXY, CHAR, T, NT>();
     __shared__ T sq[M][NT + 5];
-    __shared__ T sx[2 * M][NT];
+    __shared__ T sx[AL ? 1 : 2 * M][NT];
     __shared__ T sw[NB > 0 ? NB : 1][NT + 4];
     __shared__ double red[NT / 64];
-    T smax = stage1d_chunk<P, IXY, CHAR, T, NT>(a, c, 1 + blockIdx.x * (NT - 1), sq, sx, &sw[0][0]);
+    T(*sxp)[NT] = AL ? reinterpret_cast<T(*)[NT]>(&sw[0][0]) : sx;
+    T smax = stage1d_chunk<P, IXY, CHAR, T, NT>(a, c, 1 + blockIdx.x * (NT - 1), sq, sxp, &sw[0][0]);
     if (a.smax) merge_smax<T, NT / 64>(a.smax, smax, red);
 }
 
@@ -51,8 +53,9 @@ __global__ __launch_bounds__(NT * NLN) void sweep2d_kernel(StageArgs a, PhysCons
 {
     constexpr int M = Shape<P>::M;
     constexpr int NB = nbr_words<P, IXY, CHAR, T>();
+    constexpr bool AL = sx_in_sw<P, IXY, CHAR, T, NT>();
     __shared__ T sq[M * (NT + 5) * NLN];
-    __shared__ T sx[2 * M * NT * NLN];
+    __shared__ T sx[AL ? 1 : 2 * M * NT * NLN];
     __shared__ T sw[NB > 0 ? NB * (NT + 4) * NLN : 1];
     __shared__ double red[NT * NLN / 64];
     const int nb = gridDim.x, q8 = nb >> 3, r8 = nb & 7;
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(NT * NLN) void sweep2d_kernel(StageArgs a, PhysCons
     const int b = xcd < r8 ? xcd * (q8 + 1) + k : r8 * (q8 + 1) + (xcd - r8) * q8 + k;
     const int ngrp = ((IXY == 1 ? a.ny : a.nx) + 2 + NLN - 1) / NLN;
     const int c0 = 1 + (b / ngrp) * (NT - 1), sl0 = (b % ngrp) * NLN;
-    T smax = sweep2d_chunk<P, IXY, CHAR, T, NT, NLN>(a, c, dqx, c0, sl0, sq, sx, sw);
+    T smax = sweep2d_chunk<P, IXY, CHAR, T, NT, NLN>(a, c, dqx, c0, sl0, sq, AL ? sw : sx, sw);
     if (a.smax) merge_smax<T, NT * NLN / 64>(a.smax + (IXY - 1), smax, red);
 }
 

@@ -192,6 +192,15 @@ __device__ __forceinline__ void nbr_data(const PhysConst<T> &c, const T *ql, con
     }
 }
 
+// Whether a line chunk's exchange array sx (2M words per interface) lives in
+// the neighbour-data array sw (nbr_words per interface, NT+4 interfaces):
+// sw is dead once every interface is reconstructed, and sharing it keeps
+// the LDS of a block low enough for more blocks per CU.
+template <int P, int IXY, int RK, class T, int NT> constexpr bool sx_in_sw()
+{
+    return nbr_words<P, IXY, RK, T>() > 0 && nbr_words<P, IXY, RK, T>() * (NT + 4) >= 2 * Shape<P>::M * NT;
+}
+
 template <int P, int IXY, class T>
 __device__ __forceinline__ Eig<P, IXY, T> eig_at(const T *sw, int NW, int k)
 {
@@ -365,7 +374,9 @@ __device__ __forceinline__ void reconstruct_other(const PhysConst<T> &c, const T
 // (flux1.f90:126-133).  Leaves with a barrier, so the LDS arrays can be
 // reused by the next chunk.  CHAR is the reconstruction code (RK_*); the
 // codes with neighbour data need sw, NLN blocks of nbr_words() rows of
-// NT+4 words.  sq holds M (NT+5) NLN words, sx 2M NT NLN.
+// NT+4 words.  sq holds M (NT+5) NLN words, sx 2M NT NLN; when sx_in_sw()
+// the caller passes sx = sw, and a barrier separates the last reads of
+// the neighbour data from the first writes of sx.
 template <int P, int IXY, int CHAR, class T, int NT, int NLN, class Load, class Emit>
 __device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, const double *capa, T *sq,
                                         T *sx, T *sw, Load load, Emit emit)
@@ -416,6 +427,7 @@ __device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, co
         reconstruct_other<P, IXY, CHAR, T>(c, v, sw + ln * NB * NW, NW, t + 2, lf, rt);
     riemann<P, IXY, T>(c, lf, rt, wave, s, am, ap);
     T smax = T(0);
+    if constexpr (sx_in_sw<P, IXY, CHAR, T, NT>()) __syncthreads();  // sw is read: sx reuses it
     if (i <= n + 1) {
         if (capa) {
             // flux1.f90:128-132 with dtdx(i) = dt/(dx*capa(i)): s > 0 is
