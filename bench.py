@@ -122,8 +122,8 @@ def fp32_option_run(args, q0, q64):
     """The same run with float32 reconstruction and Riemann arithmetic
     (float64 state, accumulation, dt/CFL): cell-updates/s, the stage time,
     and the relative L1 of its result against the float64 run's (the float64
-    path matches the oracle to 1e-12 per step; the oracle itself would take
-    minutes per step at this size)."""
+    path matches the oracle to 1e-12 per step at this size; the float32
+    option's L1 against the oracle itself is a test)."""
     from sharpclaw_amd import SC_FP32, ScRunCtl, Solver, interior, make_params
     n = args.n
     p = make_params("euler", (n, n), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1),

@@ -293,6 +293,23 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     return SC_OK;
 }
 
+// The frame-output resources (sc_output_begin), released and reset
+// together, so that a partly failed creation is retried whole.
+static void output_free(sc_solver *s)
+{
+    if (s->cst) hipStreamSynchronize(s->cst);
+    if (s->snap) hipFree(s->snap);
+    if (s->hsnap) hipHostFree(s->hsnap);
+    if (s->ev_snap) hipEventDestroy(s->ev_snap);
+    if (s->ev_out) hipEventDestroy(s->ev_out);
+    if (s->cst) hipStreamDestroy(s->cst);
+    s->snap = nullptr;
+    s->hsnap = nullptr;
+    s->ev_snap = nullptr;
+    s->ev_out = nullptr;
+    s->cst = nullptr;
+}
+
 extern "C" int sc_destroy(sc_solver *s)
 {
     if (!s) return SC_OK;
@@ -309,14 +326,9 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->hslot) hipHostFree(s->hslot);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
-    if (s->cst) hipStreamSynchronize(s->cst);
     if (s->capa) hipFree(s->capa);
     if (s->aux2) hipFree(s->aux2);
-    if (s->snap) hipFree(s->snap);
-    if (s->hsnap) hipHostFree(s->hsnap);
-    if (s->ev_snap) hipEventDestroy(s->ev_snap);
-    if (s->ev_out) hipEventDestroy(s->ev_out);
-    if (s->cst) hipStreamDestroy(s->cst);
+    output_free(s);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
     return SC_OK;
@@ -364,13 +376,17 @@ extern "C" int sc_output_begin(sc_solver *s)
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     if (s->out_pending) return sc_fail(SC_ERR_STATE, "the previous frame has not been collected (sc_output_wait)");
     DevGuard g(s->dev);
-    if (!s->snap) {
-        bool ok = hipMalloc(&s->snap, s->n * sizeof(double)) == hipSuccess;
-        ok = ok && hipHostMalloc(&s->hsnap, s->n * sizeof(double), hipHostMallocDefault) == hipSuccess;
-        ok = ok && hipStreamCreateWithFlags(&s->cst, hipStreamNonBlocking) == hipSuccess;
-        ok = ok && hipEventCreateWithFlags(&s->ev_snap, hipEventDisableTiming) == hipSuccess;
-        ok = ok && hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming) == hipSuccess;
-        if (!ok) return sc_fail(SC_ERR_HIP, "frame output buffers: %s", hipGetErrorString(hipGetLastError()));
+    if (!s->snap || !s->hsnap || !s->cst || !s->ev_snap || !s->ev_out) {
+        output_free(s);
+        hipError_t e = hipMalloc(&s->snap, s->n * sizeof(double));
+        if (e == hipSuccess) e = hipHostMalloc(&s->hsnap, s->n * sizeof(double), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->cst, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_snap, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_out, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            output_free(s);
+            return sc_fail(SC_ERR_HIP, "frame output buffers: %s", hipGetErrorString(e));
+        }
     }
     const int LX = s->p.nx[0] + 6, LY = s->p.ndim == 2 ? s->p.nx[1] + 6 : 1;
     double *q = s->buf[s->cur];
@@ -632,14 +648,15 @@ extern "C" int sc_get_tile(sc_solver *s, int *tile, int *tx, int *ty)
     return SC_OK;
 }
 
-// next CFL set, cleared ring-wise
-static unsigned long long *next_slot(sc_solver *s)
+// next CFL set, cleared ring-wise: a set that is not cleared before its
+// reuse would keep the atomicMax of an earlier step
+static int next_slot(sc_solver *s, unsigned long long **slot)
 {
     if (s->slot_ctr % NSLOT == 0 && s->slot_ctr > 0)
-        hipMemsetAsync(s->slots, 0, NSLOT * CFL_WORDS * sizeof(unsigned long long), s->st);
-    unsigned long long *w = s->slots + CFL_WORDS * (s->slot_ctr % NSLOT);
+        HIPCHK(hipMemsetAsync(s->slots, 0, NSLOT * CFL_WORDS * sizeof(unsigned long long), s->st));
+    *slot = s->slots + CFL_WORDS * (s->slot_ctr % NSLOT);
     s->slot_ctr++;
-    return w;
+    return SC_OK;
 }
 
 static int read_cfl(sc_solver *s, unsigned long long *w, double dt, double *cfl)
@@ -681,7 +698,8 @@ extern "C" int sc_step(sc_solver *s, double dt, double t, double *dq, double *cf
     a.dtdx = dt / s->p.dx[0];
     a.dt = dt;
     a.dtdy = dt / s->p.dx[1];
-    a.smax = next_slot(s);
+    int rs = next_slot(s, &a.smax);
+    if (rs) return rs;
     HIPCHK(run_stage(s, a));
     int rc = read_cfl(s, a.smax, dt, cfl);
     if (rc) return rc;
@@ -729,8 +747,12 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
         a.qin = s->buf[in];
         a.qout = s->buf[outb];
         a.op = P.op[k];
-        a.smax = (k == P.n - 1) ? next_slot(s) : nullptr;
-        if (a.smax) *slot = a.smax;
+        a.smax = nullptr;
+        if (k == P.n - 1) {
+            int rs = next_slot(s, &a.smax);
+            if (rs) return rs;
+            *slot = a.smax;
+        }
         HIPCHK(run_stage(s, a));
         in = outb;
     }

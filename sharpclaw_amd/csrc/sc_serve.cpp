@@ -527,7 +527,10 @@ int round_done(hipEvent_t ev)
 // been collected, with the requests admitted meanwhile.
 void sched_loop(sc_server *S)
 {
-    (void)hipSetDevice(S->dev);
+    if (hipSetDevice(S->dev) != hipSuccess) {
+        fail_all(S, "hipSetDevice in the scheduler thread failed");
+        return;
+    }
     for (;;) {
         std::vector<Req *> admitted;
         bool busy = false;
@@ -627,7 +630,8 @@ extern "C" int sc_server_create(int device, size_t hbm_bytes, int max_active, sc
     if (device < 0 || device >= ndev) return sc_fail(SC_ERR_ARG, "device out of range");
     int prev = 0;
     (void)hipGetDevice(&prev);
-    (void)hipSetDevice(device);
+    hipError_t se = hipSetDevice(device);
+    if (se != hipSuccess) return sc_fail(SC_ERR_HIP, "hipSetDevice(%d): %s", device, hipGetErrorString(se));
     auto S = std::make_unique<sc_server>();
     S->dev = device;
     S->max_active = max_active;

@@ -1,7 +1,8 @@
 """The fused 2D stage at the benchmark's full size (2D Euler 1024 x 1024,
 characteristic WENO5 + Roe, SSPRK3), checked through properties that do not
-depend on the grid size; the oracle itself runs the small cases
-(test_gpu_parity.py), at 1024^2 it would take minutes per step.
+depend on the grid size and hold over many steps (the per-step comparison
+with the oracle at this size is test_gpu_fullsize_oracle.py; the oracle
+takes about 3 s per step there, so long runs are checked this way).
 
   transpose   the problem with x and y exchanged (q(i,j) -> q(j,i), momenta
               swapped) gives the transposed result: the y sweep of one is the
