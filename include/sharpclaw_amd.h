@@ -173,6 +173,23 @@ int sc_restore_qold(sc_solver *s);
  * zero ghost entries. */
 int sc_step(sc_solver *s, double dt, double t, double *dq, double *cfl);
 
+/* sc_step on arrays the caller already holds on the device, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  q: device array in the sc_get_q
+ * layout, its ghost cells filled in place (bc); dq: device array of the same
+ * shape receiving dt*L(q) with zero ghost entries; cfl: one device double
+ * receiving the CFL number; stream: a hipStream_t (NULL: the null stream).
+ * The call only enqueues work on that stream (no host synchronisation; the
+ * results are there when the stream reaches them).  Its only memory beyond
+ * the arguments is one CFL word set (8 KiB) in a static device array per
+ * device, kept until the process ends and cleared by each call: calls that
+ * overlap on one device race.  Grids of any size; options that need a
+ * grid-sized buffer or aux (2D slice-sweep reconstructions: lim_type 1, or
+ * char_decomp 1 or 3; capacity; 2D advection) return SC_ERR_UNSUPPORTED. */
+int sc_step_noalloc(const sc_params *p, double *q, double *dq, double dt, double t, double *cfl,
+                    void *stream);
+
 /* One SSP Runge-Kutta time step from q (= qold after sc_save_qold) with
  * every stage fused: ghost fill + flux + stage combination in one kernel
  * (ts_ssp22/42/33/104, src/sharpclaw.f90:247-464).  cfl receives the last

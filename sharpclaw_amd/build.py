@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsharpclaw_amd.so")
-SOURCES = ["sc_kernels.hip", "sc_batch.hip", "sc_batch2d.hip", "sc_capi.cpp", "sc_serve.cpp"]
+SOURCES = ["sc_kernels.hip", "sc_batch.hip", "sc_batch2d.hip", "sc_capi.cpp", "sc_serve.cpp",
+           "sc_noalloc.hip"]
 ARCH = os.environ.get("SC_OFFLOAD_ARCH", "gfx950")
 
 
