@@ -33,6 +33,48 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class roctx_range:
+    """A named range for rocprofv3 --marker-trace (rocprofiler-sdk's roctx;
+    a no-op when the library is absent or no profiler is attached)."""
+    _lib = None
+
+    def __init__(self, name):
+        self.name = name.encode()
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            import ctypes
+            try:
+                cls._lib = ctypes.CDLL("librocprofiler-sdk-roctx.so.1")
+                cls._lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+            except OSError:
+                cls._lib = False
+        return cls._lib
+
+    def __enter__(self):
+        if self.lib():
+            self.lib().roctxRangePushA(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        if self.lib():
+            self.lib().roctxRangePop()
+
+
+def physical_or_raise(p, q, what):
+    """Finite values, positive density and pressure in every interior cell."""
+    from sharpclaw_amd import interior
+    qi = interior(p, q)
+    rho = qi[..., 0]
+    with np.errstate(all="ignore"):
+        ke = 0.5 * (qi[..., 1] ** 2 + qi[..., 2] ** 2) / rho
+        pr = (p.gamma - 1.0) * (qi[..., 3] - ke)
+    bad = ~(np.isfinite(qi).all(axis=-1) & (rho > 0) & (pr > 0))
+    if bad.any():
+        raise RuntimeError(f"{what} is unphysical in {int(bad.sum())} cells (first {np.argwhere(bad)[0]})")
+
+
 def quadrant_states(seed):
     rng = np.random.default_rng(seed)
     # rows: NE, NW, SW, SE; columns: rho, u, v, p
@@ -137,15 +179,24 @@ def fp32_option_run(args, q0, q64):
             dt, wst = s.advance(ctl, 0.0, 1e9, dt)
             t = wst.t
         ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=args.steps, dt_variable=1)
+        q_start = s.get_q()
         dt_out, rst = s.advance(ctl, t, 1e9, dt)
-        ms_launch = s.time_stage(50, dt_out)
         q32 = s.get_q()
+        # per-launch kernel times from a replay of the same steps (as the
+        # float64 run measures them)
+        s.set_q(q_start)
+        s.set_stage_timing(3 * args.steps + 30)
+        s.advance(ctl, t, 1e9, dt)
+        stage_ms, stage_n = s.stage_timing()
+        s.set_stage_timing(0)
+        ms_launch = stage_ms / stage_n
     a, b = interior(p, q32), interior(p, q64)
     l1 = float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
     value = n * n * rst.stage_evals / (rst.gpu_ms * 1e-3)
     log(f"bench: fp32 option {value:.3e} cell-updates/s, stage {ms_launch * 1e3:.1f} us, "
         f"relative L1 vs fp64 {l1:.2e}")
     return {"value": value, "unit": "cell-updates/s", "us_per_launch": ms_launch * 1e3,
+            "timing": "HIP start/stop events of each fused stage launch (hipExtLaunchKernelGGL) of the timed steps, replayed from the same state",
             "l1_vs_fp64": l1, "steps": args.warmup + args.steps,
             "note": "relative L1 over the interior after the same steps as the float64 run"}
 
@@ -576,13 +627,150 @@ def cpu_baseline_served(seed):
     return out
 
 
+def trace_summary(d):
+    """Kernel statistics of a rocprofv3 run of this script (--kernel-trace
+    --marker-trace --output-format csv, output under d): per kernel over the
+    whole run, and per named range of this script (bench:timed, the timed
+    loop; bench:smooth, the smooth-state launches) for the kernels that
+    started inside it."""
+    import csv
+    import glob
+    import re
+
+    def rows(suffix):
+        out = []
+        for f in sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)):
+            with open(f, newline="") as fh:
+                out += list(csv.DictReader(fh))
+        return out
+
+    kern = rows("kernel_trace.csv")
+    marks = rows("marker_api_trace.csv")
+    if not kern:
+        raise SystemExit(f"no kernel_trace.csv under {d}")
+
+    def short(name):
+        return re.sub(r"\(.*\)$", "", name.replace("void ", "")).replace("sc::", "")
+
+    def stats(ks):
+        by = {}
+        for r in ks:
+            us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+            by.setdefault(short(r["Kernel_Name"]), []).append(us)
+        return sorted(((k, len(v), sum(v) / len(v), min(v), max(v)) for k, v in by.items()),
+                      key=lambda x: -x[1] * x[2])
+
+    lines = ["# whole run"]
+    lines += [f"{k:<58s} calls={n:6d} avg_us={a:9.2f} min_us={lo:9.2f} max_us={hi:9.2f}"
+              for k, n, a, lo, hi in stats(kern)]
+    for m in marks:
+        name = next((v for v in m.values() if isinstance(v, str) and v.startswith("bench:")), None)
+        if not name:
+            continue
+        t0, t1 = int(m["Start_Timestamp"]), int(m["End_Timestamp"])
+        inside = [r for r in kern if t0 <= int(r["Start_Timestamp"]) <= t1]
+        lines.append(f"# range {name}: {(t1 - t0) * 1e-6:.3f} ms, {len(inside)} kernels")
+        lines += [f"{k:<58s} calls={n:6d} avg_us={a:9.2f} min_us={lo:9.2f} max_us={hi:9.2f}"
+                  for k, n, a, lo, hi in stats(inside)]
+    print("\n".join(lines))
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    out = []
+    for f in sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)):
+        with open(f, newline="") as fh:
+            out += list(csv.DictReader(fh))
+    return out
+
+
+def _trace_windows(trace_dir, kernel):
+    """Ordinals (among the run's dispatches of `kernel`, in dispatch order)
+    of the dispatches inside each bench:* range of a traced run."""
+    kern = sorted((r for r in _csv_rows(trace_dir, "kernel_trace.csv") if r["Kernel_Name"] == kernel),
+                  key=lambda r: int(r["Dispatch_Id"]))
+    wins = {}
+    for m in _csv_rows(trace_dir, "marker_api_trace.csv"):
+        name = next((v for v in m.values() if isinstance(v, str) and v.startswith("bench:")), None)
+        if name:
+            t0, t1 = int(m["Start_Timestamp"]), int(m["End_Timestamp"])
+            wins[name] = [i for i, r in enumerate(kern) if t0 <= int(r["Start_Timestamp"]) <= t1]
+    return len(kern), wins
+
+
+def counter_summary(pmc_dirs, trace_dir, kernel, copy_bytes=1 << 30):
+    """Per-dispatch rocprofv3 --pmc counters of `kernel`, averaged over the
+    dispatches of each bench:* range of a kernel/marker trace of the same
+    command (a --pmc run cannot record markers: its dispatches of the kernel
+    are matched to the traced run's by their order, and the two runs must
+    hold the same number of them).  FETCH_SIZE and WRITE_SIZE (KiB) become
+    bytes per launch through the calibration of the same counter on the
+    script's copy kernel (copy_bytes read and written per dispatch).  The
+    kernel's dispatches in the counter runs are matched to the trace's from
+    the last one backwards (the tile tuner's trial launches at the start may
+    differ: counter runs pin the tile with SC_TILE)."""
+    nk, wins = _trace_windows(trace_dir, kernel)
+    out = {"kernel": kernel, "dispatches": nk, "windows": {}, "calibration": {}}
+    vals = {}   # counter -> list over the kernel's dispatches (ordinal order)
+    copyv = {}  # counter -> list over copy-kernel dispatches
+    for d in pmc_dirs:
+        rows = _csv_rows(d, "counter_collection.csv")
+        per = {}
+        for r in rows:
+            per.setdefault((r["Kernel_Name"], int(r["Dispatch_Id"])), {}).setdefault(r["Counter_Name"], 0.0)
+            per[(r["Kernel_Name"], int(r["Dispatch_Id"]))][r["Counter_Name"]] += float(r["Counter_Value"])
+        mine = [v for (kn, _), v in sorted(per.items(), key=lambda x: x[0][1]) if kn == kernel]
+        # matched from the run's last dispatch of the kernel backwards: the
+        # tile tuner's trial launches at the start are the only dispatches
+        # whose number can differ between two runs of the script
+        off = nk - len(mine)
+        if off < 0 or off > 16:
+            raise SystemExit(f"{d}: {len(mine)} dispatches of the kernel against {nk} in the trace")
+        for c in mine[0]:
+            vals[c] = [None] * off + [m[c] for m in mine]
+        for (kn, _), v in per.items():
+            if "copy_kernel" in kn:
+                for c, x in v.items():
+                    copyv.setdefault(c, []).append(x)
+    scale = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        if c in copyv:
+            kib = sum(copyv[c]) / len(copyv[c])
+            scale[c] = copy_bytes / (kib * 1024.0)
+            out["calibration"][c] = {"copy_kib_per_dispatch": kib, "bytes_per_kib_scaled": scale[c] * 1024.0}
+    for w, idx in wins.items():
+        if not idx:
+            continue
+        if any(v[i] is None for v in vals.values() for i in idx):
+            raise SystemExit(f"range {w}: dispatches of the trace missing from the counter runs")
+        avg = {c: sum(v[i] for i in idx) / len(idx) for c, v in vals.items()}
+        e = {"launches": len(idx), "first_ordinal": idx[0], "counters": avg}
+        if "FETCH_SIZE" in avg and "FETCH_SIZE" in scale:
+            e["fetch_bytes_per_launch"] = avg["FETCH_SIZE"] * 1024.0 * scale["FETCH_SIZE"]
+        if "WRITE_SIZE" in avg and "WRITE_SIZE" in scale:
+            e["write_bytes_per_launch"] = avg["WRITE_SIZE"] * 1024.0 * scale["WRITE_SIZE"]
+        if "fetch_bytes_per_launch" in e and "write_bytes_per_launch" in e:
+            e["bytes_per_launch"] = e["fetch_bytes_per_launch"] + e["write_bytes_per_launch"]
+        if "SQ_INSTS_VALU" in avg:
+            f64 = [avg.get(c) for c in ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+                                        "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")]
+            if None not in f64:
+                e["fp64_flop_per_launch"] = 64.0 * (f64[0] + f64[1] + 2.0 * f64[2] + f64[3])
+        out["windows"][w] = e
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--seed", type=int, default=961,
+                    help="seed of the four quadrant states (961 stays physical for 200+ steps)")
+    ap.add_argument("--work-seed", type=int, default=2024,
+                    help="seed of the batched and served workloads")
     ap.add_argument("--fp32", action="store_true", help="float32 reconstruction + Riemann")
     ap.add_argument("--tile", type=int, default=-1,
                     help="2D tile shape index (-1: autotune; 0: 16x15, 1: 15x16, 2: 32x7, 3: 14x17)")
@@ -595,7 +783,23 @@ def main():
     ap.add_argument("--rate", type=float, default=100.0, help="request arrival rate (1/s)")
     ap.add_argument("--no-recon", action="store_true",
                     help="skip the 1D reconstruction variants")
+    ap.add_argument("--trace-summary", metavar="DIR",
+                    help="summarise a rocprofv3 kernel/marker trace of this script and exit")
+    ap.add_argument("--counter-summary", nargs="+", metavar="DIR",
+                    help="average rocprofv3 --pmc runs of this script over the ranges of the "
+                         "trace given by --trace-summary, print JSON and exit")
+    ap.add_argument("--kernel", default="void sc::stage2d_kernel<4, 2, double, 16, 15>"
+                                        "(sc::StageArgs, sc::PhysConst<double>)",
+                    help="kernel name for --counter-summary")
     args = ap.parse_args()
+    if args.counter_summary:
+        if not args.trace_summary:
+            raise SystemExit("--counter-summary needs --trace-summary DIR (the traced run)")
+        print(json.dumps(counter_summary(args.counter_summary, args.trace_summary, args.kernel), indent=1))
+        return
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(independent solves shard without exchange; one grid would need halo "
@@ -623,21 +827,40 @@ def main():
             t = wst.t
         log(f"bench: warmup done (t={t:.4e}, dt={dt:.4e})")
         ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=args.steps, dt_variable=1)
-        w0 = time.perf_counter()
-        dt_out, rst = s.advance(ctl, t, 1e9, dt)
-        wall = time.perf_counter() - w0
+        # the timed loop (value, ms_per_step)
+        q_start = s.get_q()
+        with roctx_range("bench:timed"):
+            w0 = time.perf_counter()
+            dt_out, rst = s.advance(ctl, t, 1e9, dt)
+            wall = time.perf_counter() - w0
+        q_end = s.get_q()
+        # the same steps replayed from the same state with every fused stage
+        # launch reporting its kernel's device start and end to HIP events
+        # (hipExtLaunchKernelGGL): the per-launch time of the roofline.  The
+        # events' launch path costs ~13 us of host time per launch, so the
+        # timed loop itself runs without them; the replay repeats its launches
+        # bit for bit.
+        s.set_q(q_start)
+        s.set_stage_timing(3 * args.steps + 30)  # room for retakes
+        with roctx_range("bench:replay"):
+            _, rst_r = s.advance(ctl, t, 1e9, dt)
+        stage_ms, stage_n = s.stage_timing()
+        s.set_stage_timing(0)
+        if not np.array_equal(s.get_q(), q_end) or rst_r.stage_evals != rst.stage_evals:
+            raise RuntimeError("the replay of the timed steps differs from the timed loop")
         if rst.steps_taken != args.steps:
             raise RuntimeError(f"took {rst.steps_taken} steps, expected {args.steps}")
         log(f"bench: timed {args.steps} steps, gpu {rst.gpu_ms:.1f} ms, wall {wall * 1e3:.1f} ms, "
             f"rejected {rst.rejected}, cfl_last {rst.cfl_last:.4f}")
         value = cells * rst.stage_evals / (rst.gpu_ms * 1e-3)
-        # kernel-only: the fused stage launches of 50 SSPRK3 steps back to back
-        # (right after the timed loop, before any copy leaves the GPU idle;
-        # they write only free buffers, q stays the loop's result)
-        ms_launch = s.time_stage(50, dt_out)
-        q_end = s.get_q()
+        ms_launch = stage_ms / stage_n
+        physical_or_raise(p, q_end, f"the state after {args.warmup + args.steps} steps")
         tile, (tx, ty) = s.get_tile()
-        log(f"bench: fused stage kernel {ms_launch * 1e3:.1f} us per launch (tile {tx}x{ty})")
+        log(f"bench: fused stage kernel {ms_launch * 1e3:.1f} us per launch over the timed steps "
+            f"({stage_n} launches, tile {tx}x{ty})")
+        # the same launches back to back from the loop's final state, without
+        # the per-step CFL readback between steps (they write free buffers only)
+        ms_backtoback = s.time_stage(50, dt_out)
         sums = s.sums()
         # PCIe-inclusive: the same kind of loop with q uploaded before and
         # downloaded after it, as a host-array caller sees it
@@ -649,10 +872,11 @@ def main():
         pcie_value = cells * pst.stage_evals / (time.perf_counter() - w1)
         # the stage on a smooth state with no constant region (every
         # interface takes the full reconstruction and Riemann solve; the
-        # four-quadrant data is still constant away from its fronts, where
-        # both have exact shortcuts)
-        s.set_q(smooth_q(p))
-        ms_smooth = s.time_stage(50, 1e-4)
+        # four-quadrant data is constant away from its fronts, where the
+        # stage has an exact shortcut)
+        with roctx_range("bench:smooth"):
+            s.set_q(smooth_q(p))
+            ms_smooth = s.time_stage(50, 1e-4)
         log(f"bench: fused stage kernel on a smooth state {ms_smooth * 1e3:.1f} us per launch")
 
     fp32_option = None if args.fp32 else fp32_option_run(args, q0, q_end)
@@ -665,6 +889,7 @@ def main():
     bytes_per_cell_stage = meqn * 8 * (2 + 3 + 3) / 3.0  # SSPRK3: state reads+writes per stage
     alg_bytes = bytes_per_cell_stage * cells
     achieved = alg_bytes / (ms_launch * 1e-3) / 1e9
+    achieved_smooth = alg_bytes / (ms_smooth * 1e-3) / 1e9
     traffic = traffic_from_profile()
     out = {
         "metric": "cell-updates/s per RK stage",
@@ -690,9 +915,18 @@ def main():
                      "kernel": f"stage2d_kernel<EU2,char,{'f32' if args.fp32 else 'f64'},{tx}x{ty}>",
                      "alg_bytes_per_launch": alg_bytes,
                      "us_per_launch": ms_launch * 1e3,
-                     "us_per_launch_smooth_state": ms_smooth * 1e3},
+                     "launches": stage_n,
+                     "timing": "HIP start/stop events of each fused stage launch (hipExtLaunchKernelGGL) of the timed steps, replayed from the same state",
+                     "us_per_launch_back_to_back": ms_backtoback * 1e3,
+                     "smooth_state": {"us_per_launch": ms_smooth * 1e3, "achieved": achieved_smooth,
+                                      "frac": achieved_smooth / HBM_PEAK_GBS,
+                                      "note": "the same kernel on a smooth state (no constant tile: "
+                                              "full work everywhere), 50 SSPRK3 steps back to back"}},
         "valu_roofline": None if args.fp32 or n != 1024 else valu_from_profile(tx, ty, ms_launch, ms_smooth),
         "gpu_ms_timed": rst.gpu_ms,
+        "replay_with_timing_events": {"ms_per_step": rst_r.gpu_ms / args.steps,
+                                      "note": "the timed steps replayed from the same state with "
+                                              "per-launch timing events (the roofline's launches)"},
         "copy_kernel_gbs": copy_gbs,
         "frac_of_copy_bw": achieved / copy_gbs,
         "pct_hbm_peak": 100.0 * achieved / HBM_PEAK_GBS,
@@ -713,10 +947,10 @@ def main():
         out["euler1d"] = euler1d_bench(cpu=not args.no_cpu)
     if not args.no_batch:
         log(f"bench: batched workload ({args.batch} dam breaks)")
-        out["batched"] = batched_dambreaks(args.batch, args.seed, cpu=not args.no_cpu)
+        out["batched"] = batched_dambreaks(args.batch, args.work_seed, cpu=not args.no_cpu)
     if not args.no_serve:
         log(f"bench: served workload ({args.requests} requests at {args.rate:g}/s)")
-        out["served"] = served_bench(args.requests, args.rate, args.seed, cpu=not args.no_cpu)
+        out["served"] = served_bench(args.requests, args.rate, args.work_seed, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

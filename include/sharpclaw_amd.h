@@ -212,6 +212,16 @@ int sc_sums(sc_solver *s, double *sums);
  * solver's stream.  Leaves q unchanged. */
 int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stage);
 
+/* Per-launch device time of the fused stage kernels (no counterpart in the
+ * reference; measurement).  sc_stage_timing(s, n) with n > 0: the next n
+ * stage launches of sc_ts and sc_advance record their kernels' device start
+ * and end in HIP events (hipExtLaunchKernelGGL; no host work per launch;
+ * sc_advance takes its host-driven loop while timing is on); n = 0 turns it
+ * off.  sc_stage_timing_get waits for the recorded launches and returns the
+ * sum of their durations in ms and their number. */
+int sc_stage_timing(sc_solver *s, int max_launches);
+int sc_stage_timing_get(sc_solver *s, double *ms, long long *launches);
+
 /* Tile shape of the 2D fused stage kernel (no counterpart in the reference):
  * -1 (default) times the candidates on this solver's grid at the first
  * sc_ts/sc_advance/sc_time_stage and keeps the fastest; 0..3 fix it

@@ -54,6 +54,8 @@ def lib():
         "sc_sums": ([H, dp], C.c_int),
         "sc_time_stage": ([H, C.c_int, C.c_double, dref], C.c_int),
         "sc_copy_bench": ([C.c_int, C.c_size_t, C.c_int, dref], C.c_int),
+        "sc_stage_timing": ([H, C.c_int], C.c_int),
+        "sc_stage_timing_get": ([H, dref, C.POINTER(C.c_longlong)], C.c_int),
         "sc_set_tile": ([H, C.c_int], C.c_int),
         "sc_get_tile": ([H, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "sc_rp": ([P, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp], C.c_int),

@@ -205,6 +205,11 @@ struct sc_solver {
     hipStream_t cst;
     hipEvent_t ev_snap, ev_out;
     bool out_pending;
+    // per-launch timing of the fused stage kernels (sc_stage_timing): the
+    // start / stop event pairs of the timed launches, read at sc_stage_timing_get
+    bool timing;
+    hipEvent_t *tev;
+    int tev_cap, tev_used;
 };
 
 struct DevGuard {
@@ -328,6 +333,9 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->capa) hipFree(s->capa);
     if (s->aux2) hipFree(s->aux2);
+    for (int k = 0; k < s->tev_cap; k++)
+        if (s->tev[k]) hipEventDestroy(s->tev[k]);
+    free(s->tev);
     output_free(s);
     if (s->st) hipStreamDestroy(s->st);
     delete s;
@@ -648,6 +656,43 @@ extern "C" int sc_get_tile(sc_solver *s, int *tile, int *tx, int *ty)
     return SC_OK;
 }
 
+extern "C" int sc_stage_timing(sc_solver *s, int max_launches)
+{
+    if (!s || max_launches < 0) return sc_fail(SC_ERR_ARG, "bad argument");
+    DevGuard g(s->dev);
+    s->timing = false;
+    s->tev_used = 0;
+    if (max_launches == 0) return SC_OK;
+    const int need = 2 * max_launches;
+    if (need > s->tev_cap) {
+        hipEvent_t *t = (hipEvent_t *)realloc(s->tev, need * sizeof(hipEvent_t));
+        if (!t) return sc_fail(SC_ERR_NOMEM, "stage timing: %d events", need);
+        for (int k = s->tev_cap; k < need; k++) t[k] = nullptr;
+        s->tev = t;
+        s->tev_cap = need;
+    }
+    for (int k = 0; k < need; k++)
+        if (!s->tev[k]) HIPCHK(hipEventCreate(&s->tev[k]));
+    s->timing = true;
+    return SC_OK;
+}
+
+extern "C" int sc_stage_timing_get(sc_solver *s, double *ms, long long *launches)
+{
+    if (!s || !ms || !launches) return sc_fail(SC_ERR_ARG, "NULL argument");
+    DevGuard g(s->dev);
+    double total = 0.0;
+    if (s->tev_used > 0) HIPCHK(hipEventSynchronize(s->tev[s->tev_used - 1]));
+    for (int k = 0; k + 1 < s->tev_used; k += 2) {
+        float e = 0.f;
+        HIPCHK(hipEventElapsedTime(&e, s->tev[k], s->tev[k + 1]));
+        total += e;
+    }
+    *ms = total;
+    *launches = s->tev_used / 2;
+    return SC_OK;
+}
+
 // next CFL set, cleared ring-wise: a set that is not cleared before its
 // reuse would keep the atomicMax of an earlier step
 static int next_slot(sc_solver *s, unsigned long long **slot)
@@ -753,7 +798,14 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
             if (rs) return rs;
             *slot = a.smax;
         }
-        HIPCHK(run_stage(s, a));
+        const bool tm = s->timing && s->tev_used + 2 <= s->tev_cap;
+        if (tm) set_launch_events(s->tev[s->tev_used], s->tev[s->tev_used + 1]);
+        hipError_t e = run_stage(s, a);
+        if (tm) {
+            set_launch_events(nullptr, nullptr);
+            if (e == hipSuccess) s->tev_used += 2;
+        }
+        HIPCHK(e);
         in = outb;
     }
     s->cur = in;
@@ -889,7 +941,9 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
         long long tiles = (long long)((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty);
         // 2D advection steps its velocities per step on the host (b4step)
         // (the slice sweeps have no device-loop kernel)
-        bool device = s->sel.phys != ADV2 && !s->dqx && (s->loop_mode == 1 || (s->loop_mode < 0 && tiles < 2048));
+        // (per-launch stage timing brackets launches from the host loop)
+        bool device = s->sel.phys != ADV2 && !s->dqx && !s->timing &&
+                      (s->loop_mode == 1 || (s->loop_mode < 0 && tiles < 2048));
         if (device) return advance2d_device(s, c, tstart, tend, dt_io, st);
     }
     memset(st, 0, sizeof *st);

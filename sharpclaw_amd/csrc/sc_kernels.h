@@ -58,6 +58,10 @@ struct PhysSel {
     int lim = 2;  // lim_type
 };
 
+// start / stop events the next stage launch (launch_stage1d, launch_stage2d,
+// launch_sweep2d) of the calling host thread reports its device times to;
+// nullptr, nullptr for plain launches
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
 hipError_t launch_stage1d(const PhysSel &s, int ixy, const StageArgs &a, const PhysConst<double> &cd,
                           const PhysConst<float> &cf, hipStream_t st);
 // 2D stage by x and y slice sweeps (the reconstructions the tile kernel does

@@ -11,6 +11,8 @@
 // per block.
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "sc_kernels.h"
 #include "sc_stage.h"
 #include "sc_stage2d.h"
@@ -309,14 +311,37 @@ __global__ void copy_kernel(double2 *__restrict__ dst, const double2 *__restrict
 
 // -------------------------------------------------------------- launchers
 
+// Events that the next stage launch of this host thread reports its device
+// start and end times to (set_launch_events; null: a plain launch).  A
+// launch with events goes through hipExtLaunchKernelGGL, which records them
+// at the kernel's own start and completion, so their elapsed time is the
+// kernel's duration without the host's launch latency.
+static thread_local hipEvent_t t_ev_start = nullptr, t_ev_stop = nullptr;
+
+void set_launch_events(hipEvent_t start, hipEvent_t stop)
+{
+    t_ev_start = start;
+    t_ev_stop = stop;
+}
+
+template <typename K, typename... Args>
+static hipError_t launch_k(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t start, hipEvent_t stop,
+                           Args... args)
+{
+    if (start || stop)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, st, start, stop, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+    return hipGetLastError();
+}
+
 constexpr int NT1 = 256;
 
 template <int P, int IXY, int CHAR, class T>
 static hipError_t go1(const StageArgs &a, const PhysConst<T> &c, hipStream_t st)
 {
     int nblk = (a.nx + NT1 - 2) / (NT1 - 1);
-    hipLaunchKernelGGL((stage1d_kernel<P, IXY, CHAR, T, NT1>), dim3(nblk), dim3(NT1), 0, st, a, c);
-    return hipGetLastError();
+    return launch_k(stage1d_kernel<P, IXY, CHAR, T, NT1>, dim3(nblk), dim3(NT1), st, t_ev_start, t_ev_stop, a, c);
 }
 
 template <int P, int IXY, int CHAR>
@@ -370,12 +395,11 @@ static hipError_t gos(const StageArgs &a, const PhysConst<T> &c, double *dqx, hi
     const long long nbx = (long long)((a.nx + NTX - 2) / (NTX - 1)) * (a.ny + 2);
     const long long nby = (long long)((a.ny + NTY - 2) / (NTY - 1)) * ((a.nx + 2 + NLY - 1) / NLY);
     if (nbx > 0x7fffffffLL || nby > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((sweep2d_kernel<P, 1, CHAR, T, NTX, 1>), dim3((unsigned)nbx), dim3(NTX), 0, st, a, c, dqx);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_k(sweep2d_kernel<P, 1, CHAR, T, NTX, 1>, dim3((unsigned)nbx), dim3(NTX), st, t_ev_start,
+                            (hipEvent_t) nullptr, a, c, dqx);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((sweep2d_kernel<P, 2, CHAR, T, NTY, NLY>), dim3((unsigned)nby), dim3(NTY * NLY), 0, st, a,
-                       c, dqx);
-    return hipGetLastError();
+    return launch_k(sweep2d_kernel<P, 2, CHAR, T, NTY, NLY>, dim3((unsigned)nby), dim3(NTY * NLY), st,
+                    (hipEvent_t) nullptr, t_ev_stop, a, c, dqx);
 }
 
 template <int P, int CHAR>
@@ -425,9 +449,8 @@ static hipError_t go2t(const StageArgs &a0, const PhysConst<T> &c, hipStream_t s
     StageArgs a = a0;
     a.ntx = (a.nx + TX - 1) / TX;
     a.nty = (a.ny + TY - 1) / TY;
-    hipLaunchKernelGGL((stage2d_kernel<P, CHAR, T, TX, TY>), dim3(a.ntx * a.nty), dim3(256), 0,
-                       st, a, c);
-    return hipGetLastError();
+    return launch_k(stage2d_kernel<P, CHAR, T, TX, TY>, dim3(a.ntx * a.nty), dim3(256), st, t_ev_start, t_ev_stop,
+                    a, c);
 }
 
 template <int P, int CHAR, class T>

@@ -131,6 +131,18 @@ class Solver:
         check(lib().sc_sums(self._h, out))
         return out
 
+    def set_stage_timing(self, max_launches):
+        """Record the device start and end of the next max_launches fused
+        stage launches of ts/advance (0: off); advance then takes its
+        host-driven loop."""
+        check(lib().sc_stage_timing(self._h, int(max_launches)))
+
+    def stage_timing(self):
+        """(total ms, launches) of the recorded stage launches."""
+        ms, n = C.c_double(), C.c_longlong()
+        check(lib().sc_stage_timing_get(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def set_tile(self, tile):
         """2D tile shape: -1 autotune at first use, 0..3 fixed."""
         check(lib().sc_set_tile(self._h, tile))

@@ -30,9 +30,11 @@ pytestmark = pytest.mark.gpu
 N = 1024
 
 
-def quadrants(p, seed=2024):
+def quadrants(p, seed=961):
     """Four constant states (rho, u, v, p) in U[0.1,1.5] x U[-1,1]^2 x
-    U[0.03,1.5], split at x = y = 0.5."""
+    U[0.03,1.5], split at x = y = 0.5 (the benchmark's data: seed 961 stays
+    physical over its 200 steps; seed 2024's low-pressure quadrant gives a
+    negative pressure at the centre in step 23, in the oracle as well)."""
     rng = np.random.default_rng(seed)
     st = np.stack([rng.uniform(0.1, 1.5, 4), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4),
                    rng.uniform(0.03, 1.5, 4)], axis=1)
