@@ -120,43 +120,43 @@ def smooth_q(p):
 
 
 def traffic_from_profile():
-    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
+    """HBM bytes per stage launch of the timed steps from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
     path = os.path.join(ROOT, "profiles", "stage2d_euler2d_1024_fp64_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        w = json.load(f).get("windows", {}).get("bench:timed", {})
+    return w.get("bytes_per_launch")
 
 
 def valu_from_profile(tx, ty, ms_launch, ms_smooth):
-    """float64 VALU roofline of the tuned stage kernel: flop per launch from
-    the committed rocprofv3 counter summary (the same launches bench.py
-    times: the state after the timed steps, and the smooth state), divided
-    by the live launch times."""
+    """float64 VALU roofline of the stage kernel: flop per launch from the
+    committed rocprofv3 counter summary (the timed steps and the smooth
+    state, the same launches bench.py times), divided by the live launch
+    times."""
     path = os.path.join(ROOT, "profiles", "stage2d_euler2d_1024_fp64_valu.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        ks = json.load(f).get("kernels", {})
-    name = f"stage2d_kernel<4, 2, double, {tx}, {ty}>"
-    k = ks.get(name)
-    if k is None and ks:
-        # the profile was taken with another tile shape: its counts differ by
-        # the shape's edge-interface share (a few percent)
-        name, k = next(iter(ks.items()))
-    if k is None or "fp64_flop_per_launch" not in k:
+        d = json.load(f)
+    ws = d.get("windows", {})
+    t, sm = ws.get("bench:timed"), ws.get("bench:smooth")
+    if not t or "fp64_flop_per_launch" not in t:
         return None
-    tf = k["fp64_flop_per_launch"] / (ms_launch * 1e-3) / 1e12
+    name = d.get("kernel", "")
+    tf = t["fp64_flop_per_launch"] / (ms_launch * 1e-3) / 1e12
     out = {"bound": "valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
            "frac": tf / FP64_PEAK_TFLOPS, "profiled_kernel": name,
-           "flop_per_launch": k["fp64_flop_per_launch"],
-           "valu_lane_instr_per_cell": k["valu_lane_instr_per_cell"]}
-    sm = k.get("smooth_state")
-    if sm:
+           "flop_per_launch": t["fp64_flop_per_launch"],
+           "valu_lane_instr_per_cell": t.get("valu_lane_instr_per_cell")}
+    if f"<4, 2, double, {tx}, {ty}>" not in name:
+        out["note"] = f"profiled with another tile shape than the {tx}x{ty} timed here"
+    if sm and "fp64_flop_per_launch" in sm:
         ts = sm["fp64_flop_per_launch"] / (ms_smooth * 1e-3) / 1e12
         out["smooth_state"] = {"achieved": ts, "frac": ts / FP64_PEAK_TFLOPS,
                                "flop_per_launch": sm["fp64_flop_per_launch"],
-                               "valu_lane_instr_per_cell": sm["valu_lane_instr_per_cell"]}
+                               "valu_lane_instr_per_cell": sm.get("valu_lane_instr_per_cell")}
     return out
 
 
@@ -699,7 +699,7 @@ def _trace_windows(trace_dir, kernel):
     return len(kern), wins
 
 
-def counter_summary(pmc_dirs, trace_dir, kernel, copy_bytes=1 << 30):
+def counter_summary(pmc_dirs, trace_dir, kernel, copy_bytes=1 << 30, cells=None, alg_bytes=None):
     """Per-dispatch rocprofv3 --pmc counters of `kernel`, averaged over the
     dispatches of each bench:* range of a kernel/marker trace of the same
     command (a --pmc run cannot record markers: its dispatches of the kernel
@@ -752,12 +752,21 @@ def counter_summary(pmc_dirs, trace_dir, kernel, copy_bytes=1 << 30):
             e["write_bytes_per_launch"] = avg["WRITE_SIZE"] * 1024.0 * scale["WRITE_SIZE"]
         if "fetch_bytes_per_launch" in e and "write_bytes_per_launch" in e:
             e["bytes_per_launch"] = e["fetch_bytes_per_launch"] + e["write_bytes_per_launch"]
+        if "bytes_per_launch" in e and alg_bytes:
+            e["traffic_over_alg"] = e["bytes_per_launch"] / alg_bytes
         if "SQ_INSTS_VALU" in avg:
             f64 = [avg.get(c) for c in ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
                                         "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")]
             if None not in f64:
                 e["fp64_flop_per_launch"] = 64.0 * (f64[0] + f64[1] + 2.0 * f64[2] + f64[3])
+            if cells:  # a wave instruction is 64 lane instructions
+                e["valu_lane_instr_per_cell"] = 64.0 * avg["SQ_INSTS_VALU"] / cells
+        if "SQ_WAIT_INST_ANY" in avg and "SQ_WAVE_CYCLES" in avg:
+            e["wait_inst_any_per_wave_cycle"] = avg["SQ_WAIT_INST_ANY"] / avg["SQ_WAVE_CYCLES"]
+            e["wait_any_per_wave_cycle"] = avg.get("SQ_WAIT_ANY", 0.0) / avg["SQ_WAVE_CYCLES"]
         out["windows"][w] = e
+    if alg_bytes:
+        out["alg_bytes_per_launch"] = alg_bytes
     return out
 
 
@@ -772,8 +781,9 @@ def main():
     ap.add_argument("--work-seed", type=int, default=2024,
                     help="seed of the batched and served workloads")
     ap.add_argument("--fp32", action="store_true", help="float32 reconstruction + Riemann")
-    ap.add_argument("--tile", type=int, default=-1,
-                    help="2D tile shape index (-1: autotune; 0: 16x15, 1: 15x16, 2: 32x7, 3: 14x17)")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="2D tile shape index (0: 16x15, 1: 15x16, 2: 32x7, 3: 14x17; default: "
+                         "autotuned, or the environment's SC_TILE)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched dam-break workload")
@@ -791,11 +801,22 @@ def main():
     ap.add_argument("--kernel", default="void sc::stage2d_kernel<4, 2, double, 16, 15>"
                                         "(sc::StageArgs, sc::PhysConst<double>)",
                     help="kernel name for --counter-summary")
+    ap.add_argument("--command", action="append", default=[],
+                    help="the profiling command(s) recorded in the --counter-summary JSON")
     args = ap.parse_args()
     if args.counter_summary:
         if not args.trace_summary:
             raise SystemExit("--counter-summary needs --trace-summary DIR (the traced run)")
-        print(json.dumps(counter_summary(args.counter_summary, args.trace_summary, args.kernel), indent=1))
+        cells = args.n * args.n
+        alg = 4 * 8 * (2 + 3 + 3) / 3.0 * cells  # 2D Euler, SSPRK3 stage average
+        out = counter_summary(args.counter_summary, args.trace_summary, args.kernel, cells=cells, alg_bytes=alg)
+        out["workload"] = (f"2D Euler {args.n}x{args.n} fp64 char WENO5 + Roe rpn2, SSPRK3, four-quadrant "
+                           f"data (seed {args.seed}); windows: the bench's timed steps, their replay "
+                           f"with timing events, the smooth state")
+        summ = ["python", "bench.py", "--trace-summary", args.trace_summary, "--counter-summary"] + \
+            args.counter_summary + ([] if args.n == 1024 else ["--n", str(args.n)])
+        out["commands"] = args.command + [" ".join(summ)]
+        print(json.dumps(out, indent=1))
         return
     if args.trace_summary:
         trace_summary(args.trace_summary)
@@ -816,7 +837,8 @@ def main():
     log(f"bench: 2D Euler {n}x{n} {'fp32' if args.fp32 else 'fp64'} char-WENO5 SSPRK3, "
         f"warmup {args.warmup}, steps {args.steps}")
     with Solver(p) as s:
-        s.set_tile(args.tile)
+        if args.tile is not None:
+            s.set_tile(args.tile)
         s.set_q(q0)
         dt = 1e-4
         t = 0.0

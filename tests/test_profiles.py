@@ -1,29 +1,36 @@
-"""The committed rocprofv3 summaries hold the fields bench.py reads."""
+"""The committed rocprofv3 summaries hold the fields bench.py reads, for the
+benchmark's own command (bench.py --gpus 1 --steps 20 --warmup 5)."""
 import json
 import os
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 P = os.path.join(ROOT, "profiles")
+CMD = "bench.py --gpus 1 --steps 20 --warmup 5"
 
 
 def test_traffic_summary_fields():
     d = json.load(open(os.path.join(P, "stage2d_euler2d_1024_fp64_traffic.json")))
-    assert d["bytes_per_launch"] > 0 and d["alg_bytes_per_launch"] > 0
-    assert 0.9 < d["traffic_over_alg"] < 2.0
-    assert all("rocprofv3 --pmc" in c for c in d["commands"])
+    w = d["windows"]["bench:timed"]
+    assert w["launches"] == 60 and w["bytes_per_launch"] > 0
+    assert 0.9 < w["traffic_over_alg"] < 2.0
+    assert d["alg_bytes_per_launch"] > 0
+    assert any("--pmc FETCH_SIZE" in c and CMD in c for c in d["commands"])
+    assert set(d["calibration"]) == {"FETCH_SIZE", "WRITE_SIZE"}
 
 
 def test_valu_summary_fields():
     d = json.load(open(os.path.join(P, "stage2d_euler2d_1024_fp64_valu.json")))
-    assert "rocprofv3 --pmc" in d["command"]
-    ks = d["kernels"]
-    assert any(k.startswith("stage2d_kernel<4, 2, double,") for k in ks)
-    for k in ks.values():
-        assert k["fp64_flop_per_launch"] > 0 and k["valu_lane_instr_per_cell"] > 0
-        assert k["counters"]["SQ_INSTS_VALU"] > 0
+    assert any("--pmc SQ_INSTS_VALU" in c and CMD in c for c in d["commands"])
+    assert "stage2d_kernel<4, 2, double," in d["kernel"]
+    for w in ("bench:timed", "bench:smooth"):
+        e = d["windows"][w]
+        assert e["fp64_flop_per_launch"] > 0 and e["valu_lane_instr_per_cell"] > 0
+        assert e["counters"]["SQ_INSTS_VALU"] > 0
 
 
 def test_kernel_stats_present():
     for f in ("stage2d_euler2d_1024_fp64_kernel_stats.txt", "batch_dambreak4096_kernel_stats.txt"):
         txt = open(os.path.join(P, f)).read()
-        assert "rocprofv3 --kernel-trace --stats" in txt and "avg_us=" in txt
+        assert "rocprofv3 --kernel-trace" in txt and "avg_us=" in txt
+    txt = open(os.path.join(P, "stage2d_euler2d_1024_fp64_kernel_stats.txt")).read()
+    assert CMD in txt and "# range bench:timed" in txt
