@@ -111,7 +111,7 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
     constexpr int M = Shape<P>::M;
     constexpr int NW = NT / 64;
     __shared__ T sq[M][NT + 5];
-    __shared__ T sx[2 * M][NT];
+    __shared__ __align__(8) T sx[2 * M][NT];
     __shared__ double red[NW];
     __shared__ int sb;
     const int t = threadIdx.x;

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(NT) void stage1d_kernel(StageArgs a, PhysConst<T> c
     constexpr int NB = nbr_words<P, IXY, CHAR, T>();
     constexpr bool AL = sx_in_sw<P, IXY, CHAR, T, NT>();
     __shared__ T sq[M][NT + 5];
-    __shared__ T sx[AL ? 1 : 2 * M][NT];
+    __shared__ __align__(8) T sx[AL ? 1 : 2 * M][NT];
     __shared__ T sw[NB > 0 ? NB : 1][NT + 4];
     __shared__ double red[NT / 64];
     T(*sxp)[NT] = AL ? reinterpret_cast<T(*)[NT]>(&sw[0][0]) : sx;

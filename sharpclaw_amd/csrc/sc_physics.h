@@ -12,6 +12,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sc_fastmath.h"
 
 namespace sc {
@@ -43,6 +45,10 @@ template <class T> struct PhysConst {
 };
 
 template <class T> __device__ __forceinline__ T dsq(T x) { return x * x; }
+
+// f(q) of direction IXY in float64 (defined with the in-cell total fluctuation)
+template <int P, int IXY, class T>
+__device__ __forceinline__ void normal_flux(const PhysConst<T> &c, const T *q, double *f);
 
 // ---------------------------------------------------------------- rp
 
@@ -98,6 +104,20 @@ __device__ __forceinline__ void split_by_sign(const T *wave, const T *s, T *amdq
     }
 }
 
+// amdq alone of split_by_sign (the flux form of the stage needs no apdq)
+template <int M, int W, class T>
+__device__ __forceinline__ void split_minus(const T *wave, const T *s, T *amdq)
+{
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        T am = T(0);
+#pragma unroll
+        for (int w = 0; w < W; w++)
+            if (s[w] < T(0)) am = am + s[w] * wave[m + M * w];
+        amdq[m] = am;
+    }
+}
+
 template <int M, int W, class T>
 __device__ __forceinline__ void apdq_total(const T *wave, const T *s, const T *amdq, T *apdq)
 {
@@ -110,12 +130,90 @@ __device__ __forceinline__ void apdq_total(const T *wave, const T *s, const T *a
     }
 }
 
-// apps/euler/shockentropy/rp1eu.f:44-213 (efix: rp1eu.f:126-201)
-template <class T>
-__device__ __forceinline__ void rp_eu1(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
-                                       T *s, T *amdq, T *apdq)
+// The entropy fix of rp1eu.f:126-201 for a Roe solve of Euler (1D: MV < 0;
+// 2D: the tangential momentum MV rides along, as in rp_eu2).  wave(m, w) is
+// wave[m + M w]; irl, irr are 1/rho and pl, pr the pressures of the left and
+// right states.  Its branches test the signs of u - c for the left state and
+// for the state behind wave 1 (rp1eu.f:135-151), and of u + c for the state
+// ahead of wave 3 and the right state (rp1eu.f:167-185).  Those signs are
+// decided here from squares: u - c >= 0 iff c^2 >= 0 (the square root is a
+// number), u >= 0 and u^2 >= c^2; for an intermediate state (rho1, m1, E1)
+// with P1 = p1 rho1 = (gamma-1)(E1 rho1 - m1^2/2), u1 - c1 > 0 iff P1 >= 0,
+// m1 rho1 > 0 and m1^2 > gamma P1 (u1 = m1/rho1, c1 = sqrt(gamma P1)/|rho1|),
+// so neither the square roots nor the reciprocals of the intermediate states
+// are formed unless the wave is a transonic rarefaction, where the
+// reference's expressions give the speeds and the split fraction.  The
+// decisions equal the reference's except where u - c is within rounding of
+// zero, where both branches give the same amdq to rounding.
+template <int M, int MU, int MV, int ME, class T>
+__device__ __forceinline__ void euler_efix(const PhysConst<T> &c, const T *lf, const T *rt, T irl, T irr, T pl,
+                                           T pr, const T *wave, const T *s, T *amdq)
 {
     const T g = c.gamma, g1 = c.gamma1, h = T(0.5);
+    // left state: s0 = u_l - c_l
+    const T ul = lf[MU] * irl, xl = g * pl * irl;
+    const bool s0_ge = xl >= T(0) && ul >= T(0) && ul * ul >= xl;
+    const bool s0_lt = xl >= T(0) && !s0_ge;
+    if (s0_ge && s[0] > T(0)) {
+#pragma unroll
+        for (int m = 0; m < M; m++) amdq[m] = T(0);
+        return;
+    }
+    // the state behind wave 1: s1 = u1 - c1
+    const T rho1 = lf[0] + wave[0], m1 = lf[MU] + wave[MU], e1 = lf[ME] + wave[ME];
+    const T t1 = MV >= 0 ? lf[MV < 0 ? 0 : MV] + wave[MV < 0 ? 0 : MV] : T(0);
+    const T q1 = m1 * m1 + t1 * t1;
+    const T P1 = g1 * (e1 * rho1 - h * q1);
+    const bool s1_gt = P1 >= T(0) && m1 * rho1 > T(0) && m1 * m1 > g * P1;
+    T sfract;
+    if (s0_lt && s1_gt) {  // transonic rarefaction in the 1-wave
+        const T s0 = ul - fsqrt(xl);
+        const T ir1 = frcp(rho1);
+        const T p1 = g1 * (e1 - h * q1 * ir1);
+        const T s1 = m1 * ir1 - fsqrt(g * p1 * ir1);
+        sfract = s0 * fdiv(s1 - s[0], s1 - s0);
+    } else if (s[0] < T(0)) {
+        sfract = s[0];
+    } else {
+        sfract = T(0);
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) amdq[m] = sfract * wave[m];
+    if (s[1] >= T(0)) return;
+#pragma unroll
+    for (int m = 0; m < M; m++) amdq[m] = amdq[m] + s[1] * wave[M + m];
+    // the right state (s3 = u_r + c_r) and the state ahead of wave 3 (s2 = u2 + c2)
+    const T ur = rt[MU] * irr, xr = g * pr * irr;
+    const bool s3_gt = xr >= T(0) && (ur > T(0) || xr > ur * ur);
+    const T rho2 = rt[0] - wave[2 * M], m2 = rt[MU] - wave[2 * M + MU], e2 = rt[ME] - wave[2 * M + ME];
+    const T t2 = MV >= 0 ? rt[MV < 0 ? 0 : MV] - wave[2 * M + (MV < 0 ? 0 : MV)] : T(0);
+    const T q2 = m2 * m2 + t2 * t2;
+    const T P2 = g1 * (e2 * rho2 - h * q2);
+    const bool s2_lt = P2 >= T(0) && m2 * rho2 < T(0) && m2 * m2 > g * P2;
+    if (s2_lt && s3_gt) {  // transonic rarefaction in the 3-wave
+        const T s3 = ur + fsqrt(xr);
+        const T ir2 = frcp(rho2);
+        const T p2 = g1 * (e2 - h * q2 * ir2);
+        const T s2 = m2 * ir2 + fsqrt(g * p2 * ir2);
+        sfract = s2 * fdiv(s3 - s[2], s3 - s2);
+    } else if (s[2] < T(0)) {
+        sfract = s[2];
+    } else {
+        return;
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) amdq[m] = amdq[m] + sfract * wave[2 * M + m];
+}
+
+// apps/euler/shockentropy/rp1eu.f:44-213 (efix: rp1eu.f:126-201)
+// With AP = false only amdq is formed, and flf (when not NULL) receives the
+// normal flux of the left state in float64 (the flux form of the stage,
+// stage1d_chunk / stage2d_tile).
+template <class T, bool AP = true>
+__device__ __forceinline__ void rp_eu1(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
+                                       T *s, T *amdq, T *apdq, double *flf = nullptr)
+{
+    const T g1 = c.gamma1, h = T(0.5);
     T rl = lf[0], ml = lf[1], el = lf[2];
     T rr = rt[0], mr = rt[1], er = rt[2];
     T isl = frsq(rl), isr = frsq(rr);  // 1/sqrt(rho)
@@ -145,60 +243,35 @@ __device__ __forceinline__ void rp_eu1(const PhysConst<T> &c, const T *lf, const
     wave[8] = b3 * (enth + u * a);
     s[2] = u + a;
     if (!c.efix) {
-        split_by_sign<3, 3>(wave, s, amdq, apdq);
-        return;
+        if constexpr (AP)
+            split_by_sign<3, 3>(wave, s, amdq, apdq);
+        else
+            split_minus<3, 3>(wave, s, amdq);
+    } else {
+        euler_efix<3, 1, -1, 2>(c, lf, rt, irl, irr, pl, pr, wave, s, amdq);
+        if constexpr (AP) apdq_total<3, 3>(wave, s, amdq, apdq);
     }
-    do {
-        T s0 = ml * irl - fsqrt(g * pl * irl);
-        if (s0 >= T(0) && s[0] > T(0)) {
-            amdq[0] = amdq[1] = amdq[2] = T(0);
-            break;
+    if (flf) {
+        if constexpr (std::is_same<T, double>::value) {
+            const double un = ml * irl;
+            flf[0] = ml;
+            flf[1] = fma(ml, un, pl);
+            flf[2] = (el + pl) * un;
+        } else {
+            normal_flux<EU1, 1, T>(c, lf, flf);
         }
-        T rho1 = rl + wave[0], rhou1 = ml + wave[1], en1 = el + wave[2];
-        T ir1 = frcp(rho1);
-        T p1 = g1 * (en1 - h * (rhou1 * rhou1) * ir1);
-        T s1 = rhou1 * ir1 - fsqrt(g * p1 * ir1);
-        T sfract;
-        if (s0 < T(0) && s1 > T(0))
-            sfract = s0 * fdiv(s1 - s[0], s1 - s0);
-        else if (s[0] < T(0))
-            sfract = s[0];
-        else
-            sfract = T(0);
-        amdq[0] = sfract * wave[0];
-        amdq[1] = sfract * wave[1];
-        amdq[2] = sfract * wave[2];
-        if (s[1] >= T(0)) break;
-        amdq[0] = amdq[0] + s[1] * wave[3];
-        amdq[1] = amdq[1] + s[1] * wave[4];
-        amdq[2] = amdq[2] + s[1] * wave[5];
-        T s3 = mr * irr + fsqrt(g * pr * irr);
-        T rho2 = rr - wave[6], rhou2 = mr - wave[7], en2 = er - wave[8];
-        T ir2 = frcp(rho2);
-        T p2 = g1 * (en2 - h * (rhou2 * rhou2) * ir2);
-        T s2 = rhou2 * ir2 + fsqrt(g * p2 * ir2);
-        if (s2 < T(0) && s3 > T(0))
-            sfract = s2 * fdiv(s3 - s[2], s3 - s2);
-        else if (s[2] < T(0))
-            sfract = s[2];
-        else
-            break;
-        amdq[0] = amdq[0] + sfract * wave[6];
-        amdq[1] = amdq[1] + sfract * wave[7];
-        amdq[2] = amdq[2] + sfract * wave[8];
-    } while (0);
-    apdq_total<3, 3>(wave, s, amdq, apdq);
+    }
 }
 
 // Roe solver for 2D Euler: normal/tangential selection of rpn2sw.f:60-66,
 // Roe averages, wave strengths and entropy fix of rp1eu.f extended by the
 // tangential momentum; the entropy and shear jumps share wave 2 (speed u).
-template <int IXY, class T>
+template <int IXY, class T, bool AP = true>
 __device__ __forceinline__ void rp_eu2(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
-                                       T *s, T *amdq, T *apdq)
+                                       T *s, T *amdq, T *apdq, double *flf = nullptr)
 {
     constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
-    const T g = c.gamma, g1 = c.gamma1, h = T(0.5);
+    const T g1 = c.gamma1, h = T(0.5);
     T rl = lf[0], ul = lf[mu], vl = lf[mv], el = lf[3];
     T rr = rt[0], ur = rt[mu], vr = rt[mv], er = rt[3];
     T isl = frsq(rl), isr = frsq(rr);  // 1/sqrt(rho)
@@ -235,52 +308,37 @@ __device__ __forceinline__ void rp_eu2(const PhysConst<T> &c, const T *lf, const
     wave[11] = a4 * (enth + u * a);
     s[2] = u + a;
     if (!c.efix) {
-        split_by_sign<4, 3>(wave, s, amdq, apdq);
-        return;
+        if constexpr (AP)
+            split_by_sign<4, 3>(wave, s, amdq, apdq);
+        else
+            split_minus<4, 3>(wave, s, amdq);
+    } else {
+        euler_efix<4, mu, mv, 3>(c, lf, rt, irl, irr, pl, pr, wave, s, amdq);
+        if constexpr (AP) apdq_total<4, 3>(wave, s, amdq, apdq);
     }
-    do {
-        T s0 = ul * irl - fsqrt(g * pl * irl);
-        if (s0 >= T(0) && s[0] > T(0)) {
-            amdq[0] = amdq[1] = amdq[2] = amdq[3] = T(0);
-            break;
+    if (flf) {
+        if constexpr (std::is_same<T, double>::value) {
+            const double un = ul * irl;
+            flf[0] = ul;
+            flf[mu] = fma(ul, un, pl);
+            flf[mv] = vl * un;
+            flf[3] = (el + pl) * un;
+        } else {
+            normal_flux<EU2, IXY, T>(c, lf, flf);
         }
-        T rho1 = rl + wave[0], rhou1 = ul + wave[mu], rhov1 = vl + wave[mv], en1 = el + wave[3];
-        T ir1 = frcp(rho1);
-        T p1 = g1 * (en1 - h * (rhou1 * rhou1 + rhov1 * rhov1) * ir1);
-        T s1 = rhou1 * ir1 - fsqrt(g * p1 * ir1);
-        T sfract;
-        if (s0 < T(0) && s1 > T(0))
-            sfract = s0 * fdiv(s1 - s[0], s1 - s0);
-        else if (s[0] < T(0))
-            sfract = s[0];
-        else
-            sfract = T(0);
-#pragma unroll
-        for (int m = 0; m < 4; m++) amdq[m] = sfract * wave[m];
-        if (s[1] >= T(0)) break;
-#pragma unroll
-        for (int m = 0; m < 4; m++) amdq[m] = amdq[m] + s[1] * wave[4 + m];
-        T s3 = ur * irr + fsqrt(g * pr * irr);
-        T rho2 = rr - wave[8], rhou2 = ur - wave[8 + mu], rhov2 = vr - wave[8 + mv], en2 = er - wave[11];
-        T ir2 = frcp(rho2);
-        T p2 = g1 * (en2 - h * (rhou2 * rhou2 + rhov2 * rhov2) * ir2);
-        T s2 = rhou2 * ir2 + fsqrt(g * p2 * ir2);
-        if (s2 < T(0) && s3 > T(0))
-            sfract = s2 * fdiv(s3 - s[2], s3 - s2);
-        else if (s[2] < T(0))
-            sfract = s[2];
-        else
-            break;
-#pragma unroll
-        for (int m = 0; m < 4; m++) amdq[m] = amdq[m] + sfract * wave[8 + m];
-    } while (0);
-    apdq_total<4, 3>(wave, s, amdq, apdq);
+    }
 }
 
-// apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/rp1sw.f:43-183
-template <class T>
+// apps/shallowWater/2dRadialDam/1dRadialSymm-qwave/rp1sw.f:43-183.  The
+// entropy fix (rp1sw.f:120-170) tests the signs of u - c for the left state
+// and the state behind wave 1, and of u + c for the state ahead of wave 2
+// and the right state; as in euler_efix they are decided from squares (for
+// an intermediate state (h1, m1): u1 - c1 > 0 iff h1 > 0, m1 > 0 and
+// m1^2 > g h1^3), and the speeds are formed only for a transonic
+// rarefaction.  AP and flf as for rp_eu1.
+template <class T, bool AP = true>
 __device__ __forceinline__ void rp_sw1(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
-                                       T *s, T *amdq, T *apdq)
+                                       T *s, T *amdq, T *apdq, double *flf = nullptr)
 {
     const T gr = c.grav, h = T(0.5);
     T hl = lf[0], hul = lf[1], hr = rt[0], hur = rt[1];
@@ -299,45 +357,68 @@ __device__ __forceinline__ void rp_sw1(const PhysConst<T> &c, const T *lf, const
     wave[3] = a2 * (ubar + cbar);
     s[1] = ubar + cbar;
     if (!c.efix) {
-        split_by_sign<2, 2>(wave, s, amdq, apdq);
-        return;
+        if constexpr (AP)
+            split_by_sign<2, 2>(wave, s, amdq, apdq);
+        else
+            split_minus<2, 2>(wave, s, amdq);
+    } else {
+        do {
+            // left state: s0 = u_l - c_l
+            const T ul = hul * ihl, xl = gr * hl;
+            const bool s0_ge = xl >= T(0) && ul >= T(0) && ul * ul >= xl;
+            const bool s0_lt = xl >= T(0) && !s0_ge;
+            if (s0_ge && s[0] > T(0)) {
+                amdq[0] = amdq[1] = T(0);
+                break;
+            }
+            // behind wave 1: s1 = u1 - c1
+            const T hr1 = hl + wave[0], uhr1 = hul + wave[1];
+            const bool s1_gt = hr1 > T(0) && uhr1 > T(0) && uhr1 * uhr1 > gr * (hr1 * hr1) * hr1;
+            T sfract;
+            if (s0_lt && s1_gt) {
+                const T s0 = ul - fsqrt(xl);
+                const T s1 = fdiv(uhr1, hr1) - fsqrt(gr * hr1);
+                sfract = s0 * fdiv(s1 - s[0], s1 - s0);
+            } else if (s[0] < T(0)) {
+                sfract = s[0];
+            } else {
+                sfract = T(0);
+            }
+            amdq[0] = sfract * wave[0];
+            amdq[1] = sfract * wave[1];
+            // right state (s3 = u_r + c_r) and ahead of wave 2 (s2 = u2 + c2)
+            const T ur = hur * ihr, xr = gr * hr;
+            const bool s3_gt = xr >= T(0) && (ur > T(0) || xr > ur * ur);
+            const T hl2 = hr - wave[2], uhl2 = hur - wave[3];
+            const bool s2_lt = hl2 > T(0) && uhl2 < T(0) && uhl2 * uhl2 > gr * (hl2 * hl2) * hl2;
+            if (s2_lt && s3_gt) {
+                const T s3 = ur + fsqrt(xr);
+                const T s2 = fdiv(uhl2, hl2) + fsqrt(gr * hl2);
+                sfract = s2 * fdiv(s3 - s[1], s3 - s2);
+            } else if (s[1] < T(0)) {
+                sfract = s[1];
+            } else {
+                break;
+            }
+            amdq[0] = amdq[0] + sfract * wave[2];
+            amdq[1] = amdq[1] + sfract * wave[3];
+        } while (0);
+        if constexpr (AP) apdq_total<2, 2>(wave, s, amdq, apdq);
     }
-    do {
-        T s0 = hul * ihl - fsqrt(gr * hl);
-        if (s0 >= T(0) && s[0] > T(0)) {
-            amdq[0] = amdq[1] = T(0);
-            break;
+    if (flf) {
+        if constexpr (std::is_same<T, double>::value) {
+            flf[0] = hul;
+            flf[1] = fma(hul, hul * ihl, 0.5 * gr * (hl * hl));
+        } else {
+            normal_flux<SW1, 1, T>(c, lf, flf);
         }
-        T hr1 = hl + wave[0], uhr1 = hul + wave[1];
-        T s1 = fdiv(uhr1, hr1) - fsqrt(gr * hr1);
-        T sfract;
-        if (s0 < T(0) && s1 > T(0))
-            sfract = s0 * fdiv(s1 - s[0], s1 - s0);
-        else if (s[0] < T(0))
-            sfract = s[0];
-        else
-            sfract = T(0);
-        amdq[0] = sfract * wave[0];
-        amdq[1] = sfract * wave[1];
-        T s3 = hur * ihr + fsqrt(gr * hr);
-        T hl2 = hr - wave[2], uhl2 = hur - wave[3];
-        T s2 = fdiv(uhl2, hl2) + fsqrt(gr * hl2);
-        if (s2 < T(0) && s3 > T(0))
-            sfract = s2 * fdiv(s3 - s[1], s3 - s2);
-        else if (s[1] < T(0))
-            sfract = s[1];
-        else
-            break;
-        amdq[0] = amdq[0] + sfract * wave[2];
-        amdq[1] = amdq[1] + sfract * wave[3];
-    } while (0);
-    apdq_total<2, 2>(wave, s, amdq, apdq);
+    }
 }
 
 // apps/shallowWater/2dRadialDam/rpn2sw.f:60-228
-template <int IXY, class T>
+template <int IXY, class T, bool AP = true>
 __device__ __forceinline__ void rp_sw2(const PhysConst<T> &c, const T *lf, const T *rt, T *wave,
-                                       T *s, T *amdq, T *apdq)
+                                       T *s, T *amdq, T *apdq, double *flf = nullptr)
 {
     constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
     const T gr = c.grav, h = T(0.5);
@@ -365,9 +446,11 @@ __device__ __forceinline__ void rp_sw2(const PhysConst<T> &c, const T *lf, const
     wave[6 + mv] = a3 * v;
     s[2] = u + a;
     if (!c.efix) {
-        split_by_sign<3, 3>(wave, s, amdq, apdq);
-        return;
-    }
+        if constexpr (AP)
+            split_by_sign<3, 3>(wave, s, amdq, apdq);
+        else
+            split_minus<3, 3>(wave, s, amdq);
+    } else {
     do {
         T him1 = lf[0];
         T s0 = fdiv(lf[mu], him1) - fsqrt(gr * him1);
@@ -402,7 +485,19 @@ __device__ __forceinline__ void rp_sw2(const PhysConst<T> &c, const T *lf, const
 #pragma unroll
         for (int m = 0; m < 3; m++) amdq[m] = amdq[m] + sfract * wave[6 + m];
     } while (0);
-    apdq_total<3, 3>(wave, s, amdq, apdq);
+    if constexpr (AP) apdq_total<3, 3>(wave, s, amdq, apdq);
+    }
+    if (flf) {
+        if constexpr (std::is_same<T, double>::value) {
+            const double ih = isl * isl, hn = lf[mu];
+            const double un = hn * ih;
+            flf[0] = hn;
+            flf[mu] = fma(hn, un, 0.5 * gr * (lf[0] * lf[0]));
+            flf[mv] = lf[mv] * un;
+        } else {
+            normal_flux<SW2, IXY, T>(c, lf, flf);
+        }
+    }
 }
 
 // ------------------------------------------- in-cell total fluctuation
@@ -454,8 +549,14 @@ __device__ __forceinline__ void normal_flux(const PhysConst<T> &c, const T *q, d
     }
 }
 
-// 2D physics whose in-cell total fluctuation is taken from normal_flux (the
-// 1D stage decides per reconstruction, stage1d_chunk)
+// 2D physics whose stage takes the flux form (stage2d_tile; the 1D stage
+// decides per reconstruction, line_chunk).  With the in-cell total
+// fluctuation f(qr(i)) - f(ql(i)) and, at interface i-1/2, apdq(i) =
+// f(ql(i)) - f(qr(i-1)) - amdq(i) (Roe's property), the reference's
+// X(i) = amdq(i+1) + apdq(i) + amdq2(i) + apdq2(i) (flux1.f90:183-186) is
+// F(i+1/2) - F(i-1/2) with the interface flux F(i-1/2) = f(qr(i-1)) +
+// amdq(i): one normal flux and amdq per interface, no apdq and no in-cell
+// problem.  It agrees with the reference to rounding.
 template <int P> constexpr bool incell_by_flux() { return P == EU2 || P == SW2; }
 
 // adq = amdq2 + apdq2 of the in-cell problem (left ql, right qr) as
@@ -483,6 +584,22 @@ __device__ __forceinline__ void riemann(const PhysConst<T> &c, const T *lf, cons
     else if constexpr (P == SW1) rp_sw1(c, lf, rt, wave, s, amdq, apdq);
     else if constexpr (P == EU2) rp_eu2<IXY>(c, lf, rt, wave, s, amdq, apdq);
     else rp_sw2<IXY>(c, lf, rt, wave, s, amdq, apdq);
+}
+
+// The Riemann solve of the flux form of a stage (Euler and shallow water,
+// incell_by_flux): wave speeds s, amdq, and the float64 normal flux flf of
+// the left state; the numerical flux of the interface is flf + amdq.
+template <int P, int IXY, class T>
+__device__ __forceinline__ void riemann_flux(const PhysConst<T> &c, const T *lf, const T *rt, T *wave, T *s,
+                                             T *amdq, double *flf)
+{
+    if constexpr (P == EU1) rp_eu1<T, false>(c, lf, rt, wave, s, amdq, nullptr, flf);
+    else if constexpr (P == SW1) rp_sw1<T, false>(c, lf, rt, wave, s, amdq, nullptr, flf);
+    else if constexpr (P == EU2) rp_eu2<IXY, T, false>(c, lf, rt, wave, s, amdq, nullptr, flf);
+    else {
+        static_assert(P == SW2, "riemann_flux: Euler and shallow water only");
+        rp_sw2<IXY, T, false>(c, lf, rt, wave, s, amdq, nullptr, flf);
+    }
 }
 
 // --------------------------------------------------------------- evec

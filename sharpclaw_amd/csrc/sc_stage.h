@@ -420,12 +420,29 @@ __device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, co
     for (int k = 0; k < 6; k++)
 #pragma unroll
         for (int m = 0; m < M; m++) v[k][m] = SQ(m, t + k, ln);
+    // Euler and shallow water with weno5 / weno5_char (the reconstructions of
+    // batches and the server) take the flux form of the stage (riemann_flux:
+    // X(i) = F(i+1/2) - F(i-1/2), F = f(qr) + amdq of the interface, which is
+    // the reference's amdq(i+1) + apdq(i) + amdq2(i) + apdq2(i) through Roe's
+    // property); the other reconstructions keep apdq and the swapped Roe
+    // solve of the in-cell problem, so their results stay the reference's
+    // also where a state is unphysical (NaN at the Shu-Osher shock with
+    // weno5_wave)
+    constexpr bool BYFLUX = (P == EU1 || P == SW1) && (CHAR == RK_WENO || CHAR == RK_WENO_CHAR);
+    // the flux form's F is float64: sx holds M NT NLN of them (aligned to 8)
+    auto SXD = [&](int m, int k, int l) -> double & {
+        return reinterpret_cast<double *>(sx)[(m * NT + k) * NLN + l];
+    };
     T lf[M], rt[M], wave[M * W], s[W], am[M], ap[M];
+    double Fo[M];
     if constexpr (CHAR == RK_WENO || CHAR == RK_WENO_CHAR)
         reconstruct_iface<P, IXY, CHAR, T>(c, v, lf, rt);
     else
         reconstruct_other<P, IXY, CHAR, T>(c, v, sw + ln * NB * NW, NW, t + 2, lf, rt);
-    riemann<P, IXY, T>(c, lf, rt, wave, s, am, ap);
+    if constexpr (BYFLUX)
+        riemann_flux<P, IXY, T>(c, lf, rt, wave, s, am, Fo);
+    else
+        riemann<P, IXY, T>(c, lf, rt, wave, s, am, ap);
     T smax = T(0);
     if constexpr (sx_in_sw<P, IXY, CHAR, T, NT>()) __syncthreads();  // sw is read: sx reuses it
     if (i <= n + 1) {
@@ -442,41 +459,45 @@ __device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, co
     }
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        SX(m, t, ln) = lf[m];
-        SX(M + m, t, ln) = am[m];
+        if constexpr (BYFLUX) {
+            Fo[m] = Fo[m] + (double)am[m];
+            SXD(m, t, ln) = Fo[m];
+        } else {
+            SX(m, t, ln) = lf[m];
+            SX(M + m, t, ln) = am[m];
+        }
     }
     __syncthreads();
 
     if (t < NC && i <= n) {
-        // in-cell problem of cell i: left ql(i) = rt, right qr(i) from interface i+1/2.
-        // Euler and shallow water with weno5 / weno5_char (the reconstructions of
-        // batches and the server) take its total fluctuation as f(qr) - f(ql)
-        // (incell_flux_diff); the other reconstructions keep the swapped Roe
-        // solve, so their results stay the reference's also where a state is
-        // unphysical (NaN at the Shu-Osher shock with weno5_wave)
-        constexpr bool BYFLUX = (P == EU1 || P == SW1) && (CHAR == RK_WENO || CHAR == RK_WENO_CHAR);
-        T qrc[M], am2[M], ap2[M];
-        double adq[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) qrc[m] = SX(m, t + 1, ln);
-        if constexpr (BYFLUX)
-            incell_flux_diff<P, IXY, T>(c, rt, qrc, adq);
-        else
-            riemann<P, IXY, T>(c, rt, qrc, wave, s, am2, ap2);
+        // cell i between interfaces i-1/2 (this thread) and i+1/2
         double X[M];
         T qc[M];
+        if constexpr (BYFLUX) {
+            // the same with tfluct_solver = 1 (flux1.f90:150-152): the
+            // shallow-water tfluct (tfluct1sw.f) is the in-cell problem's Roe
+            // wave total, f(qr) - f(ql)
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            // flux1.f90:150-152 (tfluct: the in-cell problem's total
-            // fluctuation adq = amdq + apdq, tfluct*.f of the applications)
-            // or flux1.f90:183-186 (the same problem through rp, swapped)
-            const T amn = SX(M + m, t + 1, ln);
-            if constexpr (BYFLUX)
-                X[m] = c.tfluct ? ((double)ap[m] + adq[m]) + (double)amn : (double)(amn + ap[m]) + adq[m];
-            else
+            for (int m = 0; m < M; m++) {
+                X[m] = SXD(m, t + 1, ln) - Fo[m];
+                qc[m] = SQ(m, t + 3, ln);
+            }
+        } else {
+            // in-cell problem of cell i: left ql(i) = rt, right qr(i) from interface i+1/2
+            T qrc[M], am2[M], ap2[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) qrc[m] = SX(m, t + 1, ln);
+            riemann<P, IXY, T>(c, rt, qrc, wave, s, am2, ap2);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                // flux1.f90:150-152 (tfluct: the in-cell problem's total
+                // fluctuation adq = amdq + apdq, tfluct*.f of the applications)
+                // or flux1.f90:183-186 (the same problem through rp, swapped)
+                const T amn = SX(M + m, t + 1, ln);
                 X[m] = c.tfluct ? (double)((ap[m] + (am2[m] + ap2[m])) + amn)
                                 : (double)(((amn + ap[m]) + am2[m]) + ap2[m]);
-            qc[m] = SQ(m, t + 3, ln);
+                qc[m] = SQ(m, t + 3, ln);
+            }
         }
         emit(ln, i, X, qc);
     }

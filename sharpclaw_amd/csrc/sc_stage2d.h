@@ -52,8 +52,13 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
     constexpr int LXT = TX + 6, LYT = TY + 6;
     constexpr int NIX = (TX + 1) * TY, NIY = TX * (TY + 1);
     constexpr int NI = NIX > NIY ? NIX : NIY;
+    // Euler and shallow water take the flux form (incell_by_flux): each
+    // interface passes its float64 flux F to the cell on its right / above;
+    // 2D advection passes the right state and apdq
+    constexpr bool FF = incell_by_flux<P>();
     __shared__ T sq[M][LYT][LXT];
-    __shared__ T sR[M][NI], sAp[M][NI];
+    __shared__ T sR[FF ? 1 : M][NI], sAp[FF ? 1 : M][NI];
+    __shared__ double sF[FF ? M : 1][NI];
     const int t = threadIdx.x;
     const int nx = a.nx, ny = a.ny;
     const int i0 = 1 + (b % a.ntx) * TX, j0 = 1 + (b / a.ntx) * TY;
@@ -163,6 +168,7 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
     T smx = T(0), smy = T(0);
     T wave[M * W], s[W];
     T lf[M], am[M];  // own interface: left state, amdq
+    double Fo[M];    // own interface's flux (flux form)
     double dqx[M];
 
     // x-interfaces: interface ci of row r lies between tile cells ci-1, ci
@@ -182,7 +188,9 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
 #pragma unroll
                 for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + 3][ci + kk];
             reconstruct_iface<P, 1, CHAR, T>(c, v, lf, rt);
-            if constexpr (P == ADV2)
+            if constexpr (FF)
+                riemann_flux<P, 1, T>(c, lf, rt, wave, s, am, Fo);
+            else if constexpr (P == ADV2)
                 riemann<P, 1, T>(adv_const(c, a, i0 + ci, j0 + r, 0), lf, rt, wave, s, am, ap);
             else
                 riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
@@ -193,8 +201,13 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
             const int k = r * (TX + 1) + ci;
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                sR[m][k] = rt[m];
-                sAp[m][k] = ap[m];
+                if constexpr (FF) {
+                    Fo[m] = Fo[m] + (double)am[m];
+                    sF[m][k] = Fo[m];
+                } else {
+                    sR[m][k] = rt[m];
+                    sAp[m][k] = ap[m];
+                }
             }
         }
     }
@@ -203,18 +216,14 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
         // in-cell problem of cell (cx, cy): ql = right state of the interface
         // on its left, qr = left state of its own interface
         const int kl = cy * (TX + 1) + cx;
-        T ql_[M];
+        if constexpr (FF) {
+            // X = F(i+1/2) - F(i-1/2)
 #pragma unroll
-        for (int m = 0; m < M; m++) ql_[m] = sR[m][kl];
-        if constexpr (incell_by_flux<P>()) {
-            double adq[M];
-            incell_flux_diff<P, 1, T>(c, ql_, lf, adq);
-#pragma unroll
-            for (int m = 0; m < M; m++) {
-                double X = (double)(am[m] + sAp[m][kl]) + adq[m];
-                dqx[m] = -(a.dtdx * X);
-            }
+            for (int m = 0; m < M; m++) dqx[m] = -(a.dtdx * (Fo[m] - sF[m][kl]));
         } else {
+            T ql_[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) ql_[m] = sR[m][kl];
             T am2[M], ap2[M];
             if constexpr (P == ADV2)
                 riemann<P, 1, T>(adv_const(c, a, i0 + cx, j0 + cy, 0), ql_, lf, wave, s, am2, ap2);
@@ -245,7 +254,9 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
 #pragma unroll
                 for (int m = 0; m < M; m++) v[kk][m] = sq[m][r + kk][ci + 3];
             reconstruct_iface<P, 2, CHAR, T>(c, v, lf, rt);
-            if constexpr (P == ADV2)
+            if constexpr (FF)
+                riemann_flux<P, 2, T>(c, lf, rt, wave, s, am, Fo);
+            else if constexpr (P == ADV2)
                 riemann<P, 2, T>(adv_const(c, a, i0 + ci, j0 + r, 1), lf, rt, wave, s, am, ap);
             else
                 riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
@@ -256,8 +267,13 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
             const int k = r * TX + ci;
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                sR[m][k] = rt[m];
-                sAp[m][k] = ap[m];
+                if constexpr (FF) {
+                    Fo[m] = Fo[m] + (double)am[m];
+                    sF[m][k] = Fo[m];
+                } else {
+                    sR[m][k] = rt[m];
+                    sAp[m][k] = ap[m];
+                }
             }
         }
     }
@@ -265,17 +281,15 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
     if (owns_cell) {
         const int i = i0 + cx, j = j0 + cy;
         const int kb = cy * TX + cx;
-        T ql_[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) ql_[m] = sR[m][kb];
         // Y[m] = ((amdq(j+1) + apdq(j)) + amdq2(j)) + apdq2(j)
         double Y[M];
-        if constexpr (incell_by_flux<P>()) {
-            double adq[M];
-            incell_flux_diff<P, 2, T>(c, ql_, lf, adq);
+        if constexpr (FF) {
 #pragma unroll
-            for (int m = 0; m < M; m++) Y[m] = (double)(am[m] + sAp[m][kb]) + adq[m];
+            for (int m = 0; m < M; m++) Y[m] = Fo[m] - sF[m][kb];
         } else {
+            T ql_[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) ql_[m] = sR[m][kb];
             T am2[M], ap2[M];
             if constexpr (P == ADV2)
                 riemann<P, 2, T>(adv_const(c, a, i0 + cx, j0 + cy, 1), ql_, lf, wave, s, am2, ap2);
