@@ -74,6 +74,10 @@ __global__ __launch_bounds__(256) void stage2d_batch_kernel(Batch2DArgs A, PhysC
     a.aux = nullptr;
     a.vscale = 1.0;
     a.full = 0;
+    a.order = TILE_BANDS;
+    a.tflag = nullptr;
+    a.tlist = nullptr;
+    a.tnf = nullptr;
     stage2d_tile<P, CHAR, T, TX, TY>(a, c, local);
 }
 

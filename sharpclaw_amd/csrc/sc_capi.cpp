@@ -8,6 +8,7 @@
 // register r1 (sharpclaw.f90:425), a fifth dq for sc_step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -188,6 +189,13 @@ struct sc_solver {
     int cur, old;      // buffer indices of q and qold
     bool have_q;
     int tile2d;        // 2D tile shape index; -1: tune at first use
+    int tile_order;    // 2D tile order over the workgroups (StageArgs::order)
+    // TILE_ADAPTIVE: per-tile flags of the last stage, the tile list built
+    // from them after each step and its count of non-constant tiles, for the
+    // tile shape tlist_tile (-1: not built)
+    unsigned char *tflag;
+    int *tlist, *tnf;
+    int tlist_tile;
     double *mem;       // one allocation: buf[0..2], qb, dq[, dqx]
     int loop_mode;     // 2D sc_advance: -1 by grid size, 0 host-driven, 1 device-resident
     bool use_graph;    // device loop: replay rounds from a captured graph
@@ -290,6 +298,26 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     if (const char *e = getenv("SC_HOST_LOOP")) s->loop_mode = atoi(e) != 0 ? 0 : 1;
     s->use_graph = false;
     if (const char *e = getenv("SC_GRAPH")) s->use_graph = atoi(e) != 0;
+    s->tile_order = TILE_ADAPTIVE;
+    if (const char *e = getenv("SC_TILE_ORDER")) s->tile_order = atoi(e) == 0 ? TILE_BANDS : TILE_ADAPTIVE;
+    s->tlist_tile = -1;
+    if (s->p.ndim == 2 && !sweeps) {
+        // tile lists for the shape with the most tiles
+        int nmax = 0;
+        for (int k = 0; k < NTILE2D; k++) {
+            int tx, ty;
+            tile2d_shape(k, &tx, &ty);
+            nmax = std::max(nmax, ((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty));
+        }
+        void *tm = nullptr;
+        if (hipMalloc(&tm, (size_t)nmax * (sizeof(int) + 1) + 2 * sizeof(int)) != hipSuccess) {
+            sc_destroy(s);
+            return sc_fail(SC_ERR_NOMEM, "device allocation of the tile lists (%d tiles) failed", nmax);
+        }
+        s->tnf = (int *)tm;
+        s->tlist = s->tnf + 2;
+        s->tflag = (unsigned char *)(s->tlist + nmax);
+    }
     if (const char *e = getenv("SC_TILE")) {
         int k = atoi(e);
         if (k >= 0 && k < NTILE2D) s->tile2d = k;
@@ -333,6 +361,7 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->capa) hipFree(s->capa);
     if (s->aux2) hipFree(s->aux2);
+    if (s->tnf) hipFree(s->tnf);
     for (int k = 0; k < s->tev_cap; k++)
         if (s->tev[k]) hipEventDestroy(s->tev[k]);
     free(s->tev);
@@ -572,6 +601,7 @@ static StageArgs base_args(const sc_solver *s)
     a.vscale = s->vscale;
     a.xlo = s->p.xlower[0];
     a.dxc = s->p.dx[0];
+    a.order = TILE_BANDS;  // time steps set the solver's tile order (enqueue_ts)
     return a;
 }
 
@@ -786,6 +816,28 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
     a.qbin = s->qb;
     a.qbout = s->qb;
     StagePlan P = plan_of(s->p.time_integrator);
+    // the tile order of the 2D tile stage (TILE_ADAPTIVE: the tile list of
+    // the last step, rebuilt after this step's last stage)
+    const bool tiles = s->p.ndim == 2 && !s->dqx;
+    int ntiles = 0;
+    if (tiles) {
+        a.order = s->tile_order;
+        int tx, ty;
+        const int shape = s->tile2d < 0 ? 0 : s->tile2d;
+        tile2d_shape(shape, &tx, &ty);
+        ntiles = ((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty);
+        if (a.order == TILE_ADAPTIVE && s->tnf) {
+            if (s->tlist_tile != shape) {
+                HIPCHK(launch_tile_list_reset(s->tflag, ntiles, s->tlist, s->tnf, s->st));
+                s->tlist_tile = shape;
+            }
+            a.tflag = s->tflag;
+            a.tlist = s->tlist;
+            a.tnf = s->tnf;
+        } else {
+            a.order = TILE_BANDS;
+        }
+    }
     int in = s->cur;
     for (int k = 0; k < P.n; k++) {
         int outb = free_buf(s, in, s->old);
@@ -808,6 +860,7 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
         HIPCHK(e);
         in = outb;
     }
+    if (a.order == TILE_ADAPTIVE) HIPCHK(launch_tile_partition(s->tflag, ntiles, s->tlist, s->tnf, s->st));
     s->cur = in;
     return SC_OK;
 }
@@ -1060,6 +1113,14 @@ extern "C" int sc_time_stage(sc_solver *s, int n, double dt, double *ms_per_stag
     a.qold = s->buf[q0];
     a.qbin = s->qb;
     a.qbout = s->qb;
+    // the time steps' tile order, with the tile list of the last step (not
+    // rebuilt here)
+    if (s->p.ndim == 2 && !s->dqx && s->tile_order == TILE_ADAPTIVE && s->tnf &&
+        s->tlist_tile == (s->tile2d < 0 ? 0 : s->tile2d)) {
+        a.order = TILE_ADAPTIVE;
+        a.tlist = s->tlist;
+        a.tnf = s->tnf;
+    }
     StagePlan P = plan_of(s->p.time_integrator);
     // every step restarts from q0, which no stage writes; as in a real time
     // step only the last stage reduces the CFL (into a scratch word)

@@ -22,6 +22,8 @@ enum StageOp {
     OP_SSP104_10 = 9, // out = qb + 0.6*q + 0.1*r               (sharpclaw.f90:462)
 };
 
+enum TileOrder { TILE_BANDS = 0, TILE_ADAPTIVE = 1 };
+
 struct StageArgs {
     const double *qin;   // stage input, grid layout with ghosts
     const double *qold;  // q^n
@@ -48,6 +50,14 @@ struct StageArgs {
     // 2D: 1 = every tile takes the full path (no constant-tile shortcut);
     // the tile-shape tuner times the full path
     int full;
+    // 2D: order of the tiles over the workgroups (TILE_*, tile_of_workgroup).
+    // TILE_ADAPTIVE takes the tile list of the last tile_partition (tlist,
+    // *tnf); every 2D stage writes each tile's flag (1: not constant) to
+    // tflag when it is not NULL.
+    int order;
+    unsigned char *tflag;
+    const int *tlist;
+    const int *tnf;
 };
 
 // physics selector of the launchers (sc_physics.h PhysId)
@@ -74,6 +84,11 @@ constexpr int NTILE2D = 4;
 hipError_t launch_stage2d(const PhysSel &s, int tile, const StageArgs &a, const PhysConst<double> &cd,
                           const PhysConst<float> &cf, hipStream_t st);
 void tile2d_shape(int tile, int *tx, int *ty);
+// The tile list of TILE_ADAPTIVE from the flags of the last stage launch: the
+// tiles that were not constant (in tile order), then the constant ones;
+// *nf = their number.  launch_tile_list_reset: no flags, list = tile order.
+hipError_t launch_tile_partition(const unsigned char *flag, int ntiles, int *list, int *nf, hipStream_t st);
+hipError_t launch_tile_list_reset(unsigned char *flag, int ntiles, int *list, int *nf, hipStream_t st);
 // ghost fill in place (bc1.f90 / bc2.f90 semantics, corners included)
 hipError_t launch_bc(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY, const int *bc,
                      hipStream_t st);

@@ -71,14 +71,95 @@ __global__ __launch_bounds__(NT * NLN) void sweep2d_kernel(StageArgs a, PhysCons
 
 // ------------------------------------------------------ 2D fused stage
 
+// The tile a workgroup takes.  Workgroups are dealt to the 8 XCDs
+// round-robin (workgroup w runs on XCD w mod 8) and each XCD has its own L2.
+// TILE_BANDS: each XCD a contiguous band of tiles (tile rows), so halo rows
+// shared by neighbouring tiles stay in one L2.  TILE_ADAPTIVE: the tiles that
+// were not constant at the last step first, dealt round-robin over the XCDs
+// (a non-constant tile takes several times as long as a constant one, so
+// these start first and spread evenly over the XCDs), then the constant ones,
+// each XCD a contiguous run of them in tile order (tile_partition builds the
+// list after each step).  Piecewise-constant data such as a four-quadrant
+// Riemann problem has its non-constant tiles in a few rows of tiles, which
+// TILE_BANDS would give to one or two XCDs, and in tile order they would
+// start late and set the launch's tail.
+__device__ __forceinline__ int tile_of_workgroup(const StageArgs &a, int w)
+{
+    const int ntiles = a.ntx * a.nty;
+    if (w >= ntiles) return -1;
+    if (a.order == TILE_ADAPTIVE) {
+        const int nf = *a.tnf;
+        if (w < nf) return a.tlist[w];
+        const int x = w & 7;
+        int start = 0;
+        for (int y = 0; y < x; y++) {
+            const int w0 = nf + ((y - nf) & 7);  // first constant-tile workgroup of XCD y
+            if (w0 < ntiles) start += (ntiles - w0 + 7) >> 3;
+        }
+        const int w0 = nf + ((x - nf) & 7);
+        return a.tlist[nf + start + ((w - w0) >> 3)];
+    }
+    if ((ntiles & 7) == 0) return (w & 7) * (ntiles >> 3) + (w >> 3);
+    return w;
+}
+
+// Stable partition of the tiles by their flags (one block of 1024 threads):
+// each thread counts the flags of a run of tiles, a block scan places the
+// runs.
+__global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned char *flag, int n, int *list, int *nf)
+{
+    __shared__ int scan[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int lo = min(n, t * per), hi = min(n, lo + per);
+    int cnt = 0;
+    for (int i = lo; i < hi; i++) cnt += flag[i] != 0;
+    scan[t] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? scan[t - off] : 0;
+        __syncthreads();
+        scan[t] += v;
+        __syncthreads();
+    }
+    const int total = scan[1023];
+    int pf = scan[t] - cnt;       // flagged tiles before this run
+    int pc = total + (lo - pf);   // unflagged tiles before this run, after all flagged ones
+    for (int i = lo; i < hi; i++) {
+        if (flag[i])
+            list[pf++] = i;
+        else
+            list[pc++] = i;
+    }
+    if (t == 0) *nf = total;
+}
+
+__global__ void tile_list_reset_kernel(unsigned char *flag, int n, int *list, int *nf)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        flag[i] = 0;
+        list[i] = i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nf = 0;
+}
+
+hipError_t launch_tile_partition(const unsigned char *flag, int ntiles, int *list, int *nf, hipStream_t st)
+{
+    hipLaunchKernelGGL(tile_partition_kernel, dim3(1), dim3(1024), 0, st, flag, ntiles, list, nf);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_list_reset(unsigned char *flag, int ntiles, int *list, int *nf, hipStream_t st)
+{
+    const int nb = std::min(1024, (ntiles + 255) / 256);
+    hipLaunchKernelGGL(tile_list_reset_kernel, dim3(std::max(nb, 1)), dim3(256), 0, st, flag, ntiles, list, nf);
+    return hipGetLastError();
+}
+
 template <int P, int CHAR, class T, int TX, int TY>
 __global__ __launch_bounds__(256) void stage2d_kernel(StageArgs a, PhysConst<T> c)
 {
-    // XCD-aware tile order: consecutive workgroups go to different XCDs, so
-    // give each XCD a contiguous band of tiles (halo rows stay in its L2)
-    const int ntiles = a.ntx * a.nty;
-    int b = blockIdx.x;
-    if ((ntiles & 7) == 0) b = (b & 7) * (ntiles >> 3) + (b >> 3);
+    const int b = tile_of_workgroup(a, blockIdx.x);
     stage2d_tile<P, CHAR, T, TX, TY>(a, c, b);
 }
 

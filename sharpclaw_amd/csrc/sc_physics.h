@@ -602,6 +602,27 @@ __device__ __forceinline__ void riemann_flux(const PhysConst<T> &c, const T *lf,
     }
 }
 
+// The largest |s| in x and in y of the Roe waves at an interface whose two
+// states are both q (every interface of a constant tile): the speeds are
+// u - c, u, u + c (v - c, v, v + c), so |u| + c and |v| + c.  2D Euler and
+// shallow water.
+template <int P, class T>
+__device__ __forceinline__ void flat_speeds(const PhysConst<T> &c, const T *q, T &sx, T &sy)
+{
+    const T ir = frcp(q[0]);
+    const T u = q[1] * ir, v = q[2] * ir;
+    T cs;
+    if constexpr (P == EU2) {
+        const T p = c.gamma1 * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) * ir);
+        cs = fsqrt(c.gamma * p * ir);
+    } else {
+        static_assert(P == SW2, "flat_speeds: 2D Euler and shallow water");
+        cs = fsqrt(c.grav * q[0]);
+    }
+    sx = fabs(u) + cs;
+    sy = fabs(v) + cs;
+}
+
 // --------------------------------------------------------------- evec
 //
 // Left (evl[p][m]) and right (evr[m][p]) eigenvectors at the interface
@@ -764,27 +785,30 @@ __device__ __forceinline__ void roe_euler(const PhysConst<T> &c, const T *qa, co
     a2 = g1 * (H - h * (u * u + v * v));
 }
 
+// Euler: L applied to a jump d is (with Y = q2 d0 - u d1 + d2, Z = u d0 - d1,
+// K = (gamma-1)/c^2) w0 = (K Y + Z / c) / 2, w1 = d0 - K Y, w2 = (K Y - Z / c) / 2;
+// hK = K/2 and hi = 1/(2c) are kept so each costs one fma.
 template <int IXY, class T> struct Eig<EU1, IXY, T> {
-    T u, a, H, q2, b, h2, K;
+    T u, a, H, q2, hK, hi, K;
     __device__ __forceinline__ void init(const PhysConst<T> &c, const T *qa, const T *qb)
     {
         T v, a2;
         roe_euler<1, -1, 2>(c, qa, qb, u, v, H, a2);
         T ia = frsq(a2);
         a = a2 * ia;
-        b = c.gamma1;
-        K = b * (ia * ia);
-        h2 = T(0.5) * (ia * ia);
+        K = c.gamma1 * (ia * ia);
+        hK = T(0.5) * K;
+        hi = T(0.5) * ia;
         q2 = T(0.5) * (u * u);
     }
     __device__ __forceinline__ void to_char(const T *d, T *w) const
     {
         T Y = q2 * d[0] - u * d[1] + d[2];
         T Z = u * d[0] - d[1];
-        T bY = b * Y;
-        w[0] = h2 * (bY + a * Z);
+        T kY = hK * Y;
+        w[0] = fma(hi, Z, kY);
         w[1] = d[0] - K * Y;
-        w[2] = h2 * (bY - a * Z);
+        w[2] = fma(-hi, Z, kY);
     }
     __device__ __forceinline__ void from_char(const T *c, T *o) const
     {
@@ -797,27 +821,27 @@ template <int IXY, class T> struct Eig<EU1, IXY, T> {
 
 template <int IXY, class T> struct Eig<EU2, IXY, T> {
     static constexpr int mu = IXY == 1 ? 1 : 2, mv = IXY == 1 ? 2 : 1;
-    T u, v, a, H, q2, b, h2, K;
+    T u, v, a, H, q2, hK, hi, K;
     __device__ __forceinline__ void init(const PhysConst<T> &c, const T *qa, const T *qb)
     {
         T a2;
         roe_euler<mu, mv, 3>(c, qa, qb, u, v, H, a2);
         T ia = frsq(a2);
         a = a2 * ia;
-        b = c.gamma1;
-        K = b * (ia * ia);
-        h2 = T(0.5) * (ia * ia);
+        K = c.gamma1 * (ia * ia);
+        hK = T(0.5) * K;
+        hi = T(0.5) * ia;
         q2 = T(0.5) * (u * u + v * v);
     }
     __device__ __forceinline__ void to_char(const T *d, T *w) const
     {
         T Y = q2 * d[0] - u * d[mu] - v * d[mv] + d[3];
         T Z = u * d[0] - d[mu];
-        T bY = b * Y;
-        w[0] = h2 * (bY + a * Z);
+        T kY = hK * Y;
+        w[0] = fma(hi, Z, kY);
         w[1] = d[0] - K * Y;
         w[2] = d[mv] - v * d[0];
-        w[3] = h2 * (bY - a * Z);
+        w[3] = fma(-hi, Z, kY);
     }
     __device__ __forceinline__ void from_char(const T *c, T *o) const
     {

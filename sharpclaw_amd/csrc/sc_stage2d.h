@@ -77,6 +77,22 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
 #pragma unroll
         for (int m = 0; m < M; m++) ref[m] = T(a.qin[g0 + m * plane]);
     }
+    // The RK combination's other input for this thread's cell, q^n: a float64
+    // tile loads it now, so that its latency overlaps the staging (constant
+    // tiles then only store); a float tile, whose registers are the tighter
+    // limit on its occupancy, loads it and q itself at the update.
+    constexpr bool PRE = std::is_same<T, double>::value;
+    const bool owns_cell = t < TX * TY;
+    const int cx = owns_cell ? t % TX : 0, cy = owns_cell ? t / TX : 0;
+    const bool upd = owns_cell && i0 + cx <= nx && j0 + cy <= ny;
+    const size_t gc = (size_t)(i0 + cx + 2) + (size_t)a.LX * (size_t)(j0 + cy + 2);
+    const bool rq = op_reads_qold(a.op);
+    double qo_c[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) qo_c[m] = PRE && upd && rq ? a.qold[gc + m * plane] : 0.0;
+    // q^n and q of this thread's cell at the update
+    auto qold_at = [&](int m) { return PRE ? qo_c[m] : (rq ? a.qold[gc + m * plane] : 0.0); };
+    auto q_at = [&](int m) { return PRE ? (double)sq[m][cy + 3][cx + 3] : a.qin[gc + m * plane]; };
     bool same = true;
     for (int k = t; k < LYT * LXT; k += NT) {
         int ly = k / LXT, lx = k - ly * LXT;
@@ -102,69 +118,43 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
         }
     }
     const bool flat = __syncthreads_and(same) != 0;
+    if (a.tflag && t == 0) a.tflag[b] = flat ? 0 : 1;
     __shared__ double red[2][NT / 64];
 
-    // A constant tile: the speeds of one x interface (thread 0) and one y
-    // interface (thread 64, another wave) for the CFL, and the RK combination
-    // with dq = 0.  A constant stencil's reconstruction is its central value
-    // (the WENO5 corrections are zeros, at most of the other sign, which the
-    // speeds' magnitudes do not see), and the Riemann solve is the one every
-    // interface of the tile takes, so the CFL words get the bits of the full
-    // path.  2D advection's speeds vary with aux, so it always takes the
-    // full path.
+    // A constant tile: every interface of it has the same left and right
+    // state, the cell value, whose Roe waves are zero, and the stage update
+    // is the RK combination with dq = 0 (a constant stencil's WENO5
+    // corrections are zeros, at most of the other sign).  The wave speeds of
+    // every interface are u - c, u, u + c of that state, so the tile's max
+    // |s| is |u| + c in x and |v| + c in y, which thread 0 forms directly and
+    // merges into the CFL words.  (The full path's Roe averages of two equal
+    // states give the same speeds to rounding.)  2D advection's speeds vary
+    // with aux, so it always takes the full path.
     if constexpr (P != ADV2) {
         if (flat && !a.full) {
-            T cmx = T(0), cmy = T(0);
-            if (t == 0 || t == 64) {
-                T cen[M], wave[M * W], s[W], am[M], ap[M];
+            if (t == 0 && a.smax) {
+                T sx, sy;
+                flat_speeds<P, T>(c, ref, sx, sy);
+                unsigned long long *w = a.smax + ((blockIdx.x + blockIdx.y) % CFL_SUB) * CFL_STRIDE;
+                if (sx > T(0)) atomicMax(w, (unsigned long long)__double_as_longlong((double)sx));
+                if (sy > T(0)) atomicMax(w + 1, (unsigned long long)__double_as_longlong((double)sy));
+            }
+            if (upd) {
 #pragma unroll
                 for (int m = 0; m < M; m++) {
-                    const T v = sq[m][3][3];
-                    cen[m] = weno_central(v, v, v, v);
+                    const size_t gm = gc + m * plane;
+                    double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
+                    double out, qbo = 0.0;
+                    combine(a.op, qold_at(m), q_at(m), 0.0, qbi, out, qbo);
+                    a.qout[gm] = out;
+                    if (a.op == OP_SSP104_5) a.qbout[gm] = qbo;
                 }
-                T sm = T(0);
-                if (t == 0)
-                    riemann<P, 1, T>(c, cen, cen, wave, s, am, ap);
-                else
-                    riemann<P, 2, T>(c, cen, cen, wave, s, am, ap);
-#pragma unroll
-                for (int w = 0; w < W; w++) sm = fmax(sm, fabs(s[w]));
-                if (t == 0)
-                    cmx = sm;
-                else
-                    cmy = sm;
-            }
-            if (t < TX * TY) {
-                const int cx0 = t % TX, cy0 = t / TX;
-                const int i = i0 + cx0, j = j0 + cy0;
-                if (i <= nx && j <= ny) {
-                    const bool rq = op_reads_qold(a.op);
-                    size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
-#pragma unroll
-                    for (int m = 0; m < M; m++) {
-                        size_t gm = g + m * plane;
-                        // float64 tiles hold q itself
-                        double q = std::is_same<T, double>::value ? (double)sq[m][cy0 + 3][cx0 + 3] : a.qin[gm];
-                        double qo = rq ? a.qold[gm] : 0.0;
-                        double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
-                        double out, qbo = 0.0;
-                        combine(a.op, qo, q, 0.0, qbi, out, qbo);
-                        a.qout[gm] = out;
-                        if (a.op == OP_SSP104_5) a.qbout[gm] = qbo;
-                    }
-                }
-            }
-            if (a.smax) {
-                merge_smax<T, NT / 64>(a.smax, cmx, red[0]);
-                merge_smax<T, NT / 64>(a.smax + 1, cmy, red[1]);
             }
             return;
         }
     }
 
-    const bool owns_cell = t < TX * TY;
     const int e = t - TX * TY;  // edge-interface index of the spare threads
-    const int cx = owns_cell ? t % TX : 0, cy = owns_cell ? t / TX : 0;
     T smx = T(0), smy = T(0);
     T wave[M * W], s[W];
     T lf[M], am[M];  // own interface: left state, amdq
@@ -279,7 +269,6 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
     }
     __syncthreads();
     if (owns_cell) {
-        const int i = i0 + cx, j = j0 + cy;
         const int kb = cy * TX + cx;
         // Y[m] = ((amdq(j+1) + apdq(j)) + amdq2(j)) + apdq2(j)
         double Y[M];
@@ -298,19 +287,14 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
 #pragma unroll
             for (int m = 0; m < M; m++) Y[m] = (double)(((am[m] + sAp[m][kb]) + am2[m]) + ap2[m]);
         }
-        if (i <= nx && j <= ny) {
-            const bool rq = op_reads_qold(a.op);
-            size_t g = (size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2);
+        if (upd) {
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 double r = dqx[m] + -(a.dtdy * Y[m]);
-                size_t gm = g + m * plane;
-                // float64 tiles hold q itself
-                double q = std::is_same<T, double>::value ? (double)sq[m][cy + 3][cx + 3] : a.qin[gm];
-                double qo = rq ? a.qold[gm] : 0.0;
+                size_t gm = gc + m * plane;
                 double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
                 double out, qbo = 0.0;
-                combine(a.op, qo, q, r, qbi, out, qbo);
+                combine(a.op, qold_at(m), q_at(m), r, qbi, out, qbo);
                 a.qout[gm] = out;
                 if (a.op == OP_SSP104_5) a.qbout[gm] = qbo;
             }

@@ -68,9 +68,10 @@ __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, flo
     const float t1 = im * (a2 - a1);
     const float t2 = im * (a1 - a0);
     const float t3 = im * (a0 - am1);
-    const float g1 = dsq(e3 + fmaf(c13, t1 * t1, dsq(a2 - 3.0f * a1)));
-    const float g2 = dsq(e3 + fmaf(c13, t2 * t2, dsq(a1 + a0)));
-    const float g3 = dsq(e3 + fmaf(c13, t3 * t3, dsq(3.0f * a0 - am1)));
+    const float b1 = a2 - 3.0f * a1, b2 = a1 + a0, b3 = 3.0f * a0 - am1;
+    const float g1 = dsq(fmaf(c13, t1 * t1, fmaf(b1, b1, e3)));
+    const float g2 = dsq(fmaf(c13, t2 * t2, fmaf(b2, b2, e3)));
+    const float g3 = dsq(fmaf(c13, t3 * t3, fmaf(b3, b3, e3)));
     const float s1 = g2 * g3;
     const float u = g1 * g2;
     const float d = fmaf(3.0f, u, fmaf(6.0f * g1, g3, s1));
@@ -85,7 +86,8 @@ __device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, flo
 // differences f21 = t2 - t1, f32 = t3 - t2 (f15 = 1.5 f32, f12 = f32 / 12).
 // The reference's tt_k = (eps + 13 p_k + 3 b_k^2)^2 enter only through
 // ratios of products of two of them, so the common factor 9 is divided out:
-// g_k = tt_k / 9 = (eps/3 + (13/3) p_k + b_k^2)^2.  With u = g1 g2 and
+// g_k = tt_k / 9 = (eps/3 + (13/3) p_k + b_k^2)^2 (eps/3 enters through the
+// fma that squares b_k).  With u = g1 g2 and
 // D = g2 g3 + 6 g1 g3 + 3 u (the reference's s1 + s2 + s3 over 81) the
 // correction (s1 (t2-t1) + (s3/2 - 1/4)(t3-t2)) / 3 over (s1+s2+s3) is
 // (g2 g3 f21 + 1.5 u f32) / (3 D) - f32 / 12.
@@ -93,9 +95,9 @@ __device__ __forceinline__ double weno_w(double p1, double p2, double p3, double
                                          double b3, double f21, double f15, double f12)
 {
     constexpr double c13 = 13.0 / 3.0, e3 = SC_EPWENO / 3.0;
-    const double g1 = dsq(e3 + fma(c13, p1, b1 * b1));
-    const double g2 = dsq(e3 + fma(c13, p2, b2 * b2));
-    const double g3 = dsq(e3 + fma(c13, p3, b3 * b3));
+    const double g1 = dsq(fma(c13, p1, fma(b1, b1, e3)));
+    const double g2 = dsq(fma(c13, p2, fma(b2, b2, e3)));
+    const double g3 = dsq(fma(c13, p3, fma(b3, b3, e3)));
     const double s1 = g2 * g3;
     const double u = g1 * g2;
     const double d = fma(3.0, u, fma(6.0 * g1, g3, s1));
