@@ -184,7 +184,7 @@ struct sc_solver {
     double *sums;      // sums result
     int nblk_sums;
     unsigned long long *slots;  // CFL words (device)
-    unsigned long long *hslot;  // pinned readback
+    double *hcfl;      // pinned: the CFL set's x and y maxima (read_cfl)
     long long slot_ctr;
     int cur, old;      // buffer indices of q and qold
     bool have_q;
@@ -276,7 +276,7 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     ok = ok && hipMalloc(&s->partial, (size_t)s->nblk_sums * s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->sums, s->p.meqn * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&s->slots, NSLOT * CFL_WORDS * sizeof(unsigned long long)) == hipSuccess;
-    ok = ok && hipHostMalloc(&s->hslot, CFL_WORDS * sizeof(unsigned long long), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc(&s->hcfl, 2 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
     if (ok) {
         for (int k = 0; k < 3 && ok; k++)
@@ -356,7 +356,7 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->partial) hipFree(s->partial);
     if (s->sums) hipFree(s->sums);
     if (s->slots) hipFree(s->slots);
-    if (s->hslot) hipHostFree(s->hslot);
+    if (s->hcfl) hipHostFree(s->hcfl);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->capa) hipFree(s->capa);
@@ -734,14 +734,14 @@ static int next_slot(sc_solver *s, unsigned long long **slot)
     return SC_OK;
 }
 
-static int read_cfl(sc_solver *s, unsigned long long *w, double dt, double *cfl)
+// The CFL number of the CFL set w: a one-wave kernel writes the set's two
+// words to pinned host memory (s->hcfl), unless the step's tile partition
+// already did (sent), and the host reads them once the stream has passed it.
+static int read_cfl(sc_solver *s, unsigned long long *w, double dt, double *cfl, bool sent = false)
 {
-    HIPCHK(hipMemcpyAsync(s->hslot, w, CFL_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->st));
+    if (!sent) HIPCHK(launch_cfl_to_host(w, s->hcfl, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
-    double sx, sy;
-    const unsigned long long wx = cfl_word(s->hslot, 0), wy = cfl_word(s->hslot, 1);
-    memcpy(&sx, &wx, 8);
-    memcpy(&sy, &wy, 8);
+    double sx = s->hcfl[0], sy = s->hcfl[1];
     if (s->sel.phys == ADV2) {
         // the ghost rows / columns the reference also sweeps (flux2.f90:36,68)
         sx = std::fmax(sx, std::fabs(s->vscale) * s->auxg[0]);
@@ -806,8 +806,9 @@ static StagePlan plan_of(int ti)
 }
 
 // enqueue one time step; returns the CFL word of its last stage
-static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
+static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot, bool *sent)
 {
+    *sent = false;
     StageArgs a = base_args(s);
     a.dtdx = dt / s->p.dx[0];
     a.dt = dt;
@@ -860,7 +861,10 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot)
         HIPCHK(e);
         in = outb;
     }
-    if (a.order == TILE_ADAPTIVE) HIPCHK(launch_tile_partition(s->tflag, ntiles, s->tlist, s->tnf, s->st));
+    if (a.order == TILE_ADAPTIVE) {
+        HIPCHK(launch_tile_partition(s->tflag, ntiles, s->tlist, s->tnf, *slot, s->hcfl, s->st));
+        *sent = true;
+    }
     s->cur = in;
     return SC_OK;
 }
@@ -890,9 +894,10 @@ extern "C" int sc_ts(sc_solver *s, double dt, double told, double *cfl)
         s->vt_stale = false;
     }
     unsigned long long *slot = nullptr;
-    rc = enqueue_ts(s, dt, &slot);
+    bool sent = false;
+    rc = enqueue_ts(s, dt, &slot, &sent);
     if (rc) return rc;
-    return read_cfl(s, slot, dt, cfl);
+    return read_cfl(s, slot, dt, cfl, sent);
 }
 
 static int stages_of(int ti)
@@ -1030,10 +1035,11 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
         retake = false;
         t = told + dt;
         unsigned long long *slot = nullptr;
-        int rc = enqueue_ts(s, dt, &slot);
+        bool sent = false;
+        int rc = enqueue_ts(s, dt, &slot, &sent);
         if (rc) return rc;
         st->stage_evals += stages_of(s->p.time_integrator);
-        rc = read_cfl(s, slot, dt, &cfl);
+        rc = read_cfl(s, slot, dt, &cfl, sent);
         if (rc) return rc;
         if (c->dt_variable == 1) {
             if (cfl > 0.0) {

@@ -103,28 +103,61 @@ __device__ __forceinline__ int tile_of_workgroup(const StageArgs &a, int w)
     return w;
 }
 
-// Stable partition of the tiles by their flags (one block of 1024 threads):
-// each thread counts the flags of a run of tiles, a block scan places the
-// runs.
-__global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned char *flag, int n, int *list, int *nf)
+// The CFL set's two words (the max over their CFL_SUB copies) as doubles
+// into out[0..1] (pinned host memory: the host reads them after the stream
+// has passed the kernel).  One wave: lane k reads copy k mod 32 of word k / 32.
+__device__ __forceinline__ void cfl_set_to(const unsigned long long *set, double *out)
 {
-    __shared__ int scan[1024];
-    const int t = threadIdx.x;
+    const int k = threadIdx.x;
+    long long v = (long long)set[(k & 31) * CFL_STRIDE + (k >> 5)];
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) {
+        const long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((k & 31) == 0) out[k >> 5] = __longlong_as_double(v);
+}
+
+__global__ __launch_bounds__(64) void cfl_to_host_kernel(const unsigned long long *set, double *out)
+{
+    cfl_set_to(set, out);
+}
+
+// Stable partition of the tiles by their flags (one block of 1024 threads):
+// each thread counts the flags of a run of tiles and a two-level scan (wave
+// prefix sums, then the 16 wave totals) places the runs.  With set != NULL
+// the first wave also hands the step's CFL set to the host (cfl_set_to).
+__global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned char *flag, int n, int *list, int *nf,
+                                                              const unsigned long long *set, double *out)
+{
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    if (set && wid == 0) cfl_set_to(set, out);
     const int per = (n + 1023) / 1024;
     const int lo = min(n, t * per), hi = min(n, lo + per);
     int cnt = 0;
     for (int i = lo; i < hi; i++) cnt += flag[i] != 0;
-    scan[t] = cnt;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = t >= off ? scan[t - off] : 0;
-        __syncthreads();
-        scan[t] += v;
-        __syncthreads();
+    int v = cnt;  // inclusive prefix sum over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
     }
-    const int total = scan[1023];
-    int pf = scan[t] - cnt;       // flagged tiles before this run
-    int pc = total + (lo - pf);   // unflagged tiles before this run, after all flagged ones
+    if (lane == 63) wsum[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        int s = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            const int o = __shfl_up(s, off, 64);
+            if (lane >= off) s += o;
+        }
+        if (lane < 16) wsum[lane] = s;
+    }
+    __syncthreads();
+    const int total = wsum[15];
+    int pf = v - cnt + (wid > 0 ? wsum[wid - 1] : 0);  // flagged tiles before this run
+    int pc = total + (lo - pf);                          // unflagged ones before it, after all flagged
     for (int i = lo; i < hi; i++) {
         if (flag[i])
             list[pf++] = i;
@@ -143,9 +176,16 @@ __global__ void tile_list_reset_kernel(unsigned char *flag, int n, int *list, in
     if (blockIdx.x == 0 && threadIdx.x == 0) *nf = 0;
 }
 
-hipError_t launch_tile_partition(const unsigned char *flag, int ntiles, int *list, int *nf, hipStream_t st)
+hipError_t launch_tile_partition(const unsigned char *flag, int ntiles, int *list, int *nf,
+                                 const unsigned long long *set, double *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(tile_partition_kernel, dim3(1), dim3(1024), 0, st, flag, ntiles, list, nf);
+    hipLaunchKernelGGL(tile_partition_kernel, dim3(1), dim3(1024), 0, st, flag, ntiles, list, nf, set, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_to_host(const unsigned long long *set, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(cfl_to_host_kernel, dim3(1), dim3(64), 0, st, set, out);
     return hipGetLastError();
 }
 
