@@ -194,7 +194,7 @@ struct sc_solver {
     // from them after each step and its count of non-constant tiles, for the
     // tile shape tlist_tile (-1: not built)
     unsigned char *tflag;
-    int *tlist, *tnf;
+    int *tlist, *tnf, *tcount;
     int tlist_tile;
     double *mem;       // one allocation: buf[0..2], qb, dq[, dqx]
     int loop_mode;     // 2D sc_advance: -1 by grid size, 0 host-driven, 1 device-resident
@@ -310,13 +310,15 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
             nmax = std::max(nmax, ((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty));
         }
         void *tm = nullptr;
-        if (hipMalloc(&tm, (size_t)nmax * (sizeof(int) + 1) + 2 * sizeof(int)) != hipSuccess) {
+        const int nbc = (nmax + 1023) / 1024;
+        if (hipMalloc(&tm, (size_t)(nmax + nbc + 2) * sizeof(int) + nmax) != hipSuccess) {
             sc_destroy(s);
             return sc_fail(SC_ERR_NOMEM, "device allocation of the tile lists (%d tiles) failed", nmax);
         }
         s->tnf = (int *)tm;
         s->tlist = s->tnf + 2;
-        s->tflag = (unsigned char *)(s->tlist + nmax);
+        s->tcount = s->tlist + nmax;
+        s->tflag = (unsigned char *)(s->tcount + nbc);
     }
     if (const char *e = getenv("SC_TILE")) {
         int k = atoi(e);
@@ -820,16 +822,17 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot, bool *
     // the tile order of the 2D tile stage (TILE_ADAPTIVE: the tile list of
     // the last step, rebuilt after this step's last stage)
     const bool tiles = s->p.ndim == 2 && !s->dqx;
-    int ntiles = 0;
+    int ntx = 0, nty = 0;
     if (tiles) {
         a.order = s->tile_order;
         int tx, ty;
         const int shape = s->tile2d < 0 ? 0 : s->tile2d;
         tile2d_shape(shape, &tx, &ty);
-        ntiles = ((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty);
+        ntx = (s->p.nx[0] + tx - 1) / tx;
+        nty = (s->p.nx[1] + ty - 1) / ty;
         if (a.order == TILE_ADAPTIVE && s->tnf) {
             if (s->tlist_tile != shape) {
-                HIPCHK(launch_tile_list_reset(s->tflag, ntiles, s->tlist, s->tnf, s->st));
+                HIPCHK(launch_tile_list_reset(s->tflag, ntx, nty, s->tlist, s->tnf, s->st));
                 s->tlist_tile = shape;
             }
             a.tflag = s->tflag;
@@ -862,7 +865,7 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot, bool *
         in = outb;
     }
     if (a.order == TILE_ADAPTIVE) {
-        HIPCHK(launch_tile_partition(s->tflag, ntiles, s->tlist, s->tnf, *slot, s->hcfl, s->st));
+        HIPCHK(launch_tile_partition(s->tflag, ntx, nty, s->tlist, s->tnf, *slot, s->hcfl, s->tcount, s->st));
         *sent = true;
     }
     s->cur = in;

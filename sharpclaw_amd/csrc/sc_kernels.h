@@ -85,15 +85,16 @@ hipError_t launch_stage2d(const PhysSel &s, int tile, const StageArgs &a, const 
                           const PhysConst<float> &cf, hipStream_t st);
 void tile2d_shape(int tile, int *tx, int *ty);
 // The tile list of TILE_ADAPTIVE from the flags of the last stage launch: the
-// tiles that were not constant (in tile order), then the constant ones;
-// *nf = their number.  launch_tile_list_reset: no flags, list = tile order.
-// With set != NULL it also writes the CFL set's two words to out (as
-// launch_cfl_to_host).
-hipError_t launch_tile_partition(const unsigned char *flag, int ntiles, int *list, int *nf,
-                                 const unsigned long long *set, double *out, hipStream_t st);
+// tiles that were not constant, then the constant ones, each in strip order
+// (strip_tile); *nf = the number of the former.  launch_tile_list_reset: no
+// flags, the list in strip order.  With set != NULL it also writes the CFL
+// set's two words to out (as launch_cfl_to_host).  bcount: ceil(ntiles /
+// 1024) ints of scratch (grids of more than 16384 tiles take two kernels).
+hipError_t launch_tile_partition(const unsigned char *flag, int ntx, int nty, int *list, int *nf,
+                                 const unsigned long long *set, double *out, int *bcount, hipStream_t st);
 // the max over the copies of the x and y words of a CFL set into out[0..1]
 hipError_t launch_cfl_to_host(const unsigned long long *set, double *out, hipStream_t st);
-hipError_t launch_tile_list_reset(unsigned char *flag, int ntiles, int *list, int *nf, hipStream_t st);
+hipError_t launch_tile_list_reset(unsigned char *flag, int ntx, int nty, int *list, int *nf, hipStream_t st);
 // ghost fill in place (bc1.f90 / bc2.f90 semantics, corners included)
 hipError_t launch_bc(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY, const int *bc,
                      hipStream_t st);

@@ -123,20 +123,44 @@ __global__ __launch_bounds__(64) void cfl_to_host_kernel(const unsigned long lon
     cfl_set_to(set, out);
 }
 
-// Stable partition of the tiles by their flags (one block of 1024 threads):
-// each thread counts the flags of a run of tiles and a two-level scan (wave
-// prefix sums, then the 16 wave totals) places the runs.  With set != NULL
-// the first wave also hands the step's CFL set to the host (cfl_set_to).
-__global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned char *flag, int n, int *list, int *nf,
-                                                              const unsigned long long *set, double *out)
+// The tile at position p of the strip order: strips of STRIP tile columns
+// (the last one narrower), each in row order, left to right; grids of at
+// most STRIP tiles per row are in row order.  In the tile list the constant
+// tiles follow this order, so that an XCD's contiguous run of them goes down
+// a strip: the tile above is at most a strip width later, and the halo rows
+// it shares with the one below are still in the XCD's L2 (64 tiles of 16 x
+// 15 cells stage ~1.8 MB; at 4096^2 a row of tiles in between overflows the
+// 4 MB L2).
+constexpr int STRIP = 64;
+
+__device__ __forceinline__ int strip_tile(int p, int ntx, int nty)
+{
+    const int full = ntx / STRIP, per = STRIP * nty;
+    int s = p / per, w = STRIP;
+    if (s >= full) {
+        s = full;
+        w = ntx - full * STRIP;
+    }
+    const int r = p - s * per;
+    return (r / w) * ntx + s * STRIP + r % w;
+}
+
+// Stable partition of the tiles by their flags in strip order (one block of
+// 1024 threads): each thread counts the flags of a run of positions and a
+// two-level scan (wave prefix sums, then the 16 wave totals) places the
+// runs.  With set != NULL the first wave also hands the step's CFL set to the
+// host (cfl_set_to).
+__global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned char *flag, int ntx, int nty, int *list,
+                                                              int *nf, const unsigned long long *set, double *out)
 {
     __shared__ int wsum[16];
+    const int n = ntx * nty;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     if (set && wid == 0) cfl_set_to(set, out);
     const int per = (n + 1023) / 1024;
     const int lo = min(n, t * per), hi = min(n, lo + per);
     int cnt = 0;
-    for (int i = lo; i < hi; i++) cnt += flag[i] != 0;
+    for (int i = lo; i < hi; i++) cnt += flag[strip_tile(i, ntx, nty)] != 0;
     int v = cnt;  // inclusive prefix sum over the wave
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -159,27 +183,92 @@ __global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned cha
     int pf = v - cnt + (wid > 0 ? wsum[wid - 1] : 0);  // flagged tiles before this run
     int pc = total + (lo - pf);                          // unflagged ones before it, after all flagged
     for (int i = lo; i < hi; i++) {
-        if (flag[i])
-            list[pf++] = i;
+        const int tile = strip_tile(i, ntx, nty);
+        if (flag[tile])
+            list[pf++] = tile;
         else
-            list[pc++] = i;
+            list[pc++] = tile;
     }
     if (t == 0) *nf = total;
 }
 
-__global__ void tile_list_reset_kernel(unsigned char *flag, int n, int *list, int *nf)
+// The same partition for grids of many tiles, one position per thread over
+// many blocks: tile_count_kernel writes each block's count of flagged
+// positions, tile_scatter_kernel places every block's positions after the
+// counts of the blocks before it.
+__global__ __launch_bounds__(1024) void tile_count_kernel(const unsigned char *flag, int ntx, int nty, int *bcount)
 {
+    __shared__ int wsum[16];
+    const int n = ntx * nty;
+    const int p = blockIdx.x * 1024 + threadIdx.x, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const bool f = p < n && flag[strip_tile(p, ntx, nty)] != 0;
+    const unsigned long long b = __ballot(f);
+    if (lane == 0) wsum[wid] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int k = 0; k < 16; k++) s += wsum[k];
+        bcount[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(1024) void tile_scatter_kernel(const unsigned char *flag, int ntx, int nty,
+                                                            const int *bcount, int nblk, int *list, int *nf,
+                                                            const unsigned long long *set, double *out)
+{
+    __shared__ int wsum[16], red[2];
+    const int n = ntx * nty;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    if (set && blockIdx.x == 0 && wid == 0) cfl_set_to(set, out);
+    if (t == 0) {
+        int before = 0, total = 0;
+        for (int k = 0; k < nblk; k++) {
+            const int c = bcount[k];
+            before += k < (int)blockIdx.x ? c : 0;
+            total += c;
+        }
+        red[0] = before;
+        red[1] = total;
+        if (blockIdx.x == 0) *nf = total;
+    }
+    const int p = blockIdx.x * 1024 + t;
+    const int tile = p < n ? strip_tile(p, ntx, nty) : 0;
+    const bool f = p < n && flag[tile] != 0;
+    const unsigned long long b = __ballot(f);
+    if (lane == 0) wsum[wid] = __popcll(b);
+    __syncthreads();
+    int fw = 0;  // flagged positions of this block in the waves before this one
+    for (int k = 0; k < wid; k++) fw += wsum[k];
+    const int fl = fw + __popcll(b & ((1ull << lane) - 1ull));  // flagged before p in the block
+    const int pf = red[0] + fl;
+    const int pc = red[1] + (p - pf);  // unflagged positions before p, after all flagged
+    if (p < n) list[f ? pf : pc] = tile;
+}
+
+__global__ void tile_list_reset_kernel(unsigned char *flag, int ntx, int nty, int *list, int *nf)
+{
+    const int n = ntx * nty;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         flag[i] = 0;
-        list[i] = i;
+        list[i] = strip_tile(i, ntx, nty);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *nf = 0;
 }
 
-hipError_t launch_tile_partition(const unsigned char *flag, int ntiles, int *list, int *nf,
-                                 const unsigned long long *set, double *out, hipStream_t st)
+hipError_t launch_tile_partition(const unsigned char *flag, int ntx, int nty, int *list, int *nf,
+                                 const unsigned long long *set, double *out, int *bcount, hipStream_t st)
 {
-    hipLaunchKernelGGL(tile_partition_kernel, dim3(1), dim3(1024), 0, st, flag, ntiles, list, nf, set, out);
+    const int n = ntx * nty;
+    if (n <= 16384) {
+        hipLaunchKernelGGL(tile_partition_kernel, dim3(1), dim3(1024), 0, st, flag, ntx, nty, list, nf, set, out);
+        return hipGetLastError();
+    }
+    const int nblk = (n + 1023) / 1024;
+    hipLaunchKernelGGL(tile_count_kernel, dim3(nblk), dim3(1024), 0, st, flag, ntx, nty, bcount);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(tile_scatter_kernel, dim3(nblk), dim3(1024), 0, st, flag, ntx, nty, (const int *)bcount, nblk,
+                       list, nf, set, out);
     return hipGetLastError();
 }
 
@@ -189,10 +278,10 @@ hipError_t launch_cfl_to_host(const unsigned long long *set, double *out, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_tile_list_reset(unsigned char *flag, int ntiles, int *list, int *nf, hipStream_t st)
+hipError_t launch_tile_list_reset(unsigned char *flag, int ntx, int nty, int *list, int *nf, hipStream_t st)
 {
-    const int nb = std::min(1024, (ntiles + 255) / 256);
-    hipLaunchKernelGGL(tile_list_reset_kernel, dim3(std::max(nb, 1)), dim3(256), 0, st, flag, ntiles, list, nf);
+    const int nb = std::min(1024, (ntx * nty + 255) / 256);
+    hipLaunchKernelGGL(tile_list_reset_kernel, dim3(std::max(nb, 1)), dim3(256), 0, st, flag, ntx, nty, list, nf);
     return hipGetLastError();
 }
 

@@ -191,3 +191,21 @@ def test_sweeps_periodic_conservation(lim, char):
     assert st.steps_taken == 10
     scale = np.abs(interior(p, q0)).sum(axis=(0, 1)) * p.dx[0] * p.dx[1]
     assert np.all(np.abs(s1 - s0) <= 1e-12 * scale), (s1 - s0) / scale
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_tile_orders_bitwise(n, monkeypatch):
+    """The order in which the workgroups take the tiles (SC_TILE_ORDER: 0
+    bands of tiles per XCD; default the adaptive order, whose tile list of
+    the non-constant tiles is rebuilt after every step, by one block up to
+    16384 tiles and by two kernels over many blocks above) changes no
+    result: every tile computes the same values wherever it runs."""
+    p = make_params("euler", (n, n), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1))
+    q0 = quadrants(p)
+    out = []
+    for order in ("0", "1"):
+        monkeypatch.setenv("SC_TILE_ORDER", order)
+        q, dt, st, _ = run(p, q0, 6)
+        assert st.steps_taken == 6
+        out.append((q, dt))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
