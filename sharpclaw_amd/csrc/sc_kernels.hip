@@ -145,6 +145,36 @@ __device__ __forceinline__ int strip_tile(int p, int ntx, int nty)
     return (r / w) * ntx + s * STRIP + r % w;
 }
 
+// The tiles of consecutive positions of the strip order, from position p
+// on, without a division per step
+struct StripWalk {
+    int ntx, nty, x0, w, ty, tx;  // strip start and width, current row and column in the strip
+    __device__ StripWalk(int p, int ntx_, int nty_) : ntx(ntx_), nty(nty_)
+    {
+        const int full = ntx / STRIP, per = STRIP * nty;
+        int s = p / per;
+        w = STRIP;
+        if (s >= full) {
+            s = full;
+            w = ntx - full * STRIP;
+        }
+        x0 = s * STRIP;
+        const int r = p - s * per;
+        ty = w > 0 ? r / w : 0;
+        tx = w > 0 ? r % w : 0;
+    }
+    __device__ int tile() const { return ty * ntx + x0 + tx; }
+    __device__ void next()
+    {
+        if (++tx < w) return;
+        tx = 0;
+        if (++ty < nty) return;
+        ty = 0;
+        x0 += w;
+        w = min(STRIP, ntx - x0);
+    }
+};
+
 // Stable partition of the tiles by their flags in strip order (one block of
 // 1024 threads): each thread counts the flags of a run of positions and a
 // two-level scan (wave prefix sums, then the 16 wave totals) places the
@@ -160,7 +190,10 @@ __global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned cha
     const int per = (n + 1023) / 1024;
     const int lo = min(n, t * per), hi = min(n, lo + per);
     int cnt = 0;
-    for (int i = lo; i < hi; i++) cnt += flag[strip_tile(i, ntx, nty)] != 0;
+    {
+        StripWalk sw(lo, ntx, nty);
+        for (int i = lo; i < hi; i++, sw.next()) cnt += flag[sw.tile()] != 0;
+    }
     int v = cnt;  // inclusive prefix sum over the wave
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -182,8 +215,9 @@ __global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned cha
     const int total = wsum[15];
     int pf = v - cnt + (wid > 0 ? wsum[wid - 1] : 0);  // flagged tiles before this run
     int pc = total + (lo - pf);                          // unflagged ones before it, after all flagged
-    for (int i = lo; i < hi; i++) {
-        const int tile = strip_tile(i, ntx, nty);
+    StripWalk sw(lo, ntx, nty);
+    for (int i = lo; i < hi; i++, sw.next()) {
+        const int tile = sw.tile();
         if (flag[tile])
             list[pf++] = tile;
         else

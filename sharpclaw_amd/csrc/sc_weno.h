@@ -38,49 +38,6 @@ __device__ __forceinline__ double weno_side(double im, double a2, double a1, dou
     return (s1 * (t2 - t1) + (0.5 * s3 - 0.25) * (t3 - t2)) * (1.0 / 3.0);
 }
 
-// The same correction in float32.  The float64 form squares the smoothness
-// indicators and multiplies them pairwise, which underflows float32 for
-// small jumps (tt ~ jump^4, s ~ jump^8) and gives 0/0 in flat regions.  The
-// weights are homogeneous of degree 0 in (jumps, sqrt(eps)) and the
-// correction of degree 1, so the jumps are scaled by 1/M (M = max |jump|),
-// eps by 1/M^2, and the correction scaled back by M.  After scaling one of
-// the four jumps has magnitude 1, so the largest indicator is >= 3.9 (the
-// minimum over the other jump of 13(1-x)^2 + 3(1-3x)^2); with eps/M^2
-// clamped to [1e-15, 1e9] (every beta <= 100) each (eps + beta)^2 lies in
-// [1e-30, 1.1e18].  The weights are formed as in weno_w, from g_k = (eps +
-// beta_k)^2 / 9 in [1.1e-31, 1.3e17]: the products g g stay below 2e34 and
-// their sum D above 1e-31, so one reciprocal per side besides 1/M.  Below
-// the lower clamp the weights differ from the reference's only where the
-// jumps agree to float32 resolution; above the upper one by beta/eps <=
-// 1e-7 relative.  Jumps below 1e-30 (where 1/M would overflow) give no
-// correction.
-__device__ __forceinline__ float weno_side_f32(float im, float a2, float a1, float a0, float am1)
-{
-    float M = fmaxf(fmaxf(fabsf(a2), fabsf(a1)), fmaxf(fabsf(a0), fabsf(am1)));
-    if (!(M > 1.0e-30f)) return 0.0f;
-    float r = frcp(M);
-    a2 *= r;
-    a1 *= r;
-    a0 *= r;
-    am1 *= r;
-    constexpr float c13 = 13.0f / 3.0f;
-    const float e3 = fminf(fmaxf((1.0e-36f / 3.0f) * r * r, 1.0e-15f / 3.0f), 1.0e9f / 3.0f);
-    const float t1 = im * (a2 - a1);
-    const float t2 = im * (a1 - a0);
-    const float t3 = im * (a0 - am1);
-    const float b1 = a2 - 3.0f * a1, b2 = a1 + a0, b3 = 3.0f * a0 - am1;
-    const float g1 = dsq(fmaf(c13, t1 * t1, fmaf(b1, b1, e3)));
-    const float g2 = dsq(fmaf(c13, t2 * t2, fmaf(b2, b2, e3)));
-    const float g3 = dsq(fmaf(c13, t3 * t3, fmaf(b3, b3, e3)));
-    const float s1 = g2 * g3;
-    const float u = g1 * g2;
-    const float d = fmaf(3.0f, u, fmaf(6.0f * g1, g3, s1));
-    const float t0 = frcp(d);
-    const float f32 = t3 - t2;
-    const float num = fmaf(u, 1.5f * f32, s1 * (t2 - t1));
-    return fmaf(num * t0, 1.0f / 3.0f, f32 * (-1.0f / 12.0f)) * M;
-}
-
 // The weights and correction of weno_side from the squared differences
 // p1..p3 (= t1^2..t3^2), the second terms b1..b3 of the indicators and the
 // differences f21 = t2 - t1, f32 = t3 - t2 (f15 = 1.5 f32, f12 = f32 / 12).
@@ -123,12 +80,20 @@ __device__ __forceinline__ void weno_pair(double h0, double h1, double h2, doubl
     cr = weno_w(p3, p2, p1, h4 - 3.0 * h3, h3 + h2, 3.0 * h2 - h1, e2 - e3, -f15, -f12);
 }
 
-// both sides in float: weno_side_f32 scales each side by its own jumps
+// Both sides for float32 jumps.  gfx950 issues unpacked float32 and float64
+// VALU instructions at the same rate, so the float32 path forms its weights
+// in float64 from the float32 jumps: float32 weights would need either the
+// range of float64 (the products of squared indicators grow as jump^8 and
+// leave float32's range for jumps outside ~[1e-4, 1e4]) or a ratio form with
+// five reciprocals (transcendental, quarter rate) in place of one.  The
+// correction is rounded to float32 once.
 __device__ __forceinline__ void weno_pair(float h0, float h1, float h2, float h3, float h4, float &cl,
                                           float &cr)
 {
-    cl = weno_side_f32(1.0f, h0, h1, h2, h3);
-    cr = weno_side_f32(-1.0f, h4, h3, h2, h1);
+    double l, r;
+    weno_pair((double)h0, (double)h1, (double)h2, (double)h3, (double)h4, l, r);
+    cl = (float)l;
+    cr = (float)r;
 }
 
 template <class T> __device__ __forceinline__ T weno_corr(T im, T a2, T a1, T a0, T am1);
@@ -140,7 +105,7 @@ template <> __device__ __forceinline__ double weno_corr<double>(double im, doubl
 template <> __device__ __forceinline__ float weno_corr<float>(float im, float a2, float a1, float a0,
                                                               float am1)
 {
-    return weno_side_f32(im, a2, a1, a0, am1);
+    return (float)weno_side((double)im, (double)a2, (double)a1, (double)a0, (double)am1);
 }
 
 // The fourth-order central value (-q(i-2) + 7(q(i-1) + q(i)) - q(i+1)) / 12
