@@ -267,6 +267,7 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
             res, ms = b.run(ctl, 0.0, tend, dt0)
             block = b.block
             tuning = b.tuning()
+            parts = b.parts
             qend[name] = [b.get_q(i)[3:-3] for i in range(nsolves)]
         bad = [i for i, r in enumerate(res) if r.info != 0 or abs(r.t - tend) > 1e-12]
         if bad:
@@ -281,10 +282,13 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
                      "steps_total": int(sum(r.steps_taken for r in res)),
                      "rejected_total": int(sum(r.rejected for r in res)),
                      "block": block, "tuner_predicted_ms": {str(k): v for k, v in tuning.items()},
+                     "split_solves": {"count": int((parts > 1).sum()), "max_parts": int(parts.max()),
+                                      "workgroups": int(parts[parts > 1].sum()),
+                                      "mx_of_split": sorted({int(mx[i]) for i in np.nonzero(parts > 1)[0]})[-8:]},
                      "first_run_wall_s_incl_tuning": first}
         log(f"bench: batch {name}: {ms:.1f} ms, {cs / (ms * 1e-3):.3e} cell-stage/s, "
             f"p50 {out[name]['latency_ms']['p50']:.1f} ms p99 {out[name]['latency_ms']['p99']:.1f} ms, "
-            f"block {block}")
+            f"block {block}, {int((parts > 1).sum())} solves split (max {int(parts.max())} parts)")
     # float32 results against the float64 ones (relative L1 per solve)
     l1 = np.array([np.sum(np.abs(a - b)) / np.sum(np.abs(b)) for a, b in zip(qend["fp32"], qend["fp64"])])
     out["fp32_l1_vs_fp64"] = {"max": float(l1.max()), "p50": float(np.median(l1)),
@@ -377,21 +381,34 @@ def shu_osher_q(p):
 def euler1d_bench(cpu=True):
     """One 1D grid: Euler (gamma 1.4) Shu-Osher, 1600 cells on [-5, 5],
     characteristic WENO5 + Roe (efix), SSP(10,4), CFL 2.45 / 2.5, to t = 1.8,
-    float64 and float32 arithmetic, the whole run through Solver.advance
-    (host-driven steps; the grid is 7 workgroups, so latency, not
-    bandwidth, sets its rate).  With cpu, the reference Fortran on one core
-    on the same run, and the float64 result against the reference's."""
+    float64 and float32 arithmetic, the whole run through Solver.advance:
+    its time loop on the device, the grid split over one workgroup per
+    chunk (13 at 128 threads) that meet at a device barrier after every
+    stage.  Beside it the host-driven loop (SC_HOST_LOOP=1: one launch per
+    stage, the CFL read back every step), bitwise the same result.  With
+    cpu, the reference Fortran on one core on the same run, and the float64
+    result against the reference's."""
     from sharpclaw_amd import SC_FP32, SC_FP64, ScRunCtl, Solver, interior, make_params
     n, tend = 1600, 1.8
     ctl = ScRunCtl(dt_max=1e99, cfl_max=2.5, cfl_desired=2.45, max_steps=50000, dt_variable=1)
     out = {"workload": "1D Euler Shu-Osher, 1600 cells on [-5,5], char WENO5 + Roe, SSP(10,4), "
                        "CFL 2.45/2.5, to t=1.8"}
     qend = {}
-    for prec, name in ((SC_FP64, "fp64"), (SC_FP32, "fp32")):
+    for prec, name in ((SC_FP64, "fp64"), (SC_FP32, "fp32"), (SC_FP64, "fp64_host_loop")):
         p = make_params("euler", (n,), xlower=(-5.0,), xupper=(5.0,), char_decomp=2,
                         time_integrator=4, mthbc=(1, 1, 1, 1), precision=prec)
         q0 = shu_osher_q(p)
-        with Solver(p) as s:
+        old_env = os.environ.get("SC_HOST_LOOP")
+        if name == "fp64_host_loop":
+            os.environ["SC_HOST_LOOP"] = "1"
+        try:
+            solver = Solver(p)
+        finally:
+            if old_env is None:
+                os.environ.pop("SC_HOST_LOOP", None)
+            else:
+                os.environ["SC_HOST_LOOP"] = old_env
+        with solver as s:
             s.set_q(q0)
             s.advance(ScRunCtl(dt_max=1e99, cfl_max=2.5, cfl_desired=2.45, max_steps=3, dt_variable=1),
                       0.0, tend, 0.001)  # warm-up (kernel load)
@@ -408,6 +425,7 @@ def euler1d_bench(cpu=True):
             f"({cs / (st.gpu_ms * 1e-3):.3e} cell-stage/s)")
     a, b = qend["fp32"], qend["fp64"]
     out["fp32_l1_vs_fp64"] = float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))
+    out["device_loop_equals_host_loop"] = bool(np.array_equal(qend["fp64"], qend["fp64_host_loop"]))
     if cpu:
         from oracle import refrun
         if refrun.ref_available("eu1d", programs=("xsclaw",)):

@@ -304,6 +304,12 @@ int sc_batch_run(sc_batch *b, const sc_run_ctl *ctl, double tstart, double tend,
 int sc_batch_block(sc_batch *b, int *block);
 /* the tuner's candidates and predicted whole-run times (ms); *n <= max */
 int sc_batch_tuning(sc_batch *b, int max, int *blocks, double *predicted_ms, int *n);
+/* Workgroups each solve ran on in the last run (parts[i] for solve i; 1: one
+ * workgroup).  The tuner splits the largest solves over several co-resident
+ * workgroups, which meet at a device barrier after every stage, when that
+ * shortens the predicted makespan; the results are identical either way.
+ * The environment variable SC_BATCH_PARTS=k caps the split (1: none). */
+int sc_batch_parts(sc_batch *b, int *parts);
 
 /* ---- request server: continuous batching on one device -----------------
  * No counterpart in the reference (one solve per process there); each

@@ -71,6 +71,7 @@ def lib():
                           C.POINTER(ScBatchResult), dref], C.c_int),
         "sc_batch_block": ([H, C.POINTER(C.c_int)], C.c_int),
         "sc_batch_tuning": ([H, C.c_int, C.POINTER(C.c_int), dref, C.POINTER(C.c_int)], C.c_int),
+        "sc_batch_parts": ([H, C.POINTER(C.c_int)], C.c_int),
         "sc_server_create": ([C.c_int, C.c_size_t, C.c_int, C.POINTER(H)], C.c_int),
         "sc_server_submit": ([H, P, C.POINTER(ScRunCtl), C.c_double, C.c_double, C.c_double, dp,
                               C.POINTER(C.c_longlong)], C.c_int),

@@ -18,6 +18,15 @@ namespace sc {
 
 const int kBatchBlocks[NBATCH_BLOCKS] = {64, 128, 256, 512, 1024};
 
+// What one barrier of a split solve's parts costs per stage (ms, measured
+// on a 1600-cell grid in 7 parts: 4.6 us per stage against 1.9 us for one
+// chunk), the tuner's estimate; and how long a part waits at a barrier
+// before the solve is given up (INFO_GANG_TIMEOUT).
+constexpr double GANG_BARRIER_MS = 0.003;
+constexpr double GANG_WAIT_MS = 10000.0;
+// most workgroups held by split solves in one run (bounds the queue)
+constexpr int MAX_SPLIT_WG = 1024;
+
 // ------------------------------------------------------------ launchers
 
 // resident workgroups per CU of one instantiation (same on every device here)
@@ -127,11 +136,15 @@ struct sc_batch {
     hipStream_t st;
     std::vector<BatchSolve> solves;
     std::vector<SolveState> state;
-    std::vector<int> order;
+    std::vector<int> order;   // solves, largest first
+    std::vector<int> parts;   // workgroups per solve of the next run (1: not split)
+    std::vector<int> queue;   // queue entries of the next run (queue_entry)
     std::vector<int> cur;     // buffer holding each solve's current q
     BatchSolve *d_solves;
     SolveState *d_state;
     int *d_order;
+    int queue_cap;               // entries d_order holds
+    unsigned long long *d_sync;  // split solves' barrier words (BatchArgs::sync)
     unsigned long long *d_next;  // work-queue counters
     double *d_base;
     size_t total;             // doubles allocated
@@ -147,6 +160,7 @@ static void batch_free(sc_batch *b)
     if (b->d_solves) (void)hipFree(b->d_solves);
     if (b->d_state) (void)hipFree(b->d_state);
     if (b->d_order) (void)hipFree(b->d_order);
+    if (b->d_sync) (void)hipFree(b->d_sync);
     if (b->d_next) (void)hipFree(b->d_next);
     if (b->d_base) (void)hipFree(b->d_base);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -180,6 +194,7 @@ extern "C" int sc_batch_create(const sc_params *params, int nsolves, const int *
     for (int i = 0; i < nsolves; i++)
         if (mx[i] < 3 || !(xupper[i] > xlower[i]))
             return sc_fail(SC_ERR_ARG, "solve %d: mx must be >= 3 and xupper > xlower", i);
+    if (nsolves >= ENTRY_MAX_SOLVES) return sc_fail(SC_ERR_ARG, "at most %d solves per batch", ENTRY_MAX_SOLVES - 1);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sc_fail(SC_ERR_HIP, "no HIP device available");
     if (params->device < 0 || params->device >= ndev) return sc_fail(SC_ERR_ARG, "device out of range");
@@ -222,15 +237,17 @@ extern "C" int sc_batch_create(const sc_params *params, int nsolves, const int *
     std::iota(b->order.begin(), b->order.end(), 0);
     std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return mx[x] > mx[y]; });
     b->cur.assign(nsolves, 0);
+    b->parts.assign(nsolves, 1);
     b->state.resize(nsolves);
+    b->queue_cap = nsolves + MAX_SPLIT_WG;
     bool ok = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(&b->d_solves, nsolves * sizeof(BatchSolve)) == hipSuccess;
     ok = ok && hipMalloc(&b->d_state, nsolves * sizeof(SolveState)) == hipSuccess;
-    ok = ok && hipMalloc(&b->d_order, nsolves * sizeof(int)) == hipSuccess;
+    ok = ok && hipMalloc(&b->d_order, b->queue_cap * sizeof(int)) == hipSuccess;
+    ok = ok && hipMalloc(&b->d_sync, 4 * (size_t)nsolves * sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMalloc(&b->d_next, sizeof(unsigned long long)) == hipSuccess;
     ok = ok && hipMalloc(&b->d_base, b->total * sizeof(double)) == hipSuccess;
     ok = ok && hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess;
-    ok = ok && hipMemcpy(b->d_order, b->order.data(), nsolves * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemset(b->d_base, 0, b->total * sizeof(double)) == hipSuccess;
     (void)hipSetDevice(prev);
     if (!ok) {
@@ -277,16 +294,33 @@ extern "C" int sc_batch_get_q(sc_batch *b, int i, double *q)
     return SC_OK;
 }
 
-// One launch over all solves from fresh loop states (q in b->cur).
-static int run_once(sc_batch *b, int nt, double tstart, double dt0, int info0, int step_limit, float *ms)
+// The queue of a run: the split solves first (largest first), their parts
+// as consecutive entries, then the others largest first.
+static void build_queue(sc_batch *b, const std::vector<int> &parts)
+{
+    b->queue.clear();
+    for (int pass = 0; pass < 2; pass++)
+        for (int i : b->order) {
+            const int k = parts[i];
+            if ((k > 1) != (pass == 0)) continue;
+            for (int p = 0; p < k; p++) b->queue.push_back(queue_entry(i, p, k));
+        }
+}
+
+// One launch over all solves from fresh loop states (q in b->cur), each
+// solve split into parts[i] workgroups.
+static int run_once(sc_batch *b, int nt, double tstart, double dt0, int info0, int step_limit,
+                    const std::vector<int> &parts, float *ms)
 {
     for (int i = 0; i < b->nsolves; i++) b->state[i] = initial_state(tstart, dt0, b->cur[i], info0);
+    build_queue(b, parts);
+    if ((int)b->queue.size() > b->queue_cap) return sc_fail(SC_ERR_ARG, "batch: too many parts");
     BatchArgs A;
     memset(&A, 0, sizeof A);
     A.solves = b->d_solves;
     A.state = b->d_state;
     A.order = b->d_order;
-    A.nsolves = b->nsolves;
+    A.nsolves = (int)b->queue.size();
     A.next = b->d_next;
     A.ends = b->ends;
     A.base = b->d_base;
@@ -296,8 +330,13 @@ static int run_once(sc_batch *b, int nt, double tstart, double dt0, int info0, i
     A.ti = b->p.time_integrator;
     A.step_limit = step_limit;
     A.budget_chunks = 0;
+    A.sync = b->d_sync;
+    A.spin_ticks = (long long)(GANG_WAIT_MS / b->tick_ms);
     hipError_t e = hipMemcpyAsync(b->d_state, b->state.data(), b->nsolves * sizeof(SolveState),
                                   hipMemcpyHostToDevice, b->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_order, b->queue.data(), b->queue.size() * sizeof(int), hipMemcpyHostToDevice, b->st);
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_sync, 0, 4 * (size_t)b->nsolves * sizeof(unsigned long long), b->st);
     if (e == hipSuccess) e = hipMemsetAsync(b->d_next, 0, sizeof(unsigned long long), b->st);
     if (e == hipSuccess) e = hipEventRecord(b->ev0, b->st);
     if (e == hipSuccess) e = launch_batch(b->sel, nt, A, b->cd, b->cf, b->ncu, 0, b->st);
@@ -307,40 +346,90 @@ static int run_once(sc_batch *b, int nt, double tstart, double dt0, int info0, i
     if (e == hipSuccess)
         e = hipMemcpy(b->state.data(), b->d_state, b->nsolves * sizeof(SolveState), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return sc_fail(SC_ERR_HIP, "batch run: %s", hipGetErrorString(e));
+    for (int i = 0; i < b->nsolves; i++)
+        if (b->state[i].info == INFO_GANG_TIMEOUT)
+            return sc_fail(SC_ERR_HIP, "batch run: the parts of solve %d did not meet at a barrier within %g ms", i,
+                           GANG_WAIT_MS);
     return SC_OK;
 }
 
 // Makespan of a whole run predicted for workgroup size nt from a trial of
 // one time-loop iteration of every solve (trial_ms, st: the states after
-// it).  The trial keeps every resident workgroup busy, so it gives the time
-// c of one chunk-stage on one busy workgroup; the trial's CFL gives each
-// solve's step count, (tend - tstart) / dt_next (maxn for fixed dt).  Solves
-// are then placed largest first on the least loaded of the resident
-// workgroups (the device work queue does the same), and the largest load is
-// the makespan: it sees both the total work and the critical path of the
-// largest solves, which a one-iteration trial alone does not.
-static double predict_ms(const sc_batch *b, int nt, float trial_ms, double tstart, int stages)
+// it), and the split of the largest solves over several workgroups that
+// the prediction prefers (parts).  The trial gives the time c of one
+// chunk-stage on one busy workgroup; the trial's CFL gives each solve's
+// step count, (tend - tstart) / dt_next (maxn for fixed dt).  A solve of ch
+// chunks split over k workgroups takes iters x stages x (ceil(ch / k) c +
+// barrier) on each of them.  When there are fewer solves than resident
+// workgroups, starting from no splits, the solve with the longest chain
+// gets one more part while its chain is longer than the balanced load
+// (total work / slots) and the split still shortens it (at most max_parts
+// parts, at most half the slots and MAX_SPLIT_WG workgroups in split
+// solves).  The parts are then placed largest first on the least loaded of
+// the resident workgroups (the device work queue does the same), and the
+// largest load is the makespan.
+static double predict_ms(const sc_batch *b, int nt, float trial_ms, double tstart, int stages, int max_parts,
+                         std::vector<int> &parts)
 {
     const int n = b->nsolves;
-    std::vector<double> work(n);
+    std::vector<double> its(n), ch(n);
     double trial = 0.0;
     for (int i = 0; i < n; i++) {
         const BatchSolve &S = b->solves[i];
         const SolveState &s = b->state[i];
-        const double ch = (double)((S.mx + nt - 2) / (nt - 1)) * stages;
+        ch[i] = (double)((S.mx + nt - 2) / (nt - 1));
         double iters = S.maxn;
         if (S.ctl.dt_variable == 1 && s.dt > 0.0)
             iters = std::min((double)S.maxn, std::max(1.0, std::ceil((S.tend - tstart) / s.dt)));
         if (s.info != 0) iters = 0.0;
-        trial += s.it > 0 ? ch : 0.0;
-        work[i] = iters * ch;
+        trial += s.it > 0 ? ch[i] * stages : 0.0;
+        its[i] = iters * stages;
     }
-    const int slots = std::max(1, std::min(n, b->ncu * batch_occupancy(b->sel, nt)));
+    parts.assign(n, 1);
+    // resident workgroups (a launch has at most one per queue entry)
+    const int slots = std::max(1, b->ncu * batch_occupancy(b->sel, nt));
     if (trial <= 0.0) return trial_ms;
-    const double c = trial_ms * slots / trial;
+    // the trial's own makespan in chunk-stages: its balanced load, or its
+    // largest solve when there are fewer solves than slots
+    double one = 0.0;
+    for (int i = 0; i < n; i++)
+        if (b->state[i].it > 0) one = std::max(one, ch[i] * stages);
+    const double c = trial_ms / std::max(trial / std::min(n, slots), one);
+    auto chain = [&](int i, int k) {
+        return its[i] * (std::ceil(ch[i] / k) * c + (k > 1 ? GANG_BARRIER_MS : 0.0));
+    };
+    double total = 0.0;
+    for (int i = 0; i < n; i++) total += chain(i, 1);
+    // largest chain first: a max-heap of (chain, solve)
+    std::priority_queue<std::pair<double, int>> top;
+    for (int i = 0; i < n; i++) top.push({chain(i, 1), i});
+    int extra = 0;
+    // with a solve for every slot no workgroup idles at the start, and the
+    // barriers' cache maintenance (L2 write-back and invalidation at device
+    // scope) slows the workgroups around them: measured on 4096 dam breaks,
+    // splitting the 26 largest solves in two made the run 38% longer
+    while (max_parts > 1 && n < slots && !top.empty()) {
+        const int i = top.top().second;
+        const int k = parts[i];
+        const double now = chain(i, k);
+        if (now <= total / slots) break;
+        const int kn = k + 1;
+        const int in_split = extra + (k == 1 ? kn : 1);  // workgroups held by split solves
+        if (kn > std::min((int)ch[i], max_parts) || 2 * in_split > slots || in_split > MAX_SPLIT_WG) break;
+        const double next = chain(i, kn);
+        if (next >= now) break;
+        top.pop();
+        total += kn * next - k * now;
+        parts[i] = kn;
+        extra = in_split;
+        top.push({next, i});
+    }
+    std::vector<double> work;
+    for (int i = 0; i < n; i++)
+        for (int p = 0; p < parts[i]; p++) work.push_back(chain(i, parts[i]));
     std::sort(work.begin(), work.end(), std::greater<double>());
     std::priority_queue<double, std::vector<double>, std::greater<double>> load;
-    for (int k = 0; k < slots; k++) load.push(0.0);
+    for (int k = 0; k < std::min(slots, (int)work.size()); k++) load.push(0.0);
     double worst = 0.0;
     for (double w : work) {
         double l = load.top() + w;
@@ -348,40 +437,53 @@ static double predict_ms(const sc_batch *b, int nt, float trial_ms, double tstar
         load.push(l);
         worst = std::max(worst, l);
     }
-    return worst * c;
+    return worst;
+}
+
+// Most workgroups per split solve: SC_BATCH_PARTS=k (1: no splits).
+static int max_parts_of()
+{
+    int m = MAX_PARTS;
+    if (const char *e = getenv("SC_BATCH_PARTS")) m = std::min(MAX_PARTS, std::max(1, atoi(e)));
+    return m;
 }
 
 // Autotuner of the workgroup size: each candidate runs one time-loop
 // iteration of every solve from the current states, and the one with the
-// smallest predicted makespan (predict_ms) is kept.  A trial writes only the
-// buffers other than the one holding q (and the SSP(10,4) register), and
-// its loop states are discarded, so the solves are untouched.
+// smallest predicted makespan (predict_ms) is kept with its split of the
+// largest solves.  A trial writes only the buffers other than the one
+// holding q (and the SSP(10,4) register), and its loop states are
+// discarded, so the solves are untouched.
 static int tune_block(sc_batch *b, double tstart, double dt0, int info0)
 {
     if (b->block) return SC_OK;
+    int forced = 0;
     if (const char *e = getenv("SC_BATCH_BLOCK")) {
         int v = atoi(e);
         for (int k = 0; k < NBATCH_BLOCKS; k++)
-            if (kBatchBlocks[k] == v) {
-                b->block = v;
-                return SC_OK;
-            }
+            if (kBatchBlocks[k] == v) forced = v;
     }
     static const int nstage[] = {0, 2, 4, 3, 10};
+    const std::vector<int> single(b->nsolves, 1);
+    const int maxp = max_parts_of();
     int best = 256;
     double bestms = 1e300;
+    std::vector<int> plan, best_plan(b->nsolves, 1);
     for (int k = 0; k < NBATCH_BLOCKS; k++) {
+        if (forced && kBatchBlocks[k] != forced) continue;
         float ms = 0.f;
-        int rc = run_once(b, kBatchBlocks[k], tstart, dt0, info0, 1, &ms);
+        int rc = run_once(b, kBatchBlocks[k], tstart, dt0, info0, 1, single, &ms);
         if (rc) return rc;
-        double pred = predict_ms(b, kBatchBlocks[k], ms, tstart, nstage[b->p.time_integrator]);
+        double pred = predict_ms(b, kBatchBlocks[k], ms, tstart, nstage[b->p.time_integrator], maxp, plan);
         b->predicted[k] = pred;
         if (pred < bestms) {
             bestms = pred;
             best = kBatchBlocks[k];
+            best_plan = plan;
         }
     }
     b->block = best;
+    b->parts = best_plan;
     return SC_OK;
 }
 
@@ -406,7 +508,7 @@ extern "C" int sc_batch_run(sc_batch *b, const sc_run_ctl *ctl, double tstart, d
         rc = sc_fail(SC_ERR_HIP, "batch descriptors upload failed");
     if (!rc) rc = tune_block(b, tstart, dt_initial, info0);
     float ms = 0.f;
-    if (!rc) rc = run_once(b, b->block, tstart, dt_initial, info0, 0, &ms);
+    if (!rc) rc = run_once(b, b->block, tstart, dt_initial, info0, 0, b->parts, &ms);
     (void)hipSetDevice(prev);
     if (rc) return rc;
     long long t0 = 0;
@@ -440,5 +542,12 @@ extern "C" int sc_batch_tuning(sc_batch *b, int max, int *blocks, double *predic
         predicted_ms[k] = b->predicted[k];
     }
     *n = k;
+    return SC_OK;
+}
+
+extern "C" int sc_batch_parts(sc_batch *b, int *parts)
+{
+    if (!b || !parts) return sc_fail(SC_ERR_ARG, "bad argument");
+    for (int i = 0; i < b->nsolves; i++) parts[i] = b->parts[i];
     return SC_OK;
 }

@@ -7,6 +7,12 @@
 // device: the sharpclaw() loop of src/sharpclaw.f90:35-235 (dt clamp to
 // tend, qold save, SSP stages, CFL-driven dt update, accept or
 // reject-and-retake), every stage a sweep of stage1d_chunk over the grid.
+// A large solve can be split over k workgroups ("parts", consecutive queue
+// entries): part p sweeps its own contiguous range of the stage's chunks,
+// the k parts meet at a device-memory barrier after every stage, and the
+// CFL of the last stage is the maximum over the parts.  Every part runs
+// the same (uniform) control logic on the same numbers, so a split solve
+// computes exactly what one workgroup computes.
 // State stays in global memory (three rotating q buffers per solve plus the
 // SSP(10,4) register), where the workgroup's L2 keeps it hot between stages;
 // the block barrier that ends each chunk makes a stage's writes visible to
@@ -40,13 +46,27 @@ struct SolveState {
     long long stages, t_begin, t_end;
     int cur, old, retake, steps, rejected, info, it, done;
     int go;             // 2D rounds: the current iteration's stages run
-    int pad;
+    int nbar;           // barriers a split solve's parts have passed
 };
+
+// A queue entry: solve index (bits 0-19), part (bits 20-25) and part
+// count - 1 (bits 26-31); a plain solve index is a solve of one part.
+constexpr int PART_SHIFT = 20, PARTS_SHIFT = 26, MAX_PARTS = 32;
+constexpr int SOLVE_MASK = (1 << PART_SHIFT) - 1;
+constexpr int ENTRY_MAX_SOLVES = 1 << PART_SHIFT;
+__host__ __device__ inline int queue_entry(int solve, int part, int parts)
+{
+    return solve | (part << PART_SHIFT) | ((parts - 1) << PARTS_SHIFT);
+}
+
+// info of a split solve whose parts did not all reach a barrier within
+// BatchArgs::spin_ticks (the solve is abandoned; the others continue)
+constexpr int INFO_GANG_TIMEOUT = 14;
 
 struct BatchArgs {
     const BatchSolve *solves;
     SolveState *state;
-    const int *order;   // solves of this launch, largest first
+    const int *order;   // queue entries of this launch (queue_entry), largest first
     int nsolves;        // entries of order
     // work-queue counters: low 32 bits = entries taken from the front of
     // order (largest first), high 32 bits = entries taken from the back
@@ -61,6 +81,10 @@ struct BatchArgs {
     int step_limit;     // > 0: at most this many iterations per solve
     int budget_chunks;  // > 0: at most max(1, budget_chunks / (chunks x stages))
                         // iterations, chunks = ceil(mx / (NT - 1))
+    // split solves: per solve four words (arrivals at its barrier, the CFL
+    // maximum of even / odd iterations, unused), zero at a solve's start
+    unsigned long long *sync;
+    long long spin_ticks;  // longest wait at a barrier (wall_clock64 ticks)
 };
 
 struct Plan {
@@ -105,15 +129,49 @@ __device__ __forceinline__ int fetch_solve(const BatchArgs &A)
     return back ? A.nsolves - 1 - (int)b : (int)f;
 }
 
+// Barrier of the parts of one split solve: every part adds one arrival to
+// the solve's counter and waits until it reaches target (= parts x barriers
+// passed so far).  The fences make each workgroup's global writes of the
+// stage visible to the other parts before they read them (release before
+// the arrival, acquire after the wait).  The wait is bounded by spin_ticks
+// of the wall clock; false when it ran out.
+__device__ __forceinline__ bool gang_sync(const BatchArgs &A, unsigned long long *ctr, unsigned long long target,
+                                          int *flag)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        atomicAdd(ctr, 1ull);
+        const long long t0 = (long long)wall_clock64();
+        int ok = 1;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if ((long long)wall_clock64() - t0 > A.spin_ticks) {
+                ok = 0;
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *flag = ok;
+    }
+    __syncthreads();
+    const bool ok = *flag != 0;
+    __syncthreads();
+    return ok;
+}
+
+// At least 4 waves per SIMD: the float64 variants otherwise take a few more
+// than 128 VGPRs and drop to 3 (25% fewer resident workgroups).
 template <int P, int CHAR, class T, int NT>
-__global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(BatchArgs A,
+                                                                                           PhysConst<T> c)
 {
     constexpr int M = Shape<P>::M;
     constexpr int NW = NT / 64;
     __shared__ T sq[M][NT + 5];
     __shared__ __align__(8) T sx[2 * M][NT];
     __shared__ double red[NW];
-    __shared__ int sb;
+    __shared__ int sb, sflag;
     const int t = threadIdx.x;
     const Plan plan = stage_plan(A.ti);
 
@@ -125,13 +183,22 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
     // lane 0 -- a livelock under exec masking.)
     if (t == 0) sb = fetch_solve(A);
     __syncthreads();
-    int k = sb;
+    // the queue index is uniform over the workgroup: read as such, the
+    // solve's descriptor and loop state load into scalar registers
+    int k = __builtin_amdgcn_readfirstlane(sb);
     __syncthreads();
     while (k < A.nsolves) {
-        const int b = A.order[k];
+        const int ent = A.order[k];
+        const int b = ent & SOLVE_MASK;
+        const int part = (ent >> PART_SHIFT) & (MAX_PARTS * 2 - 1);
+        const int parts = ((unsigned)ent >> PARTS_SHIFT) + 1;
         const BatchSolve S = A.solves[b];
         SolveState s = A.state[b];
         const int n = S.mx;
+        // this part's chunks of every stage: [ch0, ch1) of the nch chunks
+        const int nch = (n + NT - 2) / (NT - 1);
+        const int ch0 = (int)((long long)nch * part / parts), ch1 = (int)((long long)nch * (part + 1) / parts);
+        unsigned long long *sync = parts > 1 ? A.sync + 4 * (size_t)b : nullptr;
         const size_t len = (size_t)(n + 6) * A.meqn;
         double *buf[4];
 #pragma unroll
@@ -180,17 +247,22 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
             a.qold = buf[s.old];
             int in = s.cur;
             T sm = T(0);
+            bool lost = false;  // a barrier of a split solve timed out
             for (int st = 0; st < plan.n; st++) {
                 int outb = other_buf(in, s.old);
                 a.qin = buf[in];
                 a.qout = buf[outb];
                 a.op = plan.op[st];
                 const bool last = st == plan.n - 1;
-                for (int c0 = 1; c0 <= n; c0 += NT - 1) {
-                    T s1 = stage1d_chunk<P, 1, CHAR, T, NT>(a, c, c0, sq, sx);
+                for (int ch = ch0; ch < ch1; ch++) {
+                    T s1 = stage1d_chunk<P, 1, CHAR, T, NT>(a, c, 1 + ch * (NT - 1), sq, sx);
                     if (last) sm = fmax(sm, s1);
                 }
                 in = outb;
+                if (sync && !last) {
+                    s.nbar++;
+                    lost = lost || !gang_sync(A, sync, (unsigned long long)parts * (unsigned)s.nbar, &sflag);
+                }
             }
             s.stages += plan.n;
             // block maximum of |s| of the last stage -> CFL (flux1.f90:126-133)
@@ -201,6 +273,30 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
 #pragma unroll
             for (int w = 1; w < NW; w++) m = fmax(m, red[w]);
             __syncthreads();
+            if (sync) {
+                // maximum over the parts: slot 1 + (it & 1) collects this
+                // iteration's, and part 0 clears the other one, which every
+                // part read before the first barrier of this iteration and
+                // none writes before the first barrier of the next
+                unsigned long long *slot = sync + 1 + (s.it & 1);
+                if (t == 0 && m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+                s.nbar++;
+                lost = lost || !gang_sync(A, sync, (unsigned long long)parts * (unsigned)s.nbar, &sflag);
+                if (t == 0) {
+                    red[0] = __longlong_as_double(
+                        (long long)__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                    if (part == 0) __hip_atomic_store(sync + 1 + ((s.it + 1) & 1), 0ull, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+                }
+                __syncthreads();
+                m = red[0];
+                __syncthreads();
+            }
+            if (lost) {
+                s.info = INFO_GANG_TIMEOUT;
+                fin = true;
+                break;
+            }
             s.cfl = (s.dt / S.dx) * m;
             if (ctl.dt_variable == 1) {
                 if (s.cfl > 0.0) {
@@ -237,11 +333,11 @@ __global__ __launch_bounds__(NT) void batch_kernel(BatchArgs A, PhysConst<T> c)
             s.t_end = (long long)wall_clock64();
         }
         if (t == 0) {
-            A.state[b] = s;
+            if (part == 0) A.state[b] = s;
             sb = fetch_solve(A);
         }
         __syncthreads();
-        k = sb;
+        k = __builtin_amdgcn_readfirstlane(sb);
         __syncthreads();
     }
 }

@@ -47,7 +47,10 @@ __device__ __forceinline__ double weno_side(double im, double a2, double a1, dou
 // fma that squares b_k).  With u = g1 g2 and
 // D = g2 g3 + 6 g1 g3 + 3 u (the reference's s1 + s2 + s3 over 81) the
 // correction (s1 (t2-t1) + (s3/2 - 1/4)(t3-t2)) / 3 over (s1+s2+s3) is
-// (g2 g3 f21 + 1.5 u f32) / (3 D) - f32 / 12.
+// (g2 g3 f21 + 1.5 u f32) / (3 D) - f32 / 12.  EXACT = false takes the
+// hardware estimate of 1/D (relative error below 2^-24) without refinement:
+// enough for a correction rounded to float32.
+template <bool EXACT = true>
 __device__ __forceinline__ double weno_w(double p1, double p2, double p3, double b1, double b2,
                                          double b3, double f21, double f15, double f12)
 {
@@ -58,7 +61,7 @@ __device__ __forceinline__ double weno_w(double p1, double p2, double p3, double
     const double s1 = g2 * g3;
     const double u = g1 * g2;
     const double d = fma(3.0, u, fma(6.0 * g1, g3, s1));
-    const double t0 = frcp(d);
+    const double t0 = EXACT ? frcp(d) : __builtin_amdgcn_rcp(d);
     const double num = fma(u, f15, s1 * f21);
     return fma(num * t0, 1.0 / 3.0, -f12);
 }
@@ -69,6 +72,7 @@ __device__ __forceinline__ double weno_w(double p1, double p2, double p3, double
 // h_(k+1) the left side's t1, t2, t3 are e0, e1, e2 and the right side's
 // e3, e2, e1 (its im = -1 flips each sign), so the two share e1, e2, their
 // squares and t3 - t2.
+template <bool EXACT = true>
 __device__ __forceinline__ void weno_pair(double h0, double h1, double h2, double h3, double h4,
                                           double &cl, double &cr)
 {
@@ -76,8 +80,8 @@ __device__ __forceinline__ void weno_pair(double h0, double h1, double h2, doubl
     const double p0 = e0 * e0, p1 = e1 * e1, p2 = e2 * e2, p3 = e3 * e3;
     const double f32 = e2 - e1;
     const double f15 = 1.5 * f32, f12 = f32 * (1.0 / 12.0);
-    cl = weno_w(p0, p1, p2, h0 - 3.0 * h1, h1 + h2, 3.0 * h2 - h3, e1 - e0, f15, f12);
-    cr = weno_w(p3, p2, p1, h4 - 3.0 * h3, h3 + h2, 3.0 * h2 - h1, e2 - e3, -f15, -f12);
+    cl = weno_w<EXACT>(p0, p1, p2, h0 - 3.0 * h1, h1 + h2, 3.0 * h2 - h3, e1 - e0, f15, f12);
+    cr = weno_w<EXACT>(p3, p2, p1, h4 - 3.0 * h3, h3 + h2, 3.0 * h2 - h1, e2 - e3, -f15, -f12);
 }
 
 // Both sides for float32 jumps.  gfx950 issues unpacked float32 and float64
@@ -86,12 +90,12 @@ __device__ __forceinline__ void weno_pair(double h0, double h1, double h2, doubl
 // range of float64 (the products of squared indicators grow as jump^8 and
 // leave float32's range for jumps outside ~[1e-4, 1e4]) or a ratio form with
 // five reciprocals (transcendental, quarter rate) in place of one.  The
-// correction is rounded to float32 once.
+// correction is rounded to float32 once, so 1/D needs no refinement.
 __device__ __forceinline__ void weno_pair(float h0, float h1, float h2, float h3, float h4, float &cl,
                                           float &cr)
 {
     double l, r;
-    weno_pair((double)h0, (double)h1, (double)h2, (double)h3, (double)h4, l, r);
+    weno_pair<false>((double)h0, (double)h1, (double)h2, (double)h3, (double)h4, l, r);
     cl = (float)l;
     cr = (float)r;
 }

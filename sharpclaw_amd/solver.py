@@ -220,6 +220,13 @@ class Batch:
         check(lib().sc_batch_block(self._h, C.byref(b)))
         return b.value
 
+    @property
+    def parts(self):
+        """Workgroups each solve ran on in the last run (1: one workgroup)."""
+        out = np.zeros(self.n, dtype=np.int32)
+        check(lib().sc_batch_parts(self._h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
     def tuning(self):
         """{workgroup size: predicted whole-run ms} of the tuner (0: untimed)."""
         blocks = (C.c_int * 8)()

@@ -366,6 +366,85 @@ bool serve(const std::string &name)
     return ok;
 }
 
+// The same batch with the largest solves split over several workgroups
+// (the tuner's choice) and with every solve on one: bitwise equal.
+bool batch_split(const std::string &name, sc_params p, const std::vector<int> &mx, double tend)
+{
+    const int ns = (int)mx.size();
+    std::vector<double> xl(ns, 0.0), xu(ns, 1.0);
+    std::vector<std::vector<double>> out[2];
+    std::vector<int> parts(ns);
+    int split = 0;
+    for (int run = 0; run < 2; run++) {
+        if (run == 0) setenv("SC_BATCH_PARTS", "1", 1);
+        sc_batch *b = nullptr;
+        if (!rc_ok(sc_batch_create(&p, ns, mx.data(), xl.data(), xu.data(), &b), "sc_batch_create")) return false;
+        bool ok = true;
+        for (int i = 0; i < ns && ok; i++) {
+            sc_params pi = p;
+            pi.nx[0] = mx[i];
+            pi.dx[0] = 1.0 / mx[i];
+            std::vector<double> q = state(pi, 500 + i);
+            ok = rc_ok(sc_batch_set_q(b, i, q.data()), "sc_batch_set_q");
+            out[run].push_back(std::vector<double>(q.size()));
+        }
+        sc_run_ctl c = {1e99, 1.0, 0.9, 100000, 1};
+        std::vector<sc_batch_result> res(ns);
+        double ms = 0.0;
+        ok = ok && rc_ok(sc_batch_run(b, &c, 0.0, tend, 1e-4, res.data(), &ms), "sc_batch_run");
+        ok = ok && rc_ok(sc_batch_parts(b, parts.data()), "sc_batch_parts");
+        for (int i = 0; i < ns && ok; i++) {
+            ok = rc_ok(sc_batch_get_q(b, i, out[run][i].data()), "sc_batch_get_q");
+            if (run == 1 && parts[i] > 1) split++;
+            if (run == 0 && parts[i] != 1) ok = check(false, "split although SC_BATCH_PARTS=1", name, parts[i], 1);
+        }
+        sc_batch_destroy(b);
+        unsetenv("SC_BATCH_PARTS");
+        if (!ok) return false;
+    }
+    bool ok = check(split > 0, "no solve was split", name, 0, 1);
+    for (int i = 0; i < ns; i++)
+        ok = ok && check(std::memcmp(out[0][i].data(), out[1][i].data(), out[0][i].size() * sizeof(double)) == 0,
+                         "split and unsplit results differ", name + " solve " + std::to_string(i), 1, 0);
+    return ok;
+}
+
+// sc_advance on a 1D grid: the device-resident loop (the grid split over
+// its chunks) against the host-driven loop (SC_HOST_LOOP=1), bitwise, and
+// against the oracle.
+bool loop1d(const std::string &name, sc_params p, unsigned long long seed, double tend, double tol)
+{
+    std::vector<double> q = state(p, seed);
+    sc_run_ctl c = {1e99, 1.0, 0.9, 100000, 1};
+    std::vector<double> out[2];
+    sc_run_stats st[2];
+    for (int run = 0; run < 2; run++) {
+        if (run == 0) setenv("SC_HOST_LOOP", "1", 1);
+        sc_solver *s = nullptr;
+        bool ok = rc_ok(sc_create(&p, &s), "sc_create");
+        unsetenv("SC_HOST_LOOP");
+        if (!ok) return false;
+        double dt = 0.1 * p.dx[0];
+        out[run].resize(q.size());
+        ok = rc_ok(sc_set_q(s, q.data()), "sc_set_q") && rc_ok(sc_advance(s, &c, 0.0, tend, &dt, &st[run]), "sc_advance") &&
+             rc_ok(sc_get_q(s, out[run].data()), "sc_get_q");
+        sc_destroy(s);
+        if (!ok) return false;
+    }
+    std::vector<double> qo = q;
+    double dto = 0.1 * p.dx[0];
+    sc_run_stats so;
+    orc_sharpclaw(&p, &c, qo.data(), 0.0, tend, &dto, &so);
+    const double e = p.precision == SC_FP32 ? rel_l1(p, out[1], qo) : rel_linf(p, out[1], qo);
+    return check(std::memcmp(out[0].data(), out[1].data(), q.size() * sizeof(double)) == 0,
+                 "device and host loops differ", name, 1, 0) &&
+           check(st[0].steps_taken == st[1].steps_taken && st[0].dt_last == st[1].dt_last &&
+                     st[0].cfl_maxused == st[1].cfl_maxused && st[0].stage_evals == st[1].stage_evals,
+                 "loop statistics differ", name, st[1].steps_taken, st[0].steps_taken) &&
+           check(e <= tol, "relative error vs oracle", name, e, tol) &&
+           check(st[1].steps_taken == so.steps_taken, "steps taken differ", name, st[1].steps_taken, so.steps_taken);
+}
+
 struct Scenario {
     std::string name;
     std::function<bool()> run;
@@ -476,6 +555,29 @@ std::vector<Scenario> scenarios()
                      p.char_decomp = 0;
                      p.precision = SC_FP32;
                      return batch("batch shallow water fp32", p, {48, 130, 600, 77}, 0.01, 1e-5);
+                 }});
+    v.push_back({"batch split euler fp64", []() {
+                     return batch_split("batch split euler fp64", base(1, 64, 1, SC_PHYS_EULER), {1500, 64, 90, 33},
+                                        0.01);
+                 }});
+    v.push_back({"batch split shallow water fp32 ssp104", []() {
+                     sc_params p = base(1, 64, 1, SC_PHYS_SHALLOW_WATER);
+                     p.char_decomp = 0;
+                     p.precision = SC_FP32;
+                     p.time_integrator = 4;
+                     return batch_split("batch split shallow water fp32 ssp104", p, {2000, 700, 50}, 0.005);
+                 }});
+    v.push_back({"loop1d euler ssp104", []() {
+                     sc_params p = base(1, 1600, 1, SC_PHYS_EULER);
+                     p.time_integrator = 4;
+                     p.mthbc[0] = 3;
+                     return loop1d("loop1d euler ssp104", p, 601, 0.004, 1e-10);
+                 }});
+    v.push_back({"loop1d shallow water fp32", []() {
+                     sc_params p = base(1, 700, 1, SC_PHYS_SHALLOW_WATER);
+                     p.char_decomp = 0;
+                     p.precision = SC_FP32;
+                     return loop1d("loop1d shallow water fp32", p, 603, 0.01, 1e-5);
                  }});
     v.push_back({"serve mixed", []() { return serve("serve mixed"); }});
     return v;

@@ -272,6 +272,7 @@ inline long long wall_clock64() { return emu::now_ticks(); }
 inline long long clock64() { return emu::now_ticks(); }
 
 inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+#define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(order)
 inline void __threadfence_block() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 
 template <class T> inline T atomicAdd(T *p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
@@ -298,6 +299,9 @@ template <class T> inline T atomicCAS(T *p, T cmp, T v)
 #define __hip_atomic_fetch_add(p, v, order, scope) __atomic_fetch_add(p, v, order)
 #define __builtin_amdgcn_s_sleep(n) emu_yield()
 void emu_yield();
+
+// the kernels apply it only to values that are uniform over the wave
+#define __builtin_amdgcn_readfirstlane(x) (x)
 
 // AMDGPU math builtins the kernels call: exact CPU operations
 #define __builtin_amdgcn_rcp(x) (1.0 / (x))

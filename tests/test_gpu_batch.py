@@ -264,3 +264,40 @@ def test_batch_queue_ends_bitwise(ends, monkeypatch):
         assert rg[i].info == 0 and rg[i].t == pytest.approx(0.02, abs=1e-14)
         assert rg[i].steps_taken == rr[i].steps_taken
         assert np.array_equal(got[i], ref[i]), i
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_batch_split_solves_bitwise(fp32, monkeypatch):
+    """The tuner splits the largest solves over several workgroups that meet
+    at a device barrier after every stage (sc_batch_parts > 1); the results,
+    step counts and dt equal the run with every solve on one workgroup
+    (SC_BATCH_PARTS=1) bit for bit."""
+    mx, hl, hr, x0 = dambreak_batch(60, seed=23, mx_lo=64, mx_hi=900)
+    mx[:3] = [4096, 3000, 2048]
+    n = len(mx)
+    base = make_params("shallow_water", (8,), char_decomp=2, time_integrator=3,
+                       precision=SC_FP32 if fp32 else 0)
+    xl, xu = np.zeros(n), np.ones(n)
+    qs = [dambreak_q(int(mx[i]), hl[i], hr[i], x0[i]) for i in range(n)]
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=100000, dt_variable=1)
+    runs = {}
+    for cap in ("1", None):
+        if cap:
+            monkeypatch.setenv("SC_BATCH_PARTS", cap)
+        else:
+            monkeypatch.delenv("SC_BATCH_PARTS", raising=False)
+        monkeypatch.setenv("SC_BATCH_BLOCK", "256")
+        with Batch(base, mx, xl, xu) as b:
+            for i in range(n):
+                b.set_q(i, qs[i])
+            res, ms = b.run(ctl, 0.0, 0.01, 1e-4)
+            runs[cap] = ([b.get_q(i) for i in range(n)], res, b.parts.copy())
+    q1, r1, p1 = runs["1"]
+    q2, r2, p2 = runs[None]
+    assert (p1 == 1).all()
+    assert p2.max() > 1 and p2[0] > 1, p2[:5]
+    for i in range(n):
+        assert r2[i].info == 0 and r2[i].t == pytest.approx(0.01, abs=1e-14)
+        assert (r2[i].steps_taken, r2[i].dt_last, r2[i].cfl_maxused) == \
+            (r1[i].steps_taken, r1[i].dt_last, r1[i].cfl_maxused), i
+        assert np.array_equal(q2[i], q1[i]), (i, int(mx[i]), int(p2[i]))

@@ -57,3 +57,37 @@ def test_device_loop_equals_host_loop(phys, ti, n, prec):
     assert out["device"] == out["host"]
     assert out["graph"] == out["host"]
     assert out["host"][0][1] > 0 or ti == 4  # the first interval retook its first step
+
+
+@pytest.mark.parametrize("phys,ti,mx,prec", [("euler", 4, 1600, 0), ("euler", 3, 5000, 0),
+                                             ("shallow_water", 2, 700, SC_FP32), ("acoustics", 1, 300, 0),
+                                             ("advection", 3, 255, 0)])
+def test_device_loop_1d_equals_host_loop(phys, ti, mx, prec):
+    """sc_advance on a 1D grid of up to 32 chunks runs its time loop on the
+    device, the grid split over one workgroup per chunk meeting at a device
+    barrier after every stage; q, step counts, rejections, dt and CFL equal
+    the host-driven loop's (SC_HOST_LOOP=1) bit for bit over consecutive
+    intervals, the first of which retakes its first step."""
+    rng = np.random.default_rng(43)
+    bc = (1, 1) if phys == "advection" else (3, 1)
+    p = make_params(phys, (mx,), time_integrator=ti, mthbc=bc, precision=prec,
+                    char_decomp=2 if phys == "euler" else 0)
+    q0 = O.new_grid(p)
+    interior(p, q0)[:] = _states.random_cells(p, (mx,), rng, smooth=True, amp=0.4)
+    cfl = (2.45, 2.5) if ti == 4 else (0.45, 0.5)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=cfl[1], cfl_desired=cfl[0], max_steps=100000, dt_variable=1)
+    out = {}
+    for mode in ("host", "device"):
+        with _make(p, mode) as s:
+            s.set_q(q0)
+            rec = []
+            dt, t = 0.05, 0.0
+            for k in range(3):
+                dt, st = s.advance(ctl, t, t + 0.01, dt)
+                t = st.t
+                rec.append((st.steps_taken, st.rejected, st.info, st.t, dt, st.cfl_maxused, st.cfl_last,
+                            st.stage_evals))
+                rec.append(s.get_q().tobytes())
+            out[mode] = rec
+    assert out["device"] == out["host"]
+    assert out["host"][0][1] > 0 or ti == 4
