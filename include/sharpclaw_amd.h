@@ -173,6 +173,18 @@ int sc_restore_qold(sc_solver *s);
  * zero ghost entries. */
 int sc_step(sc_solver *s, double dt, double t, double *dq, double *cfl);
 
+/* flux1(q,g,dq,aux,dt,cfl,t,rp,tfluct,1) (src/flux1.f90:2-190) on a 1D grid
+ * and flux2(q,g,dq,aux,dt,cfl,t,rp,tfluct) (src/2d/flux2.f90:2-94) on a 2D
+ * grid, as step1d.f90:26-29 / step2d.f90:31-36 call them after the user's
+ * bc and src: q is the caller's whole array (sc_get_q layout) with its
+ * ghost cells as the caller's bc filled them, dq holds the caller's src
+ * result and receives dq(interior) += dt*L(q); cfl receives the CFL number,
+ * in 2D over the ghost rows and columns too (flux2.f90:36,68).  The solver's
+ * q and qold are not touched.  This is the step of grids with a user
+ * boundary (mthbc 0), for which sc_step, sc_ts and sc_advance return
+ * SC_ERR_UNSUPPORTED, and of any caller that keeps q on the host. */
+int sc_flux(sc_solver *s, const double *q, double *dq, double dt, double t, double *cfl);
+
 /* sc_step on arrays the caller already holds on the device, allocating
  * nothing.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving

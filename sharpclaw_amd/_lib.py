@@ -46,6 +46,7 @@ def lib():
         "sc_save_qold": ([H], C.c_int),
         "sc_restore_qold": ([H], C.c_int),
         "sc_step": ([H, C.c_double, C.c_double, C.c_void_p, dref], C.c_int),
+        "sc_flux": ([H, C.c_void_p, C.c_void_p, C.c_double, C.c_double, dref], C.c_int),
         "sc_step_noalloc": ([P, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_void_p],
                             C.c_int),
         "sc_ts": ([H, C.c_double, C.c_double, dref], C.c_int),

@@ -64,13 +64,23 @@ def build(verbose=False):
 
 
 FORTRAN = os.path.join(HERE, "fortran")
+# sample applications: (ndim, their hook sources under fortran/apps)
 FAPPS = {
-    "adv1d": ["setprob_advection.f90", "qinit1.f90"],
-    "ac1d": ["setprob_acoustics.f90", "qinit1.f90"],
-    "eu1d": ["setprob_euler.f90", "qinit1.f90"],
-    "sw1d": ["setprob_sw.f90", "qinit1.f90"],
-    "eu2d": ["setprob_euler.f90", "qinit2.f90"],
-    "sw2d": ["setprob_sw.f90", "qinit2.f90"],
+    "adv1d": (1, ["setprob_advection.f90", "qinit1.f90"]),
+    "ac1d": (1, ["setprob_acoustics.f90", "qinit1.f90"]),
+    "eu1d": (1, ["setprob_euler.f90", "qinit1.f90"]),
+    "sw1d": (1, ["setprob_sw.f90", "qinit1.f90"]),
+    "eu2d": (2, ["setprob_euler.f90", "qinit2.f90"]),
+    "sw2d": (2, ["setprob_sw.f90", "qinit2.f90"]),
+    # user bc (mthbc 0), src and b4step linked in place of the defaults
+    "ac1du": (1, ["setprob_acoustics.f90", "qinit1.f90", "userhooks_ac1d.f90"]),
+}
+# default hooks, one routine per archive member, so that an application's
+# own routine replaces exactly its default (link-time substitution, as the
+# reference's application Makefiles omit the library's bc1.f90 or src1.f90)
+FDEFAULTS = {
+    1: ["b4step.f90", "setaux.f90", "skip_data_header.f90", "bc1.f90", "src1.f90"],
+    2: ["b4step.f90", "setaux.f90", "skip_data_header.f90", "bc2.f90", "src2.f90"],
 }
 
 
@@ -98,19 +108,33 @@ def build_fortran(verbose=False):
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd, cwd=bdir)
 
+    run([fc, "-O2", "-c", os.path.join(FORTRAN, "clawparams.f90"), "-o", "clawparams.o"])
     run([fc, "-O2", "-c", os.path.join(FORTRAN, "sharpclaw_gpu.f90"), "-o", "sharpclaw_gpu.o"])
-    run([fc, "-O2", "-c", os.path.join(FORTRAN, "xsclaw_gpu.f90"), "-o", "xsclaw_gpu.o"])
-    run([fc, "-O2", "-c", os.path.join(FORTRAN, "apps", "hooks_default.f90"), "-o", "hooks_default.o"])
+    drv = ["clawparams.o", "sharpclaw_gpu.o"]
+    for src in ("xsclaw_gpu.f90", "host_bc1.f90", "host_bc2.f90"):
+        obj = src[:-4] + ".o"
+        run([fc, "-O2", "-c", os.path.join(FORTRAN, src), "-o", obj])
+        drv.append(obj)
+    for nd, srcs in FDEFAULTS.items():
+        objs = []
+        for src in srcs:
+            obj = f"def{nd}_{src[:-4]}.o"
+            run([fc, "-O2", "-c", os.path.join(FORTRAN, "defaults", src), "-o", obj])
+            objs.append(obj)
+        arc = f"libschooks{nd}.a"
+        if os.path.exists(os.path.join(bdir, arc)):
+            os.remove(os.path.join(bdir, arc))
+        run(["ar", "rcs", arc] + objs)
     out = []
-    for app, srcs in FAPPS.items():
+    for app, (nd, srcs) in FAPPS.items():
         objs = []
         for src in srcs:
             obj = f"{app}_{src}.o"
             run([fc, "-O2", "-c", os.path.join(FORTRAN, "apps", src), "-o", obj])
             objs.append(obj)
         exe = f"xsclaw_gpu_{app}"
-        run([fc, "-O2", "-o", exe, "xsclaw_gpu.o", "sharpclaw_gpu.o", "hooks_default.o"] + objs +
-            [f"-L{HERE}", "-lsharpclaw_amd", "-Wl,-rpath,$ORIGIN/../.."])
+        run([fc, "-O2", "-o", exe] + drv + objs + [f"-L{bdir}", f"-lschooks{nd}", f"-L{HERE}",
+                                                   "-lsharpclaw_amd", "-Wl,-rpath,$ORIGIN/../.."])
         out.append(os.path.join(bdir, exe))
     return out
 

@@ -148,10 +148,11 @@ int sc_check_params(const sc_params *p, bool grid)
     for (int d = 0; d < p->ndim; d++) {
         if (p->nx[d] < 1) return sc_fail(SC_ERR_ARG, "nx[%d] = %d", d, p->nx[d]);
         if (!(p->dx[d] > 0.0)) return sc_fail(SC_ERR_ARG, "dx[%d] must be positive", d);
+        // mthbc 0: the user's bc fills the ghost cells (bc1.f90:34-37,
+        // bc2.f90:35,125); such grids step through sc_flux, which takes them
+        // from the caller's array
         int bl = p->mthbc[2 * d], br = p->mthbc[2 * d + 1];
-        if (bl == 0 || br == 0)
-            return sc_fail(SC_ERR_UNSUPPORTED, "mthbc=0 (user boundary conditions) has no device implementation");
-        if (bl < 1 || bl > 3 || br < 1 || br > 3) return sc_fail(SC_ERR_ARG, "mthbc must be 1, 2 or 3");
+        if (bl < 0 || bl > 3 || br < 0 || br > 3) return sc_fail(SC_ERR_ARG, "mthbc must be 0, 1, 2 or 3");
         if ((bl == 2) != (br == 2))
             return sc_fail(SC_ERR_ARG, "periodic boundary conditions require mthbc(%d) and mthbc(%d) both be 2 (main.f90:140-145)",
                         2 * d + 1, 2 * d + 2);
@@ -203,6 +204,7 @@ struct sc_solver {
     hipEvent_t ev0, ev1;
     double *capa;      // capacity aux(:,mcapa) in the q layout (1D, mcapa > 0)
     double *aux2;      // 2D advection: edge velocities aux(:,:,1:2)
+    double *gsl;       // sc_flux on 2D grids: the four ghost slices and their dq (created at first use)
     double auxg[2];    // their max |.| over the ghost rows (x) / columns (y) swept for the CFL
     double vscale;     // b4step's factor on the velocities for the current step
     bool vt_stale;     // a new step starts: vscale is due
@@ -373,6 +375,7 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->capa) hipFree(s->capa);
     if (s->aux2) hipFree(s->aux2);
+    if (s->gsl) hipFree(s->gsl);
     if (s->tnf) hipFree(s->tnf);
     if (s->l1.solve) hipFree(s->l1.solve);
     for (int k = 0; k < s->tev_cap; k++)
@@ -382,6 +385,14 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->st) hipStreamDestroy(s->st);
     delete s;
     return SC_OK;
+}
+
+// Steps of grids with a user boundary (mthbc 0) need the caller's bc before
+// every stage: they go through sc_flux.
+static int user_bc_refused()
+{
+    return sc_fail(SC_ERR_UNSUPPORTED, "mthbc 0 (user boundary conditions): the caller's bc fills the ghost "
+                                       "cells before every stage; step with sc_flux (the flux1/flux2 drop-in)");
 }
 
 static int free_buf(const sc_solver *s, int a, int b)
@@ -772,6 +783,7 @@ extern "C" int sc_step(sc_solver *s, double dt, double t, double *dq, double *cf
 {
     (void)t;
     if (!s || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (sc_user_bc(&s->p)) return user_bc_refused();
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     if ((s->p.mcapa > 0 && !s->capa) || (s->sel.phys == ADV2 && !s->aux2))
         return sc_fail(SC_ERR_STATE, "no aux uploaded (sc_set_aux)");
@@ -795,6 +807,99 @@ extern "C" int sc_step(sc_solver *s, double dt, double t, double *dq, double *cf
         HIPCHK(hipMemcpyAsync(dq, s->dq, s->n * sizeof(double), hipMemcpyDeviceToHost, s->st));
         HIPCHK(hipStreamSynchronize(s->st));
     }
+    return SC_OK;
+}
+
+// flux1 / flux2 of a whole grid on the caller's q, ghost cells as given
+// (the caller's bc has filled them): the x and y interfaces of every cell
+// are solved from q with every boundary read from memory (bc type 0), and
+// dq(interior) += dt*L(q).  In 2D the reference also sweeps the ghost rows
+// j = 0, ny+1 and columns i = 0, nx+1 for the CFL number (flux2.f90:36,68);
+// with arbitrary ghost values their speeds are not those of the interior
+// rows, so the tile path solves those four slices as 1D lines (the slice
+// sweeps already cover them).
+extern "C" int sc_flux(sc_solver *s, const double *q, double *dq, double dt, double t, double *cfl)
+{
+    (void)t;
+    if (!s || !q || !dq || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if ((s->p.mcapa > 0 && !s->capa) || (s->sel.phys == ADV2 && !s->aux2))
+        return sc_fail(SC_ERR_STATE, "no aux uploaded (sc_set_aux)");
+    if (s->sel.phys == ADV2 && s->p.tperiod != 0.0)
+        return sc_fail(SC_ERR_UNSUPPORTED, "2D advection with tperiod set scales its velocities once per step "
+                                           "(sc_ts); with sc_flux the caller's b4step scales aux and tperiod is 0");
+    DevGuard g(s->dev);
+    StageArgs a = base_args(s);
+    for (int k = 0; k < 4; k++) a.bc[k] = 0;
+    a.src = 0;  // the caller's src has set dq (step1d.f90:26, step2d.f90:31)
+    a.vscale = 1.0;
+    const int in = free_buf(s, s->cur, s->old);
+    HIPCHK(hipMemcpyAsync(s->buf[in], q, s->n * sizeof(double), hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemsetAsync(s->dq, 0, s->n * sizeof(double), s->st));
+    a.qin = s->buf[in];
+    a.qout = s->dq;
+    a.op = OP_DQ;
+    a.dtdx = dt / s->p.dx[0];
+    a.dt = dt;
+    a.dtdy = s->p.ndim == 2 ? dt / s->p.dx[1] : 0.0;
+    int rs = next_slot(s, &a.smax);
+    if (rs) return rs;
+    HIPCHK(run_stage(s, a));
+    if (s->p.ndim == 2 && !s->dqx && s->sel.phys != ADV2) {
+        // the ghost slices: rows 0 and ny+1 (x lines of nx cells), columns 0
+        // and nx+1 (y lines of ny cells), each with its mbc ghost cells
+        const int nx = s->p.nx[0], ny = s->p.nx[1], M = s->p.meqn;
+        const size_t LX = (size_t)nx + 6, LY = (size_t)ny + 6, plane = LX * LY;
+        const size_t nsl = 2 * (LX + LY) * M;
+        if (!s->gsl && hipMalloc(&s->gsl, 2 * nsl * sizeof(double)) != hipSuccess)
+            return sc_fail(SC_ERR_NOMEM, "sc_flux: ghost slices (%zu doubles)", 2 * nsl);
+        std::vector<double> h(nsl);
+        double *hp = h.data();
+        for (int r = 0; r < 2; r++) {
+            const size_t j = r == 0 ? 2 : (size_t)ny + 3;  // rows 0, ny+1 in storage
+            for (int m = 0; m < M; m++)
+                for (size_t i = 0; i < LX; i++) hp[i + m * LX] = q[i + LX * j + m * plane];
+            hp += LX * M;
+        }
+        for (int c = 0; c < 2; c++) {
+            const size_t i = c == 0 ? 2 : (size_t)nx + 3;  // columns 0, nx+1
+            for (int m = 0; m < M; m++)
+                for (size_t j = 0; j < LY; j++) hp[j + m * LY] = q[i + LX * j + m * plane];
+            hp += LY * M;
+        }
+        HIPCHK(hipMemcpyAsync(s->gsl, h.data(), nsl * sizeof(double), hipMemcpyHostToDevice, s->st));
+        size_t off = 0;
+        for (int k = 0; k < 4; k++) {
+            const int ixy = k < 2 ? 1 : 2;
+            const int len = ixy == 1 ? nx : ny;
+            StageArgs b;
+            memset(&b, 0, sizeof b);
+            b.qin = s->gsl + off;
+            b.qout = s->gsl + nsl + off;
+            b.smax = a.smax + (ixy - 1);  // the x or y word of the same CFL set
+            b.nx = len;
+            b.ny = 1;
+            b.LX = len + 6;
+            b.LY = 1;
+            b.op = OP_DQ;
+            b.dtdx = dt / s->p.dx[ixy - 1];
+            HIPCHK(launch_stage1d(s->sel, ixy, b, s->cd, s->cf, s->st));
+            off += (size_t)(len + 6) * M;
+        }
+    }
+    int rc = read_cfl(s, a.smax, dt, cfl);
+    if (rc) return rc;
+    std::vector<double> h(s->n);
+    HIPCHK(hipMemcpyAsync(h.data(), s->dq, s->n * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    // dq(interior) += dt*L(q); dq's ghost entries stay the caller's
+    const size_t LX = (size_t)s->p.nx[0] + 6;
+    const size_t LY = s->p.ndim == 2 ? (size_t)s->p.nx[1] + 6 : 1;
+    const int ny = s->p.ndim == 2 ? s->p.nx[1] : 1, j0 = s->p.ndim == 2 ? 3 : 0;
+    for (int m = 0; m < s->p.meqn; m++)
+        for (int j = 0; j < ny; j++) {
+            const size_t base = (size_t)m * LX * LY + LX * (size_t)(j + j0) + 3;
+            for (int i = 0; i < s->p.nx[0]; i++) dq[base + i] += h[base + i];
+        }
     return SC_OK;
 }
 
@@ -896,6 +1001,7 @@ static void b4step_factor(sc_solver *s, double told, double dt)
 extern "C" int sc_ts(sc_solver *s, double dt, double told, double *cfl)
 {
     if (!s || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (sc_user_bc(&s->p)) return user_bc_refused();
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     if ((s->p.mcapa > 0 && !s->capa) || (s->sel.phys == ADV2 && !s->aux2))
         return sc_fail(SC_ERR_STATE, "no aux uploaded (sc_set_aux)");
@@ -1113,6 +1219,7 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
                           sc_run_stats *st)
 {
     if (!s || !c || !dt_io || !st) return sc_fail(SC_ERR_ARG, "NULL argument");
+    if (sc_user_bc(&s->p)) return user_bc_refused();
     if (!s->have_q) return sc_fail(SC_ERR_STATE, "no q uploaded");
     if ((s->p.mcapa > 0 && !s->capa) || (s->sel.phys == ADV2 && !s->aux2))
         return sc_fail(SC_ERR_STATE, "no aux uploaded (sc_set_aux)");

@@ -12,6 +12,14 @@ int sc_fail(int code, const char *fmt, ...);
 int sc_check_params(const sc_params *p, bool grid);
 // PhysId of the device physics, -1 if none.
 int sc_phys_id(const sc_params *p);
+// A side with mthbc 0: its ghost cells come from the user's bc routine
+// (bc1.f90:34-37, bc2.f90:35,125), which the device cannot run between stages.
+inline bool sc_user_bc(const sc_params *p)
+{
+    for (int k = 0; k < 2 * p->ndim; k++)
+        if (p->mthbc[k] == 0) return true;
+    return false;
+}
 
 
 template <class T> inline sc::PhysConst<T> sc_make_const(const sc_params *p)

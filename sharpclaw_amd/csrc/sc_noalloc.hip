@@ -100,6 +100,8 @@ extern "C" int sc_step_noalloc(const sc_params *params, double *q, double *dq, d
     if (!params || !q || !dq || !cfl) return sc_fail(SC_ERR_ARG, "NULL argument");
     int rc = sc_check_params(params, true);
     if (rc) return rc;
+    if (sc_user_bc(params))
+        return sc_fail(SC_ERR_UNSUPPORTED, "mthbc 0 (user boundary conditions): step with sc_flux");
     const int phys = sc_phys_id(params);
     PhysSel sel;
     sel.phys = phys;

@@ -1,0 +1,15 @@
+! Default source of a 1D grid (src/1d/src1.f90): dq = 0 in the interior.
+! Called by the driver's host-side step only; an application links its own
+! src in its place.
+subroutine src(q, dq, aux, t, dt)
+    use ClawParams
+    implicit none
+    double precision, intent(in) :: q, aux, t, dt
+    double precision :: dq(1-mbc:nx(1)+mbc, meqn)
+    integer :: i, m
+    do m = 1, meqn
+        do i = 1, nx(1)
+            dq(i,m) = 0.d0
+        enddo
+    enddo
+end subroutine src

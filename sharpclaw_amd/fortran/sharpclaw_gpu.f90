@@ -94,6 +94,14 @@ module sharpclaw_gpu
             real(c_double), intent(inout) :: dq(*)
             real(c_double), intent(out) :: cfl
         end function
+        integer(c_int) function sc_flux(solver, q, dq, dt, t, cfl) bind(C, name="sc_flux")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: solver
+            real(c_double), intent(in) :: q(*)
+            real(c_double), intent(inout) :: dq(*)
+            real(c_double), value :: dt, t
+            real(c_double), intent(out) :: cfl
+        end function
         integer(c_int) function sc_ts(solver, dt, told, cfl) bind(C, name="sc_ts")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: solver

@@ -2,34 +2,46 @@
 ! reference program (src/main.f90:21-270) and of its time loop
 ! (src/sharpclaw.f90:35-235), with every array operation of the hot path
 ! -- qold = q, the SSP stages, q = qold -- done on the device through the
-! C ABI (module sharpclaw_gpu).  The user's hooks setprob, qinit and b4step
-! are linked in exactly as for the reference, sharpclaw.data is read with
-! the same positional layout, and frames are written as fort.qNNNN /
-! fort.tNNNN / fort.info in the reference's formats.
+! C ABI (module sharpclaw_gpu).  The user's hooks setprob, qinit, setaux,
+! b4step, bc and src are linked in exactly as for the reference (an
+! application's object replaces the default of libschooks<ndim>.a), the
+! problem parameters sit in module ClawParams under the reference's names,
+! sharpclaw.data is read with the same positional layout, and frames are
+! written as fort.qNNNN / fort.tNNNN / fort.info in the reference's formats.
 !
 ! Device options live in gpu.data (so sharpclaw.data stays as the reference
-! reads it):  physics id (1 advection, 2 acoustics, 3 Euler, 4 shallow
-! water), precision (0 float64, 1 float32), device ordinal, efix (-1 default),
-! and optionally a source id (0 none; 1 the radial shallow-water source of
-! 1dRadialSymm-qwave/src1.f90), which the reference selects by linking that
-! app's src (its sharpclaw.data src_term is read and ignored, main.f90:96),
-! and the period of the swirl app's b4step velocity factor (2D advection;
-! the host b4step's changes to aux do not reach the device, which applies
-! that factor itself).
+! reads it), one value per line:  physics id (1 advection, 2 acoustics,
+! 3 Euler, 4 shallow water), precision (0 float64, 1 float32), device
+! ordinal, efix (-1 default), and optionally
+!   - a device source id (0 none; 1 the radial shallow-water source of
+!     1dRadialSymm-qwave/src1.f90, computed inside the device stage);
+!   - the period of the swirl app's b4step velocity factor (2D advection:
+!     the device applies that factor itself; 0 = aux as uploaded);
+!   - hooks: 0 (default) the device holds q for the whole run and b4step
+!     sees the host q of the last frame; 1 live b4step: q is fetched from
+!     the device before b4step and q and aux are sent back after it, so
+!     b4step reads and changes the current solution as in the reference
+!     (sharpclaw.f90:129); 2 host step: q stays in host memory, every
+!     stage calls the application's bc and src on it (step1d.f90:20-29,
+!     step2d.f90:25-36) and the device computes flux1/flux2 through
+!     sc_flux, with the SSP combinations of ts_ssp* on the host -- the mode
+!     for user boundary conditions (mthbc 0) and user source terms, which
+!     the fused device stages cannot call.  Any mthbc of 0 selects mode 2.
 ! The physics constants are taken from the common blocks the user's setprob
 ! fills (/param/ gamma,gamma1; /comrp/ u or g; /cparam/ rho,bulk,cc,zz).
 module xsclaw_data
     use iso_c_binding
+    use ClawParams
     implicit none
-    integer :: ndim, nx(2), nout, outstyle, nstepout, nstop, max_steps, dt_variable
-    integer :: time_integrator, verbosity, src_term, mcapa, maux, tfluct_solver
-    integer :: char_decomp, meqn, mwaves, lim_type, mbc, maxnx
-    integer :: mthbc(4)
-    integer, allocatable :: mthlim(:)
+    integer :: nout, outstyle, nstepout, nstop, max_steps, dt_variable
+    integer :: time_integrator, verbosity
     double precision :: tfinal, dt_initial, dt_max, cfl_max, cfl_desired, t0
-    double precision :: xlower(2), xupper(2), dx(2)
     double precision, allocatable :: tout(:)
     double precision, allocatable :: q(:), aux(:)
+    ! host step (hooks mode 2): qold and the SSP registers of
+    ! sharpclaw.f90:68-111 in host memory
+    double precision, allocatable :: qold(:), r1(:), r2(:)
+    integer :: hooks
 contains
     ! src/opendatafile.f:45-53: skip the '#' header and the first line after it
     subroutine open_data_file(iunit, fname)
@@ -64,11 +76,11 @@ program xsclaw_gpu
 
     call open_data_file(55, 'sharpclaw.data')
     read(55,*) ndim
-    nx = 1
+    allocate(nx(ndim))
     do md = 1, ndim
         read(55,*) nx(md)
     enddo
-    maxnx = maxval(nx(1:ndim))
+    maxnx = maxval(nx)
     read(55,*) nout
     read(55,*) outstyle
     select case (outstyle)
@@ -102,25 +114,25 @@ program xsclaw_gpu
     allocate(mthlim(mwaves))
     read(55,*) (mthlim(i), i=1,mwaves)
     read(55,*) t0
-    xlower = 0.d0
-    xupper = 1.d0
+    allocate(xlower(ndim), xupper(ndim), dx(ndim))
     do md = 1, ndim
         read(55,*) xlower(md)
         read(55,*) xupper(md)
     enddo
     read(55,*) mbc
-    mthbc = 1
+    allocate(mthbc(2*ndim))
     do md = 1, 2*ndim
         read(55,*) mthbc(md)
     enddo
     close(55)
+    multid_recon = 0
+    rord = 0
 
     if ((mthbc(1) == 2 .and. mthbc(2) /= 2) .or. (mthbc(2) == 2 .and. mthbc(1) /= 2)) then
         write(6,*) '*** ERROR ***  periodic boundary conditions'
         write(6,*) ' require mthbc(1) and mthbc(2) BOTH be set to 2'
         stop 1
     endif
-    dx = 1.d0
     do i = 1, ndim
         dx(i) = (xupper(i) - xlower(i)) / float(nx(i))
     enddo
@@ -133,11 +145,19 @@ program xsclaw_gpu
     read(56,*) efix
     source = 0
     tperiod = 0.d0
+    hooks = 0
     read(56,*,iostat=ios) source
     if (ios /= 0) source = 0
     if (ios == 0) read(56,*,iostat=ios) tperiod
     if (ios /= 0) tperiod = 0.d0
+    if (ios == 0) read(56,*,iostat=ios) hooks
+    if (ios /= 0) hooks = 0
     close(56)
+    if (any(mthbc == 0)) hooks = 2
+    if (hooks < 0 .or. hooks > 2) then
+        write(6,*) '*** ERROR *** gpu.data: hooks must be 0, 1 or 2'
+        stop 1
+    endif
 
     call setprob
 
@@ -154,7 +174,8 @@ program xsclaw_gpu
     call qinit(ndim, meqn, mbc, nx, xlower, dx, q, maux, aux)
 
     prm%ndim = ndim
-    prm%nx = nx
+    prm%nx = 1
+    prm%nx(1:ndim) = nx
     prm%meqn = meqn
     prm%mwaves = mwaves
     prm%mbc = mbc
@@ -165,9 +186,12 @@ program xsclaw_gpu
     prm%tfluct_solver = tfluct_solver
     prm%time_integrator = time_integrator
     prm%src_term = source
-    prm%mthbc = mthbc
-    prm%xlower = xlower
-    prm%dx = dx
+    prm%mthbc = 1
+    prm%mthbc(1:2*ndim) = mthbc
+    prm%xlower = 0.d0
+    prm%xlower(1:ndim) = xlower
+    prm%dx = 1.d0
+    prm%dx(1:ndim) = dx
     prm%physics = physics
     prm%efix = efix
     prm%gamma = gamma_c
@@ -193,8 +217,16 @@ program xsclaw_gpu
     ! frames go through sc_output_begin/wait: the |q| < 1e-99 flush of the
     ! reference's writers (out1.f90:63-68) then happens in the device q the
     ! run continues from, as it does in the reference's q
-    call sc_check(sc_output_begin(solver), 'sc_output_begin')
-    call sc_check(sc_output_wait(solver, q), 'sc_output_wait')
+    if (hooks == 2) then
+        allocate(qold(size(q)), r1(size(q)))
+        qold = q
+        r1 = 0.d0
+        if (time_integrator == 4) then
+            allocate(r2(size(q)))
+            r2 = 0.d0
+        endif
+    endif
+    call get_frame
     call write_frame(t0, 0)
     write(6,601) 0, t0
     open(10, file='fort.info', status='unknown', form='formatted')
@@ -220,8 +252,7 @@ program xsclaw_gpu
         dt_initial = dt_last
         iframe = n / nstepout
         if (iframe*nstepout == n) then
-            call sc_check(sc_output_begin(solver), 'sc_output_begin')
-            call sc_check(sc_output_wait(solver, q), 'sc_output_wait')
+            call get_frame
             call write_frame(tend, iframe)
             write(6,601) iframe, tend
             write(10,1010) tend, info, dt_minused, dt_maxused, dt_last, cfl_maxused, &
@@ -237,9 +268,107 @@ program xsclaw_gpu
 
 contains
 
+    ! q of a frame into the host array.  On the device (hooks 0, 1) it goes
+    ! through sc_output_begin/wait: the |q| < 1e-99 flush of the reference's
+    ! writers (out1.f90:63-68) then happens in the device q the run
+    ! continues from, as it does in the reference's q; in host-step mode q
+    ! is already here and write_frame flushes it.
+    subroutine get_frame
+        if (hooks == 2) return
+        call sc_check(sc_output_begin(solver), 'sc_output_begin')
+        call sc_check(sc_output_wait(solver, q), 'sc_output_wait')
+    end subroutine get_frame
+
+    ! The user's b4step before a new step (sharpclaw.f90:129).  With live
+    ! hooks it works on the current device solution: q (ghost cells filled
+    ! by the boundary rules) is fetched first, and q and aux are sent back.
+    subroutine user_b4step(told, dt)
+        double precision, intent(in) :: told, dt
+        if (hooks == 1) call sc_check(sc_get_q(solver, q), 'sc_get_q')
+        call b4step(maxnx, mbc, nx, meqn, q, xlower, dx, told, dt, maux, aux)
+        if (hooks == 1) call sc_check(sc_set_q(solver, q), 'sc_set_q')
+        if (hooks >= 1 .and. maux > 0) call sc_check(sc_set_aux(solver, aux), 'sc_set_aux')
+    end subroutine user_b4step
+
+    ! step(q,g,dq,aux,dt,cfl,t,rp,src,tfluct) of src/1d/step1d.f90:2-31 and
+    ! src/2d/step2d.f90:2-41 on host arrays: the application's bc fills the
+    ! ghost cells, its src sets dq, and the device adds dt*L(q) (sc_flux).
+    subroutine host_step(qs, dqs, t, dt, cfl)
+        double precision, intent(inout) :: qs(:), dqs(:)
+        double precision, intent(in) :: t, dt
+        double precision, intent(out) :: cfl
+        if (ndim == 1) then
+            call host_bc1(qs, aux, t, dt)
+        else
+            call host_bc2(qs, aux, t, dt)
+        endif
+        call src(qs, dqs, aux, t, dt)
+        call sc_check(sc_flux(solver, qs, dqs, dt, t, cfl), 'sc_flux')
+    end subroutine host_step
+
+    ! One SSP step on the host arrays, with the stage times and combinations
+    ! of ts_ssp22 / ts_ssp42 / ts_ssp33 / ts_ssp104 (sharpclaw.f90:247-464);
+    ! cfl is the last stage's.
+    subroutine host_ts(dt, told, cfl)
+        double precision, intent(in) :: dt, told
+        double precision, intent(out) :: cfl
+        double precision :: third, sixth
+        select case (time_integrator)
+        case (1)
+            call host_step(q, r1, told, dt, cfl)
+            q = q + r1
+            call host_step(q, r1, told + dt, dt, cfl)
+            q = 0.5d0*(qold + q + r1)
+        case (2)
+            third = 1.d0/3.d0
+            call host_step(q, r1, told, dt, cfl)
+            q = q + third*r1
+            call host_step(q, r1, told + third*dt, dt, cfl)
+            q = q + third*r1
+            call host_step(q, r1, told + 2.d0*third*dt, dt, cfl)
+            q = q + third*r1
+            call host_step(q, r1, told + dt, dt, cfl)
+            q = 0.25d0*qold + 0.75d0*q + 0.25d0*r1
+        case (3)
+            r1 = 0.d0
+            call host_step(q, r1, told, dt, cfl)
+            q = q + r1
+            call host_step(q, r1, told + dt, dt, cfl)
+            q = 0.75d0*qold + 0.25d0*(q + r1)
+            call host_step(q, r1, told + 0.5d0*dt, dt, cfl)
+            q = (qold + 2.d0*(q + r1))/3.d0
+        case default
+            sixth = 1.d0/6.d0
+            call host_step(q, r1, told, dt, cfl)
+            q = q + sixth*r1
+            call host_step(q, r1, told + dt/6.d0, dt, cfl)
+            q = q + sixth*r1
+            call host_step(q, r1, told + dt/3.d0, dt, cfl)
+            q = q + sixth*r1
+            call host_step(q, r1, told + 0.5d0*dt, dt, cfl)
+            q = q + sixth*r1
+            call host_step(q, r1, told + 2.d0/3.d0*dt, dt, cfl)
+            r2 = q + sixth*r1
+            r1 = 0.04d0*qold + 0.36d0*r2
+            q = 0.6d0*qold + 0.4d0*r2
+            call host_step(q, r2, told + 1.d0/3.d0*dt, dt, cfl)
+            q = q + sixth*r2
+            call host_step(q, r2, told + 0.5d0*dt, dt, cfl)
+            q = q + sixth*r2
+            call host_step(q, r2, told + 2.d0/3.d0*dt, dt, cfl)
+            q = q + sixth*r2
+            call host_step(q, r2, told + 5.d0/6.d0*dt, dt, cfl)
+            q = q + sixth*r2
+            call host_step(q, r2, told + dt, dt, cfl)
+            q = r1 + 0.6d0*q + 0.1d0*r2
+        end select
+    end subroutine host_ts
+
     ! The time loop of one output interval (sharpclaw.f90:35-235).  qold is
-    ! saved on the device at every new step; the SSP step runs as fused
-    ! device stages (sc_ts); the CFL of its last stage drives dt.
+    ! saved at every new step (a buffer renaming on the device); the SSP step
+    ! runs as fused device stages (sc_ts), or stage by stage through the
+    ! application's bc / src and sc_flux in host-step mode; the CFL of its
+    ! last stage drives dt.
     subroutine gpu_sharpclaw(solver, ctl, tstart, tend, cfl_maxused, cfl_last, dt_minused, &
                              dt_maxused, dt_last, steps_taken, info)
         type(c_ptr), intent(in) :: solver
@@ -281,12 +410,20 @@ contains
             if (.not. retake) then
                 told = t
                 if (told + dt > tend .and. tstart < tend) dt = tend - told
-                call b4step(maxnx, mbc, nx, meqn, q, xlower, dx, told, dt, maux, aux)
-                call sc_check(sc_save_qold(solver), 'sc_save_qold')
+                call user_b4step(told, dt)
+                if (hooks == 2) then
+                    qold = q
+                else
+                    call sc_check(sc_save_qold(solver), 'sc_save_qold')
+                endif
             endif
             retake = .false.
             t = told + dt
-            call sc_check(sc_ts(solver, dt, told, cfl), 'sc_ts')
+            if (hooks == 2) then
+                call host_ts(dt, told, cfl)
+            else
+                call sc_check(sc_ts(solver, dt, told, cfl), 'sc_ts')
+            endif
             if (verbosity == 1) write(6,602) it, cfl, dt, t
             if (ctl%dt_variable == 1) then
                 if (cfl > 0.d0) then
@@ -301,7 +438,11 @@ contains
                 cflmax = dmax1(cfl, cflmax)
             else
                 t = told
-                call sc_check(sc_restore_qold(solver), 'sc_restore_qold')
+                if (hooks == 2) then
+                    q = qold
+                else
+                    call sc_check(sc_restore_qold(solver), 'sc_restore_qold')
+                endif
                 if (verbosity == 1) write(6,*) 'SClaw rejecting step... Courant number too large'
                 if (ctl%dt_variable == 1) then
                     retake = .true.

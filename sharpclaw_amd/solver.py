@@ -112,6 +112,19 @@ class Solver:
         check(lib().sc_step(self._h, dt, t, None, C.byref(cfl)))
         return None, cfl.value
 
+    def flux(self, q, dq, dt, t=0.0):
+        """flux1 / flux2 on the caller's q with its ghost cells as given
+        (after the caller's bc): dq(interior) += dt*L(q) in place; returns
+        the CFL number (2D: ghost rows and columns included)."""
+        q = np.asfortranarray(q, dtype=np.float64)
+        if q.shape != self.shape or dq.shape != self.shape or dq.dtype != np.float64 \
+                or not dq.flags.f_contiguous:
+            raise ValueError(f"q and dq must be float64 arrays of shape {self.shape} (dq Fortran-ordered)")
+        cfl = C.c_double()
+        check(lib().sc_flux(self._h, q.ctypes.data_as(C.c_void_p), dq.ctypes.data_as(C.c_void_p), dt, t,
+                            C.byref(cfl)))
+        return cfl.value
+
     def ts(self, dt, told=0.0):
         """One fused SSP time step from q (call save_qold first, as
         sharpclaw.f90:134 does); returns the last stage's CFL number."""
