@@ -81,20 +81,9 @@ __global__ __launch_bounds__(256) void stage2d_batch_kernel(Batch2DArgs A, PhysC
     stage2d_tile<P, CHAR, T, TX, TY>(a, c, local);
 }
 
-__device__ void finish_loop(const Grid2D &G, SolveState &s)
-{
-    if (s.info == 0 && G.ctl.dt_variable == 1 && s.t < G.tend && s.steps == G.maxn) s.info = 11;
-    if (s.info == 0 && G.ctl.dt_variable == 0 && s.cflmax > G.ctl.cfl_max) s.info = 12;
-    s.done = 1;
-    s.go = 0;
-    s.t_end = (long long)wall_clock64();
-}
-
-// One thread per grid.  Post: the CFL of the iteration's last stage
-// (flux2.f90:47,80: max of dt/dx*|s| over x sweeps and dt/dy*|s| over y
-// sweeps), the dt update and accept/reject (sharpclaw.f90:184-216).  Pre:
-// the next iteration's dt clamp to tend and qold = q (sharpclaw.f90:132-150)
-// and fresh CFL words.
+// One thread per grid.  Post: close the iteration whose stages just ran;
+// pre: open the next (loop2d_post / loop2d_pre, sc_batch2d.h) and clear the
+// grid's CFL words.
 __global__ void ctl2d_kernel(Batch2DArgs A, int do_post, int do_pre)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -103,67 +92,14 @@ __global__ void ctl2d_kernel(Batch2DArgs A, int do_post, int do_pre)
     SolveState s = A.state[g];
     const int nst = stages_of_integrator(A.ti);
     if (do_post && s.go) {
-        s.go = 0;
         double sx = __longlong_as_double((long long)cfl_word(A.smax + (size_t)CFL_WORDS * g, 0));
         double sy = __longlong_as_double((long long)cfl_word(A.smax + (size_t)CFL_WORDS * g, 1));
-        s.stages += nst;
-        s.cfl = fmax((s.dt / G.dx) * sx, (s.dt / G.dy) * sy);
-        const sc_run_ctl &ctl = G.ctl;
-        if (ctl.dt_variable == 1) {
-            if (s.cfl > 0.0) {
-                s.dt = fmin(ctl.dt_max, s.dt * ctl.cfl_desired / s.cfl);
-                s.dtmin = fmin(s.dt, s.dtmin);
-                s.dtmax = fmax(s.dt, s.dtmax);
-            } else {
-                s.dt = ctl.dt_max;
-            }
-        }
-        bool fin = false, counted = true;
-        if (s.cfl <= ctl.cfl_max) {
-            s.cflmax = fmax(s.cfl, s.cflmax);
-            int in = s.cur, out = other_buf(in, s.old);
-            for (int k = 1; k < nst; k++) {
-                in = out;
-                out = other_buf(in, s.old);
-            }
-            s.cur = out;
-        } else {
-            s.t = s.told;
-            s.cur = s.old;  // q = qold
-            s.rejected++;
-            if (ctl.dt_variable == 1) {
-                s.retake = 1;
-            } else {
-                s.cflmax = fmax(s.cfl, s.cflmax);
-                fin = true;
-                counted = false;
-            }
-        }
-        if (counted) {
-            s.steps++;
-            if (s.t >= G.tend) fin = true;
-        }
-        if (!fin && (s.info != 0 || s.it >= G.maxn)) fin = true;
-        if (fin) finish_loop(G, s);
+        loop2d_post(G, s, nst, sx, sy);
     }
-    if (do_pre && !s.done) {
-        if (s.info != 0 || s.it >= G.maxn) {
-            finish_loop(G, s);
-        } else {
-            if (s.t_begin == 0) s.t_begin = (long long)wall_clock64();
-            s.it++;
-            if (!s.retake) {
-                s.told = s.t;
-                if (s.told + s.dt > G.tend && G.tstart < G.tend) s.dt = G.tend - s.told;
-                s.old = s.cur;  // qold = q
-            }
-            s.retake = 0;
-            s.t = s.told + s.dt;
-            s.go = 1;
-            for (int k = 0; k < CFL_SUB; k++) {
-                A.smax[(size_t)CFL_WORDS * g + k * CFL_STRIDE] = 0ull;
-                A.smax[(size_t)CFL_WORDS * g + k * CFL_STRIDE + 1] = 0ull;
-            }
+    if (do_pre && loop2d_pre(G, s)) {
+        for (int k = 0; k < CFL_SUB; k++) {
+            A.smax[(size_t)CFL_WORDS * g + k * CFL_STRIDE] = 0ull;
+            A.smax[(size_t)CFL_WORDS * g + k * CFL_STRIDE + 1] = 0ull;
         }
     }
     A.state[g] = s;

@@ -200,9 +200,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
         const int ch0 = (int)((long long)nch * part / parts), ch1 = (int)((long long)nch * (part + 1) / parts);
         unsigned long long *sync = parts > 1 ? A.sync + 4 * (size_t)b : nullptr;
         const size_t len = (size_t)(n + 6) * A.meqn;
-        double *buf[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) buf[j] = A.base + S.off + j * len;
+        // the solve's buffer j (0-2: q buffers, 3: the SSP(10,4) register),
+        // formed from the argument pointer each time: a pointer kept in a
+        // dynamically indexed array is a generic pointer to the compiler,
+        // whose accesses it would issue as flat instructions
+        auto buf = [&](int j) { return A.base + S.off + (size_t)j * len; };
         if (s.t_begin == 0) s.t_begin = (long long)wall_clock64();
 
         StageArgs a;
@@ -215,8 +217,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
         a.bc[2] = a.bc[3] = 1;
         a.smax = nullptr;
         a.ntx = a.nty = 1;
-        a.qbin = buf[3];
-        a.qbout = buf[3];
+        a.qbin = buf(3);
+        a.qbout = buf(3);
         a.capa = nullptr;  // batched solves have no aux and no source
         a.src = 0;
         a.dt = 0.0;
@@ -244,14 +246,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
             s.retake = 0;
             s.t = s.told + s.dt;
             a.dtdx = s.dt / S.dx;
-            a.qold = buf[s.old];
+            a.qold = buf(s.old);
             int in = s.cur;
             T sm = T(0);
             bool lost = false;  // a barrier of a split solve timed out
             for (int st = 0; st < plan.n; st++) {
                 int outb = other_buf(in, s.old);
-                a.qin = buf[in];
-                a.qout = buf[outb];
+                a.qin = buf(in);
+                a.qout = buf(outb);
                 a.op = plan.op[st];
                 const bool last = st == plan.n - 1;
                 for (int ch = ch0; ch < ch1; ch++) {

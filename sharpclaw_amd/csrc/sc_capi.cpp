@@ -200,7 +200,12 @@ struct sc_solver {
     double *mem;       // one allocation: buf[0..2], qb, dq[, dqx]
     int loop_mode;     // sc_advance: -1 by grid size, 0 host-driven, 1 device-resident
     bool use_graph;    // device loop: replay rounds from a captured graph
-    Batch2DRunner *dr; // 2D device-resident loop (created at first use)
+    Batch2DRunner *dr; // 2D device-resident loop replayed from a graph (SC_GRAPH; created at first use)
+    // 2D device-resident loop (created at first use): the control block and
+    // its CFL set on the device, a pinned copy for the upload and the state readback
+    Res2D *res;
+    unsigned long long *rset;
+    Res2D *hres;
     hipEvent_t ev0, ev1;
     double *capa;      // capacity aux(:,mcapa) in the q layout (1D, mcapa > 0)
     double *aux2;      // 2D advection: edge velocities aux(:,:,1:2)
@@ -376,6 +381,9 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->capa) hipFree(s->capa);
     if (s->aux2) hipFree(s->aux2);
     if (s->gsl) hipFree(s->gsl);
+    if (s->res) hipFree(s->res);
+    if (s->rset) hipFree(s->rset);
+    if (s->hres) hipHostFree(s->hres);
     if (s->tnf) hipFree(s->tnf);
     if (s->l1.solve) hipFree(s->l1.solve);
     for (int k = 0; k < s->tev_cap; k++)
@@ -1026,6 +1034,110 @@ static int stages_of(int ti)
     return n[ti];
 }
 
+// The sharpclaw() loop of a 2D grid of at most 16384 tiles kept on the
+// device: the stage launches take their arguments from a control block in
+// device memory (Res2D), which a one-block control launch after each
+// iteration rewrites from the loop's decisions (CFL, dt, accept/reject,
+// buffer rotation; the same arithmetic as the host-driven loop) together
+// with the adaptive tile list, so the host enqueues a round of iterations
+// without waiting for any of them and reads the loop state back once per
+// round.  Launches enqueued past the end of the loop return at once.
+static int advance2d_resident(sc_solver *s, const sc_run_ctl *c, double tstart, double tend, double *dt_io,
+                              sc_run_stats *st)
+{
+    if (!s->res) {
+        if (hipMalloc(&s->res, sizeof(Res2D)) != hipSuccess ||
+            hipMalloc(&s->rset, CFL_WORDS * sizeof(unsigned long long)) != hipSuccess ||
+            hipHostMalloc(&s->hres, sizeof(Res2D), hipHostMallocDefault) != hipSuccess)
+            return sc_fail(SC_ERR_NOMEM, "device loop: control block allocation failed");
+    }
+    const int shape = s->tile2d < 0 ? 0 : s->tile2d;
+    int tx, ty;
+    tile2d_shape(shape, &tx, &ty);
+    const int ntx = (s->p.nx[0] + tx - 1) / tx, nty = (s->p.nx[1] + ty - 1) / ty;
+    if (s->tlist_tile != shape) {
+        HIPCHK(launch_tile_list_reset(s->tflag, ntx, nty, s->tlist, s->tnf, s->st));
+        s->tlist_tile = shape;
+    }
+    Res2D &R = *s->hres;
+    memset(&R, 0, sizeof R);
+    R.G.nx = s->p.nx[0];
+    R.G.ny = s->p.nx[1];
+    R.G.LX = R.G.nx + 2 * s->p.mbc;
+    R.G.LY = R.G.ny + 2 * s->p.mbc;
+    R.G.ntx = ntx;
+    R.G.nty = nty;
+    R.G.dx = s->p.dx[0];
+    R.G.dy = s->p.dx[1];
+    R.G.tstart = tstart;
+    R.G.tend = tend;
+    R.G.ctl = *c;
+    int info0 = 0;
+    loop_limits(*c, tstart, tend, *dt_io, &R.G.maxn, &info0);
+    R.s = initial_state(tstart, *dt_io, s->cur, info0);
+    const StagePlan P = plan_of(s->p.time_integrator);
+    R.nst = P.n;
+    // the stages' fixed arguments (stage2d_res_kernel); buffers, dt and the
+    // tile count come from the control block
+    StageArgs A[10];
+    for (int k = 0; k < P.n; k++) {
+        A[k] = base_args(s);
+        A[k].order = TILE_ADAPTIVE;
+        A[k].ntx = ntx;
+        A[k].nty = nty;
+        A[k].tflag = s->tflag;
+        A[k].tlist = s->tlist;
+        A[k].tnf = s->tnf;
+        A[k].qbin = s->qb;
+        A[k].qbout = s->qb;
+        A[k].qin = s->buf[0];  // the three buffers, whose roles the control block gives
+        A[k].qold = s->buf[1];
+        A[k].qout = s->buf[2];
+        A[k].op = P.op[k];
+        A[k].smax = k == P.n - 1 ? s->rset : nullptr;
+    }
+    HIPCHK(hipEventRecord(s->ev0, s->st));
+    HIPCHK(hipMemcpyAsync(s->res, &R, sizeof R, hipMemcpyHostToDevice, s->st));
+    HIPCHK(launch_res2d_ctl(s->res, s->tflag, s->tlist, s->tnf, s->rset, ntx, nty, 0, 1, s->st));  // open the first iteration
+    SolveState S = R.s;
+    for (;;) {
+        // iterations of this round: those left, and no more than reach tend
+        // at the current dt (one more for a retake)
+        long long k = (long long)R.G.maxn - S.it + (S.go ? 1 : 0);
+        if (tend > S.t && S.dt > 0.0) k = std::min(k, (long long)std::ceil((tend - S.t) / S.dt) + 1);
+        const int K = (int)std::max(1LL, std::min(64LL, k));
+        for (int it = 0; it < K; it++) {
+            for (int j = 0; j < P.n; j++) HIPCHK(launch_stage2d_res(s->sel, shape, A[j], s->res, j, s->cd, s->cf, s->st));
+            HIPCHK(launch_res2d_ctl(s->res, s->tflag, s->tlist, s->tnf, s->rset, ntx, nty, 1, 1, s->st));
+        }
+        HIPCHK(hipMemcpyAsync(&R.s, (const char *)s->res + offsetof(Res2D, s), sizeof(SolveState),
+                              hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        S = R.s;
+        if (S.done) break;
+    }
+    HIPCHK(hipEventRecord(s->ev1, s->st));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->cur = S.cur;
+    s->old = S.old;
+    memset(st, 0, sizeof *st);
+    st->gpu_ms = ms;
+    st->t = S.t;
+    st->cfl_maxused = S.cflmax;
+    st->cfl_last = S.cfl;
+    st->dt_minused = S.dtmin;
+    st->dt_maxused = S.dtmax;
+    st->dt_last = S.dt;
+    st->steps_taken = S.steps;
+    st->info = S.info;
+    st->rejected = S.rejected;
+    st->stage_evals = S.stages;
+    *dt_io = S.dt;
+    return SC_OK;
+}
+
 // The sharpclaw() loop of a 2D grid kept on the device (sc_batch2d.h): the
 // CFL, dt update and accept/reject of every step are taken by a control
 // kernel, and the host reads the loop state back once per round of K steps
@@ -1227,9 +1339,11 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
     int trc = tune_tile(s);
     if (trc) return trc;
     if (s->p.ndim == 2) {
-        // The device loop saves the per-step CFL readback (~5-15 us); on
-        // grids whose stages take far longer than that the host loop is as
-        // fast (1024^2 Euler: 61.8 ms host, 62.6 ms device for 200 steps).
+        // The device loop saves the per-step CFL readback and the host's
+        // turnaround between steps (~13 us at 1024^2); grids above 16384
+        // tiles (one control block partitions at most that many) and
+        // SC_HOST_LOOP=1 take the host loop, SC_GRAPH=1 the graph-replayed
+        // rounds of the batched 2D kernels.
         int tx, ty;
         tile2d_shape(s->tile2d, &tx, &ty);
         long long tiles = (long long)((s->p.nx[0] + tx - 1) / tx) * ((s->p.nx[1] + ty - 1) / ty);
@@ -1237,8 +1351,11 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
         // (the slice sweeps have no device-loop kernel)
         // (per-launch stage timing brackets launches from the host loop)
         bool device = s->sel.phys != ADV2 && !s->dqx && !s->timing &&
-                      (s->loop_mode == 1 || (s->loop_mode < 0 && tiles < 2048));
-        if (device) return advance2d_device(s, c, tstart, tend, dt_io, st);
+                      s->loop_mode != 0;
+        if (device && s->use_graph) return advance2d_device(s, c, tstart, tend, dt_io, st);
+        if (device && tiles <= 16384 && s->tnf && s->tile_order == TILE_ADAPTIVE)
+            return advance2d_resident(s, c, tstart, tend, dt_io, st);
+        if (device && s->loop_mode == 1) return advance2d_device(s, c, tstart, tend, dt_io, st);
     } else if (loop1d_fits(s) && !s->timing) {
         // grids of up to MAX_PARTS chunks loop on the device: per step the
         // host-driven loop waits for the CFL (a few microseconds per stage

@@ -94,6 +94,17 @@ hipError_t launch_tile_partition(const unsigned char *flag, int ntx, int nty, in
                                  const unsigned long long *set, double *out, int *bcount, hipStream_t st);
 // the max over the copies of the x and y words of a CFL set into out[0..1]
 hipError_t launch_cfl_to_host(const unsigned long long *set, double *out, hipStream_t st);
+// The device-resident loop of one 2D grid (Res2D, sc_batch2d.h; grids of at
+// most 16384 tiles): stage k of the current iteration on every tile (a: the
+// stage's fixed arguments; its buffers and dt come from R), a no-op once
+// the loop is done; and the control launch between iterations (do_post:
+// close the iteration that ran -- CFL, dt, accept/reject -- and rebuild the
+// tile list from its flags; do_pre: open the next).
+struct Res2D;
+hipError_t launch_stage2d_res(const PhysSel &s, int tile, const StageArgs &a, const Res2D *R, int k,
+                              const PhysConst<double> &cd, const PhysConst<float> &cf, hipStream_t st);
+hipError_t launch_res2d_ctl(Res2D *R, const unsigned char *tflag, int *tlist, int *tnf, unsigned long long *set,
+                            int ntx, int nty, int do_post, int do_pre, hipStream_t st);
 hipError_t launch_tile_list_reset(unsigned char *flag, int ntx, int nty, int *list, int *nf, hipStream_t st);
 // ghost fill in place (bc1.f90 / bc2.f90 semantics, corners included)
 hipError_t launch_bc(int ndim, int meqn, double *q, int nx, int ny, int LX, int LY, const int *bc,

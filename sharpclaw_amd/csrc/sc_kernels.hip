@@ -13,6 +13,7 @@
 
 #include <hip/hip_ext.h>
 
+#include "sc_batch2d.h"
 #include "sc_kernels.h"
 #include "sc_stage.h"
 #include "sc_stage2d.h"
@@ -79,16 +80,16 @@ __global__ __launch_bounds__(NT * NLN) void sweep2d_kernel(StageArgs a, PhysCons
 // (a non-constant tile takes several times as long as a constant one, so
 // these start first and spread evenly over the XCDs), then the constant ones,
 // each XCD a contiguous run of them in tile order (tile_partition builds the
-// list after each step).  Piecewise-constant data such as a four-quadrant
+// list after each step; nf is the count of the former, *a.tnf).
+// Piecewise-constant data such as a four-quadrant
 // Riemann problem has its non-constant tiles in a few rows of tiles, which
 // TILE_BANDS would give to one or two XCDs, and in tile order they would
 // start late and set the launch's tail.
-__device__ __forceinline__ int tile_of_workgroup(const StageArgs &a, int w)
+__device__ __forceinline__ int tile_of_workgroup(const StageArgs &a, int w, int nf)
 {
     const int ntiles = a.ntx * a.nty;
     if (w >= ntiles) return -1;
     if (a.order == TILE_ADAPTIVE) {
-        const int nf = *a.tnf;
         if (w < nf) return a.tlist[w];
         const int x = w & 7;
         int start = 0;
@@ -180,13 +181,12 @@ struct StripWalk {
 // two-level scan (wave prefix sums, then the 16 wave totals) places the
 // runs.  With set != NULL the first wave also hands the step's CFL set to the
 // host (cfl_set_to).
-__global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned char *flag, int ntx, int nty, int *list,
-                                                              int *nf, const unsigned long long *set, double *out)
+// The partition itself, by a block of 1024 threads.
+__device__ __forceinline__ int tile_partition_block(const unsigned char *flag, int ntx, int nty, int *list, int *nf)
 {
     __shared__ int wsum[16];
     const int n = ntx * nty;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    if (set && wid == 0) cfl_set_to(set, out);
     const int per = (n + 1023) / 1024;
     const int lo = min(n, t * per), hi = min(n, lo + per);
     int cnt = 0;
@@ -224,6 +224,14 @@ __global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned cha
             list[pc++] = tile;
     }
     if (t == 0) *nf = total;
+    return total;
+}
+
+__global__ __launch_bounds__(1024) void tile_partition_kernel(const unsigned char *flag, int ntx, int nty, int *list,
+                                                              int *nf, const unsigned long long *set, double *out)
+{
+    if (set && threadIdx.x < 64) cfl_set_to(set, out);
+    tile_partition_block(flag, ntx, nty, list, nf);
 }
 
 // The same partition for grids of many tiles, one position per thread over
@@ -322,8 +330,90 @@ hipError_t launch_tile_list_reset(unsigned char *flag, int ntx, int nty, int *li
 template <int P, int CHAR, class T, int TX, int TY>
 __global__ __launch_bounds__(256) void stage2d_kernel(StageArgs a, PhysConst<T> c)
 {
-    const int b = tile_of_workgroup(a, blockIdx.x);
+    const int b = tile_of_workgroup(a, blockIdx.x, a.order == TILE_ADAPTIVE ? *a.tnf : 0);
     stage2d_tile<P, CHAR, T, TX, TY>(a, c, b);
+}
+
+// Stage k of the resident loop's current iteration.  The grid's fixed
+// arguments (shape, boundaries, combination, tile list, CFL set) come by
+// value as for stage2d_kernel, with the three q buffers in qin, qold and
+// qout; the iteration's buffer roles, dt and tile count come from the
+// control block (one 64-byte record, ResStage), which the control launch
+// wrote after the last iteration: a workgroup waits for one scalar load, as
+// stage2d_kernel does for its tile count.  (The buffers are chosen among the
+// argument pointers rather than read from memory: the compiler knows
+// argument pointers to be global memory and issues global loads and stores
+// through them, where a pointer read from memory would take flat ones.)
+// Nothing runs once the loop is done.
+template <int P, int CHAR, class T, int TX, int TY>
+__global__ __launch_bounds__(256) void stage2d_res_kernel(StageArgs a, const Res2D *R, int k, PhysConst<T> c)
+{
+    const ResStage r = R->st[k];
+    if (!r.go) return;
+    const double *b0 = a.qin, *b1 = a.qold;
+    double *b2 = a.qout;
+    auto pick = [&](int i) { return i == 0 ? b0 : i == 1 ? b1 : (const double *)b2; };
+    a.qin = pick(r.in);
+    a.qold = pick(r.old);
+    a.qout = r.out == 0 ? const_cast<double *>(b0) : r.out == 1 ? const_cast<double *>(b1) : b2;
+    a.dtdx = r.dtdx;
+    a.dtdy = r.dtdy;
+    a.dt = r.dt;
+    const int b = tile_of_workgroup(a, blockIdx.x, r.nf);
+    stage2d_tile<P, CHAR, T, TX, TY>(a, c, b);
+}
+
+// The control launch of the resident loop (one block of 1024 threads).
+// do_post, after an iteration's stages: its CFL from the last stage's set,
+// the loop's decisions (loop2d_post), and the tile list from the last
+// stage's flags (tile_partition_block).  do_pre: open the next iteration
+// (loop2d_pre) and write what its stage launches need (ResStage: buffers
+// rotated and dt formed as the host-driven step does), or mark them as
+// nothing to do; clear the words of the iteration's CFL set.
+__global__ __launch_bounds__(1024) void res2d_ctl_kernel(Res2D *R, const unsigned char *tflag, int *tlist, int *tnf,
+                                                         unsigned long long *set, int ntx, int nty, int do_post,
+                                                         int do_pre)
+{
+    __shared__ double sxy[2];
+    __shared__ int opened;
+    const int t = threadIdx.x;
+    const bool ran = do_post && R->s.go;
+    // thread 0 reads the loop state and the grid's limits now, so that their
+    // latency passes under the partition
+    SolveState s;
+    Grid2D G;
+    if (t == 0) {
+        s = R->s;
+        G = R->G;
+    }
+    if (ran && t < 64) cfl_set_to(set, sxy);
+    // the tile list of the next iteration (kept when no iteration ran)
+    const int nf = ran ? tile_partition_block(tflag, ntx, nty, tlist, tnf) : *tnf;
+    __syncthreads();
+    if (t == 0) {
+        if (ran) loop2d_post(G, s, R->nst, sxy[0], sxy[1]);
+        const bool op = do_pre && loop2d_pre(G, s);
+        int in = s.cur;
+        for (int k = 0; k < R->nst; k++) {
+            const int out = other_buf(in, s.old);
+            ResStage r;
+            r.in = in;
+            r.old = s.old;
+            r.out = out;
+            r.dtdx = s.dt / G.dx;
+            r.dtdy = s.dt / G.dy;
+            r.dt = s.dt;
+            r.go = op ? 1 : 0;
+            r.nf = nf;
+            R->st[k] = r;
+            in = out;
+        }
+        R->s = s;
+        opened = op;
+    }
+    __syncthreads();
+    // the x and y words of every copy (the words the stages merge into)
+    if (opened && t < 2 * CFL_SUB) set[(t >> 1) * CFL_STRIDE + (t & 1)] = 0ull;
 }
 
 // ------------------------------------------------------------ ghost fill
@@ -729,6 +819,54 @@ hipError_t launch_stage2d(const PhysSel &s, int tile, const StageArgs &a, const 
         return s.fp32 ? go2<ADV2, 0, float>(tile, a, cf, st) : go2<ADV2, 0, double>(tile, a, cd, st);
     }
     return hipErrorInvalidValue;
+}
+
+template <int P, int CHAR, class T, int TX, int TY>
+static hipError_t go2r(const StageArgs &a, const Res2D *R, int k, const PhysConst<T> &c, hipStream_t st)
+{
+    hipLaunchKernelGGL((stage2d_res_kernel<P, CHAR, T, TX, TY>), dim3(a.ntx * a.nty), dim3(256), 0, st, a, R, k, c);
+    return hipGetLastError();
+}
+
+template <int P, int CHAR, class T>
+static hipError_t go2r_t(int tile, const StageArgs &a, const Res2D *R, int k, const PhysConst<T> &c, hipStream_t st)
+{
+    switch (tile) {
+    case 1: return go2r<P, CHAR, T, 15, 16>(a, R, k, c, st);
+    case 2: return go2r<P, CHAR, T, 32, 7>(a, R, k, c, st);
+    case 3: return go2r<P, CHAR, T, 14, 17>(a, R, k, c, st);
+    default: return go2r<P, CHAR, T, 16, 15>(a, R, k, c, st);
+    }
+}
+
+template <int P>
+static hipError_t go2r_p(const PhysSel &s, int tile, const StageArgs &a, const Res2D *R, int k,
+                         const PhysConst<double> &cd, const PhysConst<float> &cf, hipStream_t st)
+{
+    if (s.chr == 2)
+        return s.fp32 ? go2r_t<P, 2, float>(tile, a, R, k, cf, st) : go2r_t<P, 2, double>(tile, a, R, k, cd, st);
+    return s.fp32 ? go2r_t<P, 0, float>(tile, a, R, k, cf, st) : go2r_t<P, 0, double>(tile, a, R, k, cd, st);
+}
+
+hipError_t launch_stage2d_res(const PhysSel &s, int tile, const StageArgs &a, const Res2D *R, int k,
+                              const PhysConst<double> &cd, const PhysConst<float> &cf, hipStream_t st)
+{
+    if (tile < 0 || tile >= NTILE2D) tile = 0;
+    if (a.ntx * a.nty < 1) return hipErrorInvalidValue;
+    switch (s.phys) {
+    case EU2: return go2r_p<EU2>(s, tile, a, R, k, cd, cf, st);
+    case SW2: return go2r_p<SW2>(s, tile, a, R, k, cd, cf, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_res2d_ctl(Res2D *R, const unsigned char *tflag, int *tlist, int *tnf, unsigned long long *set,
+                            int ntx, int nty, int do_post, int do_pre, hipStream_t st)
+{
+    if ((long long)ntx * nty > 16384) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(res2d_ctl_kernel, dim3(1), dim3(1024), 0, st, R, tflag, tlist, tnf, set, ntx, nty, do_post,
+                       do_pre);
+    return hipGetLastError();
 }
 
 void tile2d_shape(int tile, int *tx, int *ty)

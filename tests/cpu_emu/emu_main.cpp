@@ -409,9 +409,10 @@ bool batch_split(const std::string &name, sc_params p, const std::vector<int> &m
     return ok;
 }
 
-// sc_advance on a 1D grid: the device-resident loop (the grid split over
-// its chunks) against the host-driven loop (SC_HOST_LOOP=1), bitwise, and
-// against the oracle.
+// sc_advance: the device-resident loop (1D: the grid split over its
+// chunks; 2D: stage launches driven by the control block, several rounds)
+// against the host-driven loop (SC_HOST_LOOP=1), bitwise, and against the
+// oracle.
 bool loop1d(const std::string &name, sc_params p, unsigned long long seed, double tend, double tol)
 {
     std::vector<double> q = state(p, seed);
@@ -539,6 +540,16 @@ std::vector<Scenario> scenarios()
     v.push_back({"advance1d euler ssp33", []() {
                      sc_params p = base(1, 600, 1, SC_PHYS_EULER);
                      return advance("advance1d euler ssp33", p, 83, 6, 1e-11);
+                 }});
+    v.push_back({"loop2d euler ssp33", []() {
+                     sc_params p = base(2, 48, 40, SC_PHYS_EULER);
+                     return loop1d("loop2d euler ssp33", p, 85, 0.15, 1e-11);
+                 }});
+    v.push_back({"loop2d shallow water fp32 ssp22", []() {
+                     sc_params p = base(2, 36, 52, SC_PHYS_SHALLOW_WATER);
+                     p.precision = SC_FP32;
+                     p.time_integrator = 1;
+                     return loop1d("loop2d shallow water fp32 ssp22", p, 87, 0.05, 1e-5);
                  }});
     v.push_back({"noalloc euler2d", []() { return noalloc("noalloc euler2d", base(2, 45, 38, SC_PHYS_EULER), 91); }});
     v.push_back({"noalloc sw1d", []() {

@@ -129,3 +129,37 @@ def test_host_step_with_default_hooks(mthbc):
         dg, dr, ds = run_case("ac1d", "ac1d", p, cd, 2, None, tmp)
         assert frames_agree(dg, dr, ds, 2) <= 1.0
         assert open(os.path.join(dg, "fort.info")).read() == open(os.path.join(dr, "fort.info")).read()
+
+
+@pytest.mark.skipif(not (have_driver("eu2d") and have_ref("eu2d")), reason="driver or reference build missing")
+def test_host_step_2d_default_hooks():
+    """gpu.data hooks 2 on a 2D Euler grid: the driver's default bc (the
+    rules of bc2.f90, x ghosts first, corners from y) and zero src on the
+    host, flux2 on the device through sc_flux (ghost-row CFL included).
+    Square grid: the reference's flux2 hands its y slices to flux1's dummy
+    q1d(1-mbc:maxnx+mbc, meqn), which mis-strides them when ny < maxnx."""
+    p = make_params("euler", (36, 36), char_decomp=2, time_integrator=3, mthbc=(1, 3, 3, 1))
+    rng = np.random.default_rng(17)
+    st = np.stack([rng.uniform(0.5, 1.5, 4), rng.uniform(-0.5, 0.5, 4), rng.uniform(-0.5, 0.5, 4),
+                   rng.uniform(0.5, 1.5, 4)], axis=1)
+    lines = ["1", "0.5 0.5"] + [" ".join(repr(float(v)) for v in row) for row in st]
+    cd = R.clawdata_for(p, nout=2, outstyle=1, tfinal=0.05, dt_initial=1e-3, cfl_max=0.5,
+                        cfl_desired=0.45)
+    with tempfile.TemporaryDirectory() as tmp:
+        dg, dr = os.path.join(tmp, "gpu"), os.path.join(tmp, "ref")
+        for d in (dg, dr):
+            os.makedirs(d)
+            setup_case(d, p, cd, lines, "euler")
+        with open(os.path.join(dg, "gpu.data"), "a") as f:
+            f.write("0\n0.0\n2\n")
+        rg = subprocess.run([os.path.join(FB, "xsclaw_gpu_eu2d")], cwd=dg, capture_output=True, text=True,
+                            timeout=300)
+        assert rg.returncode == 0, rg.stdout + rg.stderr
+        rr = subprocess.run([os.path.join(R.REF, "xsclaw_eu2d_fmt")], cwd=dr, capture_output=True, text=True,
+                            timeout=300)
+        assert rr.returncode == 0, rr.stdout + rr.stderr
+        assert open(os.path.join(dg, "fort.q0000")).read() == open(os.path.join(dr, "fort.q0000")).read()
+        for f in ("fort.q0001", "fort.q0002"):
+            a, b = read_q(os.path.join(dg, f)), read_q(os.path.join(dr, f))
+            assert nrm(a, b) <= 1e-11, f  # 2D frames print 16 digits (out2.f90)
+        assert open(os.path.join(dg, "fort.info")).read() == open(os.path.join(dr, "fort.info")).read()
