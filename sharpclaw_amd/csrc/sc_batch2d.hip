@@ -78,6 +78,7 @@ __global__ __launch_bounds__(256) void stage2d_batch_kernel(Batch2DArgs A, PhysC
     a.tflag = nullptr;
     a.tlist = nullptr;
     a.tnf = nullptr;
+    a.dflag = nullptr;
     stage2d_tile<P, CHAR, T, TX, TY>(a, c, local);
 }
 

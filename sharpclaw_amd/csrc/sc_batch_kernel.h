@@ -224,6 +224,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
         a.dt = 0.0;
         a.xlo = 0.0;
         a.dxc = S.dx;
+        a.tflag = nullptr;
+        a.dflag = nullptr;
 
         int lim = 0x7fffffff;
         if (A.step_limit > 0) lim = A.step_limit;

@@ -195,6 +195,7 @@ struct sc_solver {
     // from them after each step and its count of non-constant tiles, for the
     // tile shape tlist_tile (-1: not built)
     unsigned char *tflag;
+    unsigned char *dflag;  // flags of a step's first stage (the deep-constant tiles of its later stages)
     int *tlist, *tnf, *tcount;
     int tlist_tile;
     double *mem;       // one allocation: buf[0..2], qb, dq[, dqx]
@@ -328,7 +329,7 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
         }
         void *tm = nullptr;
         const int nbc = (nmax + 1023) / 1024;
-        if (hipMalloc(&tm, (size_t)(nmax + nbc + 2) * sizeof(int) + nmax) != hipSuccess) {
+        if (hipMalloc(&tm, (size_t)(nmax + nbc + 2) * sizeof(int) + 2 * (size_t)nmax) != hipSuccess) {
             sc_destroy(s);
             return sc_fail(SC_ERR_NOMEM, "device allocation of the tile lists (%d tiles) failed", nmax);
         }
@@ -336,6 +337,7 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
         s->tlist = s->tnf + 2;
         s->tcount = s->tlist + nmax;
         s->tflag = (unsigned char *)(s->tcount + nbc);
+        s->dflag = s->tflag + nmax;
     }
     if (const char *e = getenv("SC_TILE")) {
         int k = atoi(e);
@@ -911,6 +913,15 @@ extern "C" int sc_flux(sc_solver *s, const double *q, double *dq, double dt, dou
     return SC_OK;
 }
 
+// The tile flags of stage k of n in a 2D step: the first stage writes the
+// flags its second and third stages read for their deep-constant tiles
+// (dflag), the last stage those the tile list is built from (tflag).
+static void stage_flags(const sc_solver *s, int k, int n, StageArgs &a)
+{
+    a.tflag = k == 0 ? s->dflag : k == n - 1 ? s->tflag : nullptr;
+    a.dflag = k == 1 || k == 2 ? s->dflag : nullptr;
+}
+
 // The stage sequence of ts_ssp22/42/33/104 as (op, reads qold, uses register)
 struct StagePlan {
     int n;
@@ -973,6 +984,7 @@ static int enqueue_ts(sc_solver *s, double dt, unsigned long long **slot, bool *
         a.qout = s->buf[outb];
         a.op = P.op[k];
         a.smax = nullptr;
+        if (a.order == TILE_ADAPTIVE) stage_flags(s, k, P.n, a);
         if (k == P.n - 1) {
             int rs = next_slot(s, &a.smax);
             if (rs) return rs;
@@ -1085,7 +1097,7 @@ static int advance2d_resident(sc_solver *s, const sc_run_ctl *c, double tstart, 
         A[k].order = TILE_ADAPTIVE;
         A[k].ntx = ntx;
         A[k].nty = nty;
-        A[k].tflag = s->tflag;
+        stage_flags(s, k, P.n, A[k]);
         A[k].tlist = s->tlist;
         A[k].tnf = s->tnf;
         A[k].qbin = s->qb;

@@ -58,6 +58,9 @@ struct StageArgs {
     unsigned char *tflag;
     const int *tlist;
     const int *tnf;
+    // 2D, stages 2 and 3 of a step: the flags the step's first stage wrote
+    // (stage2d_tile's deep-constant tiles); NULL: not used
+    const unsigned char *dflag;
 };
 
 // physics selector of the launchers (sc_physics.h PhysId)

@@ -77,6 +77,66 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
 #pragma unroll
         for (int m = 0; m < M; m++) ref[m] = T(a.qin[g0 + m * plane]);
     }
+    // A deep-constant tile (stages 2 and 3 of a step, a.dflag set): this
+    // tile and its eight neighbours were constant tiles at the step's first
+    // stage, so q^n is one value V on all nine, and so is the first stage's
+    // output on them (V + 0).  Every cell this stage reads -- the tile with
+    // its 3-cell halos, and for the third stage the second stage's values
+    // there, which depend on first-stage values up to 6 cells from the tile,
+    // within the nine tiles since every tile shape is at least 7 cells wide
+    // -- then holds one value, as do q^n and the register on the tile: the
+    // stage is the constant tile's update below, from one cell's values,
+    // without staging the tile.  A missing neighbour beyond the grid counts
+    // as constant on a side with extrapolation (its ghost cells copy the
+    // tile's edge); periodic, wall and user boundaries do not qualify.  The
+    // values computed are those the constant-tile branch computes.
+    if constexpr (P != ADV2) {
+        if (a.dflag && !a.full) {
+            // threads 0..8 test one tile of the 3 x 3 block each (one round
+            // trip to memory), the barrier combines them
+            bool ok = true;
+            if (t < 9) {
+                const int x = b % a.ntx + t % 3 - 1, y = b / a.ntx + t / 3 - 1;
+                if (x < 0) ok = a.bc[0] == 1;
+                else if (x >= a.ntx) ok = a.bc[1] == 1;
+                if (y < 0) ok = ok && a.bc[2] == 1;
+                else if (y >= a.nty) ok = ok && a.bc[3] == 1;
+                if (x >= 0 && x < a.ntx && y >= 0 && y < a.nty) ok = ok && a.dflag[y * a.ntx + x] == 0;
+            }
+            if (__syncthreads_and(ok)) {
+                const size_t g0 = (size_t)(i0 + 2) + (size_t)a.LX * (size_t)(j0 + 2);
+                double outv[M], qbov[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const double qv = a.qin[g0 + m * plane];
+                    const double qo = op_reads_qold(a.op) ? a.qold[g0 + m * plane] : 0.0;
+                    const double qbi = a.op == OP_SSP104_10 ? a.qbin[g0 + m * plane] : 0.0;
+                    qbov[m] = 0.0;
+                    combine(a.op, qo, qv, 0.0, qbi, outv[m], qbov[m]);
+                }
+                if (a.tflag && t == 0) a.tflag[b] = 0;
+                if (t == 0 && a.smax) {
+                    T sx, sy;
+                    flat_speeds<P, T>(c, ref, sx, sy);
+                    unsigned long long *w = a.smax + ((blockIdx.x + blockIdx.y) % CFL_SUB) * CFL_STRIDE;
+                    if (sx > T(0)) atomicMax(w, (unsigned long long)__double_as_longlong((double)sx));
+                    if (sy > T(0)) atomicMax(w + 1, (unsigned long long)__double_as_longlong((double)sy));
+                }
+                const bool own = t < TX * TY;
+                const int ox = own ? t % TX : 0, oy = own ? t / TX : 0;
+                if (own && i0 + ox <= nx && j0 + oy <= ny) {
+                    const size_t gcell = (size_t)(i0 + ox + 2) + (size_t)a.LX * (size_t)(j0 + oy + 2);
+#pragma unroll
+                    for (int m = 0; m < M; m++) {
+                        a.qout[gcell + m * plane] = outv[m];
+                        if (a.op == OP_SSP104_5) a.qbout[gcell + m * plane] = qbov[m];
+                    }
+                }
+                return;
+            }
+        }
+    }
+
     // The RK combination's other input for this thread's cell, q^n: a float64
     // tile loads it now, so that its latency overlaps the staging (constant
     // tiles then only store); a float tile, whose registers are the tighter

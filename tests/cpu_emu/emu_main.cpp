@@ -446,6 +446,43 @@ bool loop1d(const std::string &name, sc_params p, unsigned long long seed, doubl
            check(st[1].steps_taken == so.steps_taken, "steps taken differ", name, st[1].steps_taken, so.steps_taken);
 }
 
+// Steps of a 2D grid with a constant block (the deep-constant tiles of a
+// step's second and third stages, which skip staging): sc_advance with the
+// adaptive tile order (device loop, deep-constant tiles) against the band
+// order (SC_TILE_ORDER=0: host loop, no tile flags), bitwise, and against
+// the oracle.
+bool deep2d(const std::string &name, sc_params p, unsigned long long seed, int steps, double tol)
+{
+    std::vector<double> q = state(p, seed, true);
+    sc_run_ctl c = {1e99, 1.0, 0.9, steps, 1};
+    std::vector<double> out[2];
+    sc_run_stats st[2];
+    for (int run = 0; run < 2; run++) {
+        if (run == 1) setenv("SC_TILE_ORDER", "0", 1);
+        sc_solver *s = nullptr;
+        bool ok = rc_ok(sc_create(&p, &s), "sc_create");
+        unsetenv("SC_TILE_ORDER");
+        if (!ok) return false;
+        double dt = 0.1 * p.dx[0];
+        out[run].resize(q.size());
+        ok = rc_ok(sc_set_q(s, q.data()), "sc_set_q") && rc_ok(sc_advance(s, &c, 0.0, 1e9, &dt, &st[run]), "sc_advance") &&
+             rc_ok(sc_get_q(s, out[run].data()), "sc_get_q");
+        sc_destroy(s);
+        if (!ok) return false;
+    }
+    std::vector<double> qo = q;
+    double dto = 0.1 * p.dx[0];
+    sc_run_stats so;
+    orc_sharpclaw(&p, &c, qo.data(), 0.0, 1e9, &dto, &so);
+    const double e = rel_linf(p, out[0], qo);
+    return check(std::memcmp(out[0].data(), out[1].data(), q.size() * sizeof(double)) == 0,
+                 "adaptive order (deep-constant tiles) and band order differ", name, 1, 0) &&
+           check(st[0].steps_taken == st[1].steps_taken && st[0].dt_last == st[1].dt_last,
+                 "loop statistics differ", name, st[1].steps_taken, st[0].steps_taken) &&
+           check(e <= tol, "relative error vs oracle", name, e, tol) &&
+           check(st[0].steps_taken == so.steps_taken, "steps taken differ", name, st[0].steps_taken, so.steps_taken);
+}
+
 struct Scenario {
     std::string name;
     std::function<bool()> run;
@@ -550,6 +587,14 @@ std::vector<Scenario> scenarios()
                      p.precision = SC_FP32;
                      p.time_integrator = 1;
                      return loop1d("loop2d shallow water fp32 ssp22", p, 87, 0.05, 1e-5);
+                 }});
+    v.push_back({"deep2d euler ssp33", []() {
+                     return deep2d("deep2d euler ssp33", base(2, 128, 120, SC_PHYS_EULER), 89, 4, 1e-11);
+                 }});
+    v.push_back({"deep2d shallow water ssp104", []() {
+                     sc_params p = base(2, 112, 128, SC_PHYS_SHALLOW_WATER);
+                     p.time_integrator = 4;
+                     return deep2d("deep2d shallow water ssp104", p, 91, 2, 1e-11);
                  }});
     v.push_back({"noalloc euler2d", []() { return noalloc("noalloc euler2d", base(2, 45, 38, SC_PHYS_EULER), 91); }});
     v.push_back({"noalloc sw1d", []() {
