@@ -362,6 +362,56 @@ def cpu_baseline_batch(mx, hl, hr, x0, order, k, q32=None):
     return out
 
 
+def cpu_full_batch(nsolves, seed):
+    """The whole batched workload on the reference (flang -O3): every dam
+    break as one serial process, largest first, over the host's cores (all
+    submitted at t = 0, as on the device).  Per-solve completion times give
+    the CPU's p50/p99 latency; the wall time its makespan.  Minutes of CPU
+    time: run on its own (bench.py --cpu-full-batch), not in the default run."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import refrun
+    from sharpclaw_amd import make_params
+    from sharpclaw_amd.workloads import dambreak_batch
+    if not refrun.ref_available("sw1d", programs=("xsclaw",)):
+        raise RuntimeError("reference build oracle/_ref/xsclaw_sw1d missing")
+    mx, hl, hr, x0 = dambreak_batch(nsolves, seed)
+    order = np.argsort(-mx, kind="stable")
+    cores = min(16, len(os.sched_getaffinity(0)))
+    t0 = time.perf_counter()
+
+    def one(i):
+        p = make_params("shallow_water", (int(mx[i]),), char_decomp=2, time_integrator=3)
+        cd = refrun.clawdata_for(p, nout=1, outstyle=1, tfinal=0.2, dt_initial=1e-5, cfl_max=0.5,
+                                 cfl_desired=0.45, max_steps=1000000)
+        a = time.perf_counter()
+        _, dtlog, _ = refrun.run(p, cd, ["3", f"{float(hl[i])!r} {float(hr[i])!r} {float(x0[i])!r}"])
+        b = time.perf_counter()
+        return b - a, b - t0, int(mx[i]) * 3 * len(dtlog)
+
+    done = [0]
+
+    def tracked(i):
+        r = one(i)
+        done[0] += 1
+        if done[0] % 256 == 0:
+            log(f"bench: cpu full batch {done[0]}/{nsolves} solves, {time.perf_counter() - t0:.0f} s")
+        return r
+
+    with ThreadPoolExecutor(cores) as ex:
+        rs = list(ex.map(tracked, order))
+    wall = time.perf_counter() - t0
+    cs = float(sum(r[2] for r in rs))
+    fin = np.array([r[1] for r in rs]) * 1e3
+    return {"workload": f"{nsolves} 1D shallow-water dam breaks (seed {seed}), the reference program "
+                        f"oracle/_ref/xsclaw_sw1d (flang -O3), one serial process per solve, largest first, "
+                        f"over {cores} host cores",
+            "cores": cores, "kind": "reference", "wall_s": wall, "cell_stages": cs,
+            "value": cs / wall, "unit": "cell-updates/s per RK stage",
+            "per_core_value": cs / sum(r[0] for r in rs),
+            "latency_ms": {"p50": float(np.percentile(fin, 50)), "p99": float(np.percentile(fin, 99)),
+                           "max": float(fin.max())}}
+
+
 def shu_osher_q(p):
     """Shu-Osher initial data: x < -4: rho = 3.857143, u = 2.629369,
     p = 10.333333; else rho = 1 + 0.2 sin(5x), u = 0, p = 1."""
@@ -529,6 +579,7 @@ def _serve_pass(srv, mix, ctl, paced):
     wall = time.perf_counter() - t0
     lat, cs, first_sub, last_done = [], [], 1e300, 0.0
     per = {}
+    work = [None] * len(mix)  # (kind, cell-stages) per request, in mix order
     for rid in srv.poll():
         k = ids[rid]
         q, rs = srv.result(rid)
@@ -542,7 +593,29 @@ def _serve_pass(srv, mix, ctl, paced):
         last_done = max(last_done, rs.done_ms)
         key = f"{mix[k]['kind']}_{'fp64' if mix[k]['precision'] == SC_FP64 else 'fp32'}"
         per.setdefault(key, []).append(lat[-1])
+        work[k] = (mix[k]["kind"], cs[-1])
+    _serve_pass.work = work
     return np.array(lat), float(np.sum(cs)), per, (last_done - first_sub) * 1e-3, wall
+
+
+def cpu_equivalent_latency(mix, work, rates, cores):
+    """The served trace on the host's cores, as a model: requests in arrival
+    order to the first free core, each taking its measured cell-stages at
+    the reference's measured per-core rate for its kind (float64; the
+    reference has no float32 arithmetic).  Returns the latency percentiles
+    (ms) and the makespan (s)."""
+    import heapq
+    free = [0.0] * cores
+    lat, end = [], 0.0
+    for r, (kind, cs) in zip(mix, work):
+        start = max(r["arrival"], heapq.heappop(free))
+        done = start + cs / rates[kind]
+        heapq.heappush(free, done)
+        lat.append((done - r["arrival"]) * 1e3)
+        end = max(end, done)
+    lat = np.array(lat)
+    return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+            "max": float(lat.max()), "makespan_s": end - min(r["arrival"] for r in mix)}
 
 
 def served_bench(n, rate, seed, cpu=True):
@@ -564,6 +637,7 @@ def served_bench(n, rate, seed, cpu=True):
         for rid in srv.poll():
             srv.result(rid)
         lat, cs, per, span_s, wall = _serve_pass(srv, mix, ctl, paced=True)
+        paced_work = _serve_pass.work
         st = srv.stats()
         slat, scs, _, sspan, _ = _serve_pass(srv, mix, ctl, paced=False)
         sst = srv.stats()
@@ -621,6 +695,14 @@ def served_bench(n, rate, seed, cpu=True):
         f"(all float32: {info['saturated_fp32_vs_fp64_single']:.2f}x)")
     if cpu:
         info["cpu_baseline"] = cpu_baseline_served(seed)
+        rates = {k: v["value"] for k, v in info["cpu_baseline"].items()}
+        if all(k in rates for k, _ in paced_work):
+            cores = min(16, len(os.sched_getaffinity(0)))
+            info["cpu_baseline"]["equivalent_latency_ms"] = dict(
+                cpu_equivalent_latency(mix, paced_work, rates, cores), cores=cores,
+                note="model: the same arrival trace served by the reference on this host's cores, each "
+                     "request on one core in arrival order, its cell-stages at the per-kind rate measured "
+                     "above (float64)")
     return info
 
 
@@ -810,6 +892,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched dam-break workload")
+    ap.add_argument("--cpu-full-batch", action="store_true",
+                    help="only run the whole batched workload on the reference over the host's cores "
+                         "(minutes) and print its JSON summary")
     ap.add_argument("--batch", type=int, default=4096, help="solves in the batched workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--requests", type=int, default=300, help="requests in the served workload")
@@ -843,6 +928,10 @@ def main():
         return
     if args.trace_summary:
         trace_summary(args.trace_summary)
+        return
+    if args.cpu_full_batch:
+        log(f"bench: the whole batched workload ({args.batch} dam breaks) on the reference, host cores")
+        print(json.dumps(cpu_full_batch(args.batch, args.work_seed)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
