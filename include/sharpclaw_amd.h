@@ -299,7 +299,9 @@ typedef struct sc_batch_result {
 
 /* nsolves 1D grids sharing params (physics, reconstruction, precision,
  * integrator, boundary options); solve i has mx[i] cells on
- * [xlower[i], xupper[i]]. */
+ * [xlower[i], xupper[i]].  Every reconstruction of flux1.f90:65-117 that
+ * single grids take (weno5, weno5_char, weno5_wave, weno5_trans, tvd2_wave,
+ * tvd2_char); no aux arrays or source terms. */
 int sc_batch_create(const sc_params *params, int nsolves, const int *mx, const double *xlower,
                     const double *xupper, sc_batch **out);
 int sc_batch_destroy(sc_batch *b);
@@ -308,13 +310,16 @@ int sc_batch_set_q(sc_batch *b, int i, const double *q);
 int sc_batch_get_q(sc_batch *b, int i, double *q);
 /* Advance every solve from tstart to tend (dt_initial each) in one launch;
  * results[i] for solve i; gpu_ms: HIP-event time of the launch.  The first
- * run of a batch also times workgroup sizes 64/128/256 on one step of every
- * solve (states untouched) and keeps the fastest. */
+ * run of a batch also times workgroup sizes 64..1024 (64..256 for the
+ * reconstructions with neighbour data: weno5_wave, weno5_trans, tvd2_*) on
+ * one step of every solve (states untouched) and keeps the one with the
+ * shortest predicted makespan. */
 int sc_batch_run(sc_batch *b, const sc_run_ctl *ctl, double tstart, double tend, double dt_initial,
                  sc_batch_result *results, double *gpu_ms);
 /* workgroup size chosen by the tuner (0 before the first run) */
 int sc_batch_block(sc_batch *b, int *block);
-/* the tuner's candidates and predicted whole-run times (ms); *n <= max */
+/* the tuner's candidates and predicted whole-run times (ms; -1 for a size
+ * not built for the batch's options); *n <= max */
 int sc_batch_tuning(sc_batch *b, int max, int *blocks, double *predicted_ms, int *n);
 /* Workgroups each solve ran on in the last run (parts[i] for solve i; 1: one
  * workgroup).  The tuner splits the largest solves over several co-resident

@@ -57,22 +57,40 @@ static hipError_t gob(const BatchArgs &A, const PhysConst<T> &c, int ncu, int ma
 template <int P, int CHAR, class T>
 static int occ_nt(int nt)
 {
+    if constexpr (CHAR == RK_WENO || CHAR == RK_WENO_CHAR) {
+        if (nt == 512) return occ_of<P, CHAR, T, 512>();
+        if (nt == 1024) return occ_of<P, CHAR, T, 1024>();
+    }
     switch (nt) {
     case 64: return occ_of<P, CHAR, T, 64>();
     case 128: return occ_of<P, CHAR, T, 128>();
-    case 512: return occ_of<P, CHAR, T, 512>();
-    case 1024: return occ_of<P, CHAR, T, 1024>();
-    default: return occ_of<P, CHAR, T, 256>();
+    case 256: return occ_of<P, CHAR, T, 256>();
     }
+    return 0;
+}
+
+template <int P, int CHAR>
+static int occ_t(const PhysSel &s, int nt)
+{
+    return s.fp32 ? occ_nt<P, CHAR, float>(nt) : occ_nt<P, CHAR, double>(nt);
 }
 
 template <int P>
 static int occ_p(const PhysSel &s, int nt)
 {
-    if (s.chr == 2) return s.fp32 ? occ_nt<P, 2, float>(nt) : occ_nt<P, 2, double>(nt);
-    return s.fp32 ? occ_nt<P, 0, float>(nt) : occ_nt<P, 0, double>(nt);
+    switch (batch_rk(s)) {
+    case RK_WENO: return occ_t<P, RK_WENO>(s, nt);
+    case RK_WENO_WAVE: return occ_t<P, RK_WENO_WAVE>(s, nt);
+    case RK_WENO_CHAR: return occ_t<P, RK_WENO_CHAR>(s, nt);
+    case RK_WENO_TRANS: return occ_t<P, RK_WENO_TRANS>(s, nt);
+    case RK_TVD_WAVE: return occ_t<P, RK_TVD_WAVE>(s, nt);
+    case RK_TVD_CHAR: return occ_t<P, RK_TVD_CHAR>(s, nt);
+    }
+    return 0;
 }
 
+// resident workgroups per CU; 0 for options or a size the kernel is not
+// built for
 int batch_occupancy(const PhysSel &s, int nt)
 {
     switch (s.phys) {
@@ -81,31 +99,46 @@ int batch_occupancy(const PhysSel &s, int nt)
     case EU1: return occ_p<EU1>(s, nt);
     case SW1: return occ_p<SW1>(s, nt);
     }
-    return 1;
+    return 0;
 }
 
 template <int P, int CHAR, class T>
 static hipError_t gob_nt(int nt, const BatchArgs &A, const PhysConst<T> &c, int ncu, int max_wg,
                          hipStream_t st)
 {
+    if constexpr (CHAR == RK_WENO || CHAR == RK_WENO_CHAR) {
+        if (nt == 512) return gob<P, CHAR, T, 512>(A, c, ncu, max_wg, st);
+        if (nt == 1024) return gob<P, CHAR, T, 1024>(A, c, ncu, max_wg, st);
+    }
     switch (nt) {
     case 64: return gob<P, CHAR, T, 64>(A, c, ncu, max_wg, st);
     case 128: return gob<P, CHAR, T, 128>(A, c, ncu, max_wg, st);
-    case 512: return gob<P, CHAR, T, 512>(A, c, ncu, max_wg, st);
-    case 1024: return gob<P, CHAR, T, 1024>(A, c, ncu, max_wg, st);
-    default: return gob<P, CHAR, T, 256>(A, c, ncu, max_wg, st);
+    case 256: return gob<P, CHAR, T, 256>(A, c, ncu, max_wg, st);
     }
+    return hipErrorInvalidValue;
+}
+
+template <int P, int CHAR>
+static hipError_t gob_t(const PhysSel &s, int nt, const BatchArgs &A, const PhysConst<double> &cd,
+                        const PhysConst<float> &cf, int ncu, int max_wg, hipStream_t st)
+{
+    return s.fp32 ? gob_nt<P, CHAR, float>(nt, A, cf, ncu, max_wg, st)
+                  : gob_nt<P, CHAR, double>(nt, A, cd, ncu, max_wg, st);
 }
 
 template <int P>
 static hipError_t gob_p(const PhysSel &s, int nt, const BatchArgs &A, const PhysConst<double> &cd,
                         const PhysConst<float> &cf, int ncu, int max_wg, hipStream_t st)
 {
-    if (s.chr == 2)
-        return s.fp32 ? gob_nt<P, 2, float>(nt, A, cf, ncu, max_wg, st)
-                      : gob_nt<P, 2, double>(nt, A, cd, ncu, max_wg, st);
-    return s.fp32 ? gob_nt<P, 0, float>(nt, A, cf, ncu, max_wg, st)
-                  : gob_nt<P, 0, double>(nt, A, cd, ncu, max_wg, st);
+    switch (batch_rk(s)) {
+    case RK_WENO: return gob_t<P, RK_WENO>(s, nt, A, cd, cf, ncu, max_wg, st);
+    case RK_WENO_WAVE: return gob_t<P, RK_WENO_WAVE>(s, nt, A, cd, cf, ncu, max_wg, st);
+    case RK_WENO_CHAR: return gob_t<P, RK_WENO_CHAR>(s, nt, A, cd, cf, ncu, max_wg, st);
+    case RK_WENO_TRANS: return gob_t<P, RK_WENO_TRANS>(s, nt, A, cd, cf, ncu, max_wg, st);
+    case RK_TVD_WAVE: return gob_t<P, RK_TVD_WAVE>(s, nt, A, cd, cf, ncu, max_wg, st);
+    case RK_TVD_CHAR: return gob_t<P, RK_TVD_CHAR>(s, nt, A, cd, cf, ncu, max_wg, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_batch(const PhysSel &s, int nt, const BatchArgs &A, const PhysConst<double> &cd,
@@ -179,9 +212,6 @@ extern "C" int sc_batch_create(const sc_params *params, int nsolves, const int *
     if (params->ndim != 1) return sc_fail(SC_ERR_UNSUPPORTED, "batched solves are 1D grids");
     if (params->maux != 0 || params->src_term != 0)
         return sc_fail(SC_ERR_UNSUPPORTED, "batched solves take no aux arrays and no source term");
-    if (params->lim_type != 2 || (params->char_decomp != 0 && params->char_decomp != 2))
-        return sc_fail(SC_ERR_UNSUPPORTED, "batched solves reconstruct with weno5 or weno5_char (lim_type 2, "
-                                           "char_decomp 0 or 2); the others run on single grids");
     if (params->time_integrator < 1 || params->time_integrator > 4)
         return sc_fail(SC_ERR_ARG, "time_integrator should be between 1 and 4");
     for (int k = 0; k < 2; k++) {
@@ -205,6 +235,7 @@ extern "C" int sc_batch_create(const sc_params *params, int nsolves, const int *
     b->p = *params;
     b->sel.phys = sc_phys_id(params);
     b->sel.chr = params->char_decomp;
+    b->sel.lim = params->lim_type;
     b->sel.fp32 = params->precision == SC_FP32;
     b->cd = sc_make_const<double>(params);
     b->cf = sc_make_const<float>(params);
@@ -471,6 +502,10 @@ static int tune_block(sc_batch *b, double tstart, double dt0, int info0)
     std::vector<int> plan, best_plan(b->nsolves, 1);
     for (int k = 0; k < NBATCH_BLOCKS; k++) {
         if (forced && kBatchBlocks[k] != forced) continue;
+        if (kBatchBlocks[k] > batch_max_block(b->sel)) {
+            b->predicted[k] = -1.0;  // not built for these options
+            continue;
+        }
         float ms = 0.f;
         int rc = run_once(b, kBatchBlocks[k], tstart, dt0, info0, 1, single, &ms);
         if (rc) return rc;

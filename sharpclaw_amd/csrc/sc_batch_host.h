@@ -16,6 +16,25 @@ namespace sc {
 constexpr int NBATCH_BLOCKS = 5;
 extern const int kBatchBlocks[NBATCH_BLOCKS];
 
+// Reconstruction code (RK_*) of the batch kernel for these options
+// (flux1.f90:65-117): lim_type 2 with char_decomp 0-3, or lim_type 1 with
+// char_decomp 1 (tvd2_wave) or 2 (tvd2_char); -1 for any other.
+inline int batch_rk(const PhysSel &s)
+{
+    if (s.lim == 2) return s.chr >= 0 && s.chr <= 3 ? s.chr : -1;
+    if (s.lim == 1) return s.chr == 1 ? RK_TVD_WAVE : s.chr == 2 ? RK_TVD_CHAR : -1;
+    return -1;
+}
+
+// Largest workgroup the batch kernel is built for with these options: the
+// reconstructions with neighbour data keep it in LDS per interface, and are
+// built up to 256 threads.
+inline int batch_max_block(const PhysSel &s)
+{
+    const int rk = batch_rk(s);
+    return rk == RK_WENO || rk == RK_WENO_CHAR ? 1024 : 256;
+}
+
 // Launch the persistent kernel over A.order[0..A.nsolves) with nt threads
 // per workgroup and at most max_wg workgroups (0: fill the device).
 hipError_t launch_batch(const PhysSel &s, int nt, const BatchArgs &A, const PhysConst<double> &cd,

@@ -168,8 +168,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
 {
     constexpr int M = Shape<P>::M;
     constexpr int NW = NT / 64;
+    // the reconstructions with neighbour data (weno5_wave, weno5_trans,
+    // tvd2_wave) hold it in sw, and the interface exchange reuses it when
+    // it is large enough (as stage1d_kernel)
+    constexpr int NB = nbr_words<P, 1, CHAR, T>();
+    constexpr bool AL = sx_in_sw<P, 1, CHAR, T, NT>();
     __shared__ T sq[M][NT + 5];
-    __shared__ __align__(8) T sx[2 * M][NT];
+    __shared__ __align__(8) T sx0[AL ? 1 : 2 * M][NT];
+    __shared__ T sw[NB > 0 ? NB : 1][NB > 0 ? NT + 4 : 1];
+    T(*sx)[NT] = AL ? reinterpret_cast<T(*)[NT]>(&sw[0][0]) : sx0;
     __shared__ double red[NW];
     __shared__ int sb, sflag;
     const int t = threadIdx.x;
@@ -259,7 +266,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
                 a.op = plan.op[st];
                 const bool last = st == plan.n - 1;
                 for (int ch = ch0; ch < ch1; ch++) {
-                    T s1 = stage1d_chunk<P, 1, CHAR, T, NT>(a, c, 1 + ch * (NT - 1), sq, sx);
+                    T s1 = stage1d_chunk<P, 1, CHAR, T, NT>(a, c, 1 + ch * (NT - 1), sq, sx, &sw[0][0]);
                     if (last) sm = fmax(sm, s1);
                 }
                 in = outb;

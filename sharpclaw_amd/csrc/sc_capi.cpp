@@ -1223,12 +1223,12 @@ static int advance2d_device(sc_solver *s, const sc_run_ctl *c, double tstart, do
     return SC_OK;
 }
 
-// Options the persistent batch kernel runs a 1D grid with: weno5 or
-// weno5_char, no capacity, aux or source, no tfluct, boundary options 1-3.
+// Options the persistent batch kernel runs a 1D grid with: any
+// reconstruction, no capacity, aux or source, no tfluct, boundary options 1-3.
 static bool loop1d_fits(const sc_solver *s)
 {
     const sc_params &p = s->p;
-    return p.ndim == 1 && p.lim_type == 2 && (p.char_decomp == 0 || p.char_decomp == 2) && p.mcapa == 0 &&
+    return p.ndim == 1 && batch_rk(s->sel) >= 0 && p.mcapa == 0 &&
            p.maux == 0 && p.src_term == 0 && p.tfluct_solver == 0 && p.mthbc[0] >= 1 && p.mthbc[0] <= 3 &&
            p.mthbc[1] >= 1 && p.mthbc[1] <= 3;
 }
@@ -1236,12 +1236,12 @@ static bool loop1d_fits(const sc_solver *s)
 // Workgroup size of the 1D device loop: SC_LOOP_BLOCK, else 128 (1600-cell
 // Euler grid, 566 SSP(10,4) steps: 24.7 ms at 128 threads in 13 parts,
 // 26.1 ms at 256 in 7, 29.7 ms at 512 in 4; host-driven loop 37.5 ms).
-static int loop1d_block()
+static int loop1d_block(const PhysSel &sel)
 {
     if (const char *e = getenv("SC_LOOP_BLOCK")) {
         const int v = atoi(e);
         for (int k = 0; k < NBATCH_BLOCKS; k++)
-            if (kBatchBlocks[k] == v) return v;
+            if (kBatchBlocks[k] == v && v <= batch_max_block(sel)) return v;
     }
     return 128;
 }
@@ -1274,7 +1274,7 @@ static int advance1d_device(sc_solver *s, const sc_run_ctl *c, double tstart, do
         HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->dev));
         L.tick_ms = khz > 0 ? 1.0 / khz : 1e-5;
     }
-    const int nt = loop1d_block();
+    const int nt = loop1d_block(s->sel);
     const int mx = s->p.nx[0];
     const int nch = (mx + nt - 2) / (nt - 1);
     const int parts = std::max(1, std::min({nch, MAX_PARTS, L.ncu * batch_occupancy(s->sel, nt)}));
@@ -1373,7 +1373,7 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
         // host-driven loop waits for the CFL (a few microseconds per stage
         // on a grid of a few chunks); larger grids fill the device with one
         // launch per stage instead
-        const int nt = loop1d_block();
+        const int nt = loop1d_block(s->sel);
         const int nch = (s->p.nx[0] + nt - 2) / (nt - 1);
         if (s->loop_mode == 1 || (s->loop_mode < 0 && nch <= MAX_PARTS))
             return advance1d_device(s, c, tstart, tend, dt_io, st);

@@ -172,7 +172,8 @@ bool same_group(const sc_params &a, const sc_params &b)
            a.precision == b.precision && a.time_integrator == b.time_integrator &&
            a.mthbc[0] == b.mthbc[0] && a.mthbc[1] == b.mthbc[1] && a.gamma == b.gamma &&
            a.grav == b.grav && a.u == b.u && a.rho == b.rho && a.bulk == b.bulk && a.efix == b.efix &&
-           a.tfluct_solver == b.tfluct_solver;
+           a.tfluct_solver == b.tfluct_solver && a.lim_type == b.lim_type &&
+           (a.lim_type != 1 || std::memcmp(a.mthlim, b.mthlim, sizeof a.mthlim) == 0);
 }
 
 }  // namespace
@@ -250,14 +251,15 @@ Group *group_for(sc_server *S, const sc_params &p)
     g->p = p;
     g->sel.phys = sc_phys_id(&p);
     g->sel.chr = p.char_decomp;
+    g->sel.lim = p.lim_type;
     g->sel.fp32 = p.precision == SC_FP32;
     g->cd = sc_make_const<double>(&p);
     g->cf = sc_make_const<float>(&p);
-    g->nt = block_class(p.nx[0]);
+    g->nt = std::min(block_class(p.nx[0]), batch_max_block(g->sel));
     if (const char *e = getenv("SC_SERVE_BLOCK")) {
         int v = atoi(e);
         for (int k = 0; k < NBATCH_BLOCKS; k++)
-            if (kBatchBlocks[k] == v) g->nt = v;
+            if (kBatchBlocks[k] == v && v <= batch_max_block(g->sel)) g->nt = v;
     }
     if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&g->ev, hipEventDisableTiming) != hipSuccess ||
@@ -665,9 +667,9 @@ extern "C" int sc_server_submit(sc_server *S, const sc_params *p, const sc_run_c
     if (p->device != S->dev) return sc_fail(SC_ERR_ARG, "request device %d, server device %d", p->device, S->dev);
     if (p->maux != 0 || p->src_term != 0)
         return sc_fail(SC_ERR_UNSUPPORTED, "served solves take no aux arrays and no source term");
-    if (p->lim_type != 2 || (p->char_decomp != 0 && p->char_decomp != 2))
-        return sc_fail(SC_ERR_UNSUPPORTED, "served solves reconstruct with weno5 or weno5_char (lim_type 2, "
-                                           "char_decomp 0 or 2); the others run on single grids");
+    if (p->ndim == 2 && (p->lim_type != 2 || (p->char_decomp != 0 && p->char_decomp != 2)))
+        return sc_fail(SC_ERR_UNSUPPORTED, "served 2D solves reconstruct with weno5 or weno5_char (lim_type 2, "
+                                           "char_decomp 0 or 2); the others run on single 2D grids");
     if (p->time_integrator < 1 || p->time_integrator > 4)
         return sc_fail(SC_ERR_ARG, "time_integrator should be between 1 and 4");
     for (int k = 0; k < 2 * p->ndim; k++)

@@ -241,7 +241,8 @@ class Batch:
         return out
 
     def tuning(self):
-        """{workgroup size: predicted whole-run ms} of the tuner (0: untimed)."""
+        """{workgroup size: predicted whole-run ms} of the tuner (0: untimed;
+        -1: a size the kernel is not built for with the batch's options)."""
         blocks = (C.c_int * 8)()
         pred = (C.c_double * 8)()
         n = C.c_int()

@@ -338,8 +338,13 @@ bool serve(const std::string &name)
     ps.push_back(base(2, 40, 33, SC_PHYS_SHALLOW_WATER));
     ps.push_back(base(1, 300, 1, SC_PHYS_EULER));
     ps.push_back(base(1, 90, 1, SC_PHYS_EULER));
+    ps.push_back(base(1, 260, 1, SC_PHYS_EULER));
+    ps.push_back(base(1, 120, 1, SC_PHYS_SHALLOW_WATER));
     ps[1].precision = SC_FP32;
     ps[3].precision = SC_FP32;
+    ps[4].char_decomp = 1;  // weno5_wave
+    ps[5].lim_type = 1;     // tvd2_char
+    ps[5].char_decomp = 2;
     sc_run_ctl c = {1e99, 1.0, 0.9, 100000, 1};
     const double tend = 0.02;
     std::vector<long long> ids(ps.size());
@@ -622,6 +627,39 @@ std::vector<Scenario> scenarios()
                      p.precision = SC_FP32;
                      p.time_integrator = 4;
                      return batch_split("batch split shallow water fp32 ssp104", p, {2000, 700, 50}, 0.005);
+                 }});
+    // the reconstructions with neighbour data in batches, split solves and
+    // the 1D device loop (flux1.f90:65-117)
+    v.push_back({"batch euler weno5_wave fp64", []() {
+                     sc_params p = base(1, 64, 1, SC_PHYS_EULER);
+                     p.char_decomp = 1;
+                     return batch("batch euler weno5_wave fp64", p, {64, 300, 257, 33}, 0.01, 1e-10);
+                 }});
+    v.push_back({"batch shallow water weno5_trans fp32", []() {
+                     sc_params p = base(1, 64, 1, SC_PHYS_SHALLOW_WATER);
+                     p.char_decomp = 3;
+                     p.precision = SC_FP32;
+                     return batch("batch shallow water weno5_trans fp32", p, {48, 400, 90}, 0.01, 1e-5);
+                 }});
+    v.push_back({"batch acoustics tvd2_wave ssp22", []() {
+                     sc_params p = base(1, 64, 1, SC_PHYS_ACOUSTICS);
+                     p.lim_type = 1;
+                     p.char_decomp = 1;
+                     p.time_integrator = 1;
+                     p.mthbc[0] = 3;
+                     return batch("batch acoustics tvd2_wave ssp22", p, {200, 64, 513}, 0.02, 1e-10);
+                 }});
+    v.push_back({"batch split euler tvd2_char", []() {
+                     sc_params p = base(1, 64, 1, SC_PHYS_EULER);
+                     p.lim_type = 1;
+                     p.char_decomp = 2;
+                     return batch_split("batch split euler tvd2_char", p, {1400, 64, 90}, 0.01);
+                 }});
+    v.push_back({"loop1d euler weno5_trans", []() {
+                     sc_params p = base(1, 900, 1, SC_PHYS_EULER);
+                     p.char_decomp = 3;
+                     p.mthbc[1] = 3;
+                     return loop1d("loop1d euler weno5_trans", p, 607, 0.005, 1e-10);
                  }});
     v.push_back({"loop1d euler ssp104", []() {
                      sc_params p = base(1, 1600, 1, SC_PHYS_EULER);

@@ -26,7 +26,8 @@ TOL_RUN = 1e-10
 def _solve_params(base, mx, xl, xu):
     kw = dict(char_decomp=base.char_decomp, time_integrator=base.time_integrator,
               mthbc=tuple(base.mthbc), gamma=base.gamma, grav=base.grav, u=base.u, rho=base.rho,
-              bulk=base.bulk, efix=base.efix)
+              bulk=base.bulk, efix=base.efix, lim_type=base.lim_type, mthlim=tuple(base.mthlim),
+              precision=base.precision)
     name = {v: k for k, v in PHYSICS.items()}[base.physics]
     return make_params(name, (int(mx),), xlower=(xl,), xupper=(xu,), **kw)
 
@@ -300,4 +301,98 @@ def test_batch_split_solves_bitwise(fp32, monkeypatch):
         assert r2[i].info == 0 and r2[i].t == pytest.approx(0.01, abs=1e-14)
         assert (r2[i].steps_taken, r2[i].dt_last, r2[i].cfl_maxused) == \
             (r1[i].steps_taken, r1[i].dt_last, r1[i].cfl_maxused), i
+        assert np.array_equal(q2[i], q1[i]), (i, int(mx[i]), int(p2[i]))
+
+
+RECON = [
+    # physics, lim_type, char_decomp, integrator, mthbc, limiters, float32
+    ("euler", 2, 1, 3, (3, 1), (5, 5, 5, 5), 0),          # weno5_wave
+    ("shallow_water", 2, 3, 2, (1, 3), (5, 5, 5, 5), 0),  # weno5_trans
+    ("acoustics", 1, 1, 1, (3, 3), (1, 2, 5, 5), 0),      # tvd2_wave, minmod / superbee
+    ("euler", 1, 2, 4, (1, 1), (4, 3, 2, 5), 0),          # tvd2_char, MC / van Leer / superbee
+    ("advection", 2, 1, 3, (2, 2), (5, 5, 5, 5), 0),      # weno5_wave, periodic
+    ("euler", 2, 3, 3, (1, 1), (5, 5, 5, 5), 1),          # weno5_trans, float32
+    ("shallow_water", 1, 2, 3, (1, 1), (3, 3, 5, 5), 1),  # tvd2_char, float32
+]
+
+
+@pytest.mark.parametrize("name,lim,chr_,ti,bc,mthlim,fp32", RECON)
+def test_batch_reconstructions_match_oracle(name, lim, chr_, ti, bc, mthlim, fp32):
+    """The reconstructions with neighbour data (weno5_wave, weno5_trans,
+    tvd2_wave, tvd2_char; flux1.f90:65-117) in the persistent batch kernel:
+    uneven sizes across the 255-cell chunk boundary, each solve against the
+    oracle's time loop (float64: whole-run tolerance and equal step
+    counts; float32: relative L1 <= 1e-5), and the tuner keeps to the
+    workgroup sizes the kernel is built for with these options (<= 256)."""
+    rng = np.random.default_rng(29 + lim * 4 + chr_)
+    mx = np.array([3, 40, 254, 256, 333, 700], dtype=np.int32)
+    n = len(mx)
+    xl = np.zeros(n)
+    xu = rng.uniform(0.5, 1.5, n)
+    base = make_params(name, (8,), char_decomp=chr_, lim_type=lim, mthlim=mthlim, time_integrator=ti,
+                       mthbc=bc + bc, precision=SC_FP32 if fp32 else 0)
+    qs = []
+    for i in range(n):
+        p = _solve_params(base, mx[i], xl[i], xu[i])
+        q = O.new_grid(p)
+        interior(p, q)[:] = _states.random_cells(p, (int(mx[i]),), rng, smooth=True, amp=0.3 if mx[i] < 10 else 0.8)
+        qs.append(q)
+    cfl = (2.45, 2.5) if ti == 4 else (0.45, 0.5)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=cfl[1], cfl_desired=cfl[0], max_steps=5000, dt_variable=1)
+    with Batch(base, mx, xl, xu) as b:
+        for i in range(n):
+            b.set_q(i, qs[i])
+        res, _ = b.run(ctl, 0.0, 0.04, 1e-3)
+        assert b.block <= 256, b.block
+        assert all(r.info == 0 for r in res), [r.info for r in res]
+        tun = b.tuning()
+        assert all((pr < 0) == (bl > 256) for bl, pr in tun.items()), tun
+        if not fp32:
+            ref = _oracle_runs(base, mx, xl, xu, qs, ctl, 0.0, 0.04, 1e-3)
+            _check_fp64(b, res, ref, (name, lim, chr_))
+        else:
+            base64 = make_params(name, (8,), char_decomp=chr_, lim_type=lim, mthlim=mthlim, time_integrator=ti,
+                                 mthbc=bc + bc)
+            for i in range(n):
+                p = _solve_params(base64, mx[i], xl[i], xu[i])
+                q, _, st = O.sharpclaw(p, ctl, qs[i], 0.0, 0.04, 1e-3)
+                g, o = interior(p, b.get_q(i)), interior(p, q)
+                err = np.sum(np.abs(g - o)) / np.sum(np.abs(o))
+                assert st.info == 0 and err <= 1e-5, (i, int(mx[i]), err)
+
+
+def test_batch_split_reconstruction_bitwise(monkeypatch):
+    """weno5_trans solves split over several workgroups equal the one-workgroup
+    run bit for bit (the neighbour data of a chunk's edge interfaces is
+    recomputed by each part)."""
+    mx = np.array([2000, 1500, 300, 90, 64, 700], dtype=np.int32)
+    n = len(mx)
+    rng = np.random.default_rng(31)
+    base = make_params("euler", (8,), char_decomp=3, time_integrator=3, mthbc=(3, 1, 3, 1))
+    xl, xu = np.zeros(n), np.ones(n)
+    qs = []
+    for i in range(n):
+        p = _solve_params(base, mx[i], xl[i], xu[i])
+        q = O.new_grid(p)
+        interior(p, q)[:] = _states.random_cells(p, (int(mx[i]),), rng, smooth=True, amp=0.5)
+        qs.append(q)
+    ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=100000, dt_variable=1)
+    runs = {}
+    for cap in ("1", None):
+        if cap:
+            monkeypatch.setenv("SC_BATCH_PARTS", cap)
+        else:
+            monkeypatch.delenv("SC_BATCH_PARTS", raising=False)
+        monkeypatch.setenv("SC_BATCH_BLOCK", "128")
+        with Batch(base, mx, xl, xu) as b:
+            for i in range(n):
+                b.set_q(i, qs[i])
+            res, _ = b.run(ctl, 0.0, 0.01, 1e-4)
+            assert b.block == 128
+            runs[cap] = ([b.get_q(i) for i in range(n)], res, b.parts.copy())
+    q1, r1, p1 = runs["1"]
+    q2, r2, p2 = runs[None]
+    assert (p1 == 1).all() and p2[0] > 1, p2
+    for i in range(n):
+        assert (r2[i].steps_taken, r2[i].dt_last) == (r1[i].steps_taken, r1[i].dt_last), i
         assert np.array_equal(q2[i], q1[i]), (i, int(mx[i]), int(p2[i]))

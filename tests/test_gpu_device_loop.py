@@ -59,10 +59,13 @@ def test_device_loop_equals_host_loop(phys, ti, n, prec):
     assert out["host"][0][1] > 0 or ti == 4  # the first interval retook its first step
 
 
-@pytest.mark.parametrize("phys,ti,mx,prec", [("euler", 4, 1600, 0), ("euler", 3, 5000, 0),
-                                             ("shallow_water", 2, 700, SC_FP32), ("acoustics", 1, 300, 0),
-                                             ("advection", 3, 255, 0)])
-def test_device_loop_1d_equals_host_loop(phys, ti, mx, prec):
+@pytest.mark.parametrize("phys,ti,mx,prec,lim,chr_", [
+    ("euler", 4, 1600, 0, 2, 2), ("euler", 3, 5000, 0, 2, 2), ("shallow_water", 2, 700, SC_FP32, 2, 0),
+    ("acoustics", 1, 300, 0, 2, 0), ("advection", 3, 255, 0, 2, 0),
+    # the reconstructions with neighbour data (flux1.f90:65-117)
+    ("euler", 3, 900, 0, 2, 1), ("euler", 4, 1200, 0, 2, 3), ("shallow_water", 3, 600, 0, 1, 1),
+    ("acoustics", 2, 500, SC_FP32, 1, 2)])
+def test_device_loop_1d_equals_host_loop(phys, ti, mx, prec, lim, chr_):
     """sc_advance on a 1D grid of up to 32 chunks runs its time loop on the
     device, the grid split over one workgroup per chunk meeting at a device
     barrier after every stage; q, step counts, rejections, dt and CFL equal
@@ -70,8 +73,8 @@ def test_device_loop_1d_equals_host_loop(phys, ti, mx, prec):
     intervals, the first of which retakes its first step."""
     rng = np.random.default_rng(43)
     bc = (1, 1) if phys == "advection" else (3, 1)
-    p = make_params(phys, (mx,), time_integrator=ti, mthbc=bc, precision=prec,
-                    char_decomp=2 if phys == "euler" else 0)
+    p = make_params(phys, (mx,), time_integrator=ti, mthbc=bc, precision=prec, char_decomp=chr_,
+                    lim_type=lim, mthlim=(4, 2, 3, 5))
     q0 = O.new_grid(p)
     interior(p, q0)[:] = _states.random_cells(p, (mx,), rng, smooth=True, amp=0.4)
     cfl = (2.45, 2.5) if ti == 4 else (0.45, 0.5)

@@ -167,11 +167,12 @@ def test_recon_options_rejected_where_not_built():
     for p in bad:
         with pytest.raises(ScError):
             Solver(p)
-    p = make_params("shallow_water", (8,), lim_type=1, char_decomp=2)
+    # batches and the server take every 1D reconstruction, not the invalid ones
+    p = make_params("shallow_water", (8,), lim_type=1, char_decomp=3)
     with pytest.raises(ScError):
         Batch(p, np.array([64, 65], dtype=np.int32), np.zeros(2), np.ones(2))
     with Server(hbm_bytes=16 << 20, max_active=2) as srv:
-        pr = make_params("shallow_water", (64,), char_decomp=3)
+        pr = make_params("shallow_water", (64,), lim_type=1, char_decomp=0)
         ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=100, dt_variable=1)
         with pytest.raises(ScError):
             srv.submit(pr, ctl, dambreak_q(64, 2.0, 0.5, 0.5), 0.0, 0.01, 1e-4)

@@ -16,7 +16,7 @@ import pytest
 from conftest import rel_linf
 from oracle import oracle as O
 from sharpclaw_amd import SC_FP32, SC_FP64, ScError, ScRunCtl, Server, interior, make_params
-from sharpclaw_amd._abi import SC_ERR_ARG, SC_ERR_NOMEM
+from sharpclaw_amd._abi import SC_ERR_ARG, SC_ERR_NOMEM, SC_ERR_UNSUPPORTED
 from sharpclaw_amd.workloads import SERVED_CTL, served_request
 
 pytestmark = pytest.mark.gpu
@@ -161,6 +161,61 @@ def test_server_rejects_bad_requests():
         with pytest.raises(ScError):
             srv.submit(p, ctl, O.new_grid(p), 0.0, 0.01, 1e-4)
         assert srv.stats()["submitted"] == 0
+
+
+def test_server_1d_reconstructions_match_oracle():
+    """1D requests with the reconstructions that use neighbour data
+    (weno5_wave, weno5_trans, tvd2_wave, tvd2_char; flux1.f90:65-117) share
+    the server with weno5_char ones; each equals the same request served
+    alone bit for bit and the oracle's time loop to the whole-run tolerance
+    (float32: relative L1 <= 1e-5).  2D requests with them are refused (they
+    run as slice sweeps on single 2D grids)."""
+    rng = np.random.default_rng(19)
+    ctl = ScRunCtl(**SERVED_CTL)
+    specs = [("euler", 2, 1, 0, 300), ("euler", 2, 3, 0, 1100), ("acoustics", 1, 1, 0, 520),
+             ("shallow_water", 1, 2, 0, 260), ("euler", 2, 2, 0, 400), ("acoustics", 2, 3, SC_FP32, 700)]
+    reqs = []
+    for name, lim, chr_, prec, mx in specs:
+        bc = (3, 3) if name == "acoustics" else (1, 3)
+        p = make_params(name, (mx,), lim_type=lim, char_decomp=chr_, mthlim=(4, 3, 5, 5), mthbc=bc,
+                        precision=prec)
+        q = O.new_grid(p)
+        xc = (np.arange(mx) + 0.5) / mx
+        bump = 0.5 * np.exp(-((xc - rng.uniform(0.3, 0.7)) / 0.05) ** 2)
+        if name == "acoustics":
+            interior(p, q)[:, 0] = bump
+        elif name == "euler":
+            interior(p, q)[:, 0] = 1.0 + bump
+            interior(p, q)[:, 2] = (1.0 + bump) / 0.4
+        else:
+            interior(p, q)[:, 0] = 1.0 + bump
+        reqs.append({"kind": name, "params": p, "q0": q, "tend": 0.03, "dt0": 1e-5})
+    big = max(r["q0"].size for r in reqs) * 8 * 5
+    mixed, st = _serve(reqs, hbm_bytes=8 * big, max_active=8, ctl=ctl)
+    for k, r in enumerate(reqs):
+        (alone,), _ = _serve([r], hbm_bytes=8 * big, max_active=8, ctl=ctl)
+        qm, sm = mixed[k]
+        assert np.array_equal(qm, alone[0]), k
+        p = r["params"]
+        p64 = _fp64(p)
+        rq, _, rst = O.sharpclaw(p64, ctl, r["q0"], 0.0, r["tend"], r["dt0"])
+        assert sm.run.info == 0 and sm.run.t == pytest.approx(r["tend"], abs=1e-14)
+        if p.precision == SC_FP64:
+            qp = r["q0"].copy(order="F")
+            interior(p, qp)[:] = np.nextafter(interior(p, qp), np.inf)
+            rq2, _, _ = O.sharpclaw(p64, ctl, qp, 0.0, r["tend"], r["dt0"])
+            tol = max(TOL_RUN, 8.0 * rel_linf(interior(p, rq2), interior(p, rq)))
+            err = rel_linf(interior(p, qm), interior(p, rq))
+            assert err <= tol, (k, specs[k], err, tol)
+            assert sm.run.steps_taken == rst.steps_taken
+        else:
+            g, o = interior(p, qm), interior(p, rq)
+            assert np.sum(np.abs(g - o)) / np.sum(np.abs(o)) <= 1e-5, k
+    with Server(hbm_bytes=64 << 20, max_active=2) as srv:
+        p = make_params("shallow_water", (40, 40), lim_type=2, char_decomp=1)
+        with pytest.raises(ScError) as ei:
+            srv.submit(p, ctl, O.new_grid(p), 0.0, 0.01, 1e-4)
+        assert ei.value.code == SC_ERR_UNSUPPORTED
 
 
 def test_server_2d_equals_solver_advance_bitwise():
