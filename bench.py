@@ -906,7 +906,7 @@ def main():
     ap.add_argument("--counter-summary", nargs="+", metavar="DIR",
                     help="average rocprofv3 --pmc runs of this script over the ranges of the "
                          "trace given by --trace-summary, print JSON and exit")
-    ap.add_argument("--kernel", default="void sc::stage2d_kernel<4, 2, double, 16, 15>"
+    ap.add_argument("--kernel", default="void sc::stage2d_kernel<4, 2, double, 16, 15, false>"
                                         "(sc::StageArgs, sc::PhysConst<double>)",
                     help="kernel name for --counter-summary")
     ap.add_argument("--command", action="append", default=[],

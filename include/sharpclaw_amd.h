@@ -57,7 +57,8 @@ typedef struct sc_params {
     int meqn;
     int mwaves;
     int mbc;             /* must be 3 (reconstruct.f90:85,158 hard-code it) */
-    int maux;            /* aux arrays per cell (1D grids; sc_set_aux) */
+    int maux;            /* aux arrays per cell (sc_set_aux): 1D grids, 2D
+                            advection's velocities, 2D capacity */
     int mcapa;           /* capacity column of aux, 1-based; 0 none (flux1.f90:56-61) */
     int lim_type;        /* 2 WENO5; 1 second-order TVD (single grids: 1D, 2D Euler
                             and shallow water) */
@@ -132,10 +133,15 @@ int sc_destroy(sc_solver *s);
 int sc_set_q(sc_solver *s, const double *q);
 int sc_get_q(sc_solver *s, double *q);
 
-/* The aux array aux(1-mbc:mx+mbc, maux) of a 1D grid, column-major as
+/* The aux array aux(1-mbc:mx+mbc, maux) of a 1D grid, or
+ * aux(1-mbc:nx+mbc, 1-mbc:ny+mbc, maux) of a 2D grid, column-major as
  * setaux(maxmx,mbc,nx,xlower,dx,maux,aux) fills it (main.f90:164-166).
- * The device keeps the capacity column mcapa (flux1.f90:56-61); the device
- * Riemann solvers take no aux.  Needed before stepping when mcapa > 0. */
+ * The device keeps the capacity field mcapa (flux1.f90:56-61; in 2D each
+ * sweep's slice of it, flux2.f90:37-40,70-73), ghost cells included: the
+ * ghost rows and columns count in the CFL with their own capacities; or 2D
+ * advection's edge velocities aux(:,:,1:2).  The device Riemann solvers of
+ * the other physics take no aux.  Needed before stepping when mcapa > 0.
+ * 2D capacity runs take weno5 / weno5_char, mthbc 1-3 and tile 0 (16x15). */
 int sc_set_aux(sc_solver *s, const double *aux);
 
 /* Frame output, replacing output(meqn,mbc,nx,xlower,dx,q,t,iframe,aux,maux)

@@ -962,7 +962,7 @@ void orc_tvd2_wave(int mx2, int meqn, int mwaves, int LD, const double *q, const
 
 /* --------------------------------------------------------------- flux1 */
 
-/* src/flux1.f90:2-190 for mcapa=0 and the reconstructions lim_type 2 with
+/* src/flux1.f90:2-190 (capacity: dtdx(i) = dt/(dx aux(i,mcapa))) and the reconstructions lim_type 2 with
  * char_decomp 0-3 and lim_type 1 with char_decomp 1-2, on one slice
  * q1d(1-mbc:mx+mbc, meqn).  In 1D dq1d is the step's dq (interior
  * zeroed by src); for ndim > 1 it is zeroed here (flux1.f90:62).  Returns
@@ -981,11 +981,13 @@ double orc_flux1(const sc_params *p, int ixy, int mx, const double *q1d, double 
 {
     const int mbc = p->mbc, meqn = p->meqn, mwaves = p->mwaves, L = mx + 2 * mbc;
     const size_t n1 = (size_t)L * meqn;
-    /* flux1.f90:56-61: dtdx(i) = dt/(dx*aux(i,mcapa)) with a capacity */
+    /* flux1.f90:56-61: dtdx(i) = dt/(dx*aux(i,mcapa)) with a capacity; in
+     * 2D aux is the sweep's slice aux(:,j,:) / aux(i,:,:) (flux2.f90:39,72) */
+    const double *ax = p->ndim == 1 ? g_aux : g_aux_slice;
     double *dtdx = (double *)malloc((size_t)L * sizeof(double));
     for (int i = 1 - mbc; i <= mx + mbc; i++)
-        dtdx[i + mbc - 1] = (p->ndim == 1 && p->mcapa > 0 && g_aux) ? dt / (p->dx[ixy - 1] * A1(g_aux, i, p->mcapa - 1))
-                                                   : dt / p->dx[ixy - 1];
+        dtdx[i + mbc - 1] = (p->mcapa > 0 && ax) ? dt / (p->dx[ixy - 1] * A1(ax, i, p->mcapa - 1))
+                                                 : dt / p->dx[ixy - 1];
 #define DTDX(i) dtdx[(i) + mbc - 1]
     if (p->ndim > 1) memset(dq1d, 0, n1 * sizeof(double));
     double *ql = (double *)calloc(n1, sizeof(double));

@@ -89,7 +89,9 @@ int sc_check_params(const sc_params *p, bool grid)
                     p->ndim, shape_m(id), shape_w(id), p->meqn, p->mwaves);
     if (p->mbc != 3) return sc_fail(SC_ERR_ARG, "mbc must be 3 for WENO5 (reconstruct.f90:85)");
     // aux arrays (main.f90:96-98): the device physics take no aux in their
-    // Riemann solvers; a capacity column (flux1.f90:56-61) is used on 1D grids
+    // Riemann solvers; a capacity column (flux1.f90:56-61) scales the update
+    // and the CFL of 1D grids and of 2D Euler / shallow water (flux2.f90
+    // passes each sweep's aux slice to flux1)
     if (p->maux < 0 || p->mcapa < 0 || p->mcapa > p->maux)
         return sc_fail(SC_ERR_ARG, "mcapa (%d) must index aux(:,1..maux) (maux = %d)", p->mcapa, p->maux);
     if (id == ADV2) {
@@ -99,7 +101,11 @@ int sc_check_params(const sc_params *p, bool grid)
         if (p->char_decomp != 0 || p->lim_type != 2 || p->tfluct_solver != 0)
             return sc_fail(SC_ERR_UNSUPPORTED, "2D advection: weno5 (char_decomp 0, lim_type 2, tfluct_solver 0)");
     } else if (p->maux > 0 && p->ndim != 1) {
-        return sc_fail(SC_ERR_UNSUPPORTED, "aux arrays (maux > 0) in 2D are implemented for advection");
+        if (p->mcapa == 0)
+            return sc_fail(SC_ERR_UNSUPPORTED, "aux arrays in 2D: 2D advection's velocities, or a capacity (mcapa > 0)");
+        if (p->lim_type != 2 || (p->char_decomp != 0 && p->char_decomp != 2))
+            return sc_fail(SC_ERR_UNSUPPORTED, "a capacity in 2D runs with weno5 or weno5_char (lim_type 2, "
+                                               "char_decomp 0 or 2)");
     }
     // reconstructions (flux1.f90:65-117)
     if (p->lim_type == 0)
@@ -596,7 +602,9 @@ extern "C" int sc_set_aux(sc_solver *s, const double *aux)
         return SC_OK;
     }
     if (s->p.mcapa < 1) return SC_OK;  // no device physics reads aux otherwise
-    const size_t LX = (size_t)s->p.nx[0] + 2 * s->p.mbc;
+    // the capacity field aux(:,mcapa) / aux(:,:,mcapa), ghost cells included
+    size_t LX = (size_t)s->p.nx[0] + 2 * s->p.mbc;
+    if (s->p.ndim == 2) LX *= (size_t)s->p.nx[1] + 2 * s->p.mbc;
     if (!s->capa && hipMalloc(&s->capa, LX * sizeof(double)) != hipSuccess)
         return sc_fail(SC_ERR_HIP, "aux: %s", hipGetErrorString(hipGetLastError()));
     HIPCHK(hipMemcpyAsync(s->capa, aux + (size_t)(s->p.mcapa - 1) * LX, LX * sizeof(double), hipMemcpyHostToDevice,
@@ -630,6 +638,9 @@ static StageArgs base_args(const sc_solver *s)
     a.LY = s->p.ndim == 2 ? s->p.nx[1] + 2 * s->p.mbc : 1;
     for (int k = 0; k < 4; k++) a.bc[k] = s->p.mthbc[k];
     a.capa = s->p.mcapa > 0 ? s->capa : nullptr;
+    // 2D with a capacity: the constant-tile shortcuts take the CFL speeds of
+    // a tile from one cell, which a varying kappa does not allow
+    a.full = s->p.ndim == 2 && a.capa ? 1 : 0;
     a.src = s->p.src_term;
     a.aux = s->aux2;
     a.vscale = s->vscale;
@@ -657,6 +668,10 @@ static std::map<std::tuple<int, int, int, int, int>, int> g_tune;
 static int tune_tile(sc_solver *s)
 {
     if (s->p.ndim != 2 || s->tile2d >= 0 || s->dqx) return SC_OK;
+    if (s->p.mcapa > 0) {
+        s->tile2d = 0;  // the capacity variant is built for the 16x15 tile
+        return SC_OK;
+    }
     // one tuning per size class (powers of two per direction), so grids of
     // many sizes (served requests) do not each pay for a tuning
     auto lg = [](int n) { int k = 0; while ((2 << k) <= n) k++; return k; };
@@ -705,6 +720,8 @@ extern "C" int sc_set_tile(sc_solver *s, int tile)
 {
     if (!s) return sc_fail(SC_ERR_ARG, "NULL solver");
     if (tile < -1 || tile >= NTILE2D) return sc_fail(SC_ERR_ARG, "tile must be -1..%d", NTILE2D - 1);
+    if (tile > 0 && s->p.ndim == 2 && s->p.mcapa > 0)
+        return sc_fail(SC_ERR_UNSUPPORTED, "a 2D grid with a capacity runs on tile 0 (16x15)");
     s->tile2d = tile;
     return SC_OK;
 }
@@ -837,6 +854,9 @@ extern "C" int sc_flux(sc_solver *s, const double *q, double *dq, double dt, dou
     if (s->sel.phys == ADV2 && s->p.tperiod != 0.0)
         return sc_fail(SC_ERR_UNSUPPORTED, "2D advection with tperiod set scales its velocities once per step "
                                            "(sc_ts); with sc_flux the caller's b4step scales aux and tperiod is 0");
+    if (s->p.ndim == 2 && s->p.mcapa > 0)
+        return sc_fail(SC_ERR_UNSUPPORTED, "sc_flux on a 2D grid with a capacity: the ghost rows' CFL would need "
+                                           "their capacities; 2D capacity runs take mthbc 1-3");
     DevGuard g(s->dev);
     StageArgs a = base_args(s);
     for (int k = 0; k < 4; k++) a.bc[k] = 0;
@@ -1362,7 +1382,8 @@ extern "C" int sc_advance(sc_solver *s, const sc_run_ctl *c, double tstart, doub
         // 2D advection steps its velocities per step on the host (b4step)
         // (the slice sweeps have no device-loop kernel)
         // (per-launch stage timing brackets launches from the host loop)
-        bool device = s->sel.phys != ADV2 && !s->dqx && !s->timing &&
+        // (a capacity runs on the host loop's stage kernel)
+        bool device = s->sel.phys != ADV2 && !s->dqx && !s->timing && s->p.mcapa == 0 &&
                       s->loop_mode != 0;
         if (device && s->use_graph) return advance2d_device(s, c, tstart, tend, dt_io, st);
         if (device && tiles <= 16384 && s->tnf && s->tile_order == TILE_ADAPTIVE)

@@ -37,9 +37,10 @@ struct StageArgs {
     double dtdx, dtdy;
     int op;
     int ntx, nty;        // tiles per direction (2D)
-    // 1D only: capacity aux(:,mcapa) in the q layout (NULL: none), and the
-    // source term (0 none, 1 radial shallow water) with the stage's dt and
-    // the cell-centre coordinates xlo + (i - 1/2) dxc it needs
+    // capacity aux(:,mcapa) (1D) or aux(:,:,mcapa) (2D) in the layout of one
+    // field of q (NULL: none); 1D only: the source term (0 none, 1 radial
+    // shallow water) with the stage's dt and the cell-centre coordinates
+    // xlo + (i - 1/2) dxc it needs
     const double *capa;
     int src;
     double dt, xlo, dxc;

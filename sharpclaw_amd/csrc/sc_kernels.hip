@@ -327,11 +327,11 @@ hipError_t launch_tile_list_reset(unsigned char *flag, int ntx, int nty, int *li
     return hipGetLastError();
 }
 
-template <int P, int CHAR, class T, int TX, int TY>
+template <int P, int CHAR, class T, int TX, int TY, bool CAPA = false>
 __global__ __launch_bounds__(256) void stage2d_kernel(StageArgs a, PhysConst<T> c)
 {
     const int b = tile_of_workgroup(a, blockIdx.x, a.order == TILE_ADAPTIVE ? *a.tnf : 0);
-    stage2d_tile<P, CHAR, T, TX, TY>(a, c, b);
+    stage2d_tile<P, CHAR, T, TX, TY, CAPA>(a, c, b);
 }
 
 // Stage k of the resident loop's current iteration.  The grid's fixed
@@ -777,19 +777,23 @@ struct TileShape {
 };
 static const TileShape kTiles[NTILE2D] = {{16, 15}, {15, 16}, {32, 7}, {14, 17}};
 
-template <int P, int CHAR, class T, int TX, int TY>
+template <int P, int CHAR, class T, int TX, int TY, bool CAPA = false>
 static hipError_t go2t(const StageArgs &a0, const PhysConst<T> &c, hipStream_t st)
 {
     StageArgs a = a0;
     a.ntx = (a.nx + TX - 1) / TX;
     a.nty = (a.ny + TY - 1) / TY;
-    return launch_k(stage2d_kernel<P, CHAR, T, TX, TY>, dim3(a.ntx * a.nty), dim3(256), st, t_ev_start, t_ev_stop,
-                    a, c);
+    return launch_k(stage2d_kernel<P, CHAR, T, TX, TY, CAPA>, dim3(a.ntx * a.nty), dim3(256), st, t_ev_start,
+                    t_ev_stop, a, c);
 }
 
 template <int P, int CHAR, class T>
 static hipError_t go2(int tile, const StageArgs &a, const PhysConst<T> &c, hipStream_t st)
 {
+    if constexpr (P == EU2 || P == SW2) {
+        // a capacity (flux1.f90:56-61) runs on the 16x15 tile only
+        if (a.capa) return tile == 0 ? go2t<P, CHAR, T, 16, 15, true>(a, c, st) : hipErrorInvalidValue;
+    }
     switch (tile) {
     case 1: return go2t<P, CHAR, T, 15, 16>(a, c, st);
     case 2: return go2t<P, CHAR, T, 32, 7>(a, c, st);

@@ -25,6 +25,25 @@ __device__ __forceinline__ PhysConst<T> adv_const(const PhysConst<T> &c, const S
     return cu;
 }
 
+// The CFL of one interface on a grid with a capacity (flux1.f90:126-133
+// with dtdx(i) = dt/(dx aux(i,mcapa)), flux1.f90:56-61): max over its waves
+// of s/kappa(right cell) and -s/kappa(left cell), in units of dt/dx.
+template <int W, class T>
+__device__ __forceinline__ T capa_speed(const T (&s)[W], double kl, double kr)
+{
+    const T il = T(1) / T(kl), ir = T(1) / T(kr);
+    T m = T(0);
+#pragma unroll
+    for (int w = 0; w < W; w++) m = fmax(m, fmax(s[w] * ir, -(s[w] * il)));
+    return m;
+}
+
+// kappa of grid cell (i, j), 0..n+1 (a.capa: one field in the q layout)
+__device__ __forceinline__ double capa_at(const StageArgs &a, int i, int j)
+{
+    return a.capa[(size_t)(i + 2) + (size_t)a.LX * (size_t)(j + 2)];
+}
+
 // A block of 256 threads updates the TX x TY tile b (row-major over the
 // a.ntx x a.nty tiles).  q of the tile plus 3-cell halos along x and y is
 // staged in LDS (the corner blocks are not needed: every stencil is
@@ -42,8 +61,15 @@ __device__ __forceinline__ PhysConst<T> adv_const(const PhysConst<T> &c, const S
 // interior row/column, whose wave speeds they therefore repeat exactly, so
 // the interior sweeps give the same maximum.  (For 2D advection the speeds
 // are the aux velocities, which the ghost rows do not repeat: the host adds
-// their maximum, which does not depend on q.)
-template <int P, int CHAR, class T, int TX, int TY>
+// their maximum, which does not depend on q.)  With a capacity (a.capa:
+// aux(:,:,mcapa) in the layout of one field of q) the update of a cell is
+// scaled by 1/kappa of the cell in both directions, and each interface's CFL
+// speeds by the kappa of the cell the wave enters (capa_speed); a ghost row
+// or column repeats the speeds of the interior one it copies but with its own
+// kappa, so the tile that holds that interior row or column adds them.
+// Every tile takes the full path then (a.full).  CAPA selects the
+// instantiation that reads a.capa (the others ignore it).
+template <int P, int CHAR, class T, int TX, int TY, bool CAPA = false>
 __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst<T> &c, int b)
 {
     constexpr int M = Shape<P>::M, W = Shape<P>::W;
@@ -245,8 +271,20 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
             else
                 riemann<P, 1, T>(c, lf, rt, wave, s, am, ap);
             if (i0 + ci <= nx + 1 && j0 + r <= ny) {
+                if constexpr (CAPA) {
+                    // interface gi between cells gi-1, gi of row gj; ghost rows
+                    // 0 and ny+1 copy row 1 / ny (extrapolation, wall) or
+                    // ny / 1 (periodic)
+                    const int gi = i0 + ci, gj = j0 + r;
+                    smx = fmax(smx, capa_speed<W, T>(s, capa_at(a, gi - 1, gj), capa_at(a, gi, gj)));
+                    if ((gj == 1 && a.bc[2] != 2) || (gj == ny && a.bc[2] == 2))
+                        smx = fmax(smx, capa_speed<W, T>(s, capa_at(a, gi - 1, 0), capa_at(a, gi, 0)));
+                    if ((gj == ny && a.bc[3] != 2) || (gj == 1 && a.bc[3] == 2))
+                        smx = fmax(smx, capa_speed<W, T>(s, capa_at(a, gi - 1, ny + 1), capa_at(a, gi, ny + 1)));
+                } else {
 #pragma unroll
-                for (int w = 0; w < W; w++) smx = fmax(smx, fabs(s[w]));
+                    for (int w = 0; w < W; w++) smx = fmax(smx, fabs(s[w]));
+                }
             }
             const int k = r * (TX + 1) + ci;
 #pragma unroll
@@ -267,9 +305,11 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
         // on its left, qr = left state of its own interface
         const int kl = cy * (TX + 1) + cx;
         if constexpr (FF) {
-            // X = F(i+1/2) - F(i-1/2)
+            // X = F(i+1/2) - F(i-1/2); dtdx(i) = dt/(dx kappa(i)) with a capacity
+            double dtdx = a.dtdx;
+            if constexpr (CAPA) dtdx = a.dtdx / capa_at(a, min(i0 + cx, nx + 1), min(j0 + cy, ny + 1));
 #pragma unroll
-            for (int m = 0; m < M; m++) dqx[m] = -(a.dtdx * (Fo[m] - sF[m][kl]));
+            for (int m = 0; m < M; m++) dqx[m] = -(dtdx * (Fo[m] - sF[m][kl]));
         } else {
             T ql_[M];
 #pragma unroll
@@ -311,8 +351,19 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
             else
                 riemann<P, 2, T>(c, lf, rt, wave, s, am, ap);
             if (i0 + ci <= nx && j0 + r <= ny + 1) {
+                if constexpr (CAPA) {
+                    // interface gj between cells gj-1, gj of column gi; ghost
+                    // columns 0 and nx+1 as the ghost rows above
+                    const int gi = i0 + ci, gj = j0 + r;
+                    smy = fmax(smy, capa_speed<W, T>(s, capa_at(a, gi, gj - 1), capa_at(a, gi, gj)));
+                    if ((gi == 1 && a.bc[0] != 2) || (gi == nx && a.bc[0] == 2))
+                        smy = fmax(smy, capa_speed<W, T>(s, capa_at(a, 0, gj - 1), capa_at(a, 0, gj)));
+                    if ((gi == nx && a.bc[1] != 2) || (gi == 1 && a.bc[1] == 2))
+                        smy = fmax(smy, capa_speed<W, T>(s, capa_at(a, nx + 1, gj - 1), capa_at(a, nx + 1, gj)));
+                } else {
 #pragma unroll
-                for (int w = 0; w < W; w++) smy = fmax(smy, fabs(s[w]));
+                    for (int w = 0; w < W; w++) smy = fmax(smy, fabs(s[w]));
+                }
             }
             const int k = r * TX + ci;
 #pragma unroll
@@ -348,9 +399,11 @@ __device__ __forceinline__ void stage2d_tile(const StageArgs &a, const PhysConst
             for (int m = 0; m < M; m++) Y[m] = (double)(((am[m] + sAp[m][kb]) + am2[m]) + ap2[m]);
         }
         if (upd) {
+            double dtdy = a.dtdy;
+            if constexpr (CAPA) dtdy = a.dtdy / capa_at(a, i0 + cx, j0 + cy);
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                double r = dqx[m] + -(a.dtdy * Y[m]);
+                double r = dqx[m] + -(dtdy * Y[m]);
                 size_t gm = gc + m * plane;
                 double qbi = a.op == OP_SSP104_10 ? a.qbin[gm] : 0.0;
                 double out, qbo = 0.0;

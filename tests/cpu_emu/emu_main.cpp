@@ -529,6 +529,33 @@ std::vector<Scenario> scenarios()
                      orc_swirl_setaux(&p, aux.data());
                      return one_step("stage2d swirl advection", p, 25, 0, false, &aux);
                  }});
+    // capacity on 2D grids (aux(:,:,mcapa)), partial edge tiles, every
+    // boundary option on some side; full path on a state with a flat block.
+    // The ghost cells of side k (and its corners) get the smallest kappa, so
+    // the CFL comes from that side's ghost row or column.
+    const int capbc[4][4] = {{1, 3, 2, 2}, {2, 2, 3, 1}, {3, 3, 3, 3}, {3, 1, 1, 3}};
+    for (int k = 0; k < 4; k++) {
+        std::string n = "stage2d capacity ghost side " + std::to_string(k);
+        v.push_back({n, [=]() {
+                         sc_params p = base(2, 37 + k, 29 + 5 * k, k == 1 ? SC_PHYS_SHALLOW_WATER : SC_PHYS_EULER);
+                         for (int b = 0; b < 4; b++) p.mthbc[b] = capbc[k][b];
+                         p.maux = 2;
+                         p.mcapa = 2;
+                         p.char_decomp = k == 2 ? 0 : 2;
+                         Rng r(900 + k);
+                         const int LX = p.nx[0] + 6, LY = p.nx[1] + 6;
+                         std::vector<double> aux(cells(p) * 2);
+                         for (auto &x : aux) x = r.uniform(1.0, 2.0);
+                         for (int j = 0; j < LY; j++)
+                             for (int i = 0; i < LX; i++) {
+                                 const int gi = i - 2, gj = j - 2;  // cell index 1..n inside
+                                 const bool side = k == 0 ? gi == 0 : k == 1 ? gi == p.nx[0] + 1
+                                                 : k == 2 ? gj == 0 : gj == p.nx[1] + 1;
+                                 if (side) aux[(size_t)LX * LY + i + (size_t)LX * j] = r.uniform(0.2, 0.3);
+                             }
+                         return one_step(n, p, 27 + k, -1, k == 0, &aux);
+                     }});
+    }
     // 2D slice sweeps (reconstructions that read neighbouring interfaces)
     const int sweeps[3][2] = {{2, 1}, {2, 3}, {1, 1}};
     for (auto &sw : sweeps) {

@@ -373,6 +373,68 @@ def aux():
          dtlog=dtl, ctl=np.array([1e99, 0.5, 0.45, 50000, 1]), dt_initial=np.array(1e-3), aux=a)
 
 
+def kappa_aux2(nx, ny, maux, mcapa):
+    """A smooth capacity field in aux(:,:,mcapa) of aux(1-mbc:nx+mbc, 1-mbc:ny+mbc, maux)."""
+    x = (np.arange(-2, nx + 4) - 0.5) / nx
+    y = (np.arange(-2, ny + 4) - 0.5) / ny
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    aux = np.zeros((nx + 6, ny + 6, maux), order="F")
+    for m in range(maux):
+        aux[:, :, m] = 1.0 + 0.1 * m
+    aux[:, :, mcapa - 1] = 1.0 + 0.4 * np.sin(2.0 * np.pi * X) ** 2 * np.cos(np.pi * Y) ** 2
+    return aux
+
+
+def aux2d():
+    """Capacity on 2D grids (flux2.f90 passes the slice aux(:,j,:) /
+    aux(i,:,:) into flux1, whose dtdx(i) = dt/(dx(ixy)*aux(i,mcapa)) scales
+    the slice's update and CFL, flux1.f90:56-61): Euler and shallow water,
+    one step per case on random capacities, and a four-quadrant Euler run.
+    Square grids (see recon2d)."""
+    rng = np.random.default_rng(20261208)
+    out = {}
+    k = 0
+    for name in ("euler", "shallow_water"):
+        for char in (0, 2):
+            for bc in [(1, 1, 1, 1), (2, 2, 2, 2), (3, 3, 3, 3), (1, 3, 2, 2)]:
+                for maux, mcapa in ((1, 1), (3, 2)):
+                    p = make_params(name, (20, 20), char_decomp=char, mthbc=bc, maux=maux, mcapa=mcapa)
+                    q = _states.grid(p, rng, smooth=True)
+                    a = np.asfortranarray(rng.uniform(0.5, 2.0, (26, 26, maux)))
+                    if maux == 3:
+                        # the smallest capacities on one side's ghost column /
+                        # row: the CFL comes from that sweep
+                        kap = a[:, :, mcapa - 1]
+                        kap[:] = rng.uniform(1.0, 2.0, kap.shape)
+                        side = (k // 2) % 4
+                        sl = {0: (2, slice(None)), 1: (23, slice(None)), 2: (slice(None), 2),
+                              3: (slice(None), 23)}[side]
+                        kap[sl] = rng.uniform(0.2, 0.3, kap[sl].shape)
+                    dt = 0.05 * p.dx[0]
+                    qb, dq, cfl = R.step(p, q, dt, aux=a)
+                    out.update({f"c{k}_name": np.array(name), f"c{k}_q": q, f"c{k}_aux": a,
+                                f"c{k}_qbc": qb, f"c{k}_dq": dq, f"c{k}_cfl": np.array(cfl),
+                                f"c{k}_dt": np.array(dt), f"c{k}_bc": np.array(bc),
+                                f"c{k}_char": np.array(char), f"c{k}_maux": np.array(maux),
+                                f"c{k}_mcapa": np.array(mcapa)})
+                    k += 1
+    out["n"] = np.array(k)
+    save("step_aux2d.npz", **out)
+    # four Euler quadrants on a capacity-weighted 32x32 grid, SSPRK3, 16 steps
+    p = make_params("euler", (32, 32), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1), maux=2,
+                    mcapa=2)
+    a = kappa_aux2(32, 32, 2, 2)
+    st = np.stack([rng.uniform(0.1, 1.5, 4), rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4),
+                   rng.uniform(0.03, 1.5, 4)], axis=1)
+    lines = ["1", "0.5 0.5"] + [" ".join(repr(float(v)) for v in row) for row in st]
+    cd = R.clawdata_for(p, nout=16, outstyle=3, nstepout=8, nstop=16, dt_initial=1e-3,
+                        cfl_max=0.5, cfl_desired=0.45)
+    fr, dtl, info = R.run(p, cd, lines, aux=a)
+    save("run_eu2d_capacity.npz", t=np.array([f[0] for f in fr]), q=np.stack([f[1] for f in fr]),
+         dtlog=dtl, states=st, ctl=np.array([1e99, 0.5, 0.45, 16, 1]), dt_initial=np.array(1e-3),
+         aux=a)
+
+
 def swirl():
     """2D advection with the swirl app's edge velocities in aux (rpn2ad1.f,
     setaux.f/psi.f) and its time-dependent b4step: steps on random data and
@@ -407,7 +469,7 @@ def swirl():
 
 
 PARTS = {"routines": routines, "steps": steps, "runs": runs, "tfluct": tfluct, "flush": flush,
-         "recon": recon, "recon2d": recon2d, "aux": aux, "swirl": swirl}
+         "recon": recon, "recon2d": recon2d, "aux": aux, "aux2d": aux2d, "swirl": swirl}
 
 if __name__ == "__main__":
     if not R.ref_available():

@@ -276,6 +276,33 @@ def test_run_eu1d_capacity():
     assert np.max(np.abs(interior(p0, q0) - d["q"][1])) > 1e-3
 
 
+def test_step_aux2d_bit_exact():
+    """Capacity on 2D grids: flux2.f90 passes each sweep's aux slice into
+    flux1, whose dtdx(i) = dt/(dx(ixy)*aux(i,mcapa)) scales the update and
+    the CFL (flux1.f90:56-61, flux2.f90:37-40,70-73); the ghost rows and
+    columns count in the CFL with their own capacities."""
+    d = load("step_aux2d.npz")
+    for k in range(int(d["n"])):
+        p = make_params(str(d[f"c{k}_name"]), (20, 20), char_decomp=int(d[f"c{k}_char"]),
+                        mthbc=tuple(int(b) for b in d[f"c{k}_bc"]), maux=int(d[f"c{k}_maux"]),
+                        mcapa=int(d[f"c{k}_mcapa"]))
+        qb, dq, cfl = O.step(p, d[f"c{k}_q"], float(d[f"c{k}_dt"]), aux=d[f"c{k}_aux"])
+        assert np.array_equal(qb, d[f"c{k}_qbc"]), k
+        assert np.array_equal(interior(p, dq), interior(p, d[f"c{k}_dq"])), k
+        assert cfl == float(d[f"c{k}_cfl"]), k
+    # the capacity changes the step: without it dq differs
+    p0 = make_params(str(d["c0_name"]), (20, 20), char_decomp=int(d["c0_char"]),
+                     mthbc=tuple(int(b) for b in d["c0_bc"]))
+    _, dq0, _ = O.step(p0, d["c0_q"], float(d["c0_dt"]))
+    assert np.max(np.abs(interior(p0, dq0) - interior(p0, d["c0_dq"]))) > 1e-6
+
+
+def test_run_eu2d_capacity():
+    d = load("run_eu2d_capacity.npz")
+    _replay(make_params("euler", (32, 32), char_decomp=2, time_integrator=3, mthbc=(1, 1, 1, 1), maux=2,
+                        mcapa=2), "run_eu2d_capacity.npz", single_step=True, aux=d["aux"])
+
+
 def test_step_swirl_bit_exact():
     """2D advection, edge velocities aux(:,:,1:2) of the swirl app
     (rpn2ad1.f, sliced per sweep as flux2.f90:37-40,70-73)."""
