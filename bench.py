@@ -120,28 +120,29 @@ def smooth_q(p):
 
 
 def traffic_from_profile():
-    """HBM bytes per stage launch of the timed steps from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    """HBM bytes per stage launch of the replayed timed steps (the launches
+    the roofline times) from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
     path = os.path.join(ROOT, "profiles", "stage2d_euler2d_1024_fp64_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        w = json.load(f).get("windows", {}).get("bench:timed", {})
+        w = json.load(f).get("windows", {}).get("bench:replay", {})
     return w.get("bytes_per_launch")
 
 
 def valu_from_profile(tx, ty, ms_launch, ms_smooth):
     """float64 VALU roofline of the stage kernel: flop per launch from the
-    committed rocprofv3 counter summary (the timed steps and the smooth
-    state, the same launches bench.py times), divided by the live launch
-    times."""
+    committed rocprofv3 counter summary (the replayed timed steps and the
+    smooth state, the same launches bench.py times), divided by the live
+    launch times."""
     path = os.path.join(ROOT, "profiles", "stage2d_euler2d_1024_fp64_valu.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
     ws = d.get("windows", {})
-    t, sm = ws.get("bench:timed"), ws.get("bench:smooth")
+    t, sm = ws.get("bench:replay"), ws.get("bench:smooth")
     if not t or "fp64_flop_per_launch" not in t:
         return None
     name = d.get("kernel", "")
@@ -150,7 +151,7 @@ def valu_from_profile(tx, ty, ms_launch, ms_smooth):
            "frac": tf / FP64_PEAK_TFLOPS, "profiled_kernel": name,
            "flop_per_launch": t["fp64_flop_per_launch"],
            "valu_lane_instr_per_cell": t.get("valu_lane_instr_per_cell")}
-    if f"<4, 2, double, {tx}, {ty}>" not in name:
+    if f"<4, 2, double, {tx}, {ty}" not in name:
         out["note"] = f"profiled with another tile shape than the {tx}x{ty} timed here"
     if sm and "fp64_flop_per_launch" in sm:
         ts = sm["fp64_flop_per_launch"] / (ms_smooth * 1e-3) / 1e12
@@ -264,7 +265,8 @@ def batched_dambreaks(nsolves, seed, cpu=True, single_sample=8, cpu_sample=16):
             first = time.perf_counter() - t0
             for i in range(nsolves):  # second run from the same data: tuner already done
                 b.set_q(i, dambreak_q(int(mx[i]), hl[i], hr[i], x0[i]))
-            res, ms = b.run(ctl, 0.0, tend, dt0)
+            with roctx_range(f"bench:batch:{name}"):
+                res, ms = b.run(ctl, 0.0, tend, dt0)
             block = b.block
             tuning = b.tuning()
             parts = b.parts
@@ -523,7 +525,8 @@ def recon_bench(n=1 << 20, stages=60):
         with Solver(p) as s:
             s.set_q(q)
             s.time_stage(3, 1e-6)
-            ms = s.time_stage(stages, 1e-6)
+            with roctx_range(f"bench:recon1d:{name}"):
+                ms = s.time_stage(stages, 1e-6)
         gbs = 3 * 8 * (2 + 3 + 3) / 3.0 * n / (ms * 1e-3) / 1e9
         out[name] = {"lim_type": lim, "char_decomp": char, "us_per_stage": ms * 1e3,
                      "value": n / (ms * 1e-3), "unit": "cell-updates/s per RK stage",
@@ -550,7 +553,8 @@ def recon2d_bench(n=1024, stages=12):
         with Solver(p) as s:
             s.set_q(smooth_q(p))
             s.time_stage(2, 1e-6)
-            ms = s.time_stage(stages, 1e-6)
+            with roctx_range(f"bench:recon2d:{name}"):
+                ms = s.time_stage(stages, 1e-6)
         cells = n * n
         gbs = 4 * 8 * (2 + 3 + 3) / 3.0 * cells / (ms * 1e-3) / 1e9
         out[name] = {"lim_type": lim, "char_decomp": char, "us_per_stage": ms * 1e3,
@@ -892,6 +896,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched dam-break workload")
+    ap.add_argument("--section", choices=("recon1d", "recon2d", "batched"),
+                    help="run only this section of the benchmark (with the copy kernel that "
+                         "calibrates counter runs) and print its JSON: the profiling runs of the "
+                         "1D stage, slice-sweep and batch kernels")
+    ap.add_argument("--cells", type=float, help="--counter-summary: cells per launch (instructions per cell)")
+    ap.add_argument("--alg-bytes", type=float, help="--counter-summary: algorithmic bytes per launch")
     ap.add_argument("--cpu-full-batch", action="store_true",
                     help="only run the whole batched workload on the reference over the host's cores "
                          "(minutes) and print its JSON summary")
@@ -915,19 +925,30 @@ def main():
     if args.counter_summary:
         if not args.trace_summary:
             raise SystemExit("--counter-summary needs --trace-summary DIR (the traced run)")
-        cells = args.n * args.n
-        alg = 4 * 8 * (2 + 3 + 3) / 3.0 * cells  # 2D Euler, SSPRK3 stage average
+        cells = args.cells if args.cells else args.n * args.n
+        # 2D Euler, SSPRK3 stage average
+        alg = args.alg_bytes if args.alg_bytes else 4 * 8 * (2 + 3 + 3) / 3.0 * cells
         out = counter_summary(args.counter_summary, args.trace_summary, args.kernel, cells=cells, alg_bytes=alg)
-        out["workload"] = (f"2D Euler {args.n}x{args.n} fp64 char WENO5 + Roe rpn2, SSPRK3, four-quadrant "
-                           f"data (seed {args.seed}); windows: the bench's timed steps, their replay "
-                           f"with timing events, the smooth state")
+        if not args.cells:
+            out["workload"] = (f"2D Euler {args.n}x{args.n} fp64 char WENO5 + Roe rpn2, SSPRK3, four-quadrant "
+                               f"data (seed {args.seed}); windows: the bench's timed steps, their replay "
+                               f"with timing events, the smooth state")
         summ = ["python", "bench.py", "--trace-summary", args.trace_summary, "--counter-summary"] + \
             args.counter_summary + ([] if args.n == 1024 else ["--n", str(args.n)])
+        if args.cells:
+            summ += ["--kernel", repr(args.kernel), "--cells", f"{args.cells:g}", "--alg-bytes", f"{alg:.6g}"]
         out["commands"] = args.command + [" ".join(summ)]
         print(json.dumps(out, indent=1))
         return
     if args.trace_summary:
         trace_summary(args.trace_summary)
+        return
+    if args.section:
+        from sharpclaw_amd import copy_bench
+        sec = {"recon1d": recon_bench, "recon2d": recon2d_bench,
+               "batched": lambda: batched_dambreaks(args.batch, args.work_seed, cpu=False)}[args.section]()
+        sec["copy_kernel_gbs"] = copy_bench(1 << 30, 10)
+        print(json.dumps({args.section: sec}), flush=True)
         return
     if args.cpu_full_batch:
         log(f"bench: the whole batched workload ({args.batch} dam breaks) on the reference, host cores")
