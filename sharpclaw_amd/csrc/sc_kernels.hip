@@ -73,32 +73,35 @@ __global__ __launch_bounds__(NT * NLN) void sweep2d_kernel(StageArgs a, PhysCons
 // ------------------------------------------------------ 2D fused stage
 
 // The tile a workgroup takes.  Workgroups are dealt to the 8 XCDs
-// round-robin (workgroup w runs on XCD w mod 8) and each XCD has its own L2.
-// TILE_BANDS: each XCD a contiguous band of tiles (tile rows), so halo rows
-// shared by neighbouring tiles stay in one L2.  TILE_ADAPTIVE: the tiles that
-// were not constant at the last step first, dealt round-robin over the XCDs
-// (a non-constant tile takes several times as long as a constant one, so
-// these start first and spread evenly over the XCDs), then the constant ones,
-// each XCD a contiguous run of them in tile order (tile_partition builds the
-// list after each step; nf is the count of the former, *a.tnf).
-// Piecewise-constant data such as a four-quadrant
-// Riemann problem has its non-constant tiles in a few rows of tiles, which
-// TILE_BANDS would give to one or two XCDs, and in tile order they would
-// start late and set the launch's tail.
+// round-robin (workgroup w runs on XCD w mod 8, its (w div 8)-th there) and
+// each XCD has its own L2.  TILE_BANDS: each XCD a contiguous band of tiles
+// (tile rows), so halo rows shared by neighbouring tiles stay in one L2.
+// TILE_ADAPTIVE: the list tile_partition builds after each step holds the
+// tiles that were not constant at the last step first, then the constant
+// ones, each part in strip order (nf = the count of the former, *a.tnf).
+// Each XCD takes an equal share of the non-constant part, a contiguous run of
+// it, with its first workgroups (a non-constant tile takes several times as
+// long as a constant one, so these start first and spread evenly over the
+// XCDs), then a contiguous run of the constant part: within an XCD
+// consecutive workgroups take neighbouring tiles, whose shared halo rows stay
+// in its L2.  Piecewise-constant data such as a four-quadrant Riemann problem
+// has its non-constant tiles in a few rows of tiles, which TILE_BANDS would
+// give to one or two XCDs, and in tile order they would start late and set
+// the launch's tail.
 __device__ __forceinline__ int tile_of_workgroup(const StageArgs &a, int w, int nf)
 {
     const int ntiles = a.ntx * a.nty;
     if (w >= ntiles) return -1;
     if (a.order == TILE_ADAPTIVE) {
-        if (w < nf) return a.tlist[w];
-        const int x = w & 7;
+        const int x = w & 7, k = w >> 3;
+        // XCD y runs workgroups y, y+8, ...: ceil((ntiles - y)/8) of them, the
+        // first ceil((nf - y)/8) on non-constant tiles
+        const int nfq = nf >> 3, nfr = nf & 7;
+        const int nfx = nfq + (x < nfr ? 1 : 0);
+        if (k < nfx) return a.tlist[x * nfq + min(x, nfr) + k];
         int start = 0;
-        for (int y = 0; y < x; y++) {
-            const int w0 = nf + ((y - nf) & 7);  // first constant-tile workgroup of XCD y
-            if (w0 < ntiles) start += (ntiles - w0 + 7) >> 3;
-        }
-        const int w0 = nf + ((x - nf) & 7);
-        return a.tlist[nf + start + ((w - w0) >> 3)];
+        for (int y = 0; y < x; y++) start += ((ntiles - y + 7) >> 3) - (nfq + (y < nfr ? 1 : 0));
+        return a.tlist[nf + start + (k - nfx)];
     }
     if ((ntiles & 7) == 0) return (w & 7) * (ntiles >> 3) + (w >> 3);
     return w;
