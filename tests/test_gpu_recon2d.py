@@ -159,6 +159,40 @@ def test_fp32_recon2d_l1(lim, char):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("prec", [0, SC_FP32])
+@pytest.mark.parametrize("name,lim,char,mth", [("euler", 2, 1, 5), ("euler", 2, 3, 5), ("euler", 1, 1, 2),
+                                               ("euler", 1, 2, 4), ("shallow_water", 2, 1, 5),
+                                               ("shallow_water", 2, 3, 5), ("shallow_water", 1, 1, 3),
+                                               ("shallow_water", 1, 2, 1)])
+@pytest.mark.skip(reason="its first run on the GPU, with a fused-tile variant of these reconstructions "
+                         "selected by SC_SWEEPS=0, went silent (killed after 180 s without output) and no "
+                         "cause was found in the source or its gfx950 assembly; the variant is not in the "
+                         "library, and this case stays off the GPU")
+def test_tile_equals_sweeps_bitwise(name, lim, char, mth, prec, monkeypatch):
+    """A fused-tile variant of these reconstructions (neighbour data per tile
+    row and column in LDS) against the x and y slice sweeps (SC_SWEEPS=1):
+    steps and whole runs (CFL, dt, rejections) bit for bit, on grids with
+    partial edge tiles and every boundary kind."""
+    rng = np.random.default_rng(67 + lim + char)
+    for shape, bc in (((50, 47), (1, 3, 2, 2)), ((130, 29), (3, 3, 1, 1)), ((33, 61), (2, 2, 3, 3))):
+        p = make_params(name, shape, char_decomp=char, lim_type=lim, mthlim=(mth,) * 4, mthbc=bc,
+                        time_integrator=3, precision=prec)
+        q = _states.grid(p, rng, smooth=True)
+        dt = 0.1 * min(p.dx[0], p.dx[1])
+        ctl = ScRunCtl(dt_max=1e99, cfl_max=0.5, cfl_desired=0.45, max_steps=8, dt_variable=1)
+        out = []
+        for sweeps in ("1", "0"):
+            monkeypatch.setenv("SC_SWEEPS", sweeps)
+            with Solver(p) as s:
+                s.set_q(q)
+                dq, cfl = s.step(dt)
+                s.set_q(q)
+                dtn, st = s.advance(ctl, 0.0, 10.0, dt * 3)
+                out.append((dq.tobytes(), cfl, s.get_q().tobytes(), dtn, st.steps_taken, st.rejected,
+                            st.cfl_maxused))
+        assert out[0] == out[1], (shape, bc)
+
+
 @pytest.mark.parametrize("env", [{"SC_HOST_LOOP": "0"}, {"SC_HOST_LOOP": "0", "SC_GRAPH": "1"}])
 def test_recon2d_ignores_device_loop_request(env, monkeypatch):
     """The slice sweeps have no device-resident loop: a request for one (or
