@@ -347,7 +347,8 @@ extern "C" int sc_create(const sc_params *params, sc_solver **out)
     }
     if (const char *e = getenv("SC_TILE")) {
         int k = atoi(e);
-        if (k >= 0 && k < NTILE2D) s->tile2d = k;
+        // a 2D capacity runs on tile 0, the shape its kernel variant is built for
+        if (k >= 0 && k < NTILE2D && !(s->p.ndim == 2 && s->p.mcapa > 0)) s->tile2d = k;
     }
     *out = s;
     return SC_OK;
