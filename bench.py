@@ -456,9 +456,10 @@ def euler1d_bench(cpu=True):
     """One 1D grid: Euler (gamma 1.4) Shu-Osher, 1600 cells on [-5, 5],
     characteristic WENO5 + Roe (efix), SSP(10,4), CFL 2.45 / 2.5, to t = 1.8,
     float64 and float32 arithmetic, the whole run through Solver.advance:
-    its time loop on the device, the grid split over one workgroup per
-    chunk (13 at 128 threads) that meet at a device barrier after every
-    stage.  Beside it the host-driven loop (SC_HOST_LOOP=1: one launch per
+    its time loop on the device, the grid split over 22 workgroups of 128
+    threads, each computing whole steps on its 73 cells widened by 3 cells
+    per remaining stage, which meet at a device barrier once per step (the
+    CFL).  Beside it the host-driven loop (SC_HOST_LOOP=1: one launch per
     stage, the CFL read back every step), bitwise the same result.  With
     cpu, the reference Fortran on one core on the same run, and the float64
     result against the reference's."""

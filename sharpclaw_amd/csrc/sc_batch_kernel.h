@@ -12,7 +12,13 @@
 // the k parts meet at a device-memory barrier after every stage, and the
 // CFL of the last stage is the maximum over the parts.  Every part runs
 // the same (uniform) control logic on the same numbers, so a split solve
-// computes exactly what one workgroup computes.
+// computes exactly what one workgroup computes.  A launch of one split
+// solve can instead give every part a whole step between barriers: part p
+// owns a contiguous range of cells and computes stage s of a step on that
+// range widened by 3 cells per stage still to come (the stencil reach of a
+// cell's update), in private stage buffers, so only the step's last stage
+// writes the shared q and only the step's CFL maximum needs a barrier; the
+// overlap cells are computed twice, by the same arithmetic.
 // State stays in global memory (three rotating q buffers per solve plus the
 // SSP(10,4) register), where the workgroup's L2 keeps it hot between stages;
 // the block barrier that ends each chunk makes a stage's writes visible to
@@ -85,6 +91,11 @@ struct BatchArgs {
     // maximum of even / odd iterations, unused), zero at a solve's start
     unsigned long long *sync;
     long long spin_ticks;  // longest wait at a barrier (wall_clock64 ticks)
+    // non-null: the launch is the parts of one split solve (nsolves parts of
+    // solve 0, boundaries without periodic wrap), which meet once per step;
+    // part p's three private stage buffers (two stage values, the SSP(10,4)
+    // register) are the len-double buffers 3p..3p+2 from priv
+    double *priv;
 };
 
 struct Plan {
@@ -213,6 +224,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
         // whose accesses it would issue as flat instructions
         auto buf = [&](int j) { return A.base + S.off + (size_t)j * len; };
         if (s.t_begin == 0) s.t_begin = (long long)wall_clock64();
+        // a whole step between barriers: this part's cells [lo, hi)
+        const bool whole = sync && A.priv;
+        const int lo = 1 + (int)((long long)n * part / parts), hi = 1 + (int)((long long)n * (part + 1) / parts);
+        auto pbuf = [&](int j) { return A.priv + (size_t)(part * 3 + j) * len; };
 
         StageArgs a;
         a.nx = n;
@@ -261,10 +276,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void ba
             bool lost = false;  // a barrier of a split solve timed out
             for (int st = 0; st < plan.n; st++) {
                 int outb = other_buf(in, s.old);
-                a.qin = buf(in);
-                a.qout = buf(outb);
                 a.op = plan.op[st];
                 const bool last = st == plan.n - 1;
+                if (whole) {
+                    // the cells the remaining stages of this part read: the
+                    // stage reads q of the step (shared) or the previous
+                    // stage's private values, valid 3 cells further out
+                    const int h = 3 * (plan.n - 1 - st);
+                    const int rlo = max(1, lo - h), rhi = min(n + 1, hi + h);
+                    a.qin = st == 0 ? buf(in) : pbuf((st - 1) & 1);
+                    a.qout = last ? buf(outb) : pbuf(st & 1);
+                    a.qbin = a.qbout = pbuf(2);
+                    for (int c0 = rlo; c0 < rhi; c0 += NT - 1) {
+                        T s1 = stage1d_chunk<P, 1, CHAR, T, NT>(a, c, c0, sq, sx, &sw[0][0], rhi - 1);
+                        if (last) sm = fmax(sm, s1);
+                    }
+                    in = outb;
+                    continue;
+                }
+                a.qin = buf(in);
+                a.qout = buf(outb);
                 for (int ch = ch0; ch < ch1; ch++) {
                     T s1 = stage1d_chunk<P, 1, CHAR, T, NT>(a, c, 1 + ch * (NT - 1), sq, sx, &sw[0][0]);
                     if (last) sm = fmax(sm, s1);

@@ -241,6 +241,8 @@ struct sc_solver {
         unsigned long long *next, *sync;
         int ncu;
         double tick_ms;
+        double *priv;       // private stage buffers of the parts (whole steps)
+        size_t priv_bytes;
     } l1;
 };
 
@@ -395,6 +397,7 @@ extern "C" int sc_destroy(sc_solver *s)
     if (s->hres) hipHostFree(s->hres);
     if (s->tnf) hipFree(s->tnf);
     if (s->l1.solve) hipFree(s->l1.solve);
+    if (s->l1.priv) hipFree(s->l1.priv);
     for (int k = 0; k < s->tev_cap; k++)
         if (s->tev[k]) hipEventDestroy(s->tev[k]);
     free(s->tev);
@@ -1255,8 +1258,9 @@ static bool loop1d_fits(const sc_solver *s)
 }
 
 // Workgroup size of the 1D device loop: SC_LOOP_BLOCK, else 128 (1600-cell
-// Euler grid, 566 SSP(10,4) steps: 24.7 ms at 128 threads in 13 parts,
-// 26.1 ms at 256 in 7, 29.7 ms at 512 in 4; host-driven loop 37.5 ms).
+// Euler grid, 566 SSP(10,4) steps, a barrier per step: 16.4 ms at 128
+// threads in 22 parts, 16.8 ms at 256 in 8, 21.5 ms at 512 in 4; a barrier
+// per stage: 24.2 / 24.8 / 28.5 ms; host-driven loop 31.4 ms).
 static int loop1d_block(const PhysSel &sel)
 {
     if (const char *e = getenv("SC_LOOP_BLOCK")) {
@@ -1269,10 +1273,14 @@ static int loop1d_block(const PhysSel &sel)
 
 // src/sharpclaw.f90:35-235 on the device for a 1D grid: the time loop of
 // the persistent batch kernel (sc_batch_kernel.h) with the grid as its one
-// solve, split over one workgroup per chunk (at most MAX_PARTS), which meet
-// at a device barrier after every stage; no host round trip per step.  The
-// stage arithmetic, the CFL maximum and the loop's decisions are those of
-// the host-driven loop, so the result is bitwise the same.
+// solve, split over workgroups (at most MAX_PARTS); no host round trip per
+// step.  Without periodic boundaries each workgroup owns a range of cells
+// of at most NT - 1 - 6 (stages - 1) (one chunk per stage with the overlap
+// its remaining stages read) and the workgroups meet once per step, for the
+// CFL; with periodic ones (or SC_LOOP_STAGEWISE=1) they own chunks and meet
+// after every stage.  The stage arithmetic, the CFL maximum and the loop's
+// decisions are those of the host-driven loop, so the result is bitwise the
+// same.
 static int advance1d_device(sc_solver *s, const sc_run_ctl *c, double tstart, double tend, double *dt_io,
                             sc_run_stats *st)
 {
@@ -1298,7 +1306,23 @@ static int advance1d_device(sc_solver *s, const sc_run_ctl *c, double tstart, do
     const int nt = loop1d_block(s->sel);
     const int mx = s->p.nx[0];
     const int nch = (mx + nt - 2) / (nt - 1);
-    const int parts = std::max(1, std::min({nch, MAX_PARTS, L.ncu * batch_occupancy(s->sel, nt)}));
+    const int cap = std::min(MAX_PARTS, L.ncu * batch_occupancy(s->sel, nt));
+    static const int kStages[5] = {0, 2, 4, 3, 10};
+    const int own = nt - 1 - 6 * (kStages[s->p.time_integrator] - 1);
+    const bool whole = s->p.mthbc[0] != 2 && s->p.mthbc[1] != 2 && own >= 16 && !getenv("SC_LOOP_STAGEWISE");
+    const int parts = std::max(1, whole ? std::min((mx + own - 1) / own, cap) : std::min(nch, cap));
+    const size_t len = (size_t)(mx + 6) * s->p.meqn;
+    const size_t pbytes = whole && parts > 1 ? (size_t)parts * 3 * len * sizeof(double) : 0;
+    if (pbytes > L.priv_bytes) {
+        if (L.priv) HIPCHK(hipFree(L.priv));
+        L.priv = nullptr;
+        L.priv_bytes = 0;
+        HIPCHK(hipMalloc(&L.priv, pbytes));
+        L.priv_bytes = pbytes;
+        // every value a stage reads is written before it; the clear keeps
+        // the rest defined
+        HIPCHK(hipMemsetAsync(L.priv, 0, pbytes, s->st));
+    }
     int maxn, info0;
     loop_limits(*c, tstart, tend, *dt_io, &maxn, &info0);
     BatchSolve S;
@@ -1328,6 +1352,7 @@ static int advance1d_device(sc_solver *s, const sc_run_ctl *c, double tstart, do
     A.ti = s->p.time_integrator;
     A.sync = L.sync;
     A.spin_ticks = (long long)(10000.0 / L.tick_ms);
+    A.priv = pbytes ? L.priv : nullptr;
     HIPCHK(hipMemcpyAsync(L.solve, &S, sizeof S, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipMemcpyAsync(L.state, &ss, sizeof ss, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipMemcpyAsync(L.order, ord, parts * sizeof(int), hipMemcpyHostToDevice, s->st));

@@ -506,10 +506,14 @@ __device__ __forceinline__ T line_chunk(const PhysConst<T> &c, int c0, int n, co
 }
 
 // One chunk of a 1D stage: line_chunk on the grid a.qin with the boundary
-// ghosts of bc1.f90, updated stage values written to a.qout.
+// ghosts of bc1.f90, updated stage values written to a.qout.  last: the
+// last cell of the chunk to update (and last + 1 the last interface of its
+// CFL maximum), at most a.nx; cells past it are left alone, and the values
+// it reads stay within cells last + 3 of a.qin.
 template <int P, int IXY, int CHAR, class T, int NT>
 __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T> &c, int c0,
-                                           T (*sq)[NT + 5], T (*sx)[NT], T *sw = nullptr)
+                                           T (*sq)[NT + 5], T (*sx)[NT], T *sw = nullptr,
+                                           int last = 0x7fffffff)
 {
     constexpr int M = Shape<P>::M;
     constexpr int MN = IXY == 1 ? 1 : 2;  // normal momentum for a wall
@@ -555,7 +559,7 @@ __device__ __forceinline__ T stage1d_chunk(const StageArgs &a, const PhysConst<T
             if (a.op == OP_SSP104_5) a.qbout[g] = qbo;
         }
     };
-    return line_chunk<P, IXY, CHAR, T, NT, 1>(c, c0, n, a.capa, &sq[0][0], &sx[0][0], sw, load, emit);
+    return line_chunk<P, IXY, CHAR, T, NT, 1>(c, c0, min(n, last), a.capa, &sq[0][0], &sx[0][0], sw, load, emit);
 }
 
 // ------------------------------------------------- 2D stage by slices

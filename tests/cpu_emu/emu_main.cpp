@@ -700,6 +700,25 @@ std::vector<Scenario> scenarios()
                      p.precision = SC_FP32;
                      return loop1d("loop1d shallow water fp32", p, 603, 0.01, 1e-5);
                  }});
+    // parts whose widened ranges reach a boundary; parts owning more
+    // than one chunk per stage (MAX_PARTS reached); periodic boundaries
+    // (a barrier after every stage)
+    v.push_back({"loop1d euler ssp104 narrow parts", []() {
+                     sc_params p = base(1, 200, 1, SC_PHYS_EULER);
+                     p.time_integrator = 4;
+                     p.mthbc[1] = 3;
+                     return loop1d("loop1d euler ssp104 narrow parts", p, 611, 0.01, 1e-10);
+                 }});
+    v.push_back({"loop1d euler ssp104 multi-chunk parts", []() {
+                     sc_params p = base(1, 4000, 1, SC_PHYS_EULER);
+                     p.time_integrator = 4;
+                     return loop1d("loop1d euler ssp104 multi-chunk parts", p, 613, 0.0004, 1e-10);
+                 }});
+    v.push_back({"loop1d euler periodic", []() {
+                     sc_params p = base(1, 600, 1, SC_PHYS_EULER);
+                     p.mthbc[0] = p.mthbc[1] = 2;
+                     return loop1d("loop1d euler periodic", p, 615, 0.005, 1e-10);
+                 }});
     v.push_back({"serve mixed", []() { return serve("serve mixed"); }});
     return v;
 }

@@ -67,12 +67,37 @@ def test_device_loop_equals_host_loop(phys, ti, n, prec):
     ("acoustics", 2, 500, SC_FP32, 1, 2)])
 def test_device_loop_1d_equals_host_loop(phys, ti, mx, prec, lim, chr_):
     """sc_advance on a 1D grid of up to 32 chunks runs its time loop on the
-    device, the grid split over one workgroup per chunk meeting at a device
-    barrier after every stage; q, step counts, rejections, dt and CFL equal
-    the host-driven loop's (SC_HOST_LOOP=1) bit for bit over consecutive
-    intervals, the first of which retakes its first step."""
+    device, the grid split over workgroups that own ranges of cells, each
+    computing a whole step on its range widened by the cells its remaining
+    stages read, meeting at a device barrier once per step; q, step
+    counts, rejections, dt and CFL equal the host-driven loop's
+    (SC_HOST_LOOP=1) bit for bit over consecutive intervals, the first of
+    which retakes its first step."""
+    _loop1d_case(phys, ti, mx, prec, lim, chr_, (1, 1) if phys == "advection" else (3, 1))
+
+
+@pytest.mark.parametrize("phys,ti,mx,prec,lim,chr_,bc,stagewise", [
+    ("euler", 4, 1600, 0, 2, 2, (2, 2), False), ("euler", 3, 900, 0, 2, 1, (2, 2), False),
+    ("euler", 4, 1600, 0, 2, 2, (3, 1), True), ("shallow_water", 2, 700, SC_FP32, 2, 0, (1, 3), True)])
+def test_device_loop_1d_barrier_per_stage(phys, ti, mx, prec, lim, chr_, bc, stagewise):
+    """The 1D device loop with a barrier after every stage, each workgroup
+    owning whole chunks: the path of periodic boundaries (whose ghost cells
+    come from the far end of the grid) and of SC_LOOP_STAGEWISE=1; bitwise
+    the host-driven loop's results."""
+    old = os.environ.get("SC_LOOP_STAGEWISE")
+    if stagewise:
+        os.environ["SC_LOOP_STAGEWISE"] = "1"
+    try:
+        _loop1d_case(phys, ti, mx, prec, lim, chr_, bc)
+    finally:
+        if old is None:
+            os.environ.pop("SC_LOOP_STAGEWISE", None)
+        else:
+            os.environ["SC_LOOP_STAGEWISE"] = old
+
+
+def _loop1d_case(phys, ti, mx, prec, lim, chr_, bc):
     rng = np.random.default_rng(43)
-    bc = (1, 1) if phys == "advection" else (3, 1)
     p = make_params(phys, (mx,), time_integrator=ti, mthbc=bc, precision=prec, char_decomp=chr_,
                     lim_type=lim, mthlim=(4, 2, 3, 5))
     q0 = O.new_grid(p)
